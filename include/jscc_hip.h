@@ -1,0 +1,192 @@
+/*
+ * libjscc_hip — C ABI of the MI355X (gfx950) reconstruction hot path of the
+ * cylindrical-detector JSCC SPECT code: the Compton cone back-projection that
+ * builds list-mode rows T from two-hit events, and the OSEM projections
+ * A.x, A^T(y/Ax) and T^T(1/Tx).
+ *
+ * Conventions
+ *   - every pointer is device memory unless stated; "stream" is a hipStream_t;
+ *   - every entry point returns 0 on success and a negative code on failure,
+ *     with a message available from jscc_last_error() (thread-local);
+ *   - the library never allocates device memory: scratch space is passed in
+ *     as (workspace, workspace_bytes), sized by the *_workspace_bytes queries;
+ *   - index tables that mirror the reference's 1-based tensors (RotMat,
+ *     RotMatInv) are int64 with an element stride, exactly as torch stores a
+ *     column of an [P, R] int64 tensor;
+ *   - "err" words are device int32 flags OR-ed by kernels when an index is out
+ *     of range (1: crystal number, 2: image gather, 4: weight gather,
+ *     8: sensitivity gather).  The host raises IndexError when set.
+ *   - storage dtypes: 0 = FP32, 1 = BF16, 2 = FP8 e4m3fn (per-row scale).
+ *
+ * Each entry point names the reference interface (file:line in
+ * lpz7777777/JSCC_Recon_SPECT_Cylinder) whose work it performs.
+ */
+#ifndef JSCC_HIP_H
+#define JSCC_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- library ------------------------------------------------------------ */
+int jscc_abi_version(void);
+const char* jscc_last_error(void);
+int jscc_device_cu_count(int* out);
+
+/* ---- list events -------------------------------------------------------- */
+
+/* Energy smearing, energy window, kinematic-domain and same-layer filters,
+ * crystal position variances and the per-event cone parameters (64-byte
+ * records, see jscc_events.h), one record per input event.
+ * Replaces process_list_plane_strict.py:152-199 together with :24-36 and
+ * :55-96 (and the identical steps of process_list_plane_sparse.py:40-72).
+ * list: [n, list_stride] FP32 rows (cpnum1, e1, cpnum2, e2, ...), 1-based
+ * crystal numbers; noise1/noise2: the N(0,1) draws of the reference's
+ * torch.randn calls (both NULL: no smearing); list_writeback: where the
+ * smeared e1/e2 are written back (the reference mutates its argument in
+ * place), may be NULL; detector: [n_det, det_stride] FP32 (x, y, z, ...).
+ * det_layers4 / rear_abs_y describe the detector's |y| layers (exactly four
+ * distinct |y| values, the largest being the rear layer).  The *_f32
+ * arguments are the FP32 roundings the reference's comparisons use. */
+int jscc_events_prepare(const float* list, int64_t n, int64_t list_stride, const float* noise1,
+                        const float* noise2, float* list_writeback, const float* detector, int64_t n_det,
+                        int64_t det_stride, int det_layers4, float rear_abs_y, float delta_r1, float delta_r2,
+                        float e0_f32, float ene_res_f32, float thr_max_f32, float thr_min_f32, float thr_sum_f32,
+                        double e0, double ene_res, void* events_out, int32_t* flags_out, int32_t* err_out,
+                        void* stream);
+
+/* Order-preserving stream compaction: out_idx[k] = k-th i with flags[i] != 0;
+ * *out_count = number of set flags.  Replaces the boolean-mask indexing
+ * t[flag] of process_list_plane_strict.py:172-198, 244-248. */
+int64_t jscc_compact_workspace_bytes(int64_t n);
+int jscc_compact_flags(const int32_t* flags, int64_t n, int32_t* out_idx, int32_t* out_count, void* workspace,
+                       int64_t workspace_bytes, void* stream);
+
+/* ---- cone rows ---------------------------------------------------------- */
+
+/* One row of T per event ev_idx[r] (events from jscc_events_prepare):
+ * t_c[r, p] = exp(-(beta - theta)^2 / (2 sigma^2)) * (KN - sin^2 beta), times
+ * the first-hit system-matrix factor, for the n_vox voxels (SoA coordinates,
+ * 16-byte aligned, padded to a multiple of 4).  sigma_model 0 = the strict
+ * model (energy + propagated position sigma, process_list_plane_strict.py:
+ * 39-52, 99-134, 201-228), 1 = a fixed angular sigma (rad).
+ * System-matrix factor: sysmat NULL -> 1; sysmat with tap tables NULL ->
+ * sysmat[arow, p] (process_list_plane_strict.py:236-240); with tap tables ->
+ * w0[p]*sysmat[arow, i0[p]] + w1[p]*sysmat[arow, i1[p]] (the coarse-grid row
+ * of process_list_plane_sparse.py:105).
+ * Outputs (each optional): out_t [n_rows, ld_out] unnormalised t in FP32 or
+ * BF16; out_tc [n_rows, ld_tc] FP32 t_c; stats [n_rows, 4] FP64 =
+ * (sum t, sum t^2, sum t_c, max t) -- the inputs of the NaN / zero-sum /
+ * effective-support filters of process_list_plane_strict.py:241-248,137-148. */
+int jscc_cone_rows(const void* events, const int32_t* ev_idx, int64_t n_rows, const float* coor_x,
+                   const float* coor_y, const float* coor_z, int64_t n_vox, const float* sysmat, int64_t ld_sysmat,
+                   const int32_t* tap_i0, const int32_t* tap_i1, const float* tap_w0, const float* tap_w1,
+                   int sigma_model, double fixed_sigma, void* out_t, int out_dtype, int64_t ld_out, float* out_tc,
+                   int64_t ld_tc, double* stats, void* stream);
+
+/* keep[i] = S finite and nonzero and S^2/Q >= min_support
+ * (process_list_plane_strict.py:241-246, 145-146); inv_sum (optional) = 1/S
+ * for kept rows (the row normalisation of :138, applied at use). */
+int jscc_rows_keep_flags(const double* stats, int64_t n, double min_support, int32_t* flags, float* inv_sum,
+                         void* stream);
+
+/* dst[i, col_off + p] = src[idx[i], p] / stats[idx[i], stat_field] (FP64 divide),
+ * optional dst[i, 0] = lead[idx[i]] (packed sparse rows).  Produces the
+ * row-normalised, compacted T / t_compton of process_list_plane_strict.py:
+ * 137-148, 244-253 and the packed rows of compton_sparse_ops.py:359-368. */
+int jscc_rows_gather_normalize(const void* src, int src_dtype, int64_t ld_src, int64_t n_cols, const int32_t* idx,
+                               int64_t n_out, const double* stats, int stat_field, const float* lead, float* dst,
+                               int64_t ld_dst, int64_t col_off, void* stream);
+
+/* t_single rows: dst[i, :] = sysmat[arow[i], :] / sum (process_list_plane_strict.py:237, 140). */
+int jscc_arows_normalize(const float* sysmat, int64_t lda, int64_t n_cols, const int32_t* arow, int64_t n_out,
+                         float* dst, int64_t ld_dst, void* stream);
+
+/* FP8 e4m3fn storage of rows with per-row scale (row_scale * code ~ t / S).
+ * No counterpart in the reference (precision mode of the MI355X engine). */
+int jscc_quantize_fp8_rows(const void* src, int src_dtype, int64_t ld_src, int64_t n_cols, int64_t n_rows,
+                           const double* stats, uint8_t* dst, int64_t ld_dst, float* row_scale, void* stream);
+
+/* ---- coarse/fine polar-grid operators ----------------------------------- */
+
+/* out[r, j] = sum_k val[k] * in[r, col[k]], k in [ptr[j], ptr[j+1]): the
+ * CSR form of compton_sparse_ops.py:293-356 (reduce_fine_rows_to_coarse,
+ * upsample_coarse_rows_to_fine, project_fine_rows_to_coarse_adjoint,
+ * reduce_fine_image_to_coarse, upsample_coarse_weight_to_fine). */
+int jscc_rows_csr_map(const float* in, int64_t ld_in, int64_t n_rows, const int32_t* ptr, const int32_t* col,
+                      const float* val, int64_t n_out, float* out, int64_t ld_out, void* stream);
+
+/* materialize_sparse_event_rows_to_fine (compton_sparse_ops.py:380-400):
+ * out[r, :] = nan_to_num(expand(t_c) * sysmat[clamp(round(c)-1)]), divided by
+ * its row sum when normalize != 0; flags[r] = row sum finite and > 0.
+ * normalize = 0 gives the unnormalised rows of the grouped
+ * get_weight_compton_sparse (recon_osem_plane_sparse.py:22-57). */
+int jscc_materialize_rows(const float* packed, int64_t ld_packed, int64_t n_rows, int64_t n_coarse,
+                          const int32_t* ptr, const int32_t* col, const float* val, int64_t n_fine,
+                          const float* sysmat, int64_t lda, int64_t M, float* out, int64_t ld_out, int normalize,
+                          int32_t* flags, void* stream);
+
+/* ---- OSEM projections --------------------------------------------------- */
+
+/* Fused one-pass projection of a block of rows (A or T), replacing
+ * get_weight_single (recon_osem_plane.py:28-30; clamped twin
+ * recon_osem_local_cntstat.py:6-8) and get_weight_compton
+ * (recon_osem_plane.py:33-42), together with the index_select rotation
+ * gather/scatter and alpha weighting of osem_*_mode (recon_osem_plane.py:45-125).
+ *   mode 0: w (+)= out_scale * G^T [ s * (y / max(s * (M x_rot), eps)) ] rows
+ *   mode 1: fwd_out[i] = s_i * (M x_rot)_i
+ *   mode 2: w (+)= out_scale * G^T [ s * coef ]
+ *   mode 3: w (+)= out_scale * G^T [ s ]  (column sums)
+ * where row i of the call is stored row (rows ? rows[row_base+i] : row_base+i),
+ * s = row_scale[stored row] (or 1), x_rot[p] = x[x_perm[p]-1] (or x), and the
+ * gather G picks partial[inv_perm[j]-1] for output j (or j).  y may be NULL
+ * (numerator 1).  use_eps = 0 reproduces the unclamped reference division.
+ * flags bit 0: y is indexed by the stored row id (random bin subsets given as
+ * a row list, recon_osem_local_cntstat.py:11-25, without copying A);
+ * bit 1: the block weight is passed through nan_to_num(0,0,0) before out_scale
+ * (recon_osem_local_sparse_jsccsd_only.py:13-19). */
+int jscc_proj_vec_width(int dtype);
+int jscc_proj_max_cols(int dtype);
+int jscc_proj_grid(int64_t n_rows);
+int64_t jscc_proj_workspace_bytes(int64_t n_rows, int64_t n_cols);
+int jscc_proj(int mode, const void* mat, int dtype, int64_t ld, int64_t n_cols, const int32_t* rows, int64_t n_rows,
+              int64_t row_base, const float* row_scale, const float* x, const int64_t* x_perm, int64_t x_perm_stride,
+              int64_t x_len, const float* y, int64_t y_stride, const float* coef, float eps, int use_eps,
+              float* fwd_out, const int64_t* inv_perm, int64_t inv_perm_stride, float out_scale, int accumulate,
+              int flags, float* w, int64_t w_len, void* workspace, int64_t workspace_bytes, int32_t* err,
+              void* stream);
+
+/* EM update with w = c1*w1 (+ c2*w2):
+ *   mode 0: x = x * w / s                        (recon_osem_plane.py:54, 73, 117-118)
+ *   mode 1: safe_em_update                        (recon_osem_plane_sparse.py:60-66)
+ *   mode 2: apply_em_update                       (distributed/python/recon_osem_dist.py:46-51)
+ * zero_after (optional) is cleared for the next subset. */
+int jscc_em_update(float* x, const float* w1, float c1, const float* w2, float c2, const float* s, int64_t n, int mode,
+                   float eps, float* zero_after, void* stream);
+
+/* Sensitivity s[j] = (1/R) sum_r sum_i A[i, inv[j, r]-1] in FP64
+ * (main_plane.py:320-323; main_local_cntstat.py:140-146). inv may be NULL. */
+int64_t jscc_sensitivity_workspace_bytes(int64_t M, int64_t P);
+int jscc_sensitivity(const float* A, int64_t M, int64_t lda, int64_t P, const int64_t* inv, int64_t inv_row_stride,
+                     int64_t inv_col_stride, int R, float* out, void* workspace, int64_t workspace_bytes,
+                     int32_t* err, void* stream);
+
+/* Self-test of the hardware FP8 decoder: out512[0..255] hardware decode of
+ * every e4m3fn code, out512[256..511] the library's software decode. */
+int jscc_fp8_selftest(float* out512, void* stream);
+
+/* Built-in kernel timer (no counterpart in the reference).  While enabled
+ * (per calling thread), every cone-row launch (tag 0) and fused projection
+ * launch (tag 1) is bracketed by a HIP event pair on its own stream;
+ * jscc_prof_read synchronises the pairs of one tag and returns their count and
+ * summed milliseconds. */
+int jscc_prof_enable(int on);
+int jscc_prof_read(int tag, int* count, double* total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* JSCC_HIP_H */

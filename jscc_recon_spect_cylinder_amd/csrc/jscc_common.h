@@ -1,0 +1,144 @@
+// Shared device helpers for the gfx950 kernels of libjscc_hip.
+//
+// Everything here is CDNA-specific: 64-lane wavefronts, DPP/bpermute wave
+// reductions, BF16 / OCP-FP8 (e4m3fn) storage formats decoded to FP32 in
+// registers.  No CUDA, no dual paths.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+#define JSCC_WAVE 64
+
+// ---------------------------------------------------------------------------
+// error reporting (host side; one thread-local message per calling thread)
+// ---------------------------------------------------------------------------
+void jscc_set_error(const char* fmt, ...);
+
+#define JSCC_CHECK_HIP(expr)                                                     \
+  do {                                                                           \
+    hipError_t _e = (expr);                                                      \
+    if (_e != hipSuccess) {                                                      \
+      jscc_set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e),      \
+                     __FILE__, __LINE__);                                        \
+      return -1;                                                                 \
+    }                                                                            \
+  } while (0)
+
+#define JSCC_REQUIRE(cond, ...)                                                  \
+  do {                                                                           \
+    if (!(cond)) {                                                               \
+      jscc_set_error(__VA_ARGS__);                                               \
+      return -2;                                                                 \
+    }                                                                            \
+  } while (0)
+
+// after a launch: catch launch-configuration errors without synchronising
+#define JSCC_CHECK_LAUNCH()                                                      \
+  do {                                                                           \
+    hipError_t _e = hipGetLastError();                                           \
+    if (_e != hipSuccess) {                                                      \
+      jscc_set_error("kernel launch failed: %s (%s:%d)", hipGetErrorString(_e),  \
+                     __FILE__, __LINE__);                                        \
+      return -1;                                                                 \
+    }                                                                            \
+  } while (0)
+
+// storage dtype codes shared with the C ABI
+enum {
+  JSCC_DT_F32 = 0,
+  JSCC_DT_BF16 = 1,
+  JSCC_DT_FP8 = 2,  // OCP e4m3fn, per-row scale carried separately
+};
+
+// ---------------------------------------------------------------------------
+// storage formats
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float jscc_bf16_to_f32(uint16_t h) {
+  return __uint_as_float(((uint32_t)h) << 16);
+}
+
+// round-to-nearest-even FP32 -> BF16 (NaN kept quiet)
+__device__ __forceinline__ uint16_t jscc_f32_to_bf16(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x0040u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+// e4m3fn -> FP32.  The 7 magnitude bits are placed at the top of the FP32
+// exponent/mantissa field and rescaled by 2^120 (= 2^(127-7)), which handles
+// normals and subnormals with one multiply.  0x7f / 0xff are NaN.
+__device__ __forceinline__ float jscc_fp8_to_f32(uint32_t b) {
+  uint32_t mag = b & 0x7fu;
+  float v = __uint_as_float(mag << 20) * 1.329227995784916e36f;  // 2^120
+  if (mag == 0x7fu) v = __uint_as_float(0x7fc00000u);
+  return (b & 0x80u) ? -v : v;
+}
+
+// FP32 -> e4m3fn, round-to-nearest-even, saturating to +-448.
+__device__ __forceinline__ uint32_t jscc_f32_to_fp8(float f) {
+  uint32_t u = __float_as_uint(f);
+  uint32_t sign = (u >> 24) & 0x80u;
+  float a = fabsf(f);
+  if (!(a == a)) return 0x7fu | sign;  // NaN
+  if (a >= 448.0f) return 0x7eu | sign;  // saturate (e4m3fn max = 448)
+  if (a < 0.0009765625f) {  // below half of the smallest subnormal 2^-9 -> maybe 0
+    // subnormal grid is k * 2^-9, k in 0..7 ; handled below as well
+  }
+  // exponent of a (unbiased); a>0 here unless zero
+  if (a == 0.0f) return sign;
+  int e = (int)((__float_as_uint(a) >> 23) & 0xff) - 127;
+  if (e < -6) {
+    // subnormal in e4m3: value = k * 2^-9
+    float k = rintf(a * 512.0f);  // RNE on the 2^-9 grid
+    uint32_t kk = (uint32_t)k;    // 0..8 ; 8 becomes the smallest normal
+    return sign | kk;             // kk==8 -> 0b0001000 = exp 1, man 0 (2^-6): correct
+  }
+  // normal: keep 3 mantissa bits with RNE
+  uint32_t m = __float_as_uint(a) & 0x7fffffu;
+  uint32_t keep = m >> 20;
+  uint32_t rem = m & 0xfffffu;
+  uint32_t half = 0x80000u;
+  if (rem > half || (rem == half && (keep & 1u))) keep += 1;
+  uint32_t ee = (uint32_t)(e + 7);
+  if (keep == 8) { keep = 0; ee += 1; }
+  uint32_t code = (ee << 3) | keep;
+  if (code > 0x7eu) code = 0x7eu;
+  return sign | code;
+}
+
+// ---------------------------------------------------------------------------
+// wave / block reductions (64-wide wavefronts)
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T jscc_wave_sum(T v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, JSCC_WAVE);
+  return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T jscc_wave_max(T v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    T o = __shfl_xor(v, off, JSCC_WAVE);
+    v = (o > v) ? o : v;
+  }
+  return v;
+}
+
+__device__ __forceinline__ int jscc_lane() { return threadIdx.x & (JSCC_WAVE - 1); }
+__device__ __forceinline__ int jscc_wave_id() { return threadIdx.x >> 6; }
+
+// number of compute units of the current device (cached per process)
+int jscc_num_cus();
+
+// Built-in kernel timer: when enabled, hot kernels are bracketed by HIP event
+// pairs recorded on their launch stream (tag = kernel family).
+enum { JSCC_TAG_CONE = 0, JSCC_TAG_PROJ = 1, JSCC_NUM_TAGS = 2 };
+bool jscc_prof_on();
+void jscc_prof_begin(int tag, hipStream_t s);
+void jscc_prof_end(int tag, hipStream_t s);

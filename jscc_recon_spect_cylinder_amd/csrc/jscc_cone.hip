@@ -1,0 +1,317 @@
+// Compton cone back-projection: one row of the list-mode matrix T per event.
+//
+// Reference: process_list_plane_strict.py:99-134 (sigma_pos), :39-52
+// (sigma_ene), :201-248 (cone weight, first-hit weighting, row statistics);
+// process_list_plane_sparse.py:78-114 (same on coarse voxels with the
+// 2-tap-reduced system-matrix row).
+//
+// Layout / mapping (gfx950):
+//   * a work-group owns E consecutive events (rows) and sweeps all P voxels in
+//     tiles of 4*256; every thread evaluates 4 consecutive voxels for each of
+//     the E events.  Event parameters are wave-uniform, so they live in scalar
+//     registers (s_load), not LDS.
+//   * voxel coordinates are SoA float arrays padded to a multiple of 4, read as
+//     float4 (one 16-byte load per component per thread, contiguous across the
+//     wave); the first-hit system-matrix row is read the same way.
+//   * beta = atan2(|v01 x v12|, v01 . v12) with a hand-fitted polynomial: well
+//     conditioned at the cone axis, unlike acos of an FP32 cosine.
+//   * rows are written unnormalised (FP32 or BF16) and the row statistics
+//     S = sum t, Q = sum t^2, Sc = sum t_c and max t are reduced in FP64; the
+//     normalisation 1/S is applied by whoever consumes the row.
+#include "jscc_common.h"
+#include "jscc_events.h"
+
+namespace {
+
+enum { SIG_STRICT = 0, SIG_FIXED = 1 };
+enum { AF_NONE = 0, AF_ROW = 1, AF_TAPS = 2 };
+
+constexpr int kConeThreads = 256;
+constexpr int kConeVec = 4;
+
+// atan(r) on [0,1] as r*P(r^2); coefficients fitted to minimise relative
+// error (max abs error 1.9e-7 rad in FP32).
+__device__ __forceinline__ float atan_01(float r) {
+  const float s = r * r;
+  float p = -4.687702868e-03f;
+  p = fmaf(p, s, 2.423363684e-02f);
+  p = fmaf(p, s, -5.946168906e-02f);
+  p = fmaf(p, s, 9.912689883e-02f);
+  p = fmaf(p, s, -1.401895101e-01f);
+  p = fmaf(p, s, 1.996964129e-01f);
+  p = fmaf(p, s, -3.333198598e-01f);
+  p = fmaf(p, s, 9.999999005e-01f);
+  return r * p;
+}
+
+// atan2(y, x) for y >= 0
+__device__ __forceinline__ float atan2_ypos(float y, float x) {
+  const float ax = fabsf(x);
+  const float mn = fminf(ax, y);
+  const float mx = fmaxf(fmaxf(ax, y), 1e-30f);
+  float a = atan_01(mn * __builtin_amdgcn_rcpf(mx));
+  a = (y > ax) ? (1.57079632679489662f - a) : a;
+  a = (x < 0.0f) ? (3.14159265358979324f - a) : a;
+  return a;
+}
+
+struct ConeConsts {
+  float beta_min, beta_max;  // acos of the reference's FP32 clamp bounds
+  float s2_min;              // 1 - cos_max^2 : sin^2 floor implied by the clamp
+  float fixed_half_inv_sig2; // 0.5 / sigma^2 for the fixed-sigma model
+};
+
+template <int SIG, int AF, int OUT_DT, bool WANT_TC, int E>
+__global__ void __launch_bounds__(kConeThreads) k_cone_rows(
+    const JsccEvent* __restrict__ ev, const int32_t* __restrict__ ev_idx, int n_rows,
+    const float* __restrict__ cx, const float* __restrict__ cy, const float* __restrict__ cz, int P,
+    const float* __restrict__ A, int64_t lda, const int32_t* __restrict__ tap_i0,
+    const int32_t* __restrict__ tap_i1, const float* __restrict__ tap_w0, const float* __restrict__ tap_w1,
+    ConeConsts cc, void* __restrict__ out_t, int64_t ldt, float* __restrict__ out_tc, int64_t ldtc,
+    double* __restrict__ stats) {
+  const int row0 = blockIdx.x * E;
+  // event parameters: uniform per work-group
+  float p1x[E], p1y[E], p1z[E], ax[E], ay[E], az[E], d12sq[E], inv_d12sq[E], th[E], kn[E];
+  float sp[E], sm[E], wx[E], wy[E], wz[E];
+  int64_t arow_off[E];
+  bool valid[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    valid[e] = (row0 + e) < n_rows;
+    const int r = valid[e] ? (row0 + e) : (n_rows - 1);
+    const JsccEvent& q = ev[ev_idx[r]];
+    p1x[e] = q.p1x; p1y[e] = q.p1y; p1z[e] = q.p1z;
+    ax[e] = q.ax; ay[e] = q.ay; az[e] = q.az;
+    d12sq[e] = q.d12sq;
+    inv_d12sq[e] = 1.0f / q.d12sq;
+    th[e] = q.theta; kn[e] = q.kn;
+    sp[e] = q.sig_plus; sm[e] = q.sig_minus;
+    wx[e] = q.wx; wy[e] = q.wy; wz[e] = q.wz;
+    arow_off[e] = (int64_t)q.arow * lda;
+  }
+  double accS[E], accQ[E], accC[E];
+  float accM[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) { accS[e] = 0.0; accQ[e] = 0.0; accC[e] = 0.0; accM[e] = 0.0f; }
+
+  for (int base = threadIdx.x * kConeVec; base < P; base += kConeThreads * kConeVec) {
+    const float4 vx4 = *reinterpret_cast<const float4*>(cx + base);
+    const float4 vy4 = *reinterpret_cast<const float4*>(cy + base);
+    const float4 vz4 = *reinterpret_cast<const float4*>(cz + base);
+    const float vcx[4] = {vx4.x, vx4.y, vx4.z, vx4.w};
+    const float vcy[4] = {vy4.x, vy4.y, vy4.z, vy4.w};
+    const float vcz[4] = {vz4.x, vz4.y, vz4.z, vz4.w};
+    float tw0[4], tw1[4];
+    int ti0[4], ti1[4];
+    if (AF == AF_TAPS) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int p = base + k < P ? base + k : P - 1;
+        ti0[k] = tap_i0[p]; ti1[k] = tap_i1[p]; tw0[k] = tap_w0[p]; tw1[k] = tap_w1[p];
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      float av[4] = {1.0f, 1.0f, 1.0f, 1.0f};
+      if (AF == AF_ROW) {
+        const float4 a4 = *reinterpret_cast<const float4*>(A + arow_off[e] + base);
+        av[0] = a4.x; av[1] = a4.y; av[2] = a4.z; av[3] = a4.w;
+      } else if (AF == AF_TAPS) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const float* Ar = A + arow_off[e];
+          av[k] = fmaf(tw1[k], Ar[ti1[k]], tw0[k] * Ar[ti0[k]]);
+        }
+      }
+      float tv[4], tcv[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const float vx = p1x[e] - vcx[k];
+        const float vy = p1y[e] - vcy[k];
+        const float vz = p1z[e] - vcz[k];
+        const float d01sq = fmaf(vx, vx, fmaf(vy, vy, vz * vz));
+        const float dot = fmaf(vx, ax[e], fmaf(vy, ay[e], vz * az[e]));
+        const float mx = fmaf(vy, az[e], -vz * ay[e]);
+        const float my = fmaf(vz, ax[e], -vx * az[e]);
+        const float mz = fmaf(vx, ay[e], -vy * ax[e]);
+        const float msq = fmaf(mx, mx, fmaf(my, my, mz * mz));
+        float beta = atan2_ypos(__builtin_sqrtf(msq), dot);
+        beta = fminf(fmaxf(beta, cc.beta_min), cc.beta_max);
+        const float inv_d01sq = __builtin_amdgcn_rcpf(d01sq);
+        float s2 = msq * inv_d01sq * inv_d12sq[e];
+        s2 = fminf(fmaxf(s2, cc.s2_min), 1.0f);
+        const float delta = beta - th[e];
+        float hinv;
+        if (SIG == SIG_STRICT) {
+          const float sige = delta >= 0.0f ? sp[e] : sm[e];
+          const float qx = fmaf(d12sq[e], vx, -dot * ax[e]);
+          const float qy = fmaf(d12sq[e], vy, -dot * ay[e]);
+          const float qz = fmaf(d12sq[e], vz, -dot * az[e]);
+          const float num = fmaf(qx * qx, wx[e], fmaf(qy * qy, wy[e], qz * qz * wz[e]));
+          const float den = d12sq[e] * d12sq[e] * fmaxf(msq, 1e-12f * d01sq * d12sq[e]);
+          const float spos2 = fmaxf(num * __builtin_amdgcn_rcpf(den), 1e-12f);
+          const float sig2 = fmaxf(fmaf(sige, sige, spos2), 1e-12f);
+          hinv = 0.5f * __builtin_amdgcn_rcpf(sig2);
+        } else {
+          hinv = cc.fixed_half_inv_sig2;
+        }
+        const float tc = __expf(-(delta * delta) * hinv) * (kn[e] - s2);
+        const bool in = (base + k) < P;
+        tcv[k] = in ? tc : 0.0f;
+        tv[k] = in ? tc * av[k] : 0.0f;
+      }
+      if (valid[e]) {
+        const int64_t r = (int64_t)(row0 + e);
+        if (OUT_DT == JSCC_DT_F32) {
+          *reinterpret_cast<float4*>(reinterpret_cast<float*>(out_t) + r * ldt + base) =
+              make_float4(tv[0], tv[1], tv[2], tv[3]);
+        } else if (OUT_DT == JSCC_DT_BF16) {
+          uint2 pk;
+          pk.x = (uint32_t)jscc_f32_to_bf16(tv[0]) | ((uint32_t)jscc_f32_to_bf16(tv[1]) << 16);
+          pk.y = (uint32_t)jscc_f32_to_bf16(tv[2]) | ((uint32_t)jscc_f32_to_bf16(tv[3]) << 16);
+          *reinterpret_cast<uint2*>(reinterpret_cast<uint16_t*>(out_t) + r * ldt + base) = pk;
+        }
+        if (WANT_TC) {
+          *reinterpret_cast<float4*>(out_tc + r * ldtc + base) = make_float4(tcv[0], tcv[1], tcv[2], tcv[3]);
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        accS[e] += (double)tv[k];
+        accQ[e] += (double)tv[k] * (double)tv[k];
+        accC[e] += (double)tcv[k];
+        accM[e] = fmaxf(accM[e], tv[k]);  // fmaxf drops NaN; NaN rows are caught by S
+      }
+    }
+  }
+
+  // block reduction of the E x 4 statistics
+  __shared__ double red[4][kConeThreads / JSCC_WAVE][E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    double s = jscc_wave_sum(accS[e]);
+    double q = jscc_wave_sum(accQ[e]);
+    double c = jscc_wave_sum(accC[e]);
+    float m = jscc_wave_max(accM[e]);
+    if (jscc_lane() == 0) {
+      red[0][jscc_wave_id()][e] = s;
+      red[1][jscc_wave_id()][e] = q;
+      red[2][jscc_wave_id()][e] = c;
+      red[3][jscc_wave_id()][e] = (double)m;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < E * 4) {
+    const int e = threadIdx.x / 4, k = threadIdx.x % 4;
+    if (row0 + e < n_rows) {
+      double v = red[k][0][e];
+      for (int w = 1; w < kConeThreads / JSCC_WAVE; ++w) v = (k == 3) ? fmax(v, red[k][w][e]) : v + red[k][w][e];
+      stats[(int64_t)(row0 + e) * 4 + k] = v;
+    }
+  }
+}
+
+template <int SIG, int AF, int OUT_DT, bool WANT_TC>
+void launch_cone(int n_rows, hipStream_t s, const JsccEvent* ev, const int32_t* idx, const float* cx,
+                 const float* cy, const float* cz, int P, const float* A, int64_t lda, const int32_t* ti0,
+                 const int32_t* ti1, const float* tw0, const float* tw1, ConeConsts cc, void* out_t,
+                 int64_t ldt, float* out_tc, int64_t ldtc, double* stats) {
+  constexpr int E = 4;
+  dim3 grid((unsigned)((n_rows + E - 1) / E));
+  hipLaunchKernelGGL((k_cone_rows<SIG, AF, OUT_DT, WANT_TC, E>), grid, dim3(kConeThreads), 0, s, ev, idx, n_rows,
+                     cx, cy, cz, P, A, lda, ti0, ti1, tw0, tw1, cc, out_t, ldt, out_tc, ldtc, stats);
+}
+
+template <int SIG, int AF>
+int dispatch_out(int out_dtype, bool want_tc, int n_rows, hipStream_t s, const JsccEvent* ev, const int32_t* idx,
+                 const float* cx, const float* cy, const float* cz, int P, const float* A, int64_t lda,
+                 const int32_t* ti0, const int32_t* ti1, const float* tw0, const float* tw1, ConeConsts cc,
+                 void* out_t, int64_t ldt, float* out_tc, int64_t ldtc, double* stats) {
+#define JSCC_CONE_CASE(DT, TC)                                                                              \
+  launch_cone<SIG, AF, DT, TC>(n_rows, s, ev, idx, cx, cy, cz, P, A, lda, ti0, ti1, tw0, tw1, cc, out_t, ldt, \
+                               out_tc, ldtc, stats)
+  if (out_dtype == JSCC_DT_F32) {
+    if (want_tc) JSCC_CONE_CASE(JSCC_DT_F32, true); else JSCC_CONE_CASE(JSCC_DT_F32, false);
+  } else if (out_dtype == JSCC_DT_BF16) {
+    if (want_tc) JSCC_CONE_CASE(JSCC_DT_BF16, true); else JSCC_CONE_CASE(JSCC_DT_BF16, false);
+  } else if (out_dtype == -1) {  // statistics (+ optional t_c) only
+    if (want_tc) JSCC_CONE_CASE(-1, true); else JSCC_CONE_CASE(-1, false);
+  } else {
+    jscc_set_error("jscc_cone_rows: unsupported output dtype %d", out_dtype);
+    return -2;
+  }
+#undef JSCC_CONE_CASE
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int jscc_cone_rows(const void* events, const int32_t* ev_idx, int64_t n_rows, const float* coor_x,
+                              const float* coor_y, const float* coor_z, int64_t n_vox, const float* sysmat,
+                              int64_t ld_sysmat, const int32_t* tap_i0, const int32_t* tap_i1, const float* tap_w0,
+                              const float* tap_w1, int sigma_model, double fixed_sigma, void* out_t,
+                              int out_dtype, int64_t ld_out, float* out_tc, int64_t ld_tc, double* stats,
+                              void* stream) {
+  JSCC_REQUIRE(n_rows >= 0 && n_rows < ((int64_t)1 << 31), "jscc_cone_rows: n_rows out of range");
+  if (n_rows == 0) return 0;
+  JSCC_REQUIRE(events && ev_idx && coor_x && coor_y && coor_z && stats, "jscc_cone_rows: null pointer");
+  JSCC_REQUIRE(n_vox > 0 && n_vox < ((int64_t)1 << 30), "jscc_cone_rows: bad voxel count");
+  JSCC_REQUIRE(((uintptr_t)coor_x | (uintptr_t)coor_y | (uintptr_t)coor_z) % 16 == 0,
+               "jscc_cone_rows: coordinate arrays must be 16-byte aligned and padded to a multiple of 4");
+  JSCC_REQUIRE(sigma_model == SIG_STRICT || sigma_model == SIG_FIXED, "jscc_cone_rows: bad sigma model");
+  if (out_t) {
+    JSCC_REQUIRE(ld_out >= n_vox && ld_out % 4 == 0, "jscc_cone_rows: ld_out must be >= n_vox and a multiple of 4");
+    JSCC_REQUIRE((uintptr_t)out_t % 16 == 0, "jscc_cone_rows: out_t must be 16-byte aligned");
+  } else {
+    out_dtype = -1;
+  }
+  if (out_tc) {
+    JSCC_REQUIRE(ld_tc >= n_vox && ld_tc % 4 == 0 && (uintptr_t)out_tc % 16 == 0,
+                 "jscc_cone_rows: out_tc must be 16-byte aligned with ld a multiple of 4");
+  }
+  const bool taps = tap_i0 != nullptr;
+  if (taps) JSCC_REQUIRE(tap_i1 && tap_w0 && tap_w1 && sysmat, "jscc_cone_rows: incomplete tap tables");
+  if (sysmat && !taps) {
+    JSCC_REQUIRE(ld_sysmat >= n_vox && ld_sysmat % 4 == 0 && (uintptr_t)sysmat % 16 == 0,
+                 "jscc_cone_rows: sysmat rows must be 16-byte aligned with ld a multiple of 4");
+  }
+  if (sigma_model == SIG_FIXED) JSCC_REQUIRE(fixed_sigma > 0.0, "jscc_cone_rows: fixed sigma must be positive");
+
+  ConeConsts cc;
+  const double cmax = (double)(float)(1.0 - 1e-7);
+  cc.beta_min = (float)acos(cmax);
+  cc.beta_max = (float)(M_PI - acos(cmax));
+  cc.s2_min = (float)(1.0 - cmax * cmax);
+  cc.fixed_half_inv_sig2 = sigma_model == SIG_FIXED ? (float)(0.5 / (fixed_sigma * fixed_sigma)) : 0.0f;
+
+  hipStream_t s = (hipStream_t)stream;
+  const JsccEvent* ev = (const JsccEvent*)events;
+  const bool want_tc = out_tc != nullptr;
+  const int P = (int)n_vox;
+  const int n = (int)n_rows;
+  int rc;
+  jscc_prof_begin(JSCC_TAG_CONE, s);
+#define JSCC_CONE_AF(SIG)                                                                                   \
+  if (taps)                                                                                                   \
+    rc = dispatch_out<SIG, AF_TAPS>(out_dtype, want_tc, n, s, ev, ev_idx, coor_x, coor_y, coor_z, P, sysmat,  \
+                                    ld_sysmat, tap_i0, tap_i1, tap_w0, tap_w1, cc, out_t, ld_out, out_tc,      \
+                                    ld_tc, stats);                                                            \
+  else if (sysmat)                                                                                            \
+    rc = dispatch_out<SIG, AF_ROW>(out_dtype, want_tc, n, s, ev, ev_idx, coor_x, coor_y, coor_z, P, sysmat,   \
+                                   ld_sysmat, tap_i0, tap_i1, tap_w0, tap_w1, cc, out_t, ld_out, out_tc,       \
+                                   ld_tc, stats);                                                             \
+  else                                                                                                        \
+    rc = dispatch_out<SIG, AF_NONE>(out_dtype, want_tc, n, s, ev, ev_idx, coor_x, coor_y, coor_z, P, sysmat,  \
+                                    ld_sysmat, tap_i0, tap_i1, tap_w0, tap_w1, cc, out_t, ld_out, out_tc,      \
+                                    ld_tc, stats);
+  if (sigma_model == SIG_STRICT) {
+    JSCC_CONE_AF(SIG_STRICT)
+  } else {
+    JSCC_CONE_AF(SIG_FIXED)
+  }
+#undef JSCC_CONE_AF
+  jscc_prof_end(JSCC_TAG_CONE, s);
+  if (rc != 0) return rc;
+  JSCC_CHECK_LAUNCH();
+  return 0;
+}

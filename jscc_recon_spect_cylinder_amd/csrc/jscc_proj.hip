@@ -1,0 +1,497 @@
+// OSEM projection kernels.
+//
+// Reference: recon_osem_plane.py:28-42 (get_weight_single / get_weight_compton),
+// :45-125 (osem_*_mode: rotation gathers/scatters, alpha weighting, update),
+// recon_osem_local_cntstat.py:6-8 (clamped twin), recon_osem_plane_sparse.py:60-66
+// (safe_em_update), distributed/python/recon_osem_dist.py:46-51 (apply_em_update),
+// main_plane.py:320-323 (sensitivity).
+//
+// One pass over the matrix computes both projections: a work-group keeps the
+// (permuted) image and its partial back-projection in registers, streams its
+// share of rows once from HBM, reduces each row's dot product across the
+// work-group, forms the ratio y/max(Ax,eps) and accumulates ratio*row into the
+// registers.  Per-work-group partial back-projections are summed by a
+// two-level deterministic column reduction that also applies the inverse
+// rotation gather and the alpha weight.  Rows are FP32, BF16 or FP8-e4m3fn
+// (with a per-row scale), always accumulated in FP32.
+#include "jscc_common.h"
+
+namespace {
+
+enum { MODE_RATIO = 0, MODE_FWD = 1, MODE_COEF = 2, MODE_ONES = 3 };
+
+constexpr int kProjThreads = 512;
+constexpr int kProjWaves = kProjThreads / JSCC_WAVE;
+
+typedef float jscc_f4v __attribute__((ext_vector_type(4)));
+typedef unsigned int jscc_u4v __attribute__((ext_vector_type(4)));
+typedef float jscc_f2v __attribute__((ext_vector_type(2)));
+
+template <int DT> struct Vec;
+template <> struct Vec<JSCC_DT_F32> { static constexpr int N = 4; using T = jscc_f4v; };
+template <> struct Vec<JSCC_DT_BF16> { static constexpr int N = 8; using T = jscc_u4v; };
+template <> struct Vec<JSCC_DT_FP8> { static constexpr int N = 16; using T = jscc_u4v; };
+
+template <int DT>
+__device__ __forceinline__ void decode(const typename Vec<DT>::T& raw, float* v);
+
+template <>
+__device__ __forceinline__ void decode<JSCC_DT_F32>(const jscc_f4v& r, float* v) {
+  v[0] = r.x; v[1] = r.y; v[2] = r.z; v[3] = r.w;
+}
+template <>
+__device__ __forceinline__ void decode<JSCC_DT_BF16>(const jscc_u4v& r, float* v) {
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    v[2 * k] = __uint_as_float(r[k] << 16);
+    v[2 * k + 1] = __uint_as_float(r[k] & 0xffff0000u);
+  }
+}
+// gfx950 converts OCP e4m3fn in hardware, two bytes per instruction
+template <>
+__device__ __forceinline__ void decode<JSCC_DT_FP8>(const jscc_u4v& r, float* v) {
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const jscc_f2v lo = __builtin_amdgcn_cvt_pk_f32_fp8((int)r[k], false);
+    const jscc_f2v hi = __builtin_amdgcn_cvt_pk_f32_fp8((int)r[k], true);
+    v[4 * k] = lo.x; v[4 * k + 1] = lo.y; v[4 * k + 2] = hi.x; v[4 * k + 3] = hi.y;
+  }
+}
+
+struct ProjArgs {
+  const void* M;             // matrix rows
+  int64_t ld;                // row stride (elements)
+  int P;                     // columns (multiple of the vector width)
+  const int32_t* rows;       // optional stored-row index list
+  int64_t n_rows;            // rows in this call
+  int64_t row_base;          // offset into rows / stored row ids
+  const float* row_scale;    // optional per stored row scale
+  const float* x;            // image [P_img]
+  const int64_t* x_perm;     // optional 1-based gather x_rot[p] = x[perm[p]-1]
+  int64_t x_perm_stride;
+  int64_t x_len;             // length of x (bounds for the gather)
+  const float* y;            // optional numerator per call row (ones if null)
+  int64_t y_stride;
+  const float* coef;         // MODE_COEF: per call row coefficient
+  float eps;                 // clamp of A x (when use_eps)
+  int use_eps;
+  float* fwd_out;            // MODE_FWD: per call row A x
+  float* partial;            // [gridDim.x, P]
+  int32_t* err;              // index error flag
+  int y_by_row;              // numerator indexed by stored row id instead of call row
+};
+
+template <int DT, int NCH, int R, int MODE>
+__global__ void __launch_bounds__(kProjThreads) k_proj_fused(ProjArgs a) {
+  constexpr int VN = Vec<DT>::N;
+  using VT = typename Vec<DT>::T;
+  const int tid = threadIdx.x;
+  const int nvec = a.P / VN;
+
+  // the image (gathered through the rotation) lives in registers
+  float xr[NCH][VN];
+  float wr[NCH][VN];
+#pragma unroll
+  for (int c = 0; c < NCH; ++c) {
+    const int vi = c * kProjThreads + tid;
+#pragma unroll
+    for (int k = 0; k < VN; ++k) {
+      wr[c][k] = 0.0f;
+      float xv = 0.0f;
+      if (MODE != MODE_ONES && MODE != MODE_COEF && vi < nvec) {
+        const int64_t p = (int64_t)vi * VN + k;
+        int64_t src = p;
+        if (a.x_perm) {
+          src = a.x_perm[p * a.x_perm_stride] - 1;
+          if (src < 0 || src >= a.x_len) { atomicOr(a.err, 2); src = 0; }
+        }
+        xv = a.x[src];
+      }
+      xr[c][k] = xv;
+    }
+  }
+
+  // this work-group's contiguous share of the rows
+  const int64_t per = a.n_rows / gridDim.x, rem = a.n_rows % gridDim.x;
+  const int64_t r_begin = blockIdx.x * per + (blockIdx.x < rem ? blockIdx.x : rem);
+  const int64_t r_end = r_begin + per + (blockIdx.x < rem ? 1 : 0);
+
+  __shared__ float red[2][R][kProjWaves];
+  int buf = 0;
+  for (int64_t i0 = r_begin; i0 < r_end; i0 += R) {
+    // rows stay in their storage format in registers and are decoded at use
+    VT raw[R][NCH];
+    float sc[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int64_t i = i0 + q;
+      const bool live = i < r_end;
+      const int64_t li = live ? i : r_begin;
+      const int64_t sid = a.rows ? (int64_t)a.rows[a.row_base + li] : a.row_base + li;
+      sc[q] = a.row_scale ? a.row_scale[sid] : 1.0f;
+      const VT* rowp = reinterpret_cast<const VT*>(reinterpret_cast<const char*>(a.M) +
+                                                  sid * a.ld * (DT == JSCC_DT_F32 ? 4 : (DT == JSCC_DT_BF16 ? 2 : 1)));
+#pragma unroll
+      for (int c = 0; c < NCH; ++c) {
+        const int vi = c * kProjThreads + tid;
+        if (live && vi < nvec) {
+          raw[q][c] = __builtin_nontemporal_load(rowp + vi);
+        } else {
+          raw[q][c] = VT(0);
+        }
+      }
+    }
+    float coefq[R];
+    if (MODE == MODE_RATIO || MODE == MODE_FWD) {
+      float d[R];
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        float acc = 0.0f;
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+          float v[VN];
+          decode<DT>(raw[q][c], v);
+#pragma unroll
+          for (int k = 0; k < VN; ++k) acc = fmaf(v[k], xr[c][k], acc);
+        }
+        d[q] = jscc_wave_sum(acc);
+      }
+      if (jscc_lane() == 0) {
+#pragma unroll
+        for (int q = 0; q < R; ++q) red[buf][q][jscc_wave_id()] = d[q];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        float f = 0.0f;
+#pragma unroll
+        for (int w = 0; w < kProjWaves; ++w) f += red[buf][q][w];
+        f *= sc[q];
+        const int64_t i = i0 + q;
+        if (MODE == MODE_FWD) {
+          if (tid == 0 && i < r_end) a.fwd_out[i] = f;
+          coefq[q] = 0.0f;
+        } else {
+          const int64_t yi = a.y_by_row ? (a.rows ? (int64_t)a.rows[a.row_base + i] : a.row_base + i) : i;
+          const float num = (a.y && i < r_end) ? a.y[yi * a.y_stride] : 1.0f;
+          const float den = a.use_eps ? fmaxf(f, a.eps) : f;
+          coefq[q] = (i < r_end) ? sc[q] * (num / den) : 0.0f;
+        }
+      }
+      buf ^= 1;
+    } else if (MODE == MODE_COEF) {
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        const int64_t i = i0 + q;
+        coefq[q] = (i < r_end) ? sc[q] * a.coef[i] : 0.0f;
+      }
+    } else {  // MODE_ONES
+#pragma unroll
+      for (int q = 0; q < R; ++q) coefq[q] = (i0 + q < r_end) ? sc[q] : 0.0f;
+    }
+    if (MODE != MODE_FWD) {
+#pragma unroll
+      for (int q = 0; q < R; ++q)
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+          float v[VN];
+          decode<DT>(raw[q][c], v);
+#pragma unroll
+          for (int k = 0; k < VN; ++k) wr[c][k] = fmaf(coefq[q], v[k], wr[c][k]);
+        }
+    }
+  }
+
+  if (MODE != MODE_FWD) {
+    float* dst = a.partial + (int64_t)blockIdx.x * a.P;
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+      const int vi = c * kProjThreads + tid;
+      if (vi < nvec) {
+#pragma unroll
+        for (int k = 0; k < VN; k += 4)
+          *reinterpret_cast<float4*>(dst + (int64_t)vi * VN + k) =
+              make_float4(wr[c][k], wr[c][k + 1], wr[c][k + 2], wr[c][k + 3]);
+      }
+    }
+  }
+}
+
+// partial [G, P] -> partial2 [ceil(G/64), P]
+constexpr int kRedRows = 64;
+__global__ void __launch_bounds__(256) k_reduce_stage1(const float* __restrict__ part, int G, int P,
+                                                       float* __restrict__ out) {
+  const int col = (blockIdx.x * 256 + threadIdx.x) * 4;
+  if (col >= P) return;
+  const int g0 = blockIdx.y * kRedRows;
+  const int g1 = min(G, g0 + kRedRows);
+  float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int g = g0; g < g1; ++g) {
+    const float4 v = *reinterpret_cast<const float4*>(part + (int64_t)g * P + col);
+    s.x += v.x; s.y += v.y; s.z += v.z; s.w += v.w;
+  }
+  *reinterpret_cast<float4*>(out + (int64_t)blockIdx.y * P + col) = s;
+}
+
+// w[j] = (accumulate ? w[j] : 0) + scale * sum_g part[g][src(j)],
+// src(j) = inv_perm[j]-1 (the reference's index_select with RotMatInv)
+__global__ void __launch_bounds__(256) k_reduce_stage2(const float* __restrict__ part, int G, int P,
+                                                       const int64_t* __restrict__ inv_perm, int64_t ips,
+                                                       float scale, int accumulate, int sanitize,
+                                                       float* __restrict__ w, int64_t w_len,
+                                                       int32_t* __restrict__ err) {
+  const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (j >= w_len) return;
+  int64_t src = j;
+  if (inv_perm) {
+    src = inv_perm[j * ips] - 1;
+    if (src < 0 || src >= P) { atomicOr(err, 4); src = 0; }
+  } else if (j >= P) {
+    return;
+  }
+  float s = 0.0f;
+  for (int g = 0; g < G; ++g) s += part[(int64_t)g * P + src];
+  if (sanitize && !(s == s && fabsf(s) != __builtin_inff())) s = 0.0f;  // nan_to_num(0, 0, 0)
+  const float add = scale == 1.0f ? s : scale * s;
+  w[j] = accumulate ? w[j] + add : add;
+}
+
+// EM updates: 0 plain x*w/s ; 1 safe_em_update ; 2 apply_em_update.
+// w = c1*w1 (+ c2*w2 when w2 is given), evaluated like the reference's
+// (2-alpha)*weight_compton + alpha*weight_single.
+__device__ __forceinline__ float nan_to_num_f(float v, float nan_v, float pinf, float ninf) {
+  if (v != v) return nan_v;
+  if (v == __builtin_inff()) return pinf;
+  if (v == -__builtin_inff()) return ninf;
+  return v;
+}
+
+__global__ void __launch_bounds__(256) k_em_update(float* __restrict__ x, const float* __restrict__ w1, float c1,
+                                                   const float* __restrict__ w2, float c2,
+                                                   const float* __restrict__ s, int64_t n, int mode, float eps,
+                                                   float* __restrict__ zero_after) {
+  const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (j >= n) return;
+  float w = w1[j];
+  if (w2) {
+    w = c1 * w;                  // (2 - alpha) * weight_compton
+    w = w + c2 * w2[j];          // + alpha * weight_single
+  } else if (c1 != 1.0f) {
+    w = c1 * w;
+  }
+  const float xv = x[j];
+  const float sv = s[j];
+  float out;
+  if (mode == 0) {
+    out = (xv * w) / sv;
+  } else if (mode == 1) {
+    const float ws = nan_to_num_f(w, 0.0f, 0.0f, 0.0f);
+    const float ss = nan_to_num_f(sv, 0.0f, 0.0f, 0.0f);
+    out = ss > eps ? (xv * fmaxf(ws, 0.0f)) / ss : 0.0f;
+  } else {
+    const float ss = (sv != sv) ? sv : fmaxf(sv, eps);
+    float u = w / ss;
+    u = nan_to_num_f(u, 1.0f, 1e6f, 0.0f);
+    out = nan_to_num_f(xv * u, 0.0f, 1e12f, 0.0f);
+  }
+  x[j] = out;
+  if (zero_after) zero_after[j] = 0.0f;
+}
+
+// FP64 column sums of a row-major FP32 matrix: partial [gridDim.y, P]
+__global__ void __launch_bounds__(256) k_colsum_f64(const float* __restrict__ A, int64_t M, int64_t lda, int64_t P,
+                                                    int64_t rows_per, double* __restrict__ part) {
+  const int64_t col = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (col >= P) return;
+  const int64_t r0 = (int64_t)blockIdx.y * rows_per;
+  const int64_t r1 = r0 + rows_per < M ? r0 + rows_per : M;
+  double s = 0.0;
+  for (int64_t r = r0; r < r1; ++r) s += (double)A[r * lda + col];
+  part[(int64_t)blockIdx.y * P + col] = s;
+}
+
+// sens[j] = (1/R) * sum_r colsum[inv[j, r]-1]   (main_plane.py:320-323)
+__global__ void __launch_bounds__(256) k_sens_gather(const double* __restrict__ part, int G, int64_t P,
+                                                     const int64_t* __restrict__ inv, int64_t inv_rs,
+                                                     int64_t inv_cs, int R, float* __restrict__ out,
+                                                     int32_t* __restrict__ err) {
+  const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (j >= P) return;
+  double acc = 0.0;
+  for (int r = 0; r < R; ++r) {
+    int64_t src = inv ? inv[j * inv_rs + r * inv_cs] - 1 : j;
+    if (src < 0 || src >= P) { atomicOr(err, 8); src = 0; }
+    double cs = 0.0;
+    for (int g = 0; g < G; ++g) cs += part[(int64_t)g * P + src];
+    acc += cs;
+  }
+  out[j] = (float)(acc / (double)R);
+}
+
+template <int DT, int NCH, int R>
+int launch_fused(int mode, const ProjArgs& a, int G, hipStream_t s) {
+  dim3 grid(G), block(kProjThreads);
+  switch (mode) {
+    case MODE_RATIO: hipLaunchKernelGGL((k_proj_fused<DT, NCH, R, MODE_RATIO>), grid, block, 0, s, a); break;
+    case MODE_FWD: hipLaunchKernelGGL((k_proj_fused<DT, NCH, R, MODE_FWD>), grid, block, 0, s, a); break;
+    case MODE_COEF: hipLaunchKernelGGL((k_proj_fused<DT, NCH, R, MODE_COEF>), grid, block, 0, s, a); break;
+    case MODE_ONES: hipLaunchKernelGGL((k_proj_fused<DT, NCH, R, MODE_ONES>), grid, block, 0, s, a); break;
+    default: jscc_set_error("jscc_proj: bad mode %d", mode); return -2;
+  }
+  return 0;
+}
+
+template <int DT, int R>
+int dispatch_nch(int mode, const ProjArgs& a, int G, hipStream_t s) {
+  constexpr int VN = Vec<DT>::N;
+  const int nch = (a.P / VN + kProjThreads - 1) / kProjThreads;
+  if (nch <= 1) return launch_fused<DT, 1, R>(mode, a, G, s);
+  if (nch <= 2) return launch_fused<DT, 2, R>(mode, a, G, s);
+  if (nch <= 4) return launch_fused<DT, 4, R>(mode, a, G, s);
+  if (nch <= 8) return launch_fused<DT, 8, R>(mode, a, G, s);
+  if (nch <= 16) return launch_fused<DT, 16, (DT == JSCC_DT_F32 ? 1 : R)>(mode, a, G, s);
+  jscc_set_error("jscc_proj: %d columns exceed the register-resident limit for this dtype", a.P);
+  return -3;
+}
+
+}  // namespace
+
+extern "C" int jscc_proj_vec_width(int dtype) {
+  return dtype == JSCC_DT_F32 ? 4 : (dtype == JSCC_DT_BF16 ? 8 : (dtype == JSCC_DT_FP8 ? 16 : -1));
+}
+
+extern "C" int jscc_proj_max_cols(int dtype) {
+  // register-resident image: NCH * 512 threads * vector width
+  return dtype == JSCC_DT_F32 ? 16 * kProjThreads * 4 : (dtype == JSCC_DT_BF16 ? 16 * kProjThreads * 8 : 16 * kProjThreads * 16);
+}
+
+extern "C" int jscc_proj_grid(int64_t n_rows) {
+  const int cus = jscc_num_cus();
+  int64_t g = (n_rows + 3) / 4;
+  if (g > 2 * cus) g = 2 * cus;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
+extern "C" int64_t jscc_proj_workspace_bytes(int64_t n_rows, int64_t P) {
+  const int64_t G = jscc_proj_grid(n_rows);
+  const int64_t G2 = (G + kRedRows - 1) / kRedRows;
+  return (G + G2) * P * (int64_t)sizeof(float);
+}
+
+// Fused projection of one block of rows.
+//   mode 0: partial <- sum_i scale_i * row_i * y_i / max(scale_i * row_i . x_rot, eps)
+//   mode 1: fwd_out[i] <- scale_i * row_i . x_rot
+//   mode 2: partial <- sum_i scale_i * coef_i * row_i
+//   mode 3: partial <- sum_i scale_i * row_i
+// then (modes 0, 2, 3) w[j] (+)= out_scale * partial[inv_perm[j]-1].
+// flags bit 0: the numerator y is indexed by the stored row id; bit 1: the
+// block's weight goes through nan_to_num(0, 0, 0) before scaling.
+extern "C" int jscc_proj(int mode, const void* mat, int dtype, int64_t ld, int64_t n_cols, const int32_t* rows,
+                         int64_t n_rows, int64_t row_base, const float* row_scale, const float* x,
+                         const int64_t* x_perm, int64_t x_perm_stride, int64_t x_len, const float* y,
+                         int64_t y_stride, const float* coef, float eps, int use_eps, float* fwd_out,
+                         const int64_t* inv_perm, int64_t inv_perm_stride, float out_scale, int accumulate,
+                         int flags, float* w, int64_t w_len, void* workspace, int64_t workspace_bytes,
+                         int32_t* err, void* stream) {
+  JSCC_REQUIRE(mode >= 0 && mode <= 3, "jscc_proj: bad mode");
+  JSCC_REQUIRE(n_rows >= 0 && n_cols > 0, "jscc_proj: bad shape");
+  JSCC_REQUIRE(err != nullptr, "jscc_proj: null error word");
+  const int vw = jscc_proj_vec_width(dtype);
+  JSCC_REQUIRE(vw > 0, "jscc_proj: bad dtype %d", dtype);
+  JSCC_REQUIRE(n_cols % vw == 0 && ld % vw == 0 && ld >= n_cols,
+               "jscc_proj: columns (%lld) and row stride (%lld) must be multiples of %d", (long long)n_cols,
+               (long long)ld, vw);
+  JSCC_REQUIRE(n_cols <= jscc_proj_max_cols(dtype), "jscc_proj: too many columns (%lld)", (long long)n_cols);
+  JSCC_REQUIRE(dtype != JSCC_DT_FP8 || row_scale != nullptr, "jscc_proj: FP8 rows need a per-row scale");
+  hipStream_t s = (hipStream_t)stream;
+  if (mode != 1) {
+    JSCC_REQUIRE(w != nullptr && w_len > 0, "jscc_proj: null weight output");
+    JSCC_REQUIRE(inv_perm != nullptr || w_len == n_cols, "jscc_proj: weight length mismatch");
+  }
+  if (mode == 0 || mode == 1) JSCC_REQUIRE(x != nullptr && x_len > 0, "jscc_proj: null image");
+  if (mode == 0 || mode == 1) JSCC_REQUIRE(x_perm != nullptr || x_len == n_cols, "jscc_proj: image length mismatch");
+  if (mode == 1) JSCC_REQUIRE(fwd_out != nullptr, "jscc_proj: null forward output");
+  if (mode == 2) JSCC_REQUIRE(coef != nullptr, "jscc_proj: null coefficients");
+
+  if (n_rows == 0) {
+    if (mode != 1 && !accumulate) JSCC_CHECK_HIP(hipMemsetAsync(w, 0, w_len * sizeof(float), s));
+    return 0;
+  }
+  JSCC_REQUIRE(mat != nullptr && ((uintptr_t)mat % 16) == 0, "jscc_proj: matrix must be 16-byte aligned");
+  const int G = jscc_proj_grid(n_rows);
+  const int G2 = (G + kRedRows - 1) / kRedRows;
+  if (mode != 1) {
+    JSCC_REQUIRE(workspace != nullptr && workspace_bytes >= jscc_proj_workspace_bytes(n_rows, n_cols),
+                 "jscc_proj: workspace too small");
+  }
+  ProjArgs a;
+  a.M = mat; a.ld = ld; a.P = (int)n_cols; a.rows = rows; a.n_rows = n_rows; a.row_base = row_base;
+  a.row_scale = row_scale; a.x = x; a.x_perm = x_perm; a.x_perm_stride = x_perm_stride; a.x_len = x_len;
+  a.y = y; a.y_stride = y_stride; a.coef = coef; a.eps = eps; a.use_eps = use_eps; a.fwd_out = fwd_out;
+  a.partial = (float*)workspace; a.err = err; a.y_by_row = (flags & 1) ? 1 : 0;
+  int rc;
+  jscc_prof_begin(JSCC_TAG_PROJ, s);
+  if (dtype == JSCC_DT_F32) rc = dispatch_nch<JSCC_DT_F32, 2>(mode, a, G, s);
+  else if (dtype == JSCC_DT_BF16) rc = dispatch_nch<JSCC_DT_BF16, 4>(mode, a, G, s);
+  else rc = dispatch_nch<JSCC_DT_FP8, 4>(mode, a, G, s);
+  jscc_prof_end(JSCC_TAG_PROJ, s);
+  if (rc != 0) return rc;
+  JSCC_CHECK_LAUNCH();
+  if (mode == 1) return 0;
+
+  float* part = (float*)workspace;
+  int Gfin = G;
+  if (G > kRedRows) {
+    float* part2 = part + (int64_t)G * n_cols;
+    dim3 grid1((unsigned)((n_cols / 4 + 255) / 256), (unsigned)G2);
+    hipLaunchKernelGGL(k_reduce_stage1, grid1, dim3(256), 0, s, part, G, (int)n_cols, part2);
+    JSCC_CHECK_LAUNCH();
+    part = part2;
+    Gfin = G2;
+  }
+  hipLaunchKernelGGL(k_reduce_stage2, dim3((unsigned)((w_len + 255) / 256)), dim3(256), 0, s, part, Gfin,
+                     (int)n_cols, inv_perm, inv_perm_stride, out_scale, accumulate, (flags & 2) ? 1 : 0, w, w_len,
+                     err);
+  JSCC_CHECK_LAUNCH();
+  return 0;
+}
+
+extern "C" int jscc_em_update(float* x, const float* w1, float c1, const float* w2, float c2, const float* s,
+                              int64_t n, int mode, float eps, float* zero_after, void* stream) {
+  JSCC_REQUIRE(n >= 0 && mode >= 0 && mode <= 2, "jscc_em_update: bad arguments");
+  if (n == 0) return 0;
+  JSCC_REQUIRE(x && w1 && s, "jscc_em_update: null pointer");
+  hipLaunchKernelGGL(k_em_update, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, x, w1, c1, w2,
+                     c2, s, n, mode, eps, zero_after);
+  JSCC_CHECK_LAUNCH();
+  return 0;
+}
+
+extern "C" int64_t jscc_sensitivity_workspace_bytes(int64_t M, int64_t P) {
+  int64_t G = (M + 255) / 256;
+  if (G > 64) G = 64;
+  if (G < 1) G = 1;
+  return G * P * (int64_t)sizeof(double);
+}
+
+// sens[j] = (1/R) * sum_r sum_i A[i, inv[j, r]-1]  (inv may be null: plain column sums, R = 1)
+extern "C" int jscc_sensitivity(const float* A, int64_t M, int64_t lda, int64_t P, const int64_t* inv,
+                                int64_t inv_row_stride, int64_t inv_col_stride, int R, float* out,
+                                void* workspace, int64_t workspace_bytes, int32_t* err, void* stream) {
+  JSCC_REQUIRE(M >= 0 && P > 0 && R >= 1 && lda >= P, "jscc_sensitivity: bad shape");
+  JSCC_REQUIRE(A && out && err && workspace, "jscc_sensitivity: null pointer");
+  JSCC_REQUIRE(workspace_bytes >= jscc_sensitivity_workspace_bytes(M, P), "jscc_sensitivity: workspace too small");
+  hipStream_t s = (hipStream_t)stream;
+  int64_t G = (M + 255) / 256;
+  if (G > 64) G = 64;
+  if (G < 1) G = 1;
+  const int64_t rows_per = (M + G - 1) / G;
+  hipLaunchKernelGGL(k_colsum_f64, dim3((unsigned)((P + 255) / 256), (unsigned)G), dim3(256), 0, s, A, M, lda, P,
+                     rows_per > 0 ? rows_per : 1, (double*)workspace);
+  JSCC_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_sens_gather, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, s, (const double*)workspace,
+                     (int)G, P, inv, inv_row_stride, inv_col_stride, R, out, err);
+  JSCC_CHECK_LAUNCH();
+  return 0;
+}

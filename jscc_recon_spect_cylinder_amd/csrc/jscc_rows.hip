@@ -1,0 +1,249 @@
+// Row utilities around the cone builder and the sparse (coarse-grid) T format.
+//
+// Reference: process_list_plane_strict.py:137-148, 236-253 (row normalisation,
+// returned t / t_compton / t_single); compton_sparse_ops.py:293-356 (row and
+// image transforms between the fine and coarse polar grids), :359-400
+// (packed rows [cpnum1, t_c...] and their materialisation on the fine grid).
+//
+// The reference applies its coarse<->fine operators as dense (block-diagonal)
+// matmuls; here every operator is a small CSR table (at most 4 taps per output
+// pixel for the expand, 2 for the reduce) applied by a gather kernel.
+#include "jscc_common.h"
+
+namespace {
+
+constexpr int kRowThreads = 256;
+
+template <typename T>
+__device__ __forceinline__ float load_as_f32(const T* p, int64_t i);
+template <>
+__device__ __forceinline__ float load_as_f32<float>(const float* p, int64_t i) { return p[i]; }
+template <>
+__device__ __forceinline__ float load_as_f32<uint16_t>(const uint16_t* p, int64_t i) {
+  return jscc_bf16_to_f32(p[i]);
+}
+
+// dst[i, off + p] = src[idx[i], p] / denom(idx[i]) ; optional dst[i, 0] = lead[idx[i]]
+template <typename T>
+__global__ void __launch_bounds__(kRowThreads) k_gather_normalize(const T* __restrict__ src, int64_t lds, int64_t P,
+                                                                  const int32_t* __restrict__ idx, int64_t n_out,
+                                                                  const double* __restrict__ stats, int field,
+                                                                  const float* __restrict__ lead,
+                                                                  float* __restrict__ dst, int64_t ldd,
+                                                                  int64_t col_off) {
+  const int64_t i = blockIdx.x;
+  if (i >= n_out) return;
+  const int64_t r = idx ? idx[i] : i;
+  const double den = stats ? stats[r * 4 + field] : 1.0;
+  const T* s = src + r * lds;
+  float* d = dst + i * ldd;
+  if (lead && threadIdx.x == 0) d[0] = lead[r];
+  for (int64_t p = threadIdx.x; p < P; p += kRowThreads) {
+    d[col_off + p] = (float)((double)load_as_f32<T>(s, p) / den);
+  }
+}
+
+// t_single: dst[i, :] = A[arow[src_ev[idx[i]]], :] / sum(...)   (process_list_plane_strict.py:237,140)
+__global__ void __launch_bounds__(kRowThreads) k_arows_normalize(const float* __restrict__ A, int64_t lda, int64_t P,
+                                                                 const int32_t* __restrict__ arow,
+                                                                 int64_t n_out, float* __restrict__ dst,
+                                                                 int64_t ldd) {
+  __shared__ double red[kRowThreads / JSCC_WAVE];
+  const int64_t i = blockIdx.x;
+  if (i >= n_out) return;
+  const float* a = A + (int64_t)arow[i] * lda;
+  double s = 0.0;
+  for (int64_t p = threadIdx.x; p < P; p += kRowThreads) s += (double)a[p];
+  s = jscc_wave_sum(s);
+  if (jscc_lane() == 0) red[jscc_wave_id()] = s;
+  __syncthreads();
+  double tot = 0.0;
+  for (int w = 0; w < kRowThreads / JSCC_WAVE; ++w) tot += red[w];
+  float* d = dst + i * ldd;
+  for (int64_t p = threadIdx.x; p < P; p += kRowThreads) d[p] = (float)((double)a[p] / tot);
+}
+
+// FP8 e4m3fn rows with a per-row scale: stored = fp8(t / (max/448)),
+// row_scale = (max/448) / S so that row_scale * stored ~ t / S.
+template <typename T>
+__global__ void __launch_bounds__(kRowThreads) k_quantize_fp8(const T* __restrict__ src, int64_t lds, int64_t P,
+                                                              int64_t n, const double* __restrict__ stats,
+                                                              uint8_t* __restrict__ dst, int64_t ldd,
+                                                              float* __restrict__ row_scale) {
+  const int64_t r = blockIdx.x;
+  if (r >= n) return;
+  const double S = stats[r * 4 + 0];
+  const double mx = stats[r * 4 + 3];
+  const double q = mx > 0.0 ? mx / 448.0 : 1.0;
+  const float inv_q = (float)(1.0 / q);
+  const T* s = src + r * lds;
+  uint8_t* d = dst + r * ldd;
+  for (int64_t p = threadIdx.x; p < ldd; p += kRowThreads) {
+    const float v = p < P ? load_as_f32<T>(s, p) : 0.0f;
+    d[p] = (uint8_t)jscc_f32_to_fp8(v * inv_q);
+  }
+  if (threadIdx.x == 0) row_scale[r] = (float)(q / S);
+}
+
+// out[r, j] = sum_k val[k] * in[r, col[k]], k in [ptr[j], ptr[j+1])   (row-wise sparse linear map)
+__global__ void __launch_bounds__(kRowThreads) k_rows_csr_map(const float* __restrict__ in, int64_t ldi,
+                                                              int64_t n_rows, const int32_t* __restrict__ ptr,
+                                                              const int32_t* __restrict__ col,
+                                                              const float* __restrict__ val, int64_t n_out,
+                                                              float* __restrict__ out, int64_t ldo) {
+  const int64_t r = blockIdx.y;
+  const int64_t j = (int64_t)blockIdx.x * kRowThreads + threadIdx.x;
+  if (r >= n_rows || j >= n_out) return;
+  const float* a = in + r * ldi;
+  float s = 0.0f;
+  for (int k = ptr[j]; k < ptr[j + 1]; ++k) s = fmaf(val[k], a[col[k]], s);
+  out[r * ldo + j] = s;
+}
+
+__device__ __forceinline__ float zero_nonfinite(float v) {
+  return (v == v && fabsf(v) != __builtin_inff()) ? v : 0.0f;
+}
+
+// compton_sparse_ops.py:380-400 : t_fine = nan_to_num(expand(t_c) * A[cp]) ; row-normalise ; valid
+// Rows are written normalised (in place, by the same threads) together with a flag.
+__global__ void __launch_bounds__(kRowThreads) k_materialize(const float* __restrict__ packed, int64_t ldp,
+                                                             int64_t n, int64_t Pc, const int32_t* __restrict__ ptr,
+                                                             const int32_t* __restrict__ col,
+                                                             const float* __restrict__ val, int64_t Pf,
+                                                             const float* __restrict__ A, int64_t lda, int64_t M,
+                                                             float* __restrict__ out, int64_t ldo,
+                                                             int normalize, int32_t* __restrict__ flags) {
+  __shared__ double red[kRowThreads / JSCC_WAVE];
+  const int64_t r = blockIdx.x;
+  if (r >= n) return;
+  const float* row = packed + r * ldp;
+  int64_t cp = (int64_t)rintf(row[0]) - 1;
+  cp = cp < 0 ? 0 : (cp > M - 1 ? M - 1 : cp);
+  const float* tc = row + 1;
+  const float* a = A + cp * lda;
+  float* o = out + r * ldo;
+  double s = 0.0;
+  for (int64_t p = threadIdx.x; p < Pf; p += kRowThreads) {
+    float u = 0.0f;
+    for (int k = ptr[p]; k < ptr[p + 1]; ++k) u = fmaf(val[k], tc[col[k]], u);
+    const float t = zero_nonfinite(u * a[p]);
+    o[p] = t;
+    s += (double)t;
+  }
+  s = jscc_wave_sum(s);
+  if (jscc_lane() == 0) red[jscc_wave_id()] = s;
+  __syncthreads();
+  double tot = 0.0;
+  for (int w = 0; w < kRowThreads / JSCC_WAVE; ++w) tot += red[w];
+  const bool valid = (tot == tot) && fabs(tot) != __builtin_inf() && tot > 0.0;
+  if (valid && normalize) {
+    for (int64_t p = threadIdx.x; p < Pf; p += kRowThreads) o[p] = (float)((double)o[p] / tot);
+  }
+  if (threadIdx.x == 0) flags[r] = valid ? 1 : 0;
+}
+
+}  // namespace
+
+extern "C" int jscc_rows_gather_normalize(const void* src, int src_dtype, int64_t ld_src, int64_t n_cols,
+                                          const int32_t* idx, int64_t n_out, const double* stats, int stat_field,
+                                          const float* lead, float* dst, int64_t ld_dst, int64_t col_off,
+                                          void* stream) {
+  JSCC_REQUIRE(n_out >= 0 && n_cols >= 0, "jscc_rows_gather_normalize: bad shape");
+  if (n_out == 0 || n_cols == 0) return 0;
+  JSCC_REQUIRE(src && dst, "jscc_rows_gather_normalize: null pointer");
+  JSCC_REQUIRE(stat_field >= 0 && stat_field < 4, "jscc_rows_gather_normalize: bad stat field");
+  hipStream_t s = (hipStream_t)stream;
+  if (src_dtype == JSCC_DT_F32) {
+    hipLaunchKernelGGL(k_gather_normalize<float>, dim3((unsigned)n_out), dim3(kRowThreads), 0, s, (const float*)src,
+                       ld_src, n_cols, idx, n_out, stats, stat_field, lead, dst, ld_dst, col_off);
+  } else if (src_dtype == JSCC_DT_BF16) {
+    hipLaunchKernelGGL(k_gather_normalize<uint16_t>, dim3((unsigned)n_out), dim3(kRowThreads), 0, s,
+                       (const uint16_t*)src, ld_src, n_cols, idx, n_out, stats, stat_field, lead, dst, ld_dst,
+                       col_off);
+  } else {
+    jscc_set_error("jscc_rows_gather_normalize: unsupported dtype %d", src_dtype);
+    return -2;
+  }
+  JSCC_CHECK_LAUNCH();
+  return 0;
+}
+
+extern "C" int jscc_arows_normalize(const float* A, int64_t lda, int64_t n_cols, const int32_t* arow, int64_t n_out,
+                                    float* dst, int64_t ld_dst, void* stream) {
+  JSCC_REQUIRE(n_out >= 0, "jscc_arows_normalize: bad shape");
+  if (n_out == 0) return 0;
+  JSCC_REQUIRE(A && arow && dst, "jscc_arows_normalize: null pointer");
+  hipLaunchKernelGGL(k_arows_normalize, dim3((unsigned)n_out), dim3(kRowThreads), 0, (hipStream_t)stream, A, lda,
+                     n_cols, arow, n_out, dst, ld_dst);
+  JSCC_CHECK_LAUNCH();
+  return 0;
+}
+
+extern "C" int jscc_quantize_fp8_rows(const void* src, int src_dtype, int64_t ld_src, int64_t n_cols, int64_t n_rows,
+                                      const double* stats, uint8_t* dst, int64_t ld_dst, float* row_scale,
+                                      void* stream) {
+  JSCC_REQUIRE(n_rows >= 0 && ld_dst >= n_cols, "jscc_quantize_fp8_rows: bad shape");
+  if (n_rows == 0) return 0;
+  JSCC_REQUIRE(src && stats && dst && row_scale, "jscc_quantize_fp8_rows: null pointer");
+  hipStream_t s = (hipStream_t)stream;
+  if (src_dtype == JSCC_DT_F32) {
+    hipLaunchKernelGGL(k_quantize_fp8<float>, dim3((unsigned)n_rows), dim3(kRowThreads), 0, s, (const float*)src,
+                       ld_src, n_cols, n_rows, stats, dst, ld_dst, row_scale);
+  } else if (src_dtype == JSCC_DT_BF16) {
+    hipLaunchKernelGGL(k_quantize_fp8<uint16_t>, dim3((unsigned)n_rows), dim3(kRowThreads), 0, s,
+                       (const uint16_t*)src, ld_src, n_cols, n_rows, stats, dst, ld_dst, row_scale);
+  } else {
+    jscc_set_error("jscc_quantize_fp8_rows: unsupported dtype %d", src_dtype);
+    return -2;
+  }
+  JSCC_CHECK_LAUNCH();
+  return 0;
+}
+
+extern "C" int jscc_rows_csr_map(const float* in, int64_t ld_in, int64_t n_rows, const int32_t* ptr,
+                                 const int32_t* col, const float* val, int64_t n_out, float* out, int64_t ld_out,
+                                 void* stream) {
+  JSCC_REQUIRE(n_rows >= 0 && n_out >= 0 && n_rows < 65536 * 32768LL, "jscc_rows_csr_map: bad shape");
+  if (n_rows == 0 || n_out == 0) return 0;
+  JSCC_REQUIRE(in && ptr && col && val && out, "jscc_rows_csr_map: null pointer");
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t chunk = 65535;
+  for (int64_t r0 = 0; r0 < n_rows; r0 += chunk) {
+    const int64_t nr = (n_rows - r0) < chunk ? (n_rows - r0) : chunk;
+    hipLaunchKernelGGL(k_rows_csr_map, dim3((unsigned)((n_out + kRowThreads - 1) / kRowThreads), (unsigned)nr),
+                       dim3(kRowThreads), 0, s, in + r0 * ld_in, ld_in, nr, ptr, col, val, n_out, out + r0 * ld_out,
+                       ld_out);
+    JSCC_CHECK_LAUNCH();
+  }
+  return 0;
+}
+
+extern "C" int jscc_materialize_rows(const float* packed, int64_t ld_packed, int64_t n_rows, int64_t n_coarse,
+                                     const int32_t* ptr, const int32_t* col, const float* val, int64_t n_fine,
+                                     const float* A, int64_t lda, int64_t M, float* out, int64_t ld_out,
+                                     int normalize, int32_t* flags, void* stream) {
+  JSCC_REQUIRE(n_rows >= 0 && ld_packed >= n_coarse + 1 && M > 0, "jscc_materialize_rows: bad shape");
+  if (n_rows == 0) return 0;
+  JSCC_REQUIRE(packed && ptr && col && val && A && out && flags, "jscc_materialize_rows: null pointer");
+  hipLaunchKernelGGL(k_materialize, dim3((unsigned)n_rows), dim3(kRowThreads), 0, (hipStream_t)stream, packed,
+                     ld_packed, n_rows, n_coarse, ptr, col, val, n_fine, A, lda, M, out, ld_out, normalize, flags);
+  JSCC_CHECK_LAUNCH();
+  return 0;
+}
+
+// Decodes all 256 e4m3fn codes with the hardware converter and with the
+// software decoder; out[0..255] = hardware, out[256..511] = software.
+__global__ void k_fp8_selftest(float* out) {
+  const int c = threadIdx.x;
+  typedef float f2v __attribute__((ext_vector_type(2)));
+  const f2v hw = __builtin_amdgcn_cvt_pk_f32_fp8(c, false);
+  out[c] = hw.x;
+  out[256 + c] = jscc_fp8_to_f32((uint32_t)c);
+}
+
+extern "C" int jscc_fp8_selftest(float* out512, void* stream) {
+  JSCC_REQUIRE(out512 != nullptr, "jscc_fp8_selftest: null pointer");
+  hipLaunchKernelGGL(k_fp8_selftest, dim3(1), dim3(256), 0, (hipStream_t)stream, out512);
+  JSCC_CHECK_LAUNCH();
+  return 0;
+}

@@ -1,0 +1,444 @@
+"""Tensor-level wrappers of the libjscc_hip entry points.
+
+PyTorch provides device memory, streams and host plumbing only; every
+numerical step runs in the gfx950 kernels.  All tensors handed to these
+functions must live on a GPU; workspaces are allocated here (through the
+torch caching allocator) so the library itself never allocates.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+
+DT_F32, DT_BF16, DT_FP8 = 0, 1, 2
+SIG_STRICT, SIG_FIXED = 0, 1
+EVENT_RECORD_FLOATS = 16  # 64-byte per-event record
+
+_DTYPE_CODE = {torch.float32: DT_F32, torch.bfloat16: DT_BF16, torch.float8_e4m3fn: DT_FP8}
+_CODE_TORCH = {DT_F32: torch.float32, DT_BF16: torch.bfloat16, DT_FP8: torch.uint8}
+
+
+def dtype_code(dtype) -> int:
+    if isinstance(dtype, int):
+        return dtype
+    if isinstance(dtype, str):
+        return {"fp32": DT_F32, "f32": DT_F32, "float32": DT_F32, "bf16": DT_BF16, "bfloat16": DT_BF16,
+                "fp8": DT_FP8, "e4m3": DT_FP8}[dtype.lower()]
+    return _DTYPE_CODE[dtype]
+
+
+def require_gpu(t: torch.Tensor, name: str) -> None:
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise ValueError(f"{name} must be a tensor on a GPU (this engine has no CPU path)")
+
+
+def _p(t) -> int | None:
+    return None if t is None else t.data_ptr()
+
+
+def stream_of(device: torch.device) -> int:
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def round_up(n: int, m: int) -> int:
+    return (n + m - 1) // m * m
+
+
+# ----------------------------------------------------------------------------- error word
+_err_words: dict[int, torch.Tensor] = {}
+
+_ERR_TEXT = {1: "crystal number out of range of the detector table",
+             2: "image gather index (RotMat) out of range",
+             4: "weight gather index (RotMatInv) out of range",
+             8: "sensitivity gather index (RotMatInv) out of range"}
+
+
+def err_word(device: torch.device) -> torch.Tensor:
+    idx = device.index if device.index is not None else torch.cuda.current_device()
+    w = _err_words.get(idx)
+    if w is None:
+        w = torch.zeros(1, dtype=torch.int32, device=torch.device("cuda", idx))
+        _err_words[idx] = w
+    return w
+
+
+def raise_on_index_error(device: torch.device) -> None:
+    """Synchronising check of the device error word (raises IndexError)."""
+    w = err_word(device)
+    code = int(w.item())
+    if code:
+        w.zero_()
+        msgs = [txt for bit, txt in _ERR_TEXT.items() if code & bit]
+        raise IndexError("; ".join(msgs))
+
+
+# ----------------------------------------------------------------------------- geometry caches
+@dataclass
+class VoxelSoA:
+    x: torch.Tensor
+    y: torch.Tensor
+    z: torch.Tensor
+    n: int
+
+
+_voxel_cache: dict = {}
+
+
+def voxel_soa(coor: torch.Tensor, device: torch.device) -> VoxelSoA:
+    """[P,3] voxel centres -> three FP32 arrays padded to a multiple of 4 on `device`."""
+    key = (coor.data_ptr(), coor._version, tuple(coor.shape), tuple(coor.stride()), str(coor.device), str(device))
+    hit = _voxel_cache.get(key)
+    if hit is not None:
+        return hit
+    if coor.ndim != 2 or coor.shape[1] < 3:
+        raise ValueError("coor must have shape [pixel_num, 3]")
+    P = coor.shape[0]
+    Pp = round_up(max(P, 1), 4)
+    c = coor[:, :3].to(device=device, dtype=torch.float32)
+    soa = torch.zeros((3, Pp), dtype=torch.float32, device=device)
+    soa[:, :P] = c.t()
+    v = VoxelSoA(soa[0], soa[1], soa[2], P)
+    if len(_voxel_cache) > 16:
+        _voxel_cache.clear()
+    _voxel_cache[key] = v
+    return v
+
+
+_det_cache: dict = {}
+
+
+def detector_layers(detector: torch.Tensor) -> tuple[int, float]:
+    """(four |y| layers?, rear-layer |y|) of a crystal table (process_list_plane_strict.py:59-71)."""
+    key = (detector.data_ptr(), detector._version, tuple(detector.shape), str(detector.device))
+    hit = _det_cache.get(key)
+    if hit is not None:
+        return hit
+    y = np.abs(detector[:, 1].detach().to("cpu", torch.float32).numpy())
+    layers = np.unique(y)
+    res = (1, float(layers[-1])) if layers.size == 4 else (0, 0.0)
+    if len(_det_cache) > 16:
+        _det_cache.clear()
+    _det_cache[key] = res
+    return res
+
+
+_aligned_cache: dict = {}
+
+
+def aligned_rows(A: torch.Tensor, vec: int = 4, device: torch.device | None = None) -> torch.Tensor:
+    """Row-major FP32 copy/view of A on `device` whose row stride is a multiple of `vec`
+    and whose base is 16-byte aligned.
+
+    The reference loads the system matrix as a transposed view (main_plane.py:174)
+    and passes it to every chunk; such inputs (or host tensors) are copied once into
+    a padded row-major device buffer that later calls on the same tensor reuse.
+    """
+    dev = device if device is not None else A.device
+    if A.device == dev and A.dtype == torch.float32 and A.stride(1) == 1 and A.stride(0) % vec == 0 \
+            and A.stride(0) >= A.shape[1] and A.data_ptr() % 16 == 0:
+        return A
+    key = (A.data_ptr(), A._version, tuple(A.shape), tuple(A.stride()), A.dtype, str(A.device), str(dev), vec)
+    hit = _aligned_cache.get(key)
+    if hit is not None:
+        return hit
+    M, P = A.shape
+    buf = torch.zeros((M, round_up(P, vec)), dtype=torch.float32, device=dev)
+    buf[:, :P] = A.to(dev)
+    out = buf[:, :P]
+    if len(_aligned_cache) >= 2:
+        _aligned_cache.clear()
+    _aligned_cache[key] = out
+    return out
+
+
+# ----------------------------------------------------------------------------- events
+def events_prepare(list_dev: torch.Tensor, noise1, noise2, writeback: bool, detector: torch.Tensor,
+                   delta_r1: float, delta_r2: float, e0: float, ene_resolution: float,
+                   thr_max: float, thr_min: float, thr_sum: float):
+    """Per-event records [n,16] (FP32 view of 64-byte structs) and keep flags [n] int32."""
+    require_gpu(list_dev, "list_origin")
+    dev = list_dev.device
+    n = list_dev.shape[0]
+    if list_dev.dtype != torch.float32 or list_dev.stride(1) != 1:
+        raise ValueError("list rows must be contiguous float32")
+    det = detector
+    if det.dtype != torch.float32 or det.stride(1) != 1:
+        det = det.float().contiguous()
+    det = det.to(dev)
+    layers4, rear = detector_layers(det)
+    events = torch.empty((max(n, 1), EVENT_RECORD_FLOATS), dtype=torch.float32, device=dev)
+    flags = torch.empty(max(n, 1), dtype=torch.int32, device=dev)
+    err = err_word(dev)
+    f32 = lambda v: float(np.float32(v))
+    rc = _lib.lib().jscc_events_prepare(
+        list_dev.data_ptr(), n, list_dev.stride(0), _p(noise1), _p(noise2),
+        list_dev.data_ptr() if writeback else None, det.data_ptr(), det.shape[0], det.stride(0),
+        layers4, f32(rear), f32(delta_r1), f32(delta_r2),
+        f32(e0), f32(ene_resolution), f32(thr_max), f32(thr_min), f32(thr_sum), float(e0), float(ene_resolution),
+        events.data_ptr(), flags.data_ptr(), err.data_ptr(), stream_of(dev))
+    _lib.check(rc, "jscc_events_prepare")
+    return events, flags[:n]
+
+
+def compact(flags: torch.Tensor):
+    """Order-preserving indices of the set flags: (idx int32 [n] (valid prefix), count int32 [1] device)."""
+    dev = flags.device
+    n = flags.shape[0]
+    idx = torch.empty(max(n, 1), dtype=torch.int32, device=dev)
+    count = torch.empty(1, dtype=torch.int32, device=dev)
+    ws = torch.empty(max(_lib.lib().jscc_compact_workspace_bytes(n), 4), dtype=torch.uint8, device=dev)
+    rc = _lib.lib().jscc_compact_flags(flags.data_ptr(), n, idx.data_ptr(), count.data_ptr(), ws.data_ptr(),
+                                       ws.numel(), stream_of(dev))
+    _lib.check(rc, "jscc_compact_flags")
+    return idx, count
+
+
+# ----------------------------------------------------------------------------- cone rows
+@dataclass
+class ATaps:
+    """First-hit factor on coarse voxels: w0*A[row, i0] + w1*A[row, i1]."""
+    i0: torch.Tensor
+    i1: torch.Tensor
+    w0: torch.Tensor
+    w1: torch.Tensor
+
+
+def cone_rows(events: torch.Tensor, ev_idx: torch.Tensor, n_rows: int, vox: VoxelSoA, sysmat=None,
+              taps: ATaps | None = None, sigma_model: int = SIG_STRICT, fixed_sigma: float = 0.0,
+              out_dtype=None, want_tc: bool = False, ld: int | None = None):
+    """Evaluate cone rows; returns (t_store | None, t_c | None, stats [n,4] f64, ld)."""
+    dev = events.device
+    P = vox.n
+    ld = ld or round_up(P, 64)
+    out_t = None
+    code = -1
+    if out_dtype is not None:
+        code = dtype_code(out_dtype)
+        if code not in (DT_F32, DT_BF16):
+            raise ValueError("cone rows are stored as FP32 or BF16 (FP8 is a re-quantisation)")
+        out_t = torch.empty((max(n_rows, 1), ld), dtype=_CODE_TORCH[code], device=dev)
+    out_tc = torch.empty((max(n_rows, 1), ld), dtype=torch.float32, device=dev) if want_tc else None
+    stats = torch.empty((max(n_rows, 1), 4), dtype=torch.float64, device=dev)
+    A = None
+    lda = 0
+    if sysmat is not None:
+        A = aligned_rows(sysmat) if taps is None else sysmat
+        lda = A.stride(0)
+    rc = _lib.lib().jscc_cone_rows(
+        events.data_ptr(), ev_idx.data_ptr(), n_rows, vox.x.data_ptr(), vox.y.data_ptr(), vox.z.data_ptr(), P,
+        _p(A), lda,
+        _p(taps.i0) if taps else None, _p(taps.i1) if taps else None,
+        _p(taps.w0) if taps else None, _p(taps.w1) if taps else None,
+        int(sigma_model), float(fixed_sigma), _p(out_t), code, ld, _p(out_tc), ld, stats.data_ptr(), stream_of(dev))
+    _lib.check(rc, "jscc_cone_rows")
+    return out_t, out_tc, stats, ld
+
+
+def rows_keep(stats: torch.Tensor, n: int, min_support: float, want_inv_sum: bool = False):
+    """Keep flags [n] int32 (and 1/S per row when want_inv_sum)."""
+    flags = torch.empty(max(n, 1), dtype=torch.int32, device=stats.device)
+    inv = torch.empty(max(n, 1), dtype=torch.float32, device=stats.device) if want_inv_sum else None
+    rc = _lib.lib().jscc_rows_keep_flags(stats.data_ptr(), n, float(min_support), flags.data_ptr(), _p(inv),
+                                         stream_of(stats.device))
+    _lib.check(rc, "jscc_rows_keep_flags")
+    return (flags[:n], inv) if want_inv_sum else flags[:n]
+
+
+def gather_normalize(src: torch.Tensor, P: int, idx: torch.Tensor | None, n_out: int, stats, field: int,
+                     lead: torch.Tensor | None = None) -> torch.Tensor:
+    """Compacted, row-normalised FP32 rows [n_out, P] (or [n_out, 1+P] with a leading column)."""
+    dev = src.device
+    off = 1 if lead is not None else 0
+    dst = torch.empty((n_out, P + off), dtype=torch.float32, device=dev)
+    if n_out == 0:
+        return dst
+    code = DT_F32 if src.dtype == torch.float32 else DT_BF16
+    rc = _lib.lib().jscc_rows_gather_normalize(src.data_ptr(), code, src.stride(0), P, _p(idx), n_out, _p(stats),
+                                               field, _p(lead), dst.data_ptr(), dst.stride(0), off, stream_of(dev))
+    _lib.check(rc, "jscc_rows_gather_normalize")
+    return dst
+
+
+def arows_normalize(A: torch.Tensor, arow: torch.Tensor, n_out: int) -> torch.Tensor:
+    dev = A.device
+    P = A.shape[1]
+    dst = torch.empty((n_out, P), dtype=torch.float32, device=dev)
+    if n_out == 0:
+        return dst
+    rc = _lib.lib().jscc_arows_normalize(A.data_ptr(), A.stride(0), P, arow.data_ptr(), n_out, dst.data_ptr(),
+                                         dst.stride(0), stream_of(dev))
+    _lib.check(rc, "jscc_arows_normalize")
+    return dst
+
+
+def quantize_fp8(src: torch.Tensor, P: int, n_rows: int, stats: torch.Tensor):
+    """FP8 e4m3fn copy of the rows plus per-row scales (scale * code ~ t / S)."""
+    dev = src.device
+    ld = round_up(P, 64)
+    dst = torch.empty((max(n_rows, 1), ld), dtype=torch.uint8, device=dev)
+    scale = torch.empty(max(n_rows, 1), dtype=torch.float32, device=dev)
+    code = DT_F32 if src.dtype == torch.float32 else DT_BF16
+    rc = _lib.lib().jscc_quantize_fp8_rows(src.data_ptr(), code, src.stride(0), P, n_rows, stats.data_ptr(),
+                                           dst.data_ptr(), ld, scale.data_ptr(), stream_of(dev))
+    _lib.check(rc, "jscc_quantize_fp8_rows")
+    return dst, scale
+
+
+# ----------------------------------------------------------------------------- CSR row maps
+@dataclass
+class CSR:
+    ptr: torch.Tensor
+    col: torch.Tensor
+    val: torch.Tensor
+    n_out: int
+    n_in: int
+
+    def to(self, device):
+        return CSR(self.ptr.to(device), self.col.to(device), self.val.to(device), self.n_out, self.n_in)
+
+
+def csr_from_dense(M: np.ndarray, tol: float = 0.0) -> CSR:
+    """CSR (on CPU) of a dense operator M [n_out, n_in]: out = M @ in."""
+    M = np.asarray(M, dtype=np.float64)
+    nz = np.abs(M) > tol
+    counts = nz.sum(axis=1)
+    ptr = np.zeros(M.shape[0] + 1, dtype=np.int32)
+    ptr[1:] = np.cumsum(counts)
+    rows, cols = np.nonzero(nz)
+    return CSR(torch.from_numpy(ptr), torch.from_numpy(cols.astype(np.int32)),
+               torch.from_numpy(M[rows, cols].astype(np.float32)), M.shape[0], M.shape[1])
+
+
+def csr_map_rows(inp: torch.Tensor, csr: CSR) -> torch.Tensor:
+    """out[r, :] = csr @ inp[r, :] for every row."""
+    require_gpu(inp, "rows")
+    if inp.dtype != torch.float32 or inp.stride(-1) != 1:
+        inp = inp.float().contiguous()
+    n = inp.shape[0]
+    out = torch.empty((n, csr.n_out), dtype=torch.float32, device=inp.device)
+    rc = _lib.lib().jscc_rows_csr_map(inp.data_ptr(), inp.stride(0), n, csr.ptr.data_ptr(), csr.col.data_ptr(),
+                                      csr.val.data_ptr(), csr.n_out, out.data_ptr(), out.stride(0),
+                                      stream_of(inp.device))
+    _lib.check(rc, "jscc_rows_csr_map")
+    return out
+
+
+def materialize(packed: torch.Tensor, expand: CSR, sysmat: torch.Tensor, normalize: bool = True):
+    """Fine rows of packed coarse rows: (rows [n, P_f] (normalised where valid), flags int32 [n])."""
+    dev = packed.device
+    n = packed.shape[0]
+    Pf = expand.n_out
+    if packed.dtype != torch.float32 or packed.stride(1) != 1:
+        packed = packed.float().contiguous()
+    A = sysmat if (sysmat.dtype == torch.float32 and sysmat.stride(1) == 1) else sysmat.float().contiguous()
+    out = torch.empty((max(n, 1), Pf), dtype=torch.float32, device=dev)
+    flags = torch.empty(max(n, 1), dtype=torch.int32, device=dev)
+    rc = _lib.lib().jscc_materialize_rows(packed.data_ptr(), packed.stride(0), n, expand.n_in, expand.ptr.data_ptr(),
+                                          expand.col.data_ptr(), expand.val.data_ptr(), Pf, A.data_ptr(),
+                                          A.stride(0), A.shape[0], out.data_ptr(), out.stride(0),
+                                          1 if normalize else 0, flags.data_ptr(), stream_of(dev))
+    _lib.check(rc, "jscc_materialize_rows")
+    return out[:n], flags[:n]
+
+
+# ----------------------------------------------------------------------------- projections
+MODE_RATIO, MODE_FWD, MODE_COEF, MODE_ONES = 0, 1, 2, 3
+
+_ws_cache: dict = {}
+
+
+def _workspace(dev: torch.device, nbytes: int) -> torch.Tensor:
+    key = dev.index if dev.index is not None else torch.cuda.current_device()
+    ws = _ws_cache.get(key)
+    if ws is None or ws.numel() < nbytes:
+        ws = torch.empty(max(nbytes, 256), dtype=torch.uint8, device=dev)
+        _ws_cache[key] = ws
+    return ws
+
+
+def proj_ready(mat: torch.Tensor, dtype_c: int) -> torch.Tensor:
+    """Matrix rows in the layout the projection kernel streams: unit column stride,
+    16-byte aligned rows whose stride is a multiple of the vector width.  The
+    column count itself must be a multiple of the vector width (16 for every
+    polar grid of the reference)."""
+    vw = _lib.lib().jscc_proj_vec_width(dtype_c)
+    P = mat.shape[1]
+    if P % vw != 0:
+        raise ValueError(f"{P} columns: the projection kernels need a multiple of {vw} pixels")
+    if mat.stride(1) == 1 and mat.stride(0) % vw == 0 and mat.data_ptr() % 16 == 0:
+        return mat
+    # a fresh allocation is 16-byte aligned; a contiguous but offset view must be copied
+    return mat.clone(memory_format=torch.contiguous_format) if mat.is_contiguous() else mat.contiguous()
+
+
+def proj(mode: int, mat: torch.Tensor, *, dtype_c: int | None = None, n_cols: int | None = None,
+         rows: torch.Tensor | None = None, n_rows: int | None = None, row_base: int = 0,
+         row_scale: torch.Tensor | None = None, x: torch.Tensor | None = None, x_perm: torch.Tensor | None = None,
+         y: torch.Tensor | None = None, coef: torch.Tensor | None = None, eps: float | None = None,
+         fwd_out: torch.Tensor | None = None, inv_perm: torch.Tensor | None = None, out_scale: float = 1.0,
+         accumulate: bool = False, w: torch.Tensor | None = None, y_by_row: bool = False,
+         sanitize: bool = False) -> None:
+    """Launch jscc_proj on the current stream (see include/jscc_hip.h)."""
+    dev = mat.device
+    if dtype_c is None:
+        dtype_c = dtype_code(mat.dtype) if mat.dtype != torch.uint8 else DT_FP8
+    P = n_cols if n_cols is not None else mat.shape[1]
+    if n_rows is None:
+        n_rows = rows.shape[0] - row_base if rows is not None else mat.shape[0] - row_base
+    ld = mat.stride(0)
+    wl = 0
+    if w is not None:
+        wl = w.shape[0]
+    ws = _workspace(dev, _lib.lib().jscc_proj_workspace_bytes(max(n_rows, 1), P))
+    xl = x.shape[0] if x is not None else 0
+    rc = _lib.lib().jscc_proj(
+        mode, mat.data_ptr(), dtype_c, ld, P, _p(rows), n_rows, row_base, _p(row_scale), _p(x), _p(x_perm),
+        x_perm.stride(0) if x_perm is not None else 0, xl, _p(y), y.stride(0) if y is not None else 0, _p(coef),
+        float(eps) if eps is not None else 0.0, 1 if eps is not None else 0, _p(fwd_out), _p(inv_perm),
+        inv_perm.stride(0) if inv_perm is not None else 0, float(out_scale), 1 if accumulate else 0,
+        (1 if y_by_row else 0) | (2 if sanitize else 0), _p(w), wl,
+        ws.data_ptr(), ws.numel(), err_word(dev).data_ptr(), stream_of(dev))
+    _lib.check(rc, "jscc_proj")
+
+
+EM_PLAIN, EM_SAFE, EM_APPLY = 0, 1, 2
+
+
+def em_update(x: torch.Tensor, w1: torch.Tensor, s: torch.Tensor, mode: int = EM_PLAIN, c1: float = 1.0,
+              w2: torch.Tensor | None = None, c2: float = 0.0, eps: float = 1e-12,
+              zero_after: torch.Tensor | None = None) -> None:
+    """In-place x <- update(x, c1*w1 + c2*w2, s)."""
+    rc = _lib.lib().jscc_em_update(x.data_ptr(), w1.data_ptr(), float(c1), _p(w2), float(c2), s.data_ptr(),
+                                   x.numel(), int(mode), float(eps), _p(zero_after), stream_of(x.device))
+    _lib.check(rc, "jscc_em_update")
+
+
+def sensitivity(A: torch.Tensor, inv: torch.Tensor | None) -> torch.Tensor:
+    """(1/R) sum_r colsum(A[:, inv[:, r]-1]) as [P, 1] FP32 (reductions in FP64)."""
+    require_gpu(A, "sysmat")
+    dev = A.device
+    if A.dtype != torch.float32 or A.stride(1) != 1:
+        A = A.float().contiguous()
+    M, P = A.shape
+    R = inv.shape[1] if inv is not None else 1
+    if inv is not None:
+        inv = inv.to(device=dev, dtype=torch.int64)
+    out = torch.empty((P, 1), dtype=torch.float32, device=dev)
+    ws = torch.empty(_lib.lib().jscc_sensitivity_workspace_bytes(M, P), dtype=torch.uint8, device=dev)
+    rc = _lib.lib().jscc_sensitivity(A.data_ptr(), M, A.stride(0), P, _p(inv),
+                                     inv.stride(0) if inv is not None else 0, inv.stride(1) if inv is not None else 0,
+                                     R, out.data_ptr(), ws.data_ptr(), ws.numel(), err_word(dev).data_ptr(),
+                                     stream_of(dev))
+    _lib.check(rc, "jscc_sensitivity")
+    return out
+
+
+def fp8_selftest(device) -> torch.Tensor:
+    out = torch.empty(512, dtype=torch.float32, device=device)
+    rc = _lib.lib().jscc_fp8_selftest(out.data_ptr(), stream_of(out.device))
+    _lib.check(rc, "jscc_fp8_selftest")
+    return out

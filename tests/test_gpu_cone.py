@@ -1,0 +1,153 @@
+"""GPU parity of the cone back-projection (libjscc_hip through the drop-in API)
+against the reference's golden vectors and the FP64 oracle."""
+import importlib.util
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import cone
+from tests.conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+FIX = os.path.join(GOLDEN, "factors_511keV_RotateNum20.npz")
+STRICT = os.path.join(GOLDEN, "strict_cone_small.npz")
+DEV = "cuda:0"
+
+
+def _golden_sysmat(det, coor):
+    spec = importlib.util.spec_from_file_location("mk", os.path.join(GOLDEN, "make_strict_golden.py"))
+    mk = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mk)
+    return mk.golden_sysmat(det, coor)
+
+
+def _rel_rows(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return (np.linalg.norm(a - b, axis=1) / np.linalg.norm(b, axis=1)).max()
+
+
+def test_strict_dropin_matches_reference_golden():
+    from jscc_recon_spect_cylinder_amd.process_list_plane_strict import get_compton_backproj_list_single
+    g = np.load(STRICT)
+    det = np.load(FIX)["detector"]
+    coor = g["coor"]
+    A = _golden_sysmat(det, coor)
+    dr1, dr2, e0, res, tmax, tmin, tsum = g["params"]
+    lst = torch.from_numpy(g["list_in"].copy()).to(DEV)
+    torch.manual_seed(1234)  # the drop-in draws its smearing noise like the reference (CPU generator)
+    t, tc, ts = get_compton_backproj_list_single(torch.from_numpy(A).to(DEV), torch.from_numpy(det).to(DEV),
+                                                 torch.from_numpy(coor).to(DEV), lst, dr1, dr2, e0, res, tmax,
+                                                 tmin, tsum, torch.device(DEV))
+    # the caller's list is mutated in place, bit-identically
+    assert np.array_equal(lst.cpu().numpy(), g["list_out"])
+    assert t.device.type == "cpu" and t.dtype == torch.float32
+    assert t.shape == g["t"].shape, (t.shape, g["t"].shape)
+    assert _rel_rows(t.numpy(), g["t"]) < 1e-4
+    assert _rel_rows(tc.numpy(), g["t_compton"]) < 1e-4
+    np.testing.assert_allclose(ts.numpy().sum(1), g["t_single_sum"], rtol=1e-5)
+
+
+def test_strict_rows_match_fp64_oracle_and_kept_indices_bit_exact():
+    from jscc_recon_spect_cylinder_amd.process_list_plane_strict import build_compton_rows
+    g = np.load(STRICT)
+    det = np.load(FIX)["detector"]
+    coor = g["coor"]
+    A = _golden_sysmat(det, coor)
+    dr1, dr2, e0, res, tmax, tmin, tsum = g["params"]
+    ref = cone.backproj_list_single(A, det, coor, g["list_in"], dr1, dr2, e0, res, tmax, tmin, tsum,
+                                    g["noise1"], g["noise2"])
+    n1 = torch.from_numpy(g["noise1"]).to(DEV)
+    n2 = torch.from_numpy(g["noise2"]).to(DEV)
+    rows = build_compton_rows(torch.from_numpy(A), torch.from_numpy(det), torch.from_numpy(coor),
+                              torch.from_numpy(g["list_in"].copy()), dr1, dr2, e0, res, tmax, tmin, tsum, DEV,
+                              noise=(n1, n2), want_tc=True)
+    kept = rows.kept_event_index().cpu().numpy()
+    borderline = np.abs(ref["support"] - 50.0) / 50.0 < 1e-5
+    assert not borderline.any()
+    assert np.array_equal(kept, ref["kept"])
+    t = rows.normalized_t().cpu().numpy()
+    assert _rel_rows(t, ref["t"]) < 1e-4
+
+
+def _small_list(n=600, rings=16, angles=64, seed=5):
+    from jscc_recon_spect_cylinder_amd import synth
+    return synth.compton_list(n_events=n, n_rings=rings, n_angles=angles, seed=seed)
+
+
+@pytest.mark.parametrize("store", ["fp32", "bf16"])
+def test_fixed_sigma_rows_match_oracle(store):
+    from jscc_recon_spect_cylinder_amd import synth
+    from jscc_recon_spect_cylinder_amd.process_list_plane_strict import build_compton_rows
+    cl = _small_list()
+    tmax, tmin, tsum = synth.energy_thresholds(cl.e0)
+    sig = np.deg2rad(3.0)
+    ref = cone.backproj_list_single(None, cl.detector, cl.coor, cl.list_origin, 0.0, 0.0, cl.e0, 0.02, tmax, tmin,
+                                    tsum, None, None, fixed_sigma=sig)
+    rows = build_compton_rows(None, torch.from_numpy(cl.detector), torch.from_numpy(cl.coor),
+                              torch.from_numpy(cl.list_origin.copy()), 0.0, 0.0, cl.e0, 0.02, tmax, tmin, tsum, DEV,
+                              noise="none", sigma_model="fixed", fixed_sigma=sig,
+                              store_dtype=torch.float32 if store == "fp32" else torch.bfloat16)
+    kept = rows.kept_event_index().cpu().numpy()
+    border = np.abs(ref["support"] - 50.0) / 50.0 < 1e-4
+    ok_ref = set(ref["kept"].tolist()) - set(np.nonzero(border)[0].tolist())
+    ok_gpu = set(kept.tolist()) - set(np.nonzero(border)[0].tolist())
+    assert ok_ref == ok_gpu
+    if store == "fp32":
+        common = np.isin(ref["kept"], kept)
+        t = rows.normalized_t().cpu().numpy()
+        tg = t[np.isin(kept, ref["kept"])]
+        assert _rel_rows(tg, ref["t"][common]) < 1e-4
+    else:
+        t = rows.normalized_t().cpu().numpy()
+        tg = t[np.isin(kept, ref["kept"])]
+        common = np.isin(ref["kept"], kept)
+        assert _rel_rows(tg, ref["t"][common]) < 5e-3  # BF16 storage: 2^-9 relative per element
+
+
+def test_cpu_list_is_mutated_and_device_list_too():
+    from jscc_recon_spect_cylinder_amd import synth
+    from jscc_recon_spect_cylinder_amd.process_list_plane_strict import build_compton_rows
+    cl = _small_list(n=64)
+    tmax, tmin, tsum = synth.energy_thresholds(cl.e0)
+    lst_cpu = torch.from_numpy(cl.list_origin.copy())
+    torch.manual_seed(7)
+    build_compton_rows(None, torch.from_numpy(cl.detector), torch.from_numpy(cl.coor), lst_cpu, 0.0, 0.0, cl.e0,
+                       0.1, tmax, tmin, tsum, DEV, noise="cpu", sigma_model="fixed", fixed_sigma=0.05)
+    torch.manual_seed(7)
+    n1 = torch.randn(64).numpy()
+    n2 = torch.randn(64).numpy()
+    ev = cone.prepare_events(cl.list_origin, cl.detector, cl.e0, 0.1, tmax, tmin, tsum, n1, n2)
+    assert np.array_equal(lst_cpu.numpy(), ev["list"])
+
+
+def test_index_error_raises():
+    from jscc_recon_spect_cylinder_amd.process_list_plane_strict import build_compton_rows
+    cl = _small_list(n=32)
+    lst = cl.list_origin.copy()
+    lst[3, 0] = cl.detector.shape[0] + 5  # crystal number beyond the table
+    with pytest.raises(IndexError):
+        build_compton_rows(None, torch.from_numpy(cl.detector), torch.from_numpy(cl.coor), torch.from_numpy(lst),
+                           0.0, 0.0, cl.e0, 0.02, 0.34, 0.05, 0.46, DEV, noise="none", sigma_model="fixed",
+                           fixed_sigma=0.05)
+
+
+def test_cpu_device_rejected():
+    from jscc_recon_spect_cylinder_amd.process_list_plane_strict import get_compton_backproj_list_single
+    with pytest.raises(ValueError):
+        get_compton_backproj_list_single(torch.ones(2, 4), torch.ones(2, 3), torch.ones(4, 3), torch.ones(1, 4), 0, 0,
+                                         0.511, 0.1, 0.34, 0.05, 0.46, torch.device("cpu"))
+
+
+def test_fp8_hardware_decode_matches_ocp_software_decode():
+    from jscc_recon_spect_cylinder_amd import ops
+    out = ops.fp8_selftest(torch.device(DEV)).cpu().numpy()
+    hw, sw = out[:256], out[256:]
+    nan = np.isnan(sw)
+    assert nan.sum() == 2                       # 0x7f and 0xff
+    assert np.array_equal(np.isnan(hw), nan)
+    assert np.array_equal(hw[~nan], sw[~nan])
+    assert sw[0x7E] == 448.0 and sw[0x01] == 2.0 ** -9 and sw[0x08] == 2.0 ** -6
