@@ -46,13 +46,16 @@ int jscc_device_cu_count(int* out);
  * crystal numbers; noise1/noise2: the N(0,1) draws of the reference's
  * torch.randn calls (both NULL: no smearing); list_writeback: where the
  * smeared e1/e2 are written back (the reference mutates its argument in
- * place), may be NULL; detector: [n_det, det_stride] FP32 (x, y, z, ...).
+ * place), may be NULL; detector: [n_det, det_stride] FP32 (x, y, z, ...);
+ * n_sysmat_rows: rows of the system matrix the first hit indexes (0: no
+ * matrix), checked with torch's index rules like sysmat[cpnum1 - 1].
  * det_layers4 / rear_abs_y describe the detector's |y| layers (exactly four
  * distinct |y| values, the largest being the rear layer).  The *_f32
  * arguments are the FP32 roundings the reference's comparisons use. */
 int jscc_events_prepare(const float* list, int64_t n, int64_t list_stride, const float* noise1,
                         const float* noise2, float* list_writeback, const float* detector, int64_t n_det,
-                        int64_t det_stride, int det_layers4, float rear_abs_y, float delta_r1, float delta_r2,
+                        int64_t det_stride, int64_t n_sysmat_rows, int det_layers4, float rear_abs_y,
+                        float delta_r1, float delta_r2,
                         float e0_f32, float ene_res_f32, float thr_max_f32, float thr_min_f32, float thr_sum_f32,
                         double e0, double ene_res, void* events_out, int32_t* flags_out, int32_t* err_out,
                         void* stream);
@@ -172,6 +175,30 @@ int64_t jscc_sensitivity_workspace_bytes(int64_t M, int64_t P);
 int jscc_sensitivity(const float* A, int64_t M, int64_t lda, int64_t P, const int64_t* inv, int64_t inv_row_stride,
                      int64_t inv_col_stride, int R, float* out, void* workspace, int64_t workspace_bytes,
                      int32_t* err, void* stream);
+
+/* No-store cone projection: the rows of T for events ev_idx[0..n_rows) are
+ * evaluated in registers (same model as jscc_cone_rows, first-hit factor
+ * sysmat[arow, p] or 1) and consumed at once, so T is never written.  Rows
+ * are divided by their sums when normalize != 0 (the reference's T).
+ *   mode 0: w (+)= out_scale * sum_i T_i / max(T_i x, eps)   (T^T(1/Tx) of
+ *           recon_osem_plane.py:33-42 for a T that is not stored)
+ *   mode 1: fwd_out[i] = T_i x
+ *   mode 2: w (+)= out_scale * sum_i coef_i T_i
+ * dsig_out (optional, fixed-sigma model): d(T_i x)/d sigma.  Used for lists
+ * whose T exceeds HBM, the lazy per-iteration recompute of
+ * distributed/python/recon_osem_dist_sparse_jsccsd_only_lazy.py:46-125, and
+ * the adjoint of the differentiable cone operator (no reference counterpart). */
+int jscc_cone_proj_grid(int64_t n_rows);
+int64_t jscc_cone_proj_workspace_bytes(int64_t n_rows, int64_t n_vox);
+int jscc_cone_proj(int mode, const void* events, const int32_t* ev_idx, int64_t n_rows, const float* coor_x,
+                   const float* coor_y, const float* coor_z, int64_t n_vox, const float* sysmat, int64_t ld_sysmat,
+                   int sigma_model, double fixed_sigma, const float* x, const float* coef, float eps, int use_eps,
+                   int normalize, float* fwd_out, float* dsig_out, float out_scale, int accumulate, float* w,
+                   int64_t w_len, void* workspace, int64_t workspace_bytes, int32_t* err, void* stream);
+
+/* FP64 sum of n strided FP32 values into *out (device), fixed order: the
+ * per-iteration count / likelihood reductions (main_plane.py:312-313). */
+int jscc_sum_f64(const float* v, int64_t n, int64_t stride, double* out, void* stream);
 
 /* Self-test of the hardware FP8 decoder: out512[0..255] hardware decode of
  * every e4m3fn code, out512[256..511] the library's software decode. */

@@ -27,8 +27,8 @@ SIGNATURES = {
     "jscc_abi_version": (c_int, []),
     "jscc_last_error": (ctypes.c_char_p, []),
     "jscc_device_cu_count": (c_int, [vp]),
-    "jscc_events_prepare": (c_int, [vp, c_i64, c_i64, vp, vp, vp, vp, c_i64, c_i64, c_int, c_f32, c_f32, c_f32,
-                                    c_f32, c_f32, c_f32, c_f32, c_f32, c_f64, c_f64, vp, vp, vp, vp]),
+    "jscc_events_prepare": (c_int, [vp, c_i64, c_i64, vp, vp, vp, vp, c_i64, c_i64, c_i64, c_int, c_f32, c_f32,
+                                    c_f32, c_f32, c_f32, c_f32, c_f32, c_f32, c_f64, c_f64, vp, vp, vp, vp]),
     "jscc_compact_workspace_bytes": (c_i64, [c_i64]),
     "jscc_compact_flags": (c_int, [vp, c_i64, vp, vp, vp, c_i64, vp]),
     "jscc_cone_rows": (c_int, [vp, vp, c_i64, vp, vp, vp, c_i64, vp, c_i64, vp, vp, vp, vp, c_int, c_f64, vp,
@@ -50,6 +50,11 @@ SIGNATURES = {
     "jscc_sensitivity_workspace_bytes": (c_i64, [c_i64, c_i64]),
     "jscc_sensitivity": (c_int, [vp, c_i64, c_i64, c_i64, vp, c_i64, c_i64, c_int, vp, vp, c_i64, vp, vp]),
     "jscc_fp8_selftest": (c_int, [vp, vp]),
+    "jscc_cone_proj_grid": (c_int, [c_i64]),
+    "jscc_cone_proj_workspace_bytes": (c_i64, [c_i64, c_i64]),
+    "jscc_cone_proj": (c_int, [c_int, vp, vp, c_i64, vp, vp, vp, c_i64, vp, c_i64, c_int, c_f64, vp, vp, c_f32, c_int,
+                               c_int, vp, vp, c_f32, c_int, vp, c_i64, vp, c_i64, vp, vp]),
+    "jscc_sum_f64": (c_int, [vp, c_i64, c_i64, vp, vp]),
     "jscc_prof_enable": (c_int, [c_int]),
     "jscc_prof_read": (c_int, [c_int, vp, vp]),
 }

@@ -136,6 +136,13 @@ __device__ __forceinline__ int jscc_wave_id() { return threadIdx.x >> 6; }
 // number of compute units of the current device (cached per process)
 int jscc_num_cus();
 
+// Deterministic two-level reduction of per-work-group partial back-projections
+// [G, P] into w (inverse-rotation gather, scale, accumulate, optional
+// nan_to_num); the buffer must hold jscc_partials_bytes(G, P).
+int jscc_reduce_partials(float* part, int G, int P, const int64_t* inv_perm, int64_t ips, float scale,
+                         int accumulate, int sanitize, float* w, int64_t w_len, int32_t* err, hipStream_t s);
+int64_t jscc_partials_bytes(int G, int64_t P);
+
 // Built-in kernel timer: when enabled, hot kernels are bracketed by HIP event
 // pairs recorded on their launch stream (tag = kernel family).
 enum { JSCC_TAG_CONE = 0, JSCC_TAG_PROJ = 1, JSCC_NUM_TAGS = 2 };

@@ -59,7 +59,96 @@ struct ConeConsts {
   float beta_min, beta_max;  // acos of the reference's FP32 clamp bounds
   float s2_min;              // 1 - cos_max^2 : sin^2 floor implied by the clamp
   float fixed_half_inv_sig2; // 0.5 / sigma^2 for the fixed-sigma model
+  float inv_sig3;            // 1 / sigma^3 (d log t / d sigma = delta^2 / sigma^3)
 };
+
+// wave-uniform parameters of one event (scalar registers)
+struct ConeEv {
+  float p1x, p1y, p1z, ax, ay, az, d12sq, inv_d12sq, th, kn, sp, sm, wx, wy, wz;
+  int64_t arow_off;
+};
+
+__device__ __forceinline__ ConeEv load_event(const JsccEvent& q, int64_t lda) {
+  ConeEv e;
+  e.p1x = q.p1x; e.p1y = q.p1y; e.p1z = q.p1z;
+  e.ax = q.ax; e.ay = q.ay; e.az = q.az;
+  e.d12sq = q.d12sq;
+  e.inv_d12sq = 1.0f / q.d12sq;
+  e.th = q.theta; e.kn = q.kn;
+  e.sp = q.sig_plus; e.sm = q.sig_minus;
+  e.wx = q.wx; e.wy = q.wy; e.wz = q.wz;
+  e.arow_off = (int64_t)q.arow * lda;
+  return e;
+}
+
+// t_c of one (event, voxel) pair.  For the fixed-sigma model and DSIG,
+// *dlog = d log(t_c) / d sigma = delta^2 / sigma^3.
+template <int SIG, bool DSIG>
+__device__ __forceinline__ float cone_tc(const ConeEv& e, float cxv, float cyv, float czv, const ConeConsts& cc,
+                                         float* dlog) {
+  const float vx = e.p1x - cxv;
+  const float vy = e.p1y - cyv;
+  const float vz = e.p1z - czv;
+  const float d01sq = fmaf(vx, vx, fmaf(vy, vy, vz * vz));
+  const float dot = fmaf(vx, e.ax, fmaf(vy, e.ay, vz * e.az));
+  const float mx = fmaf(vy, e.az, -vz * e.ay);
+  const float my = fmaf(vz, e.ax, -vx * e.az);
+  const float mz = fmaf(vx, e.ay, -vy * e.ax);
+  const float msq = fmaf(mx, mx, fmaf(my, my, mz * mz));
+  float beta = atan2_ypos(__builtin_sqrtf(msq), dot);
+  beta = fminf(fmaxf(beta, cc.beta_min), cc.beta_max);
+  const float inv_d01sq = __builtin_amdgcn_rcpf(d01sq);
+  float s2 = msq * inv_d01sq * e.inv_d12sq;
+  s2 = fminf(fmaxf(s2, cc.s2_min), 1.0f);
+  const float delta = beta - e.th;
+  float hinv;
+  if (SIG == SIG_STRICT) {
+    const float sige = delta >= 0.0f ? e.sp : e.sm;
+    const float qx = fmaf(e.d12sq, vx, -dot * e.ax);
+    const float qy = fmaf(e.d12sq, vy, -dot * e.ay);
+    const float qz = fmaf(e.d12sq, vz, -dot * e.az);
+    const float num = fmaf(qx * qx, e.wx, fmaf(qy * qy, e.wy, qz * qz * e.wz));
+    const float den = e.d12sq * e.d12sq * fmaxf(msq, 1e-12f * d01sq * e.d12sq);
+    const float spos2 = fmaxf(num * __builtin_amdgcn_rcpf(den), 1e-12f);
+    const float sig2 = fmaxf(fmaf(sige, sige, spos2), 1e-12f);
+    hinv = 0.5f * __builtin_amdgcn_rcpf(sig2);
+  } else {
+    hinv = cc.fixed_half_inv_sig2;
+  }
+  const float d2 = delta * delta;
+  if (DSIG) *dlog = d2 * cc.inv_sig3;
+  return __expf(-d2 * hinv) * (e.kn - s2);
+}
+
+template <int AF>
+__device__ __forceinline__ void afactor4(const ConeEv& e, const float* __restrict__ A, int base, const int* ti0,
+                                         const int* ti1, const float* tw0, const float* tw1, float* av) {
+  if (AF == AF_ROW) {
+    const float4 a4 = *reinterpret_cast<const float4*>(A + e.arow_off + base);
+    av[0] = a4.x; av[1] = a4.y; av[2] = a4.z; av[3] = a4.w;
+  } else if (AF == AF_TAPS) {
+    const float* Ar = A + e.arow_off;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) av[k] = fmaf(tw1[k], Ar[ti1[k]], tw0[k] * Ar[ti0[k]]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) av[k] = 1.0f;
+  }
+}
+
+template <int AF>
+__device__ __forceinline__ void load_taps(int base, int P, const int32_t* __restrict__ tap_i0,
+                                          const int32_t* __restrict__ tap_i1, const float* __restrict__ tap_w0,
+                                          const float* __restrict__ tap_w1, int* ti0, int* ti1, float* tw0,
+                                          float* tw1) {
+  if (AF == AF_TAPS) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int p = base + k < P ? base + k : P - 1;
+      ti0[k] = tap_i0[p]; ti1[k] = tap_i1[p]; tw0[k] = tap_w0[p]; tw1[k] = tap_w1[p];
+    }
+  }
+}
 
 template <int SIG, int AF, int OUT_DT, bool WANT_TC, int E>
 __global__ void __launch_bounds__(kConeThreads) k_cone_rows(
@@ -70,24 +159,13 @@ __global__ void __launch_bounds__(kConeThreads) k_cone_rows(
     ConeConsts cc, void* __restrict__ out_t, int64_t ldt, float* __restrict__ out_tc, int64_t ldtc,
     double* __restrict__ stats) {
   const int row0 = blockIdx.x * E;
-  // event parameters: uniform per work-group
-  float p1x[E], p1y[E], p1z[E], ax[E], ay[E], az[E], d12sq[E], inv_d12sq[E], th[E], kn[E];
-  float sp[E], sm[E], wx[E], wy[E], wz[E];
-  int64_t arow_off[E];
+  ConeEv evs[E];
   bool valid[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     valid[e] = (row0 + e) < n_rows;
     const int r = valid[e] ? (row0 + e) : (n_rows - 1);
-    const JsccEvent& q = ev[ev_idx[r]];
-    p1x[e] = q.p1x; p1y[e] = q.p1y; p1z[e] = q.p1z;
-    ax[e] = q.ax; ay[e] = q.ay; az[e] = q.az;
-    d12sq[e] = q.d12sq;
-    inv_d12sq[e] = 1.0f / q.d12sq;
-    th[e] = q.theta; kn[e] = q.kn;
-    sp[e] = q.sig_plus; sm[e] = q.sig_minus;
-    wx[e] = q.wx; wy[e] = q.wy; wz[e] = q.wz;
-    arow_off[e] = (int64_t)q.arow * lda;
+    evs[e] = load_event(ev[ev_idx[r]], lda);
   }
   double accS[E], accQ[E], accC[E];
   float accM[E];
@@ -103,59 +181,15 @@ __global__ void __launch_bounds__(kConeThreads) k_cone_rows(
     const float vcz[4] = {vz4.x, vz4.y, vz4.z, vz4.w};
     float tw0[4], tw1[4];
     int ti0[4], ti1[4];
-    if (AF == AF_TAPS) {
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const int p = base + k < P ? base + k : P - 1;
-        ti0[k] = tap_i0[p]; ti1[k] = tap_i1[p]; tw0[k] = tap_w0[p]; tw1[k] = tap_w1[p];
-      }
-    }
+    load_taps<AF>(base, P, tap_i0, tap_i1, tap_w0, tap_w1, ti0, ti1, tw0, tw1);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-      float av[4] = {1.0f, 1.0f, 1.0f, 1.0f};
-      if (AF == AF_ROW) {
-        const float4 a4 = *reinterpret_cast<const float4*>(A + arow_off[e] + base);
-        av[0] = a4.x; av[1] = a4.y; av[2] = a4.z; av[3] = a4.w;
-      } else if (AF == AF_TAPS) {
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          const float* Ar = A + arow_off[e];
-          av[k] = fmaf(tw1[k], Ar[ti1[k]], tw0[k] * Ar[ti0[k]]);
-        }
-      }
+      float av[4];
+      afactor4<AF>(evs[e], A, base, ti0, ti1, tw0, tw1, av);
       float tv[4], tcv[4];
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        const float vx = p1x[e] - vcx[k];
-        const float vy = p1y[e] - vcy[k];
-        const float vz = p1z[e] - vcz[k];
-        const float d01sq = fmaf(vx, vx, fmaf(vy, vy, vz * vz));
-        const float dot = fmaf(vx, ax[e], fmaf(vy, ay[e], vz * az[e]));
-        const float mx = fmaf(vy, az[e], -vz * ay[e]);
-        const float my = fmaf(vz, ax[e], -vx * az[e]);
-        const float mz = fmaf(vx, ay[e], -vy * ax[e]);
-        const float msq = fmaf(mx, mx, fmaf(my, my, mz * mz));
-        float beta = atan2_ypos(__builtin_sqrtf(msq), dot);
-        beta = fminf(fmaxf(beta, cc.beta_min), cc.beta_max);
-        const float inv_d01sq = __builtin_amdgcn_rcpf(d01sq);
-        float s2 = msq * inv_d01sq * inv_d12sq[e];
-        s2 = fminf(fmaxf(s2, cc.s2_min), 1.0f);
-        const float delta = beta - th[e];
-        float hinv;
-        if (SIG == SIG_STRICT) {
-          const float sige = delta >= 0.0f ? sp[e] : sm[e];
-          const float qx = fmaf(d12sq[e], vx, -dot * ax[e]);
-          const float qy = fmaf(d12sq[e], vy, -dot * ay[e]);
-          const float qz = fmaf(d12sq[e], vz, -dot * az[e]);
-          const float num = fmaf(qx * qx, wx[e], fmaf(qy * qy, wy[e], qz * qz * wz[e]));
-          const float den = d12sq[e] * d12sq[e] * fmaxf(msq, 1e-12f * d01sq * d12sq[e]);
-          const float spos2 = fmaxf(num * __builtin_amdgcn_rcpf(den), 1e-12f);
-          const float sig2 = fmaxf(fmaf(sige, sige, spos2), 1e-12f);
-          hinv = 0.5f * __builtin_amdgcn_rcpf(sig2);
-        } else {
-          hinv = cc.fixed_half_inv_sig2;
-        }
-        const float tc = __expf(-(delta * delta) * hinv) * (kn[e] - s2);
+        const float tc = cone_tc<SIG, false>(evs[e], vcx[k], vcy[k], vcz[k], cc, nullptr);
         const bool in = (base + k) < P;
         tcv[k] = in ? tc : 0.0f;
         tv[k] = in ? tc * av[k] : 0.0f;
@@ -283,6 +317,7 @@ extern "C" int jscc_cone_rows(const void* events, const int32_t* ev_idx, int64_t
   cc.beta_max = (float)(M_PI - acos(cmax));
   cc.s2_min = (float)(1.0 - cmax * cmax);
   cc.fixed_half_inv_sig2 = sigma_model == SIG_FIXED ? (float)(0.5 / (fixed_sigma * fixed_sigma)) : 0.0f;
+  cc.inv_sig3 = sigma_model == SIG_FIXED ? (float)(1.0 / (fixed_sigma * fixed_sigma * fixed_sigma)) : 0.0f;
 
   hipStream_t s = (hipStream_t)stream;
   const JsccEvent* ev = (const JsccEvent*)events;
@@ -314,4 +349,238 @@ extern "C" int jscc_cone_rows(const void* events, const int32_t* ev_idx, int64_t
   if (rc != 0) return rc;
   JSCC_CHECK_LAUNCH();
   return 0;
+}
+
+// ============================================================================
+// No-store cone projection: rows are evaluated in registers and consumed at
+// once, so T never touches HBM.  Used when T does not fit in HBM (lists of
+// many millions of events), by the differentiable operator's backward pass
+// ("T recomputed in the backward") and for lazy event lists.
+//   CP_RATIO: w += T_i^T / max(T_i x, eps)         (list-mode OSEM weight)
+//   CP_FWD:   fwd[i] = T_i x                       (+ d(T_i x)/d sigma)
+//   CP_COEF:  w += coef_i T_i^T                    (+ d(T_i x)/d sigma)
+// T_i is the row divided by its sum when `normalize` (the reference's rows).
+// ============================================================================
+namespace {
+
+enum { CP_RATIO = 0, CP_FWD = 1, CP_COEF = 2 };
+
+struct ConeProjArgs {
+  const JsccEvent* ev;
+  const int32_t* ev_idx;
+  int64_t n_rows;
+  const float* cx;
+  const float* cy;
+  const float* cz;
+  int P;
+  const float* A;
+  int64_t lda;
+  ConeConsts cc;
+  const float* x;
+  const float* coef;
+  float eps;
+  int use_eps;
+  int normalize;
+  float* fwd_out;
+  float* dsig_out;
+  float* partial;
+};
+
+template <int SIG, int AF, int BLOCK, int NCH, int MODE, bool DSIG>
+__global__ void __launch_bounds__(BLOCK) k_cone_proj(ConeProjArgs a) {
+  constexpr int NW = BLOCK / JSCC_WAVE;
+  const int tid = threadIdx.x;
+  float tr[NCH][4];
+  float wr[NCH][4];
+#pragma unroll
+  for (int c = 0; c < NCH; ++c)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) wr[c][k] = 0.0f;
+  const int64_t per = a.n_rows / gridDim.x, rem = a.n_rows % gridDim.x;
+  const int64_t r_begin = blockIdx.x * per + (blockIdx.x < rem ? blockIdx.x : rem);
+  const int64_t r_end = r_begin + per + (blockIdx.x < rem ? 1 : 0);
+  __shared__ double red[2][4][NW];
+  int buf = 0;
+  for (int64_t i = r_begin; i < r_end; ++i) {
+    const ConeEv e = load_event(a.ev[a.ev_idx[i]], a.lda);
+    float S = 0.0f, D = 0.0f, dS = 0.0f, dD = 0.0f;
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+      const int base = (c * BLOCK + tid) * 4;
+      if (base < a.P) {
+        const float4 vx4 = *reinterpret_cast<const float4*>(a.cx + base);
+        const float4 vy4 = *reinterpret_cast<const float4*>(a.cy + base);
+        const float4 vz4 = *reinterpret_cast<const float4*>(a.cz + base);
+        const float4 x4 = *reinterpret_cast<const float4*>(a.x + base);
+        const float vcx[4] = {vx4.x, vx4.y, vx4.z, vx4.w};
+        const float vcy[4] = {vy4.x, vy4.y, vy4.z, vy4.w};
+        const float vcz[4] = {vz4.x, vz4.y, vz4.z, vz4.w};
+        const float xv[4] = {x4.x, x4.y, x4.z, x4.w};
+        float av[4];
+        afactor4<AF>(e, a.A, base, nullptr, nullptr, nullptr, nullptr, av);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          float dl = 0.0f;
+          const float t = cone_tc<SIG, DSIG>(e, vcx[k], vcy[k], vcz[k], a.cc, &dl) * av[k];
+          tr[c][k] = t;
+          S += t;
+          D = fmaf(t, xv[k], D);
+          if (DSIG) {
+            const float dt = t * dl;
+            dS += dt;
+            dD = fmaf(dt, xv[k], dD);
+          }
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) tr[c][k] = 0.0f;
+      }
+    }
+    double vS = jscc_wave_sum((double)S);
+    double vD = jscc_wave_sum((double)D);
+    double vdS = DSIG ? jscc_wave_sum((double)dS) : 0.0;
+    double vdD = DSIG ? jscc_wave_sum((double)dD) : 0.0;
+    if (jscc_lane() == 0) {
+      red[buf][0][jscc_wave_id()] = vS;
+      red[buf][1][jscc_wave_id()] = vD;
+      red[buf][2][jscc_wave_id()] = vdS;
+      red[buf][3][jscc_wave_id()] = vdD;
+    }
+    __syncthreads();
+    double tS = 0.0, tD = 0.0, tdS = 0.0, tdD = 0.0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      tS += red[buf][0][w];
+      tD += red[buf][1][w];
+      if (DSIG) {
+        tdS += red[buf][2][w];
+        tdD += red[buf][3][w];
+      }
+    }
+    buf ^= 1;
+    const double inv = a.normalize ? 1.0 / tS : 1.0;
+    const double f = tD * inv;
+    if (tid == 0) {
+      if (MODE == CP_FWD) a.fwd_out[i] = (float)f;
+      if (DSIG && a.dsig_out)
+        a.dsig_out[i] = (float)(a.normalize ? (tdD * tS - tD * tdS) * inv * inv : tdD);
+    }
+    if (MODE != CP_FWD) {
+      float coef;
+      if (MODE == CP_RATIO) {
+        const double den = a.use_eps ? (f > (double)a.eps ? f : (double)a.eps) : f;
+        coef = (float)(inv / den);
+      } else {
+        coef = (float)((double)a.coef[i] * inv);
+      }
+#pragma unroll
+      for (int c = 0; c < NCH; ++c)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) wr[c][k] = fmaf(coef, tr[c][k], wr[c][k]);
+    }
+  }
+  if (MODE != CP_FWD) {
+    float* dst = a.partial + (int64_t)blockIdx.x * a.P;
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+      const int base = (c * BLOCK + tid) * 4;
+      if (base < a.P)
+        *reinterpret_cast<float4*>(dst + base) = make_float4(wr[c][0], wr[c][1], wr[c][2], wr[c][3]);
+    }
+  }
+}
+
+template <int SIG, int AF, int BLOCK, int NCH, bool DSIG>
+void launch_cp_mode(int mode, const ConeProjArgs& a, int G, hipStream_t s) {
+  if (mode == CP_RATIO)
+    hipLaunchKernelGGL((k_cone_proj<SIG, AF, BLOCK, NCH, CP_RATIO, DSIG>), dim3(G), dim3(BLOCK), 0, s, a);
+  else if (mode == CP_FWD)
+    hipLaunchKernelGGL((k_cone_proj<SIG, AF, BLOCK, NCH, CP_FWD, DSIG>), dim3(G), dim3(BLOCK), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_cone_proj<SIG, AF, BLOCK, NCH, CP_COEF, DSIG>), dim3(G), dim3(BLOCK), 0, s, a);
+}
+
+template <int SIG, int AF, bool DSIG>
+int launch_cp_size(int mode, const ConeProjArgs& a, int G, hipStream_t s) {
+  // the row lives in registers: 4 * NCH voxels per thread
+  if (a.P <= 4 * 256 * 4) launch_cp_mode<SIG, AF, 256, 4, DSIG>(mode, a, G, s);
+  else if (a.P <= 4 * 512 * 4) launch_cp_mode<SIG, AF, 512, 4, DSIG>(mode, a, G, s);
+  else if (a.P <= 4 * 512 * 8) launch_cp_mode<SIG, AF, 512, 8, DSIG>(mode, a, G, s);
+  else if (a.P <= 4 * 1024 * 8) launch_cp_mode<SIG, AF, 1024, 8, DSIG>(mode, a, G, s);
+  else {
+    jscc_set_error("jscc_cone_proj: %d voxels exceed the register-resident row limit (32768)", a.P);
+    return -3;
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int jscc_cone_proj_grid(int64_t n_rows) {
+  int64_t g = 4 * (int64_t)jscc_num_cus();
+  if (g > n_rows) g = n_rows;
+  return (int)(g < 1 ? 1 : g);
+}
+
+extern "C" int64_t jscc_cone_proj_workspace_bytes(int64_t n_rows, int64_t n_vox) {
+  return jscc_partials_bytes(jscc_cone_proj_grid(n_rows), n_vox);
+}
+
+extern "C" int jscc_cone_proj(int mode, const void* events, const int32_t* ev_idx, int64_t n_rows,
+                              const float* coor_x, const float* coor_y, const float* coor_z, int64_t n_vox,
+                              const float* sysmat, int64_t ld_sysmat, int sigma_model, double fixed_sigma,
+                              const float* x, const float* coef, float eps, int use_eps, int normalize,
+                              float* fwd_out, float* dsig_out, float out_scale, int accumulate, float* w,
+                              int64_t w_len, void* workspace, int64_t workspace_bytes, int32_t* err, void* stream) {
+  JSCC_REQUIRE(mode >= 0 && mode <= 2, "jscc_cone_proj: bad mode");
+  JSCC_REQUIRE(n_rows >= 0 && n_rows < ((int64_t)1 << 31) && n_vox > 0 && n_vox % 4 == 0,
+               "jscc_cone_proj: voxel count must be a positive multiple of 4");
+  JSCC_REQUIRE(sigma_model == SIG_STRICT || sigma_model == SIG_FIXED, "jscc_cone_proj: bad sigma model");
+  JSCC_REQUIRE(dsig_out == nullptr || sigma_model == SIG_FIXED,
+               "jscc_cone_proj: d/d sigma exists for the fixed-sigma model only");
+  JSCC_REQUIRE(events && ev_idx && coor_x && coor_y && coor_z && x && err, "jscc_cone_proj: null pointer");
+  JSCC_REQUIRE(((uintptr_t)coor_x | (uintptr_t)coor_y | (uintptr_t)coor_z | (uintptr_t)x) % 16 == 0,
+               "jscc_cone_proj: coordinates and image must be 16-byte aligned");
+  if (sysmat) JSCC_REQUIRE(ld_sysmat % 4 == 0 && (uintptr_t)sysmat % 16 == 0, "jscc_cone_proj: sysmat alignment");
+  if (mode == CP_FWD) JSCC_REQUIRE(fwd_out != nullptr, "jscc_cone_proj: null forward output");
+  if (mode == CP_COEF) JSCC_REQUIRE(coef != nullptr, "jscc_cone_proj: null coefficients");
+  if (mode != CP_FWD) JSCC_REQUIRE(w != nullptr && w_len == n_vox, "jscc_cone_proj: weight must have n_vox entries");
+  if (sigma_model == SIG_FIXED) JSCC_REQUIRE(fixed_sigma > 0.0, "jscc_cone_proj: fixed sigma must be positive");
+  hipStream_t s = (hipStream_t)stream;
+  if (n_rows == 0) {
+    if (mode != CP_FWD && !accumulate) JSCC_CHECK_HIP(hipMemsetAsync(w, 0, w_len * sizeof(float), s));
+    return 0;
+  }
+  const int G = jscc_cone_proj_grid(n_rows);
+  if (mode != CP_FWD)
+    JSCC_REQUIRE(workspace && workspace_bytes >= jscc_partials_bytes(G, n_vox), "jscc_cone_proj: workspace too small");
+  ConeProjArgs a;
+  a.ev = (const JsccEvent*)events; a.ev_idx = ev_idx; a.n_rows = n_rows;
+  a.cx = coor_x; a.cy = coor_y; a.cz = coor_z; a.P = (int)n_vox; a.A = sysmat; a.lda = ld_sysmat;
+  const double cmax = (double)(float)(1.0 - 1e-7);
+  a.cc.beta_min = (float)acos(cmax);
+  a.cc.beta_max = (float)(M_PI - acos(cmax));
+  a.cc.s2_min = (float)(1.0 - cmax * cmax);
+  a.cc.fixed_half_inv_sig2 = sigma_model == SIG_FIXED ? (float)(0.5 / (fixed_sigma * fixed_sigma)) : 0.0f;
+  a.cc.inv_sig3 = sigma_model == SIG_FIXED ? (float)(1.0 / (fixed_sigma * fixed_sigma * fixed_sigma)) : 0.0f;
+  a.x = x; a.coef = coef; a.eps = eps; a.use_eps = use_eps; a.normalize = normalize;
+  a.fwd_out = fwd_out; a.dsig_out = dsig_out; a.partial = (float*)workspace;
+  const bool dsig = dsig_out != nullptr;
+  int rc;
+  jscc_prof_begin(JSCC_TAG_CONE, s);
+  if (sigma_model == SIG_STRICT) {
+    rc = sysmat ? launch_cp_size<SIG_STRICT, AF_ROW, false>(mode, a, G, s)
+                : launch_cp_size<SIG_STRICT, AF_NONE, false>(mode, a, G, s);
+  } else if (dsig) {
+    rc = sysmat ? launch_cp_size<SIG_FIXED, AF_ROW, true>(mode, a, G, s)
+                : launch_cp_size<SIG_FIXED, AF_NONE, true>(mode, a, G, s);
+  } else {
+    rc = sysmat ? launch_cp_size<SIG_FIXED, AF_ROW, false>(mode, a, G, s)
+                : launch_cp_size<SIG_FIXED, AF_NONE, false>(mode, a, G, s);
+  }
+  jscc_prof_end(JSCC_TAG_CONE, s);
+  if (rc != 0) return rc;
+  JSCC_CHECK_LAUNCH();
+  if (mode == CP_FWD) return 0;
+  return jscc_reduce_partials((float*)workspace, G, (int)n_vox, nullptr, 0, out_scale, accumulate, 0, w, w_len, err, s);
 }

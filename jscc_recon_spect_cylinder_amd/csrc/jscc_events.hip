@@ -53,7 +53,7 @@ __global__ void __launch_bounds__(256) k_events_prepare(
     const float* __restrict__ list, int64_t n, int64_t ls,
     const float* __restrict__ noise1, const float* __restrict__ noise2,
     float* __restrict__ writeback,
-    const float* __restrict__ det, int64_t ndet, int64_t ds,
+    const float* __restrict__ det, int64_t ndet, int64_t ds, int64_t n_arows,
     int layers4, float rear_abs_y, float dr1, float dr2,
     float e0f, float resf, float thr_max, float thr_min, float thr_sum,
     double e0d, double resd,
@@ -135,7 +135,19 @@ __global__ void __launch_bounds__(256) k_events_prepare(
         ev.wx = __fadd_rn(ax1, ax2);
         ev.wy = __fadd_rn(ay1, ay2);
         ev.wz = __fadd_rn(az1, az2);
-        ev.arow = (int32_t)i1;
+        // :237-240 sysmat[cpnum1 - 1] (torch wrap relative to the matrix rows)
+        int64_t ia = (int64_t)(int32_t)c1f - 1;
+        if (n_arows > 0) {
+          if (ia < -n_arows || ia >= n_arows) {
+            atomicOr(err, 1);
+            keep = false;
+          } else if (ia < 0) {
+            ia += n_arows;
+          }
+        } else {
+          ia = i1;
+        }
+        ev.arow = (int32_t)ia;
       }
     }
   }
@@ -228,7 +240,7 @@ __global__ void __launch_bounds__(256) k_rows_keep(const double* __restrict__ st
 
 extern "C" int jscc_events_prepare(const float* list, int64_t n, int64_t list_stride,
                                    const float* noise1, const float* noise2, float* list_writeback,
-                                   const float* detector, int64_t n_det, int64_t det_stride,
+                                   const float* detector, int64_t n_det, int64_t det_stride, int64_t n_sysmat_rows,
                                    int det_layers4, float rear_abs_y, float delta_r1, float delta_r2,
                                    float e0_f32, float ene_res_f32, float thr_max_f32, float thr_min_f32,
                                    float thr_sum_f32, double e0, double ene_res,
@@ -236,12 +248,14 @@ extern "C" int jscc_events_prepare(const float* list, int64_t n, int64_t list_st
   JSCC_REQUIRE(n >= 0, "jscc_events_prepare: negative event count");
   if (n == 0) return 0;
   JSCC_REQUIRE(list && detector && events_out && flags_out && err_out, "jscc_events_prepare: null pointer");
-  JSCC_REQUIRE(list_stride >= 4 && det_stride >= 3 && n_det > 0, "jscc_events_prepare: bad strides");
+  JSCC_REQUIRE(list_stride >= 4 && det_stride >= 3 && n_det > 0 && n_sysmat_rows >= 0,
+               "jscc_events_prepare: bad strides");
   JSCC_REQUIRE((noise1 == nullptr) == (noise2 == nullptr), "jscc_events_prepare: noise1/noise2 must both be set or unset");
   hipStream_t s = (hipStream_t)stream;
   int64_t nb = (n + 255) / 256;
   hipLaunchKernelGGL(k_events_prepare, dim3((unsigned)nb), dim3(256), 0, s, list, n, list_stride, noise1, noise2,
-                     list_writeback, detector, n_det, det_stride, det_layers4, rear_abs_y, delta_r1, delta_r2,
+                     list_writeback, detector, n_det, det_stride, n_sysmat_rows, det_layers4, rear_abs_y, delta_r1,
+                     delta_r2,
                      e0_f32, ene_res_f32, thr_max_f32, thr_min_f32, thr_sum_f32, e0, ene_res,
                      (JsccEvent*)events_out, flags_out, err_out);
   JSCC_CHECK_LAUNCH();

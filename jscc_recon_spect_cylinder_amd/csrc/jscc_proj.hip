@@ -88,16 +88,20 @@ __global__ void __launch_bounds__(kProjThreads) k_proj_fused(ProjArgs a) {
   const int tid = threadIdx.x;
   const int nvec = a.P / VN;
 
-  // the image (gathered through the rotation) lives in registers
-  float xr[NCH][VN];
+  // The image (gathered through the rotation) is kept in LDS as per-thread
+  // storage: each thread only reads back the positions it wrote, in the same
+  // chunk layout as the rows, so no barrier is needed and 4*NCH*VN/4 VGPRs
+  // are freed for more resident work-groups.
+  extern __shared__ jscc_f4v xs4[];
   float wr[NCH][VN];
 #pragma unroll
   for (int c = 0; c < NCH; ++c) {
     const int vi = c * kProjThreads + tid;
+    float xv[VN];
 #pragma unroll
     for (int k = 0; k < VN; ++k) {
       wr[c][k] = 0.0f;
-      float xv = 0.0f;
+      xv[k] = 0.0f;
       if (MODE != MODE_ONES && MODE != MODE_COEF && vi < nvec) {
         const int64_t p = (int64_t)vi * VN + k;
         int64_t src = p;
@@ -105,9 +109,16 @@ __global__ void __launch_bounds__(kProjThreads) k_proj_fused(ProjArgs a) {
           src = a.x_perm[p * a.x_perm_stride] - 1;
           if (src < 0 || src >= a.x_len) { atomicOr(a.err, 2); src = 0; }
         }
-        xv = a.x[src];
+        xv[k] = a.x[src];
       }
-      xr[c][k] = xv;
+    }
+    if (MODE == MODE_RATIO || MODE == MODE_FWD) {
+#pragma unroll
+      for (int kk = 0; kk < VN / 4; ++kk) {
+        jscc_f4v t;
+        t.x = xv[4 * kk]; t.y = xv[4 * kk + 1]; t.z = xv[4 * kk + 2]; t.w = xv[4 * kk + 3];
+        xs4[(c * (VN / 4) + kk) * kProjThreads + tid] = t;
+      }
     }
   }
 
@@ -145,17 +156,25 @@ __global__ void __launch_bounds__(kProjThreads) k_proj_fused(ProjArgs a) {
     if (MODE == MODE_RATIO || MODE == MODE_FWD) {
       float d[R];
 #pragma unroll
-      for (int q = 0; q < R; ++q) {
-        float acc = 0.0f;
+      for (int q = 0; q < R; ++q) d[q] = 0.0f;
 #pragma unroll
-        for (int c = 0; c < NCH; ++c) {
+      for (int c = 0; c < NCH; ++c) {
+        float xv[VN];
+#pragma unroll
+        for (int kk = 0; kk < VN / 4; ++kk) {
+          const jscc_f4v t = xs4[(c * (VN / 4) + kk) * kProjThreads + tid];
+          xv[4 * kk] = t.x; xv[4 * kk + 1] = t.y; xv[4 * kk + 2] = t.z; xv[4 * kk + 3] = t.w;
+        }
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
           float v[VN];
           decode<DT>(raw[q][c], v);
 #pragma unroll
-          for (int k = 0; k < VN; ++k) acc = fmaf(v[k], xr[c][k], acc);
+          for (int k = 0; k < VN; ++k) d[q] = fmaf(v[k], xv[k], d[q]);
         }
-        d[q] = jscc_wave_sum(acc);
       }
+#pragma unroll
+      for (int q = 0; q < R; ++q) d[q] = jscc_wave_sum(d[q]);
       if (jscc_lane() == 0) {
 #pragma unroll
         for (int q = 0; q < R; ++q) red[buf][q][jscc_wave_id()] = d[q];
@@ -190,6 +209,14 @@ __global__ void __launch_bounds__(kProjThreads) k_proj_fused(ProjArgs a) {
       for (int q = 0; q < R; ++q) coefq[q] = (i0 + q < r_end) ? sc[q] : 0.0f;
     }
     if (MODE != MODE_FWD) {
+      if (DT != JSCC_DT_F32) {
+        // Packed rows are decoded a second time after the reduction instead of
+        // keeping the decoded floats live across it (halves the row registers).
+#pragma unroll
+        for (int q = 0; q < R; ++q)
+#pragma unroll
+          for (int c = 0; c < NCH; ++c) asm volatile("" : "+v"(raw[q][c]));
+      }
 #pragma unroll
       for (int q = 0; q < R; ++q)
 #pragma unroll
@@ -328,17 +355,29 @@ __global__ void __launch_bounds__(256) k_sens_gather(const double* __restrict__ 
   out[j] = (float)(acc / (double)R);
 }
 
+template <int DT, int NCH, int R, int MODE>
+int launch_one(const ProjArgs& a, int G, hipStream_t s) {
+  // image slots in LDS: NCH chunks x 512 threads x VN floats (only for the modes that read x)
+  const size_t lds = (MODE == MODE_RATIO || MODE == MODE_FWD) ? (size_t)NCH * kProjThreads * Vec<DT>::N * 4 : 0;
+  static bool attr_set = false;
+  if (lds > 65536 && !attr_set) {
+    JSCC_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_proj_fused<DT, NCH, R, MODE>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    attr_set = true;
+  }
+  hipLaunchKernelGGL((k_proj_fused<DT, NCH, R, MODE>), dim3(G), dim3(kProjThreads), lds, s, a);
+  return 0;
+}
+
 template <int DT, int NCH, int R>
 int launch_fused(int mode, const ProjArgs& a, int G, hipStream_t s) {
-  dim3 grid(G), block(kProjThreads);
   switch (mode) {
-    case MODE_RATIO: hipLaunchKernelGGL((k_proj_fused<DT, NCH, R, MODE_RATIO>), grid, block, 0, s, a); break;
-    case MODE_FWD: hipLaunchKernelGGL((k_proj_fused<DT, NCH, R, MODE_FWD>), grid, block, 0, s, a); break;
-    case MODE_COEF: hipLaunchKernelGGL((k_proj_fused<DT, NCH, R, MODE_COEF>), grid, block, 0, s, a); break;
-    case MODE_ONES: hipLaunchKernelGGL((k_proj_fused<DT, NCH, R, MODE_ONES>), grid, block, 0, s, a); break;
+    case MODE_RATIO: return launch_one<DT, NCH, R, MODE_RATIO>(a, G, s);
+    case MODE_FWD: return launch_one<DT, NCH, R, MODE_FWD>(a, G, s);
+    case MODE_COEF: return launch_one<DT, NCH, R, MODE_COEF>(a, G, s);
+    case MODE_ONES: return launch_one<DT, NCH, R, MODE_ONES>(a, G, s);
     default: jscc_set_error("jscc_proj: bad mode %d", mode); return -2;
   }
-  return 0;
 }
 
 template <int DT, int R>
@@ -432,27 +471,62 @@ extern "C" int jscc_proj(int mode, const void* mat, int dtype, int64_t ld, int64
   a.partial = (float*)workspace; a.err = err; a.y_by_row = (flags & 1) ? 1 : 0;
   int rc;
   jscc_prof_begin(JSCC_TAG_PROJ, s);
-  if (dtype == JSCC_DT_F32) rc = dispatch_nch<JSCC_DT_F32, 2>(mode, a, G, s);
-  else if (dtype == JSCC_DT_BF16) rc = dispatch_nch<JSCC_DT_BF16, 4>(mode, a, G, s);
-  else rc = dispatch_nch<JSCC_DT_FP8, 4>(mode, a, G, s);
+  if (dtype == JSCC_DT_F32) rc = dispatch_nch<JSCC_DT_F32, 1>(mode, a, G, s);
+  else if (dtype == JSCC_DT_BF16) rc = dispatch_nch<JSCC_DT_BF16, 2>(mode, a, G, s);
+  else rc = dispatch_nch<JSCC_DT_FP8, 2>(mode, a, G, s);
   jscc_prof_end(JSCC_TAG_PROJ, s);
   if (rc != 0) return rc;
   JSCC_CHECK_LAUNCH();
   if (mode == 1) return 0;
 
-  float* part = (float*)workspace;
+  return jscc_reduce_partials((float*)workspace, G, (int)n_cols, inv_perm, inv_perm_stride, out_scale, accumulate,
+                              (flags & 2) ? 1 : 0, w, w_len, err, s);
+}
+
+// partial [G, P] (followed by room for ceil(G/64) more rows) -> w, deterministic order
+int jscc_reduce_partials(float* part, int G, int P, const int64_t* inv_perm, int64_t ips, float scale,
+                         int accumulate, int sanitize, float* w, int64_t w_len, int32_t* err, hipStream_t s) {
+  JSCC_REQUIRE(P % 4 == 0, "jscc_reduce_partials: columns must be a multiple of 4");
   int Gfin = G;
   if (G > kRedRows) {
-    float* part2 = part + (int64_t)G * n_cols;
-    dim3 grid1((unsigned)((n_cols / 4 + 255) / 256), (unsigned)G2);
-    hipLaunchKernelGGL(k_reduce_stage1, grid1, dim3(256), 0, s, part, G, (int)n_cols, part2);
+    const int G2 = (G + kRedRows - 1) / kRedRows;
+    float* part2 = part + (int64_t)G * P;
+    dim3 grid1((unsigned)((P / 4 + 255) / 256), (unsigned)G2);
+    hipLaunchKernelGGL(k_reduce_stage1, grid1, dim3(256), 0, s, part, G, P, part2);
     JSCC_CHECK_LAUNCH();
     part = part2;
     Gfin = G2;
   }
-  hipLaunchKernelGGL(k_reduce_stage2, dim3((unsigned)((w_len + 255) / 256)), dim3(256), 0, s, part, Gfin,
-                     (int)n_cols, inv_perm, inv_perm_stride, out_scale, accumulate, (flags & 2) ? 1 : 0, w, w_len,
-                     err);
+  hipLaunchKernelGGL(k_reduce_stage2, dim3((unsigned)((w_len + 255) / 256)), dim3(256), 0, s, part, Gfin, P,
+                     inv_perm, ips, scale, accumulate, sanitize, w, w_len, err);
+  JSCC_CHECK_LAUNCH();
+  return 0;
+}
+
+int64_t jscc_partials_bytes(int G, int64_t P) {
+  const int64_t G2 = (G + kRedRows - 1) / kRedRows;
+  return ((int64_t)G + G2) * P * (int64_t)sizeof(float);
+}
+
+// ------------------------------------------------------------------ FP64 sums
+__global__ void __launch_bounds__(1024) k_sum_f64(const float* __restrict__ v, int64_t n, int64_t stride,
+                                                  double* __restrict__ out) {
+  __shared__ double red[1024 / JSCC_WAVE];
+  double s = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += 1024) s += (double)v[i * stride];
+  s = jscc_wave_sum(s);
+  if (jscc_lane() == 0) red[jscc_wave_id()] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int k = 0; k < 1024 / JSCC_WAVE; ++k) t += red[k];
+    *out = t;
+  }
+}
+
+extern "C" int jscc_sum_f64(const float* v, int64_t n, int64_t stride, double* out, void* stream) {
+  JSCC_REQUIRE(n >= 0 && out != nullptr && (n == 0 || v != nullptr), "jscc_sum_f64: bad arguments");
+  hipLaunchKernelGGL(k_sum_f64, dim3(1), dim3(1024), 0, (hipStream_t)stream, v, n, stride > 0 ? stride : 1, out);
   JSCC_CHECK_LAUNCH();
   return 0;
 }

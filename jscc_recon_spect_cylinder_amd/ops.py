@@ -158,13 +158,16 @@ def aligned_rows(A: torch.Tensor, vec: int = 4, device: torch.device | None = No
 # ----------------------------------------------------------------------------- events
 def events_prepare(list_dev: torch.Tensor, noise1, noise2, writeback: bool, detector: torch.Tensor,
                    delta_r1: float, delta_r2: float, e0: float, ene_resolution: float,
-                   thr_max: float, thr_min: float, thr_sum: float):
+                   thr_max: float, thr_min: float, thr_sum: float, n_sysmat_rows: int = 0):
     """Per-event records [n,16] (FP32 view of 64-byte structs) and keep flags [n] int32."""
     require_gpu(list_dev, "list_origin")
     dev = list_dev.device
     n = list_dev.shape[0]
     if list_dev.dtype != torch.float32 or list_dev.stride(1) != 1:
         raise ValueError("list rows must be contiguous float32")
+    for nz in (noise1, noise2):
+        if nz is not None and (nz.numel() < n or nz.dtype != torch.float32 or not nz.is_contiguous()):
+            raise ValueError("noise tensors must be contiguous float32 with one value per event")
     det = detector
     if det.dtype != torch.float32 or det.stride(1) != 1:
         det = det.float().contiguous()
@@ -177,7 +180,7 @@ def events_prepare(list_dev: torch.Tensor, noise1, noise2, writeback: bool, dete
     rc = _lib.lib().jscc_events_prepare(
         list_dev.data_ptr(), n, list_dev.stride(0), _p(noise1), _p(noise2),
         list_dev.data_ptr() if writeback else None, det.data_ptr(), det.shape[0], det.stride(0),
-        layers4, f32(rear), f32(delta_r1), f32(delta_r2),
+        int(n_sysmat_rows), layers4, f32(rear), f32(delta_r1), f32(delta_r2),
         f32(e0), f32(ene_resolution), f32(thr_max), f32(thr_min), f32(thr_sum), float(e0), float(ene_resolution),
         events.data_ptr(), flags.data_ptr(), err.data_ptr(), stream_of(dev))
     _lib.check(rc, "jscc_events_prepare")
@@ -393,6 +396,33 @@ def proj(mode: int, mat: torch.Tensor, *, dtype_c: int | None = None, n_cols: in
     wl = 0
     if w is not None:
         wl = w.shape[0]
+    # operand extents the kernels rely on
+    if P > mat.shape[1] and mat.stride(0) < P:
+        raise ValueError("n_cols exceeds the matrix row")
+    if rows is not None:
+        if row_base + n_rows > rows.shape[0] or rows.dtype != torch.int32:
+            raise ValueError("row list shorter than row_base + n_rows (int32 expected)")
+    elif row_base + n_rows > mat.shape[0]:
+        raise ValueError("row range exceeds the matrix")
+    if row_scale is not None and row_scale.numel() < mat.shape[0]:
+        raise ValueError("row_scale needs one entry per stored row")
+    if x_perm is not None and x_perm.shape[0] < P:
+        raise ValueError(f"RotMat column has {x_perm.shape[0]} entries for {P} pixels")
+    if x is not None and x_perm is None and x.shape[0] != P:
+        raise ValueError(f"image has {x.shape[0]} pixels, matrix {P} columns")
+    if inv_perm is not None and inv_perm.shape[0] < wl:
+        raise ValueError("RotMatInv column shorter than the weight vector")
+    if y is not None:
+        need = mat.shape[0] if y_by_row else n_rows
+        if y.shape[0] < need:
+            raise ValueError(f"projection data has {y.shape[0]} bins for {need} rows")
+    if coef is not None and coef.shape[0] < n_rows:
+        raise ValueError("coefficients shorter than the rows")
+    if fwd_out is not None and fwd_out.shape[0] < n_rows:
+        raise ValueError("forward output shorter than the rows")
+    for t in (x, x_perm, y, coef, fwd_out, inv_perm, w, row_scale, rows):
+        if t is not None and t.device != dev:
+            raise ValueError("all operands must be on the matrix's device")
     ws = _workspace(dev, _lib.lib().jscc_proj_workspace_bytes(max(n_rows, 1), P))
     xl = x.shape[0] if x is not None else 0
     rc = _lib.lib().jscc_proj(
@@ -412,6 +442,11 @@ def em_update(x: torch.Tensor, w1: torch.Tensor, s: torch.Tensor, mode: int = EM
               w2: torch.Tensor | None = None, c2: float = 0.0, eps: float = 1e-12,
               zero_after: torch.Tensor | None = None) -> None:
     """In-place x <- update(x, c1*w1 + c2*w2, s)."""
+    n = x.numel()
+    for t in (w1, s, w2, zero_after):
+        if t is not None and (t.numel() < n or t.device != x.device or t.dtype != torch.float32
+                              or not t.is_contiguous()):
+            raise ValueError("EM update operands must be contiguous float32 of the image's size and device")
     rc = _lib.lib().jscc_em_update(x.data_ptr(), w1.data_ptr(), float(c1), _p(w2), float(c2), s.data_ptr(),
                                    x.numel(), int(mode), float(eps), _p(zero_after), stream_of(x.device))
     _lib.check(rc, "jscc_em_update")
@@ -426,6 +461,8 @@ def sensitivity(A: torch.Tensor, inv: torch.Tensor | None) -> torch.Tensor:
     M, P = A.shape
     R = inv.shape[1] if inv is not None else 1
     if inv is not None:
+        if inv.ndim != 2 or inv.shape[0] != P:
+            raise ValueError(f"RotMatInv must be [{P}, R]")
         inv = inv.to(device=dev, dtype=torch.int64)
     out = torch.empty((P, 1), dtype=torch.float32, device=dev)
     ws = torch.empty(_lib.lib().jscc_sensitivity_workspace_bytes(M, P), dtype=torch.uint8, device=dev)
@@ -434,6 +471,55 @@ def sensitivity(A: torch.Tensor, inv: torch.Tensor | None) -> torch.Tensor:
                                      R, out.data_ptr(), ws.data_ptr(), ws.numel(), err_word(dev).data_ptr(),
                                      stream_of(dev))
     _lib.check(rc, "jscc_sensitivity")
+    return out
+
+
+CP_RATIO, CP_FWD, CP_COEF = 0, 1, 2
+
+
+def cone_proj(mode: int, events: torch.Tensor, ev_idx: torch.Tensor, n_rows: int, vox: VoxelSoA, x: torch.Tensor,
+              *, sysmat=None, sigma_model: int = SIG_FIXED, fixed_sigma: float = 0.0, coef=None, eps=None,
+              normalize: bool = True, fwd_out=None, dsig_out=None, out_scale: float = 1.0, accumulate=False,
+              w=None) -> None:
+    """No-store cone projection (rows recomputed in registers), see include/jscc_hip.h."""
+    dev = events.device
+    P = vox.n
+    if P % 4 != 0:
+        raise ValueError("the recompute kernel needs a multiple of 4 voxels")
+    if x.numel() != P or x.dtype != torch.float32 or not x.is_contiguous() or x.device != dev:
+        raise ValueError("image must be a contiguous float32 vector with one value per voxel")
+    if ev_idx.dtype != torch.int32 or ev_idx.numel() < n_rows or not ev_idx.is_contiguous():
+        raise ValueError("event index list must be contiguous int32 with n_rows entries")
+    for t, need in ((coef, n_rows), (fwd_out, n_rows), (dsig_out, n_rows), (w, P)):
+        if t is not None and (t.numel() < need or t.dtype != torch.float32 or t.device != dev):
+            raise ValueError("cone_proj operand has the wrong size, dtype or device")
+    A = None
+    if sysmat is not None:
+        A = aligned_rows(sysmat, 4, dev)
+        if A.shape[1] != P:
+            raise ValueError("sysmat columns do not match the voxels")
+    ws = None
+    nbytes = 0
+    if mode != CP_FWD:
+        nbytes = _lib.lib().jscc_cone_proj_workspace_bytes(max(n_rows, 1), P)
+        ws = _workspace(dev, nbytes)
+        nbytes = ws.numel()
+    rc = _lib.lib().jscc_cone_proj(
+        mode, events.data_ptr(), ev_idx.data_ptr(), n_rows, vox.x.data_ptr(), vox.y.data_ptr(), vox.z.data_ptr(),
+        P, _p(A), A.stride(0) if A is not None else 0, int(sigma_model), float(fixed_sigma), x.data_ptr(), _p(coef),
+        float(eps) if eps is not None else 0.0, 1 if eps is not None else 0, 1 if normalize else 0, _p(fwd_out),
+        _p(dsig_out), float(out_scale), 1 if accumulate else 0, _p(w), w.numel() if w is not None else 0,
+        _p(ws), nbytes, err_word(dev).data_ptr(), stream_of(dev))
+    _lib.check(rc, "jscc_cone_proj")
+
+
+def sum_f64(v: torch.Tensor) -> torch.Tensor:
+    """FP64 sum of a float32 vector as a device scalar (deterministic order)."""
+    if v.dtype != torch.float32 or v.ndim != 1:
+        raise ValueError("sum_f64 takes a 1-D float32 tensor")
+    out = torch.empty((), dtype=torch.float64, device=v.device)
+    rc = _lib.lib().jscc_sum_f64(v.data_ptr(), v.numel(), v.stride(0), out.data_ptr(), stream_of(v.device))
+    _lib.check(rc, "jscc_sum_f64")
     return out
 
 

@@ -126,29 +126,39 @@ def prepare_list(list_origin: torch.Tensor, dev: torch.device):
 def build_compton_rows(sysmat, detector, coor, list_origin, delta_r1, delta_r2, e0, ene_resolution,
                        ene_threshold_max, ene_threshold_min, ene_threshold_sum, device, *, noise="cpu",
                        sigma_model="strict", fixed_sigma=None, store_dtype=torch.float32, want_tc=False,
-                       taps=None, sysmat_dev=None) -> ComptonRows:
+                       taps=None, sysmat_dev=None, coor_dev=None, mutate_list=True) -> ComptonRows:
     """Cone rows of a list chunk on the GPU (see ComptonRows).
 
     noise: "cpu" (two torch.randn on the CPU generator, as the strict
     reference), "device" (as the sparse reference), "none", or a pair of
     N(0,1) tensors.  sigma_model "strict" or "fixed" (fixed_sigma in rad).
-    sysmat may be None (first-hit factor 1).
+    sysmat may be None (first-hit factor 1).  mutate_list: write the smeared
+    energies back into list_origin (the strict reference does, the sparse one
+    does not).
     """
     dev = _device_of(device)
-    lst, copy_back = prepare_list(list_origin, dev)
+    if mutate_list:
+        lst, copy_back = prepare_list(list_origin, dev)
+    else:
+        if list_origin.ndim != 2 or list_origin.shape[1] < 4:
+            raise ValueError("list_origin must have shape [num_events, >=4]")
+        lst, copy_back = list_origin.to(device=dev, dtype=torch.float32).clone(memory_format=torch.contiguous_format), None
     n = lst.shape[0]
+    if sysmat_dev is None and sysmat is not None:
+        sysmat_dev = ops.aligned_rows(sysmat, device=dev) if taps is None else ops.aligned_rows(sysmat, 1, dev)
+    vox = coor_dev if coor_dev is not None else ops.voxel_soa(coor, dev)
+    if sysmat_dev is not None and taps is None and sysmat_dev.shape[1] != vox.n:
+        raise ValueError(f"sysmat has {sysmat_dev.shape[1]} columns for {vox.n} voxels")
     n1, n2 = _noise(noise, n, dev)
     det = detector.to(dev)
     events, flags = ops.events_prepare(lst, n1, n2, n1 is not None, det, delta_r1, delta_r2, e0, ene_resolution,
-                                       ene_threshold_max, ene_threshold_min, ene_threshold_sum)
+                                       ene_threshold_max, ene_threshold_min, ene_threshold_sum,
+                                       n_sysmat_rows=sysmat_dev.shape[0] if sysmat_dev is not None else 0)
     if copy_back is not None and n1 is not None:
         copy_back()
     sel_idx, sel_count = ops.compact(flags)
     n_sel = int(sel_count.item())
     ops.raise_on_index_error(dev)
-    vox = ops.voxel_soa(coor, dev)
-    if sysmat_dev is None and sysmat is not None:
-        sysmat_dev = ops.aligned_rows(sysmat, device=dev) if taps is None else ops.aligned_rows(sysmat, 1, dev)
     sig = ops.SIG_STRICT if sigma_model == "strict" else ops.SIG_FIXED
     if sig == ops.SIG_FIXED and not fixed_sigma:
         raise ValueError("sigma_model='fixed' needs fixed_sigma (rad)")

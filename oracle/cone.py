@@ -159,10 +159,21 @@ def sigma_pos_sq(pos1, pos2, coor, wsum):
     return float(np.sum(g * g * wsum))
 
 
-def row_filters(t):
-    """NaN / zero-sum / effective-support decisions of :241-248, :137-148 (FP64).
+def as_fp32_rows(t):
+    """The rows as the reference holds them: FP32 values (tails below the FP32
+    subnormal range are exactly zero), carried on in FP64."""
+    with np.errstate(over="ignore", under="ignore"):
+        return np.asarray(t, np.float64).astype(np.float32).astype(np.float64)
 
-    Returns (keep [N] bool, S [N], support [N])."""
+
+def row_filters(t):
+    """NaN / zero-sum / effective-support decisions of :241-248, :137-148.
+
+    The reference evaluates the rows in FP32, so a row whose every weight lies
+    below the FP32 range sums to zero and is dropped; the decisions are taken on
+    the FP32-rounded rows (sums in FP64).  Returns (keep [N] bool, S [N],
+    support [N])."""
+    t = as_fp32_rows(t)
     with np.errstate(invalid="ignore", divide="ignore"):
         S = t.sum(axis=1)
         nan = np.isnan(t).any(axis=1)
@@ -186,9 +197,9 @@ def backproj_list_single(sysmat, detector, coor, list_origin, delta_r1, delta_r2
     th = theta_from_e1(ev["e1"], e0)
     kn = klein_nishina(ev["e1"], e0)
     sp, sm = angle_sigma_ene_pm(ev["e1"], e0, ene_resolution)
-    tc = cone_tc(ev["pos1"], ev["pos2"], s1, s2, th, kn, sp, sm, coor, fixed_sigma)
+    tc = as_fp32_rows(cone_tc(ev["pos1"], ev["pos2"], s1, s2, th, kn, sp, sm, coor, fixed_sigma))
     a = np.ones_like(tc) if sysmat is None else np.asarray(sysmat, np.float64)[ev["arow"]]
-    t = tc * a
+    t = as_fp32_rows(tc * a)
     keep, S, support = row_filters(t)
     kk = np.nonzero(keep)[0]
     with np.errstate(invalid="ignore", divide="ignore"):
