@@ -60,12 +60,14 @@ __device__ __forceinline__ float jscc_bf16_to_f32(uint16_t h) {
   return __uint_as_float(((uint32_t)h) << 16);
 }
 
-// round-to-nearest-even FP32 -> BF16 (NaN kept quiet)
-__device__ __forceinline__ uint16_t jscc_f32_to_bf16(float f) {
-  uint32_t u = __float_as_uint(f);
-  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x0040u);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
+// two FP32 -> packed BF16x2 with the gfx950 converter (round-to-nearest-even)
+typedef float jscc_f2_t __attribute__((ext_vector_type(2)));
+typedef __bf16 jscc_bf2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t jscc_pack_bf16x2(float lo, float hi) {
+  jscc_f2_t v;
+  v.x = lo;
+  v.y = hi;
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, jscc_bf2_t));
 }
 
 // e4m3fn -> FP32.  The 7 magnitude bits are placed at the top of the FP32

@@ -58,9 +58,21 @@ __device__ __forceinline__ float atan2_ypos(float y, float x) {
 struct ConeConsts {
   float beta_min, beta_max;  // acos of the reference's FP32 clamp bounds
   float s2_min;              // 1 - cos_max^2 : sin^2 floor implied by the clamp
-  float fixed_half_inv_sig2; // 0.5 / sigma^2 for the fixed-sigma model
+  float fixed_hl2;           // log2(e) / (2 sigma^2) for the fixed-sigma model
   float inv_sig3;            // 1 / sigma^3 (d log t / d sigma = delta^2 / sigma^3)
 };
+
+ConeConsts make_consts(int sigma_model, double fixed_sigma) {
+  ConeConsts cc;
+  const double cmax = (double)(float)(1.0 - 1e-7);  // the reference's FP32 clamp of cos(beta)
+  cc.beta_min = (float)acos(cmax);
+  cc.beta_max = (float)(M_PI - acos(cmax));
+  cc.s2_min = (float)(1.0 - cmax * cmax);
+  const bool fixed = sigma_model == SIG_FIXED && fixed_sigma > 0.0;
+  cc.fixed_hl2 = fixed ? (float)(0.5 * 1.4426950408889634 / (fixed_sigma * fixed_sigma)) : 0.0f;
+  cc.inv_sig3 = fixed ? (float)(1.0 / (fixed_sigma * fixed_sigma * fixed_sigma)) : 0.0f;
+  return cc;
+}
 
 // wave-uniform parameters of one event (scalar registers)
 struct ConeEv {
@@ -95,13 +107,14 @@ __device__ __forceinline__ float cone_tc(const ConeEv& e, float cxv, float cyv, 
   const float my = fmaf(vz, e.ax, -vx * e.az);
   const float mz = fmaf(vx, e.ay, -vy * e.ax);
   const float msq = fmaf(mx, mx, fmaf(my, my, mz * mz));
-  float beta = atan2_ypos(__builtin_sqrtf(msq), dot);
+  // hardware sqrt / exp2 (about 1 ulp): no range-reduction branches in the loop
+  float beta = atan2_ypos(__builtin_amdgcn_sqrtf(msq), dot);
   beta = fminf(fmaxf(beta, cc.beta_min), cc.beta_max);
   const float inv_d01sq = __builtin_amdgcn_rcpf(d01sq);
-  float s2 = msq * inv_d01sq * e.inv_d12sq;
-  s2 = fminf(fmaxf(s2, cc.s2_min), 1.0f);
+  // sin^2(beta) of the clamped beta: never below the clamp's floor
+  const float s2 = fmaxf(msq * inv_d01sq * e.inv_d12sq, cc.s2_min);
   const float delta = beta - e.th;
-  float hinv;
+  float hl2;  // log2(e) / (2 sigma^2)
   if (SIG == SIG_STRICT) {
     const float sige = delta >= 0.0f ? e.sp : e.sm;
     const float qx = fmaf(e.d12sq, vx, -dot * e.ax);
@@ -111,13 +124,13 @@ __device__ __forceinline__ float cone_tc(const ConeEv& e, float cxv, float cyv, 
     const float den = e.d12sq * e.d12sq * fmaxf(msq, 1e-12f * d01sq * e.d12sq);
     const float spos2 = fmaxf(num * __builtin_amdgcn_rcpf(den), 1e-12f);
     const float sig2 = fmaxf(fmaf(sige, sige, spos2), 1e-12f);
-    hinv = 0.5f * __builtin_amdgcn_rcpf(sig2);
+    hl2 = 0.72134752044448170368f * __builtin_amdgcn_rcpf(sig2);
   } else {
-    hinv = cc.fixed_half_inv_sig2;
+    hl2 = cc.fixed_hl2;
   }
   const float d2 = delta * delta;
   if (DSIG) *dlog = d2 * cc.inv_sig3;
-  return __expf(-d2 * hinv) * (e.kn - s2);
+  return __builtin_amdgcn_exp2f(-d2 * hl2) * (e.kn - s2);
 }
 
 template <int AF>
@@ -167,10 +180,11 @@ __global__ void __launch_bounds__(kConeThreads) k_cone_rows(
     const int r = valid[e] ? (row0 + e) : (n_rows - 1);
     evs[e] = load_event(ev[ev_idx[r]], lda);
   }
-  double accS[E], accQ[E], accC[E];
-  float accM[E];
+  // per-thread partial statistics in FP32 (at most P/1024 tiles x 4 values each),
+  // combined across threads in FP64
+  float accS[E], accQ[E], accC[E], accM[E];
 #pragma unroll
-  for (int e = 0; e < E; ++e) { accS[e] = 0.0; accQ[e] = 0.0; accC[e] = 0.0; accM[e] = 0.0f; }
+  for (int e = 0; e < E; ++e) { accS[e] = 0.0f; accQ[e] = 0.0f; accC[e] = 0.0f; accM[e] = 0.0f; }
 
   for (int base = threadIdx.x * kConeVec; base < P; base += kConeThreads * kConeVec) {
     const float4 vx4 = *reinterpret_cast<const float4*>(cx + base);
@@ -189,10 +203,13 @@ __global__ void __launch_bounds__(kConeThreads) k_cone_rows(
       float tv[4], tcv[4];
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        const float tc = cone_tc<SIG, false>(evs[e], vcx[k], vcy[k], vcz[k], cc, nullptr);
-        const bool in = (base + k) < P;
-        tcv[k] = in ? tc : 0.0f;
-        tv[k] = in ? tc * av[k] : 0.0f;
+        tcv[k] = cone_tc<SIG, false>(evs[e], vcx[k], vcy[k], vcz[k], cc, nullptr);
+        tv[k] = tcv[k] * av[k];
+      }
+      if (base + 4 > P) {  // the padded tail of the last tile (P not a multiple of 4)
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          if (base + k >= P) { tcv[k] = 0.0f; tv[k] = 0.0f; }
       }
       if (valid[e]) {
         const int64_t r = (int64_t)(row0 + e);
@@ -201,8 +218,8 @@ __global__ void __launch_bounds__(kConeThreads) k_cone_rows(
               make_float4(tv[0], tv[1], tv[2], tv[3]);
         } else if (OUT_DT == JSCC_DT_BF16) {
           uint2 pk;
-          pk.x = (uint32_t)jscc_f32_to_bf16(tv[0]) | ((uint32_t)jscc_f32_to_bf16(tv[1]) << 16);
-          pk.y = (uint32_t)jscc_f32_to_bf16(tv[2]) | ((uint32_t)jscc_f32_to_bf16(tv[3]) << 16);
+          pk.x = jscc_pack_bf16x2(tv[0], tv[1]);
+          pk.y = jscc_pack_bf16x2(tv[2], tv[3]);
           *reinterpret_cast<uint2*>(reinterpret_cast<uint16_t*>(out_t) + r * ldt + base) = pk;
         }
         if (WANT_TC) {
@@ -211,9 +228,9 @@ __global__ void __launch_bounds__(kConeThreads) k_cone_rows(
       }
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        accS[e] += (double)tv[k];
-        accQ[e] += (double)tv[k] * (double)tv[k];
-        accC[e] += (double)tcv[k];
+        accS[e] += tv[k];
+        accQ[e] = fmaf(tv[k], tv[k], accQ[e]);
+        accC[e] += tcv[k];
         accM[e] = fmaxf(accM[e], tv[k]);  // fmaxf drops NaN; NaN rows are caught by S
       }
     }
@@ -223,9 +240,9 @@ __global__ void __launch_bounds__(kConeThreads) k_cone_rows(
   __shared__ double red[4][kConeThreads / JSCC_WAVE][E];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
-    double s = jscc_wave_sum(accS[e]);
-    double q = jscc_wave_sum(accQ[e]);
-    double c = jscc_wave_sum(accC[e]);
+    double s = jscc_wave_sum((double)accS[e]);
+    double q = jscc_wave_sum((double)accQ[e]);
+    double c = jscc_wave_sum((double)accC[e]);
     float m = jscc_wave_max(accM[e]);
     if (jscc_lane() == 0) {
       red[0][jscc_wave_id()][e] = s;
@@ -311,13 +328,7 @@ extern "C" int jscc_cone_rows(const void* events, const int32_t* ev_idx, int64_t
   }
   if (sigma_model == SIG_FIXED) JSCC_REQUIRE(fixed_sigma > 0.0, "jscc_cone_rows: fixed sigma must be positive");
 
-  ConeConsts cc;
-  const double cmax = (double)(float)(1.0 - 1e-7);
-  cc.beta_min = (float)acos(cmax);
-  cc.beta_max = (float)(M_PI - acos(cmax));
-  cc.s2_min = (float)(1.0 - cmax * cmax);
-  cc.fixed_half_inv_sig2 = sigma_model == SIG_FIXED ? (float)(0.5 / (fixed_sigma * fixed_sigma)) : 0.0f;
-  cc.inv_sig3 = sigma_model == SIG_FIXED ? (float)(1.0 / (fixed_sigma * fixed_sigma * fixed_sigma)) : 0.0f;
+  const ConeConsts cc = make_consts(sigma_model, fixed_sigma);
 
   hipStream_t s = (hipStream_t)stream;
   const JsccEvent* ev = (const JsccEvent*)events;
@@ -557,12 +568,7 @@ extern "C" int jscc_cone_proj(int mode, const void* events, const int32_t* ev_id
   ConeProjArgs a;
   a.ev = (const JsccEvent*)events; a.ev_idx = ev_idx; a.n_rows = n_rows;
   a.cx = coor_x; a.cy = coor_y; a.cz = coor_z; a.P = (int)n_vox; a.A = sysmat; a.lda = ld_sysmat;
-  const double cmax = (double)(float)(1.0 - 1e-7);
-  a.cc.beta_min = (float)acos(cmax);
-  a.cc.beta_max = (float)(M_PI - acos(cmax));
-  a.cc.s2_min = (float)(1.0 - cmax * cmax);
-  a.cc.fixed_half_inv_sig2 = sigma_model == SIG_FIXED ? (float)(0.5 / (fixed_sigma * fixed_sigma)) : 0.0f;
-  a.cc.inv_sig3 = sigma_model == SIG_FIXED ? (float)(1.0 / (fixed_sigma * fixed_sigma * fixed_sigma)) : 0.0f;
+  a.cc = make_consts(sigma_model, fixed_sigma);
   a.x = x; a.coef = coef; a.eps = eps; a.use_eps = use_eps; a.normalize = normalize;
   a.fwd_out = fwd_out; a.dsig_out = dsig_out; a.partial = (float*)workspace;
   const bool dsig = dsig_out != nullptr;

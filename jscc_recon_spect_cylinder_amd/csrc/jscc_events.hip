@@ -21,6 +21,11 @@ constexpr double kElectronRestMeV = 0.511;  // process_list_plane_strict.py:7
 // (-1 + 1e-7, 1 - 1e-7), i.e. to the FP32 roundings of those bounds.
 __device__ __forceinline__ double cos_clamp_f32bound() { return (double)(float)(1.0 - 1e-7); }
 
+// Correctly rounded FP32 square root: the FP64 root rounded once to FP32 (53 >=
+// 2*24 + 2 bits, so the double rounding cannot differ from a direct one).  The
+// device's FP32 sqrt (sqrtf / __fsqrt_rn) is not correctly rounded on gfx950.
+__device__ __forceinline__ float sqrt_rn_f32(float x) { return (float)__dsqrt_rn((double)x); }
+
 __device__ __forceinline__ double theta_from_e1(double e1, double e0) {
   // process_list_plane_strict.py:24-27
   double c = 1.0 - (kElectronRestMeV * e1) / ((e0 - e1) * e0);
@@ -70,9 +75,9 @@ __global__ void __launch_bounds__(256) k_events_prepare(
   // torch evaluates E0/e_k as reciprocal(e_k)*E0 and **0.5 as sqrt.
   if (noise1 != nullptr) {
     float s1 = __fmul_rn(__fdiv_rn(__fmul_rn(e1, resf), 2.355f),
-                         __fsqrt_rn(__fmul_rn(__fdiv_rn(1.0f, e1), e0f)));
+                         sqrt_rn_f32(__fmul_rn(__fdiv_rn(1.0f, e1), e0f)));
     float s2 = __fmul_rn(__fdiv_rn(__fmul_rn(e2, resf), 2.355f),
-                         __fsqrt_rn(__fmul_rn(__fdiv_rn(1.0f, e2), e0f)));
+                         sqrt_rn_f32(__fmul_rn(__fdiv_rn(1.0f, e2), e0f)));
     e1 = __fadd_rn(e1, __fmul_rn(s1, noise1[i]));
     e2 = __fadd_rn(e2, __fmul_rn(s2, noise2[i]));
     if (writeback != nullptr) {  // the reference mutates list columns 1 and 3 in place
@@ -227,7 +232,10 @@ __global__ void __launch_bounds__(256) k_rows_keep(const double* __restrict__ st
   if (i >= n) return;
   const double S = stats[i * 4 + 0];
   const double Q = stats[i * 4 + 1];
-  bool keep = (S == S) && (S != 0.0);
+  const double M = stats[i * 4 + 3];
+  // a row lying entirely below the smallest normal FP32 value is a zero row:
+  // its subnormal tail depends on the exponential's subnormal handling
+  bool keep = (S == S) && (S != 0.0) && (M >= 1.17549435082228750797e-38);
   if (keep) {
     const double support = (S * S) / Q;  // 1 / sum((t/S)^2)
     keep = support >= min_support;       // NaN compares false

@@ -78,3 +78,67 @@ class ListModeStore:
                 self.subset_weight(x, w, lo, hi, eps=eps)
                 ops.em_update(x, w, s, mode=em_mode, zero_after=w)
         return x
+
+
+@dataclass
+class LazyListStore:
+    """List-mode data kept as per-event cone parameters (64 B per event) instead of
+    T: every projection re-evaluates the rows in registers (jscc_cone_proj).
+
+    The natural home of lists whose T exceeds HBM and of the lazy per-iteration
+    recompute of distributed/python/recon_osem_dist_sparse_jsccsd_only_lazy.py
+    (46-125); the kept set and the row normalisation are those of the stored form.
+    """
+    events: torch.Tensor      # [n_in, 16] event records
+    ev_kept: torch.Tensor     # int32 record indices of the kept events, in order
+    n_kept: int
+    vox: ops.VoxelSoA
+    sysmat: torch.Tensor | None
+    sigma_model: int
+    fixed_sigma: float
+
+    @classmethod
+    def from_rows(cls, rows: ComptonRows, sysmat=None, sigma_model=ops.SIG_FIXED, fixed_sigma=0.0):
+        """Keep the event records of built (or statistics-only) rows; T itself may be dropped."""
+        kept_rows = rows.kept[: rows.n_kept].long()
+        ev_kept = rows.sel_idx[kept_rows].to(torch.int32).contiguous()
+        return cls(rows.events, ev_kept, rows.n_kept, rows.vox, sysmat, sigma_model, fixed_sigma)
+
+    def _call(self, mode, x, lo=0, hi=None, **kw):
+        hi = self.n_kept if hi is None else hi
+        ops.cone_proj(mode, self.events, self.ev_kept[lo:hi], hi - lo, self.vox, x, sysmat=self.sysmat,
+                      sigma_model=self.sigma_model, fixed_sigma=self.fixed_sigma, **kw)
+
+    def subset_bounds(self, k: int):
+        return [(self.n_kept * i // k, self.n_kept * (i + 1) // k) for i in range(k)]
+
+    def forward(self, x: torch.Tensor, dsig: bool = False):
+        """(T x, d(T x)/d sigma or None) for the kept events."""
+        f = torch.empty(max(self.n_kept, 1), dtype=torch.float32, device=x.device)
+        g = torch.empty_like(f) if dsig else None
+        self._call(ops.CP_FWD, x, fwd_out=f, dsig_out=g)
+        return f[: self.n_kept], (g[: self.n_kept] if dsig else None)
+
+    def adjoint(self, x: torch.Tensor, coef: torch.Tensor, dsig: bool = False):
+        """(T^T coef, d(T x)/d sigma or None) in one recompute pass."""
+        w = torch.zeros(self.vox.n, dtype=torch.float32, device=x.device)
+        g = torch.empty(max(self.n_kept, 1), dtype=torch.float32, device=x.device) if dsig else None
+        self._call(ops.CP_COEF, x, coef=coef.contiguous(), dsig_out=g, w=w)
+        return w, (g[: self.n_kept] if dsig else None)
+
+    def backproject_ones(self, like: torch.Tensor) -> torch.Tensor:
+        ones = torch.ones(max(self.n_kept, 1), dtype=torch.float32, device=like.device)
+        return self.adjoint(like, ones)[0]
+
+    def subset_weight(self, x, w, lo, hi, eps=None, scale=1.0):
+        if hi > lo:
+            self._call(ops.CP_RATIO, x, lo, hi, eps=eps, out_scale=scale, accumulate=True, w=w)
+
+    def osem(self, x, s, n_iter, n_subsets, eps=None, em_mode: int = ops.EM_PLAIN):
+        w = torch.zeros_like(x)
+        bounds = self.subset_bounds(n_subsets)
+        for _ in range(n_iter):
+            for lo, hi in bounds:
+                self.subset_weight(x, w, lo, hi, eps=eps)
+                ops.em_update(x, w, s, mode=em_mode, zero_after=w)
+        return x

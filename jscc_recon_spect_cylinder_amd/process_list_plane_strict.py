@@ -71,6 +71,7 @@ class ComptonRows:
     n_kept: int
     inv_sum: torch.Tensor
     device: torch.device
+    vox: ops.VoxelSoA | None = None
 
     def kept_arows(self) -> torch.Tensor:
         ev_i32 = self.events.view(torch.int32)
@@ -168,7 +169,7 @@ def build_compton_rows(sysmat, detector, coor, list_origin, delta_r1, delta_r2, 
     keep, inv_sum = ops.rows_keep(stats, n_sel, MIN_EVENT_EFFECTIVE_SUPPORT, want_inv_sum=True)
     kept, kept_count = ops.compact(keep)
     n_kept = int(kept_count.item())
-    return ComptonRows(store, tc_store, ld, vox.n, stats, events, sel_idx, n_sel, kept, n_kept, inv_sum, dev)
+    return ComptonRows(store, tc_store, ld, vox.n, stats, events, sel_idx, n_sel, kept, n_kept, inv_sum, dev, vox)
 
 
 def get_compton_backproj_list_single(sysmat, detector, coor, list_origin, delta_r1, delta_r2, e0, ene_resolution,
