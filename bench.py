@@ -37,7 +37,22 @@ def parse():
     ap.add_argument("--subsets", type=int, default=8)
     ap.add_argument("--cpu-sample", type=int, default=600, help="events in the CPU-baseline sample (0: skip)")
     ap.add_argument("--skip-fp32", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB three times (HBM counter calibration) and exit")
     return ap.parse_args()
+
+
+def copy_calibration(mib):
+    """A plain copy kernel of known size: under rocprofv3 --pmc its FETCH_SIZE / WRITE_SIZE per dispatch
+    calibrate the counters against the bytes actually moved (mib MiB read + mib MiB written)."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": n * 4, "copy_calibration_bytes_written": n * 4}))
 
 
 def cpu_baseline(cl, n_sample, n_iter, n_sub, sigma):
@@ -96,6 +111,9 @@ def main():
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
               "(the list shards by events with one all-reduce of the image weight per subset).", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration > 0:
+        copy_calibration(args.copy_calibration)
+        return
     import torch
     from jscc_recon_spect_cylinder_amd import _lib, ops, synth
     from jscc_recon_spect_cylinder_amd.engine import ListModeStore

@@ -161,6 +161,20 @@ int jscc_proj(int mode, const void* mat, int dtype, int64_t ld, int64_t n_cols, 
               int flags, float* w, int64_t w_len, void* workspace, int64_t workspace_bytes, int32_t* err,
               void* stream);
 
+/* One OSEM subset step in one call: the mode-0 back-projection of jscc_proj
+ * into w, then, in the same reduction pass, em_x[j] = update(em_x[j], w[j],
+ * em_s[j]) with the modes of jscc_em_update (c1 = 1, no second weight) and w
+ * cleared afterwards when zero_w != 0.  Replaces the pair
+ * get_weight_compton / get_weight_single + "img = img * weight / s_map" of
+ * osem_bin_mode / osem_list_mode (recon_osem_plane.py:45-75) for one subset;
+ * em_x may be the projected image x. */
+int jscc_proj_em(const void* mat, int dtype, int64_t ld, int64_t n_cols, const int32_t* rows, int64_t n_rows,
+                 int64_t row_base, const float* row_scale, const float* x, const int64_t* x_perm,
+                 int64_t x_perm_stride, int64_t x_len, const float* y, int64_t y_stride, float eps, int use_eps,
+                 const int64_t* inv_perm, int64_t inv_perm_stride, float out_scale, int accumulate, int flags,
+                 float* w, int64_t w_len, float* em_x, const float* em_s, int em_mode, float em_eps, int zero_w,
+                 void* workspace, int64_t workspace_bytes, int32_t* err, void* stream);
+
 /* EM update with w = c1*w1 (+ c2*w2):
  *   mode 0: x = x * w / s                        (recon_osem_plane.py:54, 73, 117-118)
  *   mode 1: safe_em_update                        (recon_osem_plane_sparse.py:60-66)

@@ -132,17 +132,48 @@ __device__ __forceinline__ T jscc_wave_max(T v) {
   return v;
 }
 
+// Wave sum with DPP lane moves folded into the adds (quad swaps, half-row and
+// row mirrors, then the gfx9 row broadcasts); the total is read from lane 63
+// and is uniform across the wave.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float jscc_dpp_add(float v) {
+  const int moved = __builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xf, false);
+  return v + __int_as_float(moved);
+}
+
+__device__ __forceinline__ float jscc_wave_sum_dpp(float v) {
+  v = jscc_dpp_add<0xB1, 0xf>(v);   // quad_perm [1,0,3,2]
+  v = jscc_dpp_add<0x4E, 0xf>(v);   // quad_perm [2,3,0,1]
+  v = jscc_dpp_add<0x141, 0xf>(v);  // row_half_mirror: sums of 8
+  v = jscc_dpp_add<0x140, 0xf>(v);  // row_mirror: sums of 16
+  v = jscc_dpp_add<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+  v = jscc_dpp_add<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
 __device__ __forceinline__ int jscc_lane() { return threadIdx.x & (JSCC_WAVE - 1); }
 __device__ __forceinline__ int jscc_wave_id() { return threadIdx.x >> 6; }
 
 // number of compute units of the current device (cached per process)
 int jscc_num_cus();
 
-// Deterministic two-level reduction of per-work-group partial back-projections
-// [G, P] into w (inverse-rotation gather, scale, accumulate, optional
-// nan_to_num); the buffer must hold jscc_partials_bytes(G, P).
+// Optional EM update applied where a subset weight is finished:
+// x[j] <- em(x[j], w[j], s[j]) (modes of jscc_em_update); w[j] is zeroed after
+// when zero_w is set.  x == nullptr: no update.
+struct JsccEm {
+  float* x;
+  const float* s;
+  int mode;
+  float eps;
+  int zero_w;
+};
+
+// Deterministic reduction of per-work-group partial back-projections [G, P]
+// into w (inverse-rotation gather, scale, accumulate, optional nan_to_num,
+// optional EM update); the buffer must hold jscc_partials_bytes(G, P).
 int jscc_reduce_partials(float* part, int G, int P, const int64_t* inv_perm, int64_t ips, float scale,
-                         int accumulate, int sanitize, float* w, int64_t w_len, int32_t* err, hipStream_t s);
+                         int accumulate, int sanitize, float* w, int64_t w_len, int32_t* err, hipStream_t s,
+                         const JsccEm& em);
 int64_t jscc_partials_bytes(int G, int64_t P);
 
 // Built-in kernel timer: when enabled, hot kernels are bracketed by HIP event

@@ -180,11 +180,14 @@ __global__ void __launch_bounds__(kConeThreads) k_cone_rows(
     const int r = valid[e] ? (row0 + e) : (n_rows - 1);
     evs[e] = load_event(ev[ev_idx[r]], lda);
   }
-  // per-thread partial statistics in FP32 (at most P/1024 tiles x 4 values each),
-  // combined across threads in FP64
-  float accS[E], accQ[E], accC[E], accM[E];
+  // per-thread partial sums in FP32 (at most P/1024 tiles x 4 values each),
+  // combined across threads in FP64.  The sum of squares is kept in FP64: a row
+  // of tiny weights (1e-25 and below) squares under the FP32 range, and its
+  // effective support S^2/Q must match the reference's scale-free 1/sum((t/S)^2).
+  float accS[E], accC[E], accM[E];
+  double accQ[E];
 #pragma unroll
-  for (int e = 0; e < E; ++e) { accS[e] = 0.0f; accQ[e] = 0.0f; accC[e] = 0.0f; accM[e] = 0.0f; }
+  for (int e = 0; e < E; ++e) { accS[e] = 0.0f; accQ[e] = 0.0; accC[e] = 0.0f; accM[e] = 0.0f; }
 
   for (int base = threadIdx.x * kConeVec; base < P; base += kConeThreads * kConeVec) {
     const float4 vx4 = *reinterpret_cast<const float4*>(cx + base);
@@ -229,7 +232,7 @@ __global__ void __launch_bounds__(kConeThreads) k_cone_rows(
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         accS[e] += tv[k];
-        accQ[e] = fmaf(tv[k], tv[k], accQ[e]);
+        accQ[e] = fma((double)tv[k], (double)tv[k], accQ[e]);
         accC[e] += tcv[k];
         accM[e] = fmaxf(accM[e], tv[k]);  // fmaxf drops NaN; NaN rows are caught by S
       }
@@ -241,7 +244,7 @@ __global__ void __launch_bounds__(kConeThreads) k_cone_rows(
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     double s = jscc_wave_sum((double)accS[e]);
-    double q = jscc_wave_sum((double)accQ[e]);
+    double q = jscc_wave_sum(accQ[e]);
     double c = jscc_wave_sum((double)accC[e]);
     float m = jscc_wave_max(accM[e]);
     if (jscc_lane() == 0) {
@@ -588,5 +591,7 @@ extern "C" int jscc_cone_proj(int mode, const void* events, const int32_t* ev_id
   if (rc != 0) return rc;
   JSCC_CHECK_LAUNCH();
   if (mode == CP_FWD) return 0;
-  return jscc_reduce_partials((float*)workspace, G, (int)n_vox, nullptr, 0, out_scale, accumulate, 0, w, w_len, err, s);
+  const JsccEm none = {nullptr, nullptr, 0, 0.0f, 0};
+  return jscc_reduce_partials((float*)workspace, G, (int)n_vox, nullptr, 0, out_scale, accumulate, 0, w, w_len, err, s,
+                              none);
 }

@@ -20,7 +20,7 @@ namespace {
 
 enum { MODE_RATIO = 0, MODE_FWD = 1, MODE_COEF = 2, MODE_ONES = 3 };
 
-constexpr int kProjThreads = 512;
+constexpr int kProjThreads = 1024;  // one work-group per CU, 4 waves per SIMD
 constexpr int kProjWaves = kProjThreads / JSCC_WAVE;
 
 typedef float jscc_f4v __attribute__((ext_vector_type(4)));
@@ -81,43 +81,215 @@ struct ProjArgs {
   int y_by_row;              // numerator indexed by stored row id instead of call row
 };
 
-template <int DT, int NCH, int R, int MODE>
-__global__ void __launch_bounds__(kProjThreads) k_proj_fused(ProjArgs a) {
-  constexpr int VN = Vec<DT>::N;
+// Row metadata staged in LDS for a chunk of a work-group's rows: stored row
+// id, row scale and numerator (y, or the MODE_COEF coefficient).  Reading them
+// from LDS inside the row loop never waits on the row loads in flight.
+constexpr int kStage = 128;
+
+// gfx9 buffer resource word 3 (raw, untyped) and the nontemporal cache policy
+constexpr int kBufWord3 = 0x00020000;
+constexpr int kBufNT = 2;
+
+struct RowMeta {
+  int64_t sid[kStage];
+  float sc[kStage];
+  float num[kStage];
+};
+
+// R rows held in their storage format (decoded at use)
+template <int DT, int NCH, int R>
+struct RowBuf {
+  typename Vec<DT>::T raw[R][NCH];
+};
+
+// Issue the loads of rows [i0, i0 + R) of the staged chunk starting at c0.
+// Buffer loads: the resource's byte range is the row (or empty for slots at or
+// past c1), so out-of-range lanes and dead slots read zero without a branch or
+// a memory access, and the loads stay unconditional for the wait counters.
+template <int DT, int NCH, int R>
+__device__ __forceinline__ void fetch_rows(const ProjArgs& a, const RowMeta& meta, int64_t i0, int64_t c0,
+                                           int64_t c1, int nvec, int tid, RowBuf<DT, NCH, R>& b) {
   using VT = typename Vec<DT>::T;
+  constexpr int EB = DT == JSCC_DT_F32 ? 4 : (DT == JSCC_DT_BF16 ? 2 : 1);
+  // The load offsets pass through an empty volatile asm, which stays ordered
+  // after the one closing the previous block's accumulation: the next
+  // buffer's loads cannot be hoisted above the current rows' arithmetic (which
+  // would keep three buffers live).
+  int voff[NCH];
+#pragma unroll
+  for (int c = 0; c < NCH; ++c) {
+    voff[c] = (c * kProjThreads + tid) * 16;
+    asm volatile("" : "+v"(voff[c]));
+  }
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const int64_t i = i0 + q;
+    const bool live = i < c1;
+    const int64_t sid = meta.sid[live ? (int)(i - c0) : 0];
+    const char* rowp = reinterpret_cast<const char*>(a.M) + sid * a.ld * EB;
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<char*>(rowp), (short)0, live ? nvec * 16 : 0, kBufWord3);
+#pragma unroll
+    for (int c = 0; c < NCH; ++c)
+      b.raw[q][c] = __builtin_bit_cast(VT, __builtin_amdgcn_raw_buffer_load_b128(rs, voff[c], 0, kBufNT));
+  }
+}
+
+// One block of R rows: dot products with the LDS image, work-group reduction,
+// ratio / coefficient, and accumulation of coef * row into the registers.
+template <int DT, int NCH, int R, int MODE>
+__device__ __forceinline__ void process_rows(const ProjArgs& a, const RowMeta& meta, int64_t i0, int64_t c0,
+                                             int64_t c1, int tid, RowBuf<DT, NCH, R>& b, const jscc_f4v* xs4,
+                                             float (&red)[2][R][kProjWaves], int& buf,
+                                             float (&wr)[NCH][Vec<DT>::N]) {
+  constexpr int VN = Vec<DT>::N;
+  float coefq[R];
+  if (MODE == MODE_RATIO || MODE == MODE_FWD) {
+    float d[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) d[q] = 0.0f;
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+#pragma unroll
+      for (int kk = 0; kk < VN / 4; ++kk) {
+        const jscc_f4v t = xs4[(c * (VN / 4) + kk) * kProjThreads + tid];
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+          float v[VN];
+          decode<DT>(b.raw[q][c], v);
+          d[q] = fmaf(v[4 * kk], t.x, d[q]);
+          d[q] = fmaf(v[4 * kk + 1], t.y, d[q]);
+          d[q] = fmaf(v[4 * kk + 2], t.z, d[q]);
+          d[q] = fmaf(v[4 * kk + 3], t.w, d[q]);
+        }
+      }
+    }
+    // wave sums (DPP, result uniform) -> LDS -> one lane per row finishes the
+    // work-group sum and its coefficient, which is then read back as a scalar
+#pragma unroll
+    for (int q = 0; q < R; ++q) d[q] = jscc_wave_sum_dpp(d[q]);
+    if (jscc_lane() == 0) {
+#pragma unroll
+      for (int q = 0; q < R; ++q) red[buf][q][jscc_wave_id()] = d[q];
+    }
+    __syncthreads();
+  }
+  const int lane = jscc_lane();
+  float cl = 0.0f;
+  if (lane < R) {
+    const int64_t i = i0 + lane;
+    const int m = i < c1 ? (int)(i - c0) : 0;
+    const float sc = meta.sc[m];
+    if (MODE == MODE_RATIO || MODE == MODE_FWD) {
+      const jscc_f4v* rp = reinterpret_cast<const jscc_f4v*>(&red[buf][lane][0]);
+      jscc_f4v acc = rp[0];
+#pragma unroll
+      for (int w4 = 1; w4 < kProjWaves / 4; ++w4) acc += rp[w4];
+      const float f = ((acc.x + acc.y) + (acc.z + acc.w)) * sc;
+      if (MODE == MODE_FWD) {
+        if (jscc_wave_id() == 0 && i < c1) a.fwd_out[i] = f;
+      } else {
+        const float den = a.use_eps ? fmaxf(f, a.eps) : f;
+        cl = (i < c1) ? sc * (meta.num[m] / den) : 0.0f;
+      }
+    } else {  // MODE_COEF, MODE_ONES: num holds the coefficient (1 for ONES)
+      cl = (i < c1) ? sc * meta.num[m] : 0.0f;
+    }
+  }
+  if (MODE == MODE_RATIO || MODE == MODE_FWD) buf ^= 1;
+#pragma unroll
+  for (int q = 0; q < R; ++q) coefq[q] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cl), q));
+  if (MODE != MODE_FWD) {
+    if (DT != JSCC_DT_F32) {
+      // Packed rows are decoded a second time after the reduction instead of
+      // keeping the decoded floats live across it (halves the row registers).
+#pragma unroll
+      for (int q = 0; q < R; ++q)
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) asm volatile("" : "+v"(b.raw[q][c]));
+    }
+#pragma unroll
+    for (int q = 0; q < R; ++q)
+#pragma unroll
+      for (int c = 0; c < NCH; ++c) {
+        float v[VN];
+        decode<DT>(b.raw[q][c], v);
+#pragma unroll
+        for (int k = 0; k < VN; ++k) wr[c][k] = fmaf(coefq[q], v[k], wr[c][k]);
+      }
+    // the accumulation completes here (see fetch_rows)
+#pragma unroll
+    for (int c = 0; c < NCH; ++c)
+#pragma unroll
+      for (int k = 0; k < VN; ++k) asm volatile("" : "+v"(wr[c][k]));
+  }
+}
+
+// The pointers are separate __restrict__ parameters (not struct members) so
+// the compiler knows the inputs are not written by this kernel.
+template <int DT, int NCH, int R, int MODE>
+__global__ void __launch_bounds__(kProjThreads)
+    k_proj_fused(const void* __restrict__ M, const int32_t* __restrict__ rows, const float* __restrict__ row_scale,
+                 const float* __restrict__ x, const int64_t* __restrict__ x_perm, const float* __restrict__ y,
+                 const float* __restrict__ coef, float* __restrict__ fwd_out, float* __restrict__ partial,
+                 int32_t* __restrict__ err, ProjArgs a) {
+  a.M = M; a.rows = rows; a.row_scale = row_scale; a.x = x; a.x_perm = x_perm; a.y = y; a.coef = coef;
+  a.fwd_out = fwd_out; a.partial = partial; a.err = err;
+  constexpr int VN = Vec<DT>::N;
   const int tid = threadIdx.x;
   const int nvec = a.P / VN;
 
   // The image (gathered through the rotation) is kept in LDS as per-thread
   // storage: each thread only reads back the positions it wrote, in the same
-  // chunk layout as the rows, so no barrier is needed and 4*NCH*VN/4 VGPRs
-  // are freed for more resident work-groups.
+  // chunk layout as the rows, so no barrier is needed and the registers go to
+  // rows in flight instead.
   extern __shared__ jscc_f4v xs4[];
   float wr[NCH][VN];
 #pragma unroll
-  for (int c = 0; c < NCH; ++c) {
-    const int vi = c * kProjThreads + tid;
-    float xv[VN];
+  for (int c = 0; c < NCH; ++c)
 #pragma unroll
-    for (int k = 0; k < VN; ++k) {
-      wr[c][k] = 0.0f;
-      xv[k] = 0.0f;
-      if (MODE != MODE_ONES && MODE != MODE_COEF && vi < nvec) {
-        const int64_t p = (int64_t)vi * VN + k;
-        int64_t src = p;
-        if (a.x_perm) {
-          src = a.x_perm[p * a.x_perm_stride] - 1;
-          if (src < 0 || src >= a.x_len) { atomicOr(a.err, 2); src = 0; }
+    for (int k = 0; k < VN; ++k) wr[c][k] = 0.0f;
+  if (MODE == MODE_RATIO || MODE == MODE_FWD) {
+    // All of this thread's image loads are issued before any is used (a
+    // rotation gather first loads its indices, then the pixels).
+    if (x_perm) {
+      int32_t src[NCH][VN];
+      int bad = 0;
+#pragma unroll
+      for (int c = 0; c < NCH; ++c) {
+        const int vi = c * kProjThreads + tid;
+#pragma unroll
+        for (int k = 0; k < VN; ++k) {
+          const int64_t p = (int64_t)(vi < nvec ? vi : 0) * VN + k;
+          const int64_t v = x_perm[p * a.x_perm_stride];
+          const bool ok = v >= 1 && v <= a.x_len;
+          bad |= (ok || vi >= nvec) ? 0 : 1;
+          src[c][k] = ok ? (int32_t)(v - 1) : 0;
         }
-        xv[k] = a.x[src];
       }
-    }
-    if (MODE == MODE_RATIO || MODE == MODE_FWD) {
+      if (bad) atomicOr(err, 2);
 #pragma unroll
-      for (int kk = 0; kk < VN / 4; ++kk) {
-        jscc_f4v t;
-        t.x = xv[4 * kk]; t.y = xv[4 * kk + 1]; t.z = xv[4 * kk + 2]; t.w = xv[4 * kk + 3];
-        xs4[(c * (VN / 4) + kk) * kProjThreads + tid] = t;
+      for (int c = 0; c < NCH; ++c) {
+        const int vi = c * kProjThreads + tid;
+        float xv[VN];
+#pragma unroll
+        for (int k = 0; k < VN; ++k) xv[k] = vi < nvec ? x[src[c][k]] : 0.0f;
+#pragma unroll
+        for (int kk = 0; kk < VN / 4; ++kk) {
+          jscc_f4v t;
+          t.x = xv[4 * kk]; t.y = xv[4 * kk + 1]; t.z = xv[4 * kk + 2]; t.w = xv[4 * kk + 3];
+          xs4[(c * (VN / 4) + kk) * kProjThreads + tid] = t;
+        }
+      }
+    } else {
+      // x is P contiguous floats, 16-byte aligned
+      const jscc_f4v* x4 = reinterpret_cast<const jscc_f4v*>(x);
+#pragma unroll
+      for (int c = 0; c < NCH; ++c) {
+        const int vi = c * kProjThreads + tid;
+#pragma unroll
+        for (int kk = 0; kk < VN / 4; ++kk)
+          xs4[(c * (VN / 4) + kk) * kProjThreads + tid] = vi < nvec ? x4[vi * (VN / 4) + kk] : jscc_f4v(0.0f);
       }
     }
   }
@@ -127,110 +299,46 @@ __global__ void __launch_bounds__(kProjThreads) k_proj_fused(ProjArgs a) {
   const int64_t r_begin = blockIdx.x * per + (blockIdx.x < rem ? blockIdx.x : rem);
   const int64_t r_end = r_begin + per + (blockIdx.x < rem ? 1 : 0);
 
-  __shared__ float red[2][R][kProjWaves];
+  __shared__ RowMeta meta;
+  __shared__ __align__(16) float red[2][R][kProjWaves];
   int buf = 0;
-  for (int64_t i0 = r_begin; i0 < r_end; i0 += R) {
-    // rows stay in their storage format in registers and are decoded at use
-    VT raw[R][NCH];
-    float sc[R];
-#pragma unroll
-    for (int q = 0; q < R; ++q) {
-      const int64_t i = i0 + q;
-      const bool live = i < r_end;
-      const int64_t li = live ? i : r_begin;
-      const int64_t sid = a.rows ? (int64_t)a.rows[a.row_base + li] : a.row_base + li;
-      sc[q] = a.row_scale ? a.row_scale[sid] : 1.0f;
-      const VT* rowp = reinterpret_cast<const VT*>(reinterpret_cast<const char*>(a.M) +
-                                                  sid * a.ld * (DT == JSCC_DT_F32 ? 4 : (DT == JSCC_DT_BF16 ? 2 : 1)));
-#pragma unroll
-      for (int c = 0; c < NCH; ++c) {
-        const int vi = c * kProjThreads + tid;
-        if (live && vi < nvec) {
-          raw[q][c] = __builtin_nontemporal_load(rowp + vi);
-        } else {
-          raw[q][c] = VT(0);
-        }
+  for (int64_t c0 = r_begin; c0 < r_end; c0 += kStage) {
+    const int64_t c1 = c0 + kStage < r_end ? c0 + kStage : r_end;
+    __syncthreads();  // the previous chunk's metadata is no longer read
+    if (tid < c1 - c0) {
+      const int64_t i = c0 + tid;
+      const int64_t sid = rows ? (int64_t)rows[a.row_base + i] : a.row_base + i;
+      meta.sid[tid] = sid;
+      meta.sc[tid] = row_scale ? row_scale[sid] : 1.0f;
+      float num = 1.0f;
+      if (MODE == MODE_COEF) {
+        num = coef[i];
+      } else if (MODE == MODE_RATIO && y) {
+        num = y[(a.y_by_row ? sid : i) * a.y_stride];
       }
+      meta.num[tid] = num;
     }
-    float coefq[R];
-    if (MODE == MODE_RATIO || MODE == MODE_FWD) {
-      float d[R];
-#pragma unroll
-      for (int q = 0; q < R; ++q) d[q] = 0.0f;
-#pragma unroll
-      for (int c = 0; c < NCH; ++c) {
-        float xv[VN];
-#pragma unroll
-        for (int kk = 0; kk < VN / 4; ++kk) {
-          const jscc_f4v t = xs4[(c * (VN / 4) + kk) * kProjThreads + tid];
-          xv[4 * kk] = t.x; xv[4 * kk + 1] = t.y; xv[4 * kk + 2] = t.z; xv[4 * kk + 3] = t.w;
-        }
-#pragma unroll
-        for (int q = 0; q < R; ++q) {
-          float v[VN];
-          decode<DT>(raw[q][c], v);
-#pragma unroll
-          for (int k = 0; k < VN; ++k) d[q] = fmaf(v[k], xv[k], d[q]);
-        }
-      }
-#pragma unroll
-      for (int q = 0; q < R; ++q) d[q] = jscc_wave_sum(d[q]);
-      if (jscc_lane() == 0) {
-#pragma unroll
-        for (int q = 0; q < R; ++q) red[buf][q][jscc_wave_id()] = d[q];
-      }
-      __syncthreads();
-#pragma unroll
-      for (int q = 0; q < R; ++q) {
-        float f = 0.0f;
-#pragma unroll
-        for (int w = 0; w < kProjWaves; ++w) f += red[buf][q][w];
-        f *= sc[q];
-        const int64_t i = i0 + q;
-        if (MODE == MODE_FWD) {
-          if (tid == 0 && i < r_end) a.fwd_out[i] = f;
-          coefq[q] = 0.0f;
-        } else {
-          const int64_t yi = a.y_by_row ? (a.rows ? (int64_t)a.rows[a.row_base + i] : a.row_base + i) : i;
-          const float num = (a.y && i < r_end) ? a.y[yi * a.y_stride] : 1.0f;
-          const float den = a.use_eps ? fmaxf(f, a.eps) : f;
-          coefq[q] = (i < r_end) ? sc[q] * (num / den) : 0.0f;
-        }
-      }
-      buf ^= 1;
-    } else if (MODE == MODE_COEF) {
-#pragma unroll
-      for (int q = 0; q < R; ++q) {
-        const int64_t i = i0 + q;
-        coefq[q] = (i < r_end) ? sc[q] * a.coef[i] : 0.0f;
-      }
-    } else {  // MODE_ONES
-#pragma unroll
-      for (int q = 0; q < R; ++q) coefq[q] = (i0 + q < r_end) ? sc[q] : 0.0f;
+    __syncthreads();
+
+    // Two register buffers in ping-pong: the next R rows are in flight while
+    // the current R rows are reduced and accumulated.  Every fetch is issued
+    // (slots past the chunk read nothing), so the loop body is straight-line.
+    RowBuf<DT, NCH, R> ba, bb;
+    const int64_t n_steps = (c1 - c0 + R - 1) / R;
+    int64_t i0 = c0;
+    fetch_rows<DT, NCH, R>(a, meta, i0, c0, c1, nvec, tid, ba);
+    for (int64_t st = 0; st + 2 <= n_steps; st += 2) {
+      fetch_rows<DT, NCH, R>(a, meta, i0 + R, c0, c1, nvec, tid, bb);
+      process_rows<DT, NCH, R, MODE>(a, meta, i0, c0, c1, tid, ba, xs4, red, buf, wr);
+      fetch_rows<DT, NCH, R>(a, meta, i0 + 2 * R, c0, c1, nvec, tid, ba);
+      process_rows<DT, NCH, R, MODE>(a, meta, i0 + R, c0, c1, tid, bb, xs4, red, buf, wr);
+      i0 += 2 * R;
     }
-    if (MODE != MODE_FWD) {
-      if (DT != JSCC_DT_F32) {
-        // Packed rows are decoded a second time after the reduction instead of
-        // keeping the decoded floats live across it (halves the row registers).
-#pragma unroll
-        for (int q = 0; q < R; ++q)
-#pragma unroll
-          for (int c = 0; c < NCH; ++c) asm volatile("" : "+v"(raw[q][c]));
-      }
-#pragma unroll
-      for (int q = 0; q < R; ++q)
-#pragma unroll
-        for (int c = 0; c < NCH; ++c) {
-          float v[VN];
-          decode<DT>(raw[q][c], v);
-#pragma unroll
-          for (int k = 0; k < VN; ++k) wr[c][k] = fmaf(coefq[q], v[k], wr[c][k]);
-        }
-    }
+    if (n_steps & 1) process_rows<DT, NCH, R, MODE>(a, meta, i0, c0, c1, tid, ba, xs4, red, buf, wr);
   }
 
   if (MODE != MODE_FWD) {
-    float* dst = a.partial + (int64_t)blockIdx.x * a.P;
+    float* dst = partial + (int64_t)blockIdx.x * a.P;
 #pragma unroll
     for (int c = 0; c < NCH; ++c) {
       const int vi = c * kProjThreads + tid;
@@ -244,20 +352,62 @@ __global__ void __launch_bounds__(kProjThreads) k_proj_fused(ProjArgs a) {
   }
 }
 
-// partial [G, P] -> partial2 [ceil(G/64), P]
-constexpr int kRedRows = 64;
+// partial [G, P] -> partial2 [ceil(G/16), P]: 16 partial rows per output row,
+// so that enough work-groups stream the partials at full rate
+constexpr int kRedRows = 16;
 __global__ void __launch_bounds__(256) k_reduce_stage1(const float* __restrict__ part, int G, int P,
                                                        float* __restrict__ out) {
   const int col = (blockIdx.x * 256 + threadIdx.x) * 4;
   if (col >= P) return;
   const int g0 = blockIdx.y * kRedRows;
   const int g1 = min(G, g0 + kRedRows);
-  float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
-  for (int g = g0; g < g1; ++g) {
-    const float4 v = *reinterpret_cast<const float4*>(part + (int64_t)g * P + col);
-    s.x += v.x; s.y += v.y; s.z += v.z; s.w += v.w;
+  float4 v[kRedRows];
+#pragma unroll
+  for (int k = 0; k < kRedRows; ++k)
+    v[k] = g0 + k < g1 ? *reinterpret_cast<const float4*>(part + (int64_t)(g0 + k) * P + col)
+                       : make_float4(0.f, 0.f, 0.f, 0.f);
+  float4 s = v[0];
+#pragma unroll
+  for (int k = 1; k < kRedRows; ++k) {
+    s.x += v[k].x; s.y += v[k].y; s.z += v[k].z; s.w += v[k].w;
   }
   *reinterpret_cast<float4*>(out + (int64_t)blockIdx.y * P + col) = s;
+}
+
+// EM updates: 0 plain x*w/s ; 1 safe_em_update ; 2 apply_em_update.
+__device__ __forceinline__ float nan_to_num_f(float v, float nan_v, float pinf, float ninf) {
+  if (v != v) return nan_v;
+  if (v == __builtin_inff()) return pinf;
+  if (v == -__builtin_inff()) return ninf;
+  return v;
+}
+
+__device__ __forceinline__ float em_value(float xv, float w, float sv, int mode, float eps) {
+  if (mode == 0) return (xv * w) / sv;
+  if (mode == 1) {
+    const float ws = nan_to_num_f(w, 0.0f, 0.0f, 0.0f);
+    const float ss = nan_to_num_f(sv, 0.0f, 0.0f, 0.0f);
+    return ss > eps ? (xv * fmaxf(ws, 0.0f)) / ss : 0.0f;
+  }
+  const float ss = (sv != sv) ? sv : fmaxf(sv, eps);
+  float u = w / ss;
+  u = nan_to_num_f(u, 1.0f, 1e6f, 0.0f);
+  return nan_to_num_f(xv * u, 0.0f, 1e12f, 0.0f);
+}
+
+typedef JsccEm EmEpilogue;
+
+__device__ __forceinline__ void finish_weight(float sum, int64_t j, float scale, int accumulate, int sanitize,
+                                              float* __restrict__ w, const EmEpilogue& em) {
+  if (sanitize && !(sum == sum && fabsf(sum) != __builtin_inff())) sum = 0.0f;  // nan_to_num(0, 0, 0)
+  const float add = scale == 1.0f ? sum : scale * sum;
+  const float wj = accumulate ? w[j] + add : add;
+  if (em.x) {
+    em.x[j] = em_value(em.x[j], wj, em.s[j], em.mode, em.eps);
+    w[j] = em.zero_w ? 0.0f : wj;
+  } else {
+    w[j] = wj;
+  }
 }
 
 // w[j] = (accumulate ? w[j] : 0) + scale * sum_g part[g][src(j)],
@@ -266,7 +416,7 @@ __global__ void __launch_bounds__(256) k_reduce_stage2(const float* __restrict__
                                                        const int64_t* __restrict__ inv_perm, int64_t ips,
                                                        float scale, int accumulate, int sanitize,
                                                        float* __restrict__ w, int64_t w_len,
-                                                       int32_t* __restrict__ err) {
+                                                       int32_t* __restrict__ err, EmEpilogue em) {
   const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (j >= w_len) return;
   int64_t src = j;
@@ -276,21 +426,38 @@ __global__ void __launch_bounds__(256) k_reduce_stage2(const float* __restrict__
   } else if (j >= P) {
     return;
   }
-  float s = 0.0f;
-  for (int g = 0; g < G; ++g) s += part[(int64_t)g * P + src];
-  if (sanitize && !(s == s && fabsf(s) != __builtin_inff())) s = 0.0f;  // nan_to_num(0, 0, 0)
-  const float add = scale == 1.0f ? s : scale * s;
-  w[j] = accumulate ? w[j] + add : add;
+  float sum = 0.0f;
+#pragma unroll 8
+  for (int g = 0; g < G; ++g) sum += part[(int64_t)g * P + src];
+  finish_weight(sum, j, scale, accumulate, sanitize, w, em);
 }
 
-// EM updates: 0 plain x*w/s ; 1 safe_em_update ; 2 apply_em_update.
-// w = c1*w1 (+ c2*w2 when w2 is given), evaluated like the reference's
-// (2-alpha)*weight_compton + alpha*weight_single.
-__device__ __forceinline__ float nan_to_num_f(float v, float nan_v, float pinf, float ninf) {
-  if (v != v) return nan_v;
-  if (v == __builtin_inff()) return pinf;
-  if (v == -__builtin_inff()) return ninf;
-  return v;
+// Without a rotation gather the columns are reduced in one pass: a block owns
+// 64 columns; 16 row groups of 16 threads (one float4 column each) sum every
+// 16th partial row, then the groups are combined in LDS in a fixed order.
+constexpr int kDirCols = 64;
+__global__ void __launch_bounds__(256) k_reduce_direct(const float* __restrict__ part, int G, int P, float scale,
+                                                       int accumulate, int sanitize, float* __restrict__ w,
+                                                       EmEpilogue em) {
+  __shared__ jscc_f4v acc[16][kDirCols / 4];
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  const int col = blockIdx.x * kDirCols + tx * 4;
+  jscc_f4v sum = jscc_f4v(0.0f);
+  if (col < P) {
+#pragma unroll 8
+    for (int g = ty; g < G; g += 16) sum += *reinterpret_cast<const jscc_f4v*>(part + (int64_t)g * P + col);
+  }
+  acc[ty][tx] = sum;
+  __syncthreads();
+  if (ty == 0 && col < P) {
+    jscc_f4v t = acc[0][tx];
+#pragma unroll
+    for (int k = 1; k < 16; ++k) t += acc[k][tx];
+    finish_weight(t.x, col, scale, accumulate, sanitize, w, em);
+    finish_weight(t.y, col + 1, scale, accumulate, sanitize, w, em);
+    finish_weight(t.z, col + 2, scale, accumulate, sanitize, w, em);
+    finish_weight(t.w, col + 3, scale, accumulate, sanitize, w, em);
+  }
 }
 
 __global__ void __launch_bounds__(256) k_em_update(float* __restrict__ x, const float* __restrict__ w1, float c1,
@@ -299,29 +466,16 @@ __global__ void __launch_bounds__(256) k_em_update(float* __restrict__ x, const 
                                                    float* __restrict__ zero_after) {
   const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (j >= n) return;
+  // w = c1*w1 (+ c2*w2 when w2 is given), evaluated like the reference's
+  // (2-alpha)*weight_compton + alpha*weight_single.
   float w = w1[j];
   if (w2) {
-    w = c1 * w;                  // (2 - alpha) * weight_compton
-    w = w + c2 * w2[j];          // + alpha * weight_single
+    w = c1 * w;
+    w = w + c2 * w2[j];
   } else if (c1 != 1.0f) {
     w = c1 * w;
   }
-  const float xv = x[j];
-  const float sv = s[j];
-  float out;
-  if (mode == 0) {
-    out = (xv * w) / sv;
-  } else if (mode == 1) {
-    const float ws = nan_to_num_f(w, 0.0f, 0.0f, 0.0f);
-    const float ss = nan_to_num_f(sv, 0.0f, 0.0f, 0.0f);
-    out = ss > eps ? (xv * fmaxf(ws, 0.0f)) / ss : 0.0f;
-  } else {
-    const float ss = (sv != sv) ? sv : fmaxf(sv, eps);
-    float u = w / ss;
-    u = nan_to_num_f(u, 1.0f, 1e6f, 0.0f);
-    out = nan_to_num_f(xv * u, 0.0f, 1e12f, 0.0f);
-  }
-  x[j] = out;
+  x[j] = em_value(x[j], w, s[j], mode, eps);
   if (zero_after) zero_after[j] = 0.0f;
 }
 
@@ -365,7 +519,8 @@ int launch_one(const ProjArgs& a, int G, hipStream_t s) {
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     attr_set = true;
   }
-  hipLaunchKernelGGL((k_proj_fused<DT, NCH, R, MODE>), dim3(G), dim3(kProjThreads), lds, s, a);
+  hipLaunchKernelGGL((k_proj_fused<DT, NCH, R, MODE>), dim3(G), dim3(kProjThreads), lds, s, a.M, a.rows, a.row_scale,
+                     a.x, a.x_perm, a.y, a.coef, a.fwd_out, a.partial, a.err, a);
   return 0;
 }
 
@@ -380,15 +535,21 @@ int launch_fused(int mode, const ProjArgs& a, int G, hipStream_t s) {
   }
 }
 
-template <int DT, int R>
+// Rows per register buffer: two buffers of R rows x NCH 16-byte vectors take
+// at most 64 VGPRs next to the NCH x VN accumulators (128 per lane in all);
+// FP8 decodes 16 values per vector and gets half as many rows.
+constexpr int rows_per_buf(int vn, int nch) {
+  return nch * vn >= 64 ? 1 : ((vn == 16 ? 4 : 8) / nch > 1 ? (vn == 16 ? 4 : 8) / nch : 1);
+}
+
+template <int DT>
 int dispatch_nch(int mode, const ProjArgs& a, int G, hipStream_t s) {
   constexpr int VN = Vec<DT>::N;
   const int nch = (a.P / VN + kProjThreads - 1) / kProjThreads;
-  if (nch <= 1) return launch_fused<DT, 1, R>(mode, a, G, s);
-  if (nch <= 2) return launch_fused<DT, 2, R>(mode, a, G, s);
-  if (nch <= 4) return launch_fused<DT, 4, R>(mode, a, G, s);
-  if (nch <= 8) return launch_fused<DT, 8, R>(mode, a, G, s);
-  if (nch <= 16) return launch_fused<DT, 16, (DT == JSCC_DT_F32 ? 1 : R)>(mode, a, G, s);
+  if (nch <= 1) return launch_fused<DT, 1, rows_per_buf(VN, 1)>(mode, a, G, s);
+  if (nch <= 2) return launch_fused<DT, 2, rows_per_buf(VN, 2)>(mode, a, G, s);
+  if (nch <= 4) return launch_fused<DT, 4, rows_per_buf(VN, 4)>(mode, a, G, s);
+  if (DT != JSCC_DT_FP8 && nch <= 8) return launch_fused<DT, (DT == JSCC_DT_FP8 ? 4 : 8), 1>(mode, a, G, s);
   jscc_set_error("jscc_proj: %d columns exceed the register-resident limit for this dtype", a.P);
   return -3;
 }
@@ -400,14 +561,14 @@ extern "C" int jscc_proj_vec_width(int dtype) {
 }
 
 extern "C" int jscc_proj_max_cols(int dtype) {
-  // register-resident image: NCH * 512 threads * vector width
-  return dtype == JSCC_DT_F32 ? 16 * kProjThreads * 4 : (dtype == JSCC_DT_BF16 ? 16 * kProjThreads * 8 : 16 * kProjThreads * 16);
+  // register-resident accumulators: up to 8 (FP8: 4) vectors per thread
+  return dtype == JSCC_DT_F32 ? 8 * kProjThreads * 4 : (dtype == JSCC_DT_BF16 ? 8 * kProjThreads * 8 : 4 * kProjThreads * 16);
 }
 
 extern "C" int jscc_proj_grid(int64_t n_rows) {
   const int cus = jscc_num_cus();
   int64_t g = (n_rows + 3) / 4;
-  if (g > 2 * cus) g = 2 * cus;
+  if (g > cus) g = cus;
   if (g < 1) g = 1;
   return (int)g;
 }
@@ -426,13 +587,13 @@ extern "C" int64_t jscc_proj_workspace_bytes(int64_t n_rows, int64_t P) {
 // then (modes 0, 2, 3) w[j] (+)= out_scale * partial[inv_perm[j]-1].
 // flags bit 0: the numerator y is indexed by the stored row id; bit 1: the
 // block's weight goes through nan_to_num(0, 0, 0) before scaling.
-extern "C" int jscc_proj(int mode, const void* mat, int dtype, int64_t ld, int64_t n_cols, const int32_t* rows,
-                         int64_t n_rows, int64_t row_base, const float* row_scale, const float* x,
-                         const int64_t* x_perm, int64_t x_perm_stride, int64_t x_len, const float* y,
-                         int64_t y_stride, const float* coef, float eps, int use_eps, float* fwd_out,
-                         const int64_t* inv_perm, int64_t inv_perm_stride, float out_scale, int accumulate,
-                         int flags, float* w, int64_t w_len, void* workspace, int64_t workspace_bytes,
-                         int32_t* err, void* stream) {
+static int proj_impl(int mode, const void* mat, int dtype, int64_t ld, int64_t n_cols, const int32_t* rows,
+                     int64_t n_rows, int64_t row_base, const float* row_scale, const float* x,
+                     const int64_t* x_perm, int64_t x_perm_stride, int64_t x_len, const float* y,
+                     int64_t y_stride, const float* coef, float eps, int use_eps, float* fwd_out,
+                     const int64_t* inv_perm, int64_t inv_perm_stride, float out_scale, int accumulate,
+                     int flags, float* w, int64_t w_len, void* workspace, int64_t workspace_bytes,
+                     int32_t* err, void* stream, const JsccEm& em) {
   JSCC_REQUIRE(mode >= 0 && mode <= 3, "jscc_proj: bad mode");
   JSCC_REQUIRE(n_rows >= 0 && n_cols > 0, "jscc_proj: bad shape");
   JSCC_REQUIRE(err != nullptr, "jscc_proj: null error word");
@@ -471,22 +632,63 @@ extern "C" int jscc_proj(int mode, const void* mat, int dtype, int64_t ld, int64
   a.partial = (float*)workspace; a.err = err; a.y_by_row = (flags & 1) ? 1 : 0;
   int rc;
   jscc_prof_begin(JSCC_TAG_PROJ, s);
-  if (dtype == JSCC_DT_F32) rc = dispatch_nch<JSCC_DT_F32, 1>(mode, a, G, s);
-  else if (dtype == JSCC_DT_BF16) rc = dispatch_nch<JSCC_DT_BF16, 2>(mode, a, G, s);
-  else rc = dispatch_nch<JSCC_DT_FP8, 2>(mode, a, G, s);
+  if (dtype == JSCC_DT_F32) rc = dispatch_nch<JSCC_DT_F32>(mode, a, G, s);
+  else if (dtype == JSCC_DT_BF16) rc = dispatch_nch<JSCC_DT_BF16>(mode, a, G, s);
+  else rc = dispatch_nch<JSCC_DT_FP8>(mode, a, G, s);
   jscc_prof_end(JSCC_TAG_PROJ, s);
   if (rc != 0) return rc;
   JSCC_CHECK_LAUNCH();
   if (mode == 1) return 0;
 
   return jscc_reduce_partials((float*)workspace, G, (int)n_cols, inv_perm, inv_perm_stride, out_scale, accumulate,
-                              (flags & 2) ? 1 : 0, w, w_len, err, s);
+                              (flags & 2) ? 1 : 0, w, w_len, err, s, em);
 }
 
-// partial [G, P] (followed by room for ceil(G/64) more rows) -> w, deterministic order
+extern "C" int jscc_proj(int mode, const void* mat, int dtype, int64_t ld, int64_t n_cols, const int32_t* rows,
+                         int64_t n_rows, int64_t row_base, const float* row_scale, const float* x,
+                         const int64_t* x_perm, int64_t x_perm_stride, int64_t x_len, const float* y,
+                         int64_t y_stride, const float* coef, float eps, int use_eps, float* fwd_out,
+                         const int64_t* inv_perm, int64_t inv_perm_stride, float out_scale, int accumulate,
+                         int flags, float* w, int64_t w_len, void* workspace, int64_t workspace_bytes,
+                         int32_t* err, void* stream) {
+  const JsccEm none = {nullptr, nullptr, 0, 0.0f, 0};
+  return proj_impl(mode, mat, dtype, ld, n_cols, rows, n_rows, row_base, row_scale, x, x_perm, x_perm_stride, x_len,
+                   y, y_stride, coef, eps, use_eps, fwd_out, inv_perm, inv_perm_stride, out_scale, accumulate, flags,
+                   w, w_len, workspace, workspace_bytes, err, stream, none);
+}
+
+// One OSEM subset step: jscc_proj's ratio back-projection into w followed, in
+// the same reduction pass, by the EM update of jscc_em_update (c1 = 1, no
+// second weight) on em_x with sensitivity em_s; w is zeroed afterwards when
+// zero_w is set.  em_x may be the projected image x itself.
+extern "C" int jscc_proj_em(const void* mat, int dtype, int64_t ld, int64_t n_cols, const int32_t* rows,
+                            int64_t n_rows, int64_t row_base, const float* row_scale, const float* x,
+                            const int64_t* x_perm, int64_t x_perm_stride, int64_t x_len, const float* y,
+                            int64_t y_stride, float eps, int use_eps, const int64_t* inv_perm,
+                            int64_t inv_perm_stride, float out_scale, int accumulate, int flags, float* w,
+                            int64_t w_len, float* em_x, const float* em_s, int em_mode, float em_eps, int zero_w,
+                            void* workspace, int64_t workspace_bytes, int32_t* err, void* stream) {
+  JSCC_REQUIRE(em_x != nullptr && em_s != nullptr, "jscc_proj_em: null image or sensitivity");
+  JSCC_REQUIRE(em_mode >= 0 && em_mode <= 2, "jscc_proj_em: bad EM mode %d", em_mode);
+  JSCC_REQUIRE(n_rows > 0, "jscc_proj_em: empty subset (use jscc_em_update)");
+  const JsccEm em = {em_x, em_s, em_mode, em_eps, zero_w};
+  return proj_impl(0, mat, dtype, ld, n_cols, rows, n_rows, row_base, row_scale, x, x_perm, x_perm_stride, x_len, y,
+                   y_stride, nullptr, eps, use_eps, nullptr, inv_perm, inv_perm_stride, out_scale, accumulate, flags,
+                   w, w_len, workspace, workspace_bytes, err, stream, em);
+}
+
+// partial [G, P] (followed by room for ceil(G/16) more rows) -> w, deterministic order
 int jscc_reduce_partials(float* part, int G, int P, const int64_t* inv_perm, int64_t ips, float scale,
-                         int accumulate, int sanitize, float* w, int64_t w_len, int32_t* err, hipStream_t s) {
+                         int accumulate, int sanitize, float* w, int64_t w_len, int32_t* err, hipStream_t s,
+                         const JsccEm& em) {
   JSCC_REQUIRE(P % 4 == 0, "jscc_reduce_partials: columns must be a multiple of 4");
+  if (!inv_perm) {
+    JSCC_REQUIRE(w_len == P, "jscc_reduce_partials: weight length mismatch");
+    hipLaunchKernelGGL(k_reduce_direct, dim3((unsigned)((P + kDirCols - 1) / kDirCols)), dim3(256), 0, s, part, G, P,
+                       scale, accumulate, sanitize, w, em);
+    JSCC_CHECK_LAUNCH();
+    return 0;
+  }
   int Gfin = G;
   if (G > kRedRows) {
     const int G2 = (G + kRedRows - 1) / kRedRows;
@@ -498,7 +700,7 @@ int jscc_reduce_partials(float* part, int G, int P, const int64_t* inv_perm, int
     Gfin = G2;
   }
   hipLaunchKernelGGL(k_reduce_stage2, dim3((unsigned)((w_len + 255) / 256)), dim3(256), 0, s, part, Gfin, P,
-                     inv_perm, ips, scale, accumulate, sanitize, w, w_len, err);
+                     inv_perm, ips, scale, accumulate, sanitize, w, w_len, err, em);
   JSCC_CHECK_LAUNCH();
   return 0;
 }

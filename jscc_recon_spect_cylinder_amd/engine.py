@@ -75,8 +75,12 @@ class ListModeStore:
         bounds = self.subset_bounds(n_subsets)
         for _ in range(n_iter):
             for lo, hi in bounds:
-                self.subset_weight(x, w, lo, hi, eps=eps)
-                ops.em_update(x, w, s, mode=em_mode, zero_after=w)
+                if hi > lo:  # back-projection and EM update in one call (jscc_proj_em)
+                    ops.proj(ops.MODE_RATIO, self.mat, dtype_c=self.dtype_c, n_cols=self.n_vox, rows=self.kept,
+                             row_base=lo, n_rows=hi - lo, row_scale=self.row_scale, x=x, eps=eps, accumulate=True,
+                             w=w, em=(x, s, em_mode, 1e-12, True))
+                else:
+                    ops.em_update(x, w, s, mode=em_mode, zero_after=w)
         return x
 
 

@@ -384,8 +384,12 @@ def proj(mode: int, mat: torch.Tensor, *, dtype_c: int | None = None, n_cols: in
          y: torch.Tensor | None = None, coef: torch.Tensor | None = None, eps: float | None = None,
          fwd_out: torch.Tensor | None = None, inv_perm: torch.Tensor | None = None, out_scale: float = 1.0,
          accumulate: bool = False, w: torch.Tensor | None = None, y_by_row: bool = False,
-         sanitize: bool = False) -> None:
-    """Launch jscc_proj on the current stream (see include/jscc_hip.h)."""
+         sanitize: bool = False, em: tuple | None = None) -> None:
+    """Launch jscc_proj on the current stream (see include/jscc_hip.h).
+
+    em = (x_em, s, em_mode, em_eps, zero_w) (mode MODE_RATIO only) runs
+    jscc_proj_em instead: the EM update of x_em with sensitivity s is applied
+    in the same reduction pass that finishes w."""
     dev = mat.device
     if dtype_c is None:
         dtype_c = dtype_code(mat.dtype) if mat.dtype != torch.uint8 else DT_FP8
@@ -425,12 +429,30 @@ def proj(mode: int, mat: torch.Tensor, *, dtype_c: int | None = None, n_cols: in
             raise ValueError("all operands must be on the matrix's device")
     ws = _workspace(dev, _lib.lib().jscc_proj_workspace_bytes(max(n_rows, 1), P))
     xl = x.shape[0] if x is not None else 0
+    flags = (1 if y_by_row else 0) | (2 if sanitize else 0)
+    if em is not None:
+        x_em, s_em, em_mode, em_eps, zero_w = em
+        if mode != MODE_RATIO:
+            raise ValueError("the EM epilogue follows the ratio back-projection only")
+        for t in (x_em, s_em):
+            if (t.dtype != torch.float32 or not t.is_contiguous() or t.device != dev or w is None
+                    or t.numel() < w.shape[0]):
+                raise ValueError("EM operands must be contiguous float32 covering the weight, on the matrix's device")
+        rc = _lib.lib().jscc_proj_em(
+            mat.data_ptr(), dtype_c, ld, P, _p(rows), n_rows, row_base, _p(row_scale), _p(x), _p(x_perm),
+            x_perm.stride(0) if x_perm is not None else 0, xl, _p(y), y.stride(0) if y is not None else 0,
+            float(eps) if eps is not None else 0.0, 1 if eps is not None else 0, _p(inv_perm),
+            inv_perm.stride(0) if inv_perm is not None else 0, float(out_scale), 1 if accumulate else 0, flags,
+            _p(w), wl, x_em.data_ptr(), s_em.data_ptr(), int(em_mode), float(em_eps), 1 if zero_w else 0,
+            ws.data_ptr(), ws.numel(), err_word(dev).data_ptr(), stream_of(dev))
+        _lib.check(rc, "jscc_proj_em")
+        return
     rc = _lib.lib().jscc_proj(
         mode, mat.data_ptr(), dtype_c, ld, P, _p(rows), n_rows, row_base, _p(row_scale), _p(x), _p(x_perm),
         x_perm.stride(0) if x_perm is not None else 0, xl, _p(y), y.stride(0) if y is not None else 0, _p(coef),
         float(eps) if eps is not None else 0.0, 1 if eps is not None else 0, _p(fwd_out), _p(inv_perm),
         inv_perm.stride(0) if inv_perm is not None else 0, float(out_scale), 1 if accumulate else 0,
-        (1 if y_by_row else 0) | (2 if sanitize else 0), _p(w), wl,
+        flags, _p(w), wl,
         ws.data_ptr(), ws.numel(), err_word(dev).data_ptr(), stream_of(dev))
     _lib.check(rc, "jscc_proj")
 

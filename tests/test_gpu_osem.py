@@ -165,6 +165,35 @@ def test_em_update_modes_vs_oracle():
                                                         s.astype(np.float64)), rtol=1e-6)
 
 
+@pytest.mark.parametrize("em_mode", [0, 1, 2])
+@pytest.mark.parametrize("rotated", [False, True])
+def test_fused_em_step_equals_projection_then_update(em_mode, rotated):
+    """jscc_proj_em is bit-identical to jscc_proj followed by jscc_em_update (same
+    reduction order), with and without the rotation gather, for all EM modes."""
+    from jscc_recon_spect_cylinder_amd import ops
+    rng = np.random.default_rng(11 + em_mode)
+    rot, inv = _layer_tables()
+    P = rot.shape[0]
+    M = (rng.random((3000, P)) * (rng.random((3000, P)) < 0.4)).astype(np.float32)
+    mat = torch.from_numpy(M).to(DEV)
+    x0 = torch.from_numpy((rng.random(P) + 0.5).astype(np.float32)).to(DEV)
+    y = torch.from_numpy(rng.poisson(3.0, 3000).astype(np.float32)).to(DEV)
+    s = torch.from_numpy((rng.random(P) * 2).astype(np.float32)).to(DEV)
+    s[::97] = 0.0
+    kw = dict(y=y, eps=1e-12, out_scale=0.75, accumulate=True)
+    if rotated:
+        kw.update(x_perm=torch.from_numpy(rot[:, 3].copy()).to(DEV), inv_perm=torch.from_numpy(inv[:, 3].copy()).to(DEV))
+    w_ref = torch.full((P,), 0.25, device=DEV)
+    x_ref = x0.clone()
+    ops.proj(ops.MODE_RATIO, mat, x=x0, w=w_ref, **kw)
+    ops.em_update(x_ref, w_ref, s, mode=em_mode, zero_after=w_ref)
+    w = torch.full((P,), 0.25, device=DEV)
+    x = x0.clone()
+    ops.proj(ops.MODE_RATIO, mat, x=x, w=w, em=(x, s, em_mode, 1e-12, True), **kw)
+    assert torch.equal(x, x_ref)
+    assert torch.equal(w, torch.zeros_like(w))
+
+
 def test_sensitivity_with_rotations_vs_oracle():
     from jscc_recon_spect_cylinder_amd import ops
     rot, inv = _layer_tables()
