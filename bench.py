@@ -1,21 +1,24 @@
 """Benchmark of the reconstruction hot path on one MI355X.
 
-Workload (the JSCCD list-mode case): 2e5 synthetic two-hit events from a rod
-phantom on a 64 x 256 polar grid (16384 voxels), cone blur sigma = 3 deg,
+Default workload (the JSCCD list-mode case): 2e5 synthetic two-hit events from
+a rod phantom on a 64 x 256 polar grid (16384 voxels), cone blur sigma = 3 deg,
 T (2e5 x 16384) built on the GPU, then 10 OSEM iterations x 8 subsets.
 
-One step = build T from the list (cone back-projection) + list-mode
-sensitivity + 10 OSEM iterations, all on inputs resident in HBM.  The step is
-timed for the BF16-stored T (the reported value) and for FP32-stored T.
+One step = build T from the list (event filters + cone back-projection) +
+list-mode sensitivity (column sums of T) + 10 OSEM iterations, all on inputs
+resident in HBM.  The step is timed for BF16-stored T (the reported value) and
+for FP32-stored T.
 
 Prints ONE JSON line: value = events reconstructed per second (Mevents/s) of
-the BF16 path; components, the roofline of the dominant kernel (measured with
-the library's HIP-event kernel timer over the timed region) and a bounded CPU
-baseline (the numpy oracle on the host, extrapolated) are included.
+the BF16 path, the roofline of the dominant kernel (one OSEM subset pass
+T_s^T(1/T_s x), timed by the library's HIP-event kernel timer on its launch
+stream over the timed region), its HBM traffic from the committed rocprofv3
+counter summary, and a bounded CPU baseline (the numpy oracle on the host).
 """
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -24,7 +27,8 @@ import time
 import numpy as np
 
 HBM_PEAK_GBS = 8000.0
-FP32_PEAK_TFLOPS = 157.3
+HERE = os.path.dirname(os.path.abspath(__file__))
+TRAFFIC_FILE = os.path.join(HERE, "profiles", "k_proj_fused_jsccd_2e5x16384_traffic.json")
 
 
 def parse():
@@ -35,7 +39,7 @@ def parse():
     ap.add_argument("--events", type=int, default=200_000)
     ap.add_argument("--iters", type=int, default=10)
     ap.add_argument("--subsets", type=int, default=8)
-    ap.add_argument("--cpu-sample", type=int, default=600, help="events in the CPU-baseline sample (0: skip)")
+    ap.add_argument("--cpu-sample", type=int, default=12000, help="events in the CPU-baseline sample (0: skip)")
     ap.add_argument("--skip-fp32", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (HBM counter calibration) and exit")
@@ -55,9 +59,10 @@ def copy_calibration(mib):
     print(json.dumps({"copy_calibration_bytes_read": n * 4, "copy_calibration_bytes_written": n * 4}))
 
 
-def cpu_baseline(cl, n_sample, n_iter, n_sub, sigma):
-    """The numpy oracle (FP64) on a bounded sample of the same list, extrapolated per event."""
-    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+def cpu_baseline(cl, n_sample, n_iter, n_sub, sigma, chunk=1000):
+    """The numpy oracle (FP64, one thread) on a bounded sample of the same list: T built in chunks of
+    `chunk` events, then one OSEM iteration over the sample's rows; extrapolated linearly per event."""
+    sys.path.insert(0, HERE)
     try:
         from threadpoolctl import threadpool_limits
     except ImportError:  # pragma: no cover
@@ -65,13 +70,18 @@ def cpu_baseline(cl, n_sample, n_iter, n_sub, sigma):
     from oracle import cone, osem
     from jscc_recon_spect_cylinder_amd import synth
     tmax, tmin, tsum = synth.energy_thresholds(cl.e0)
-    lst = cl.list_origin[:n_sample]
     with (threadpool_limits(limits=1) if threadpool_limits else _null()):
-        t0 = time.perf_counter()
-        out = cone.backproj_list_single(None, cl.detector, cl.coor, lst, 0.0, 0.0, cl.e0, 0.02, tmax, tmin, tsum,
-                                        None, None, fixed_sigma=sigma)
-        t_build = time.perf_counter() - t0
-        T = out["t"]
+        blocks = []
+        t_build = 0.0
+        for lo in range(0, n_sample, chunk):
+            lst = cl.list_origin[lo:min(lo + chunk, n_sample)]
+            t0 = time.perf_counter()
+            out = cone.backproj_list_single(None, cl.detector, cl.coor, lst, 0.0, 0.0, cl.e0, 0.02, tmax, tmin,
+                                            tsum, None, None, fixed_sigma=sigma)
+            t_build += time.perf_counter() - t0
+            blocks.append(out["t"])
+        T = np.concatenate(blocks)
+        del blocks
         s = T.sum(0)
         x = np.ones(T.shape[1])
         bounds = osem.split_bounds_even(T.shape[0], n_sub)
@@ -82,9 +92,9 @@ def cpu_baseline(cl, n_sample, n_iter, n_sub, sigma):
         t_iter = time.perf_counter() - t0
     per_event = (t_build + n_iter * t_iter) / n_sample
     return {"value": 1e-6 / per_event, "unit": "Mevents/s", "cores": 1, "kind": "port",
-            "sample": f"numpy FP64 oracle, {n_sample} of the events (T build {t_build:.2f} s, 1 OSEM iteration "
-                      f"{t_iter:.3f} s), extrapolated linearly to T build + {n_iter} iterations per event",
-            "host_cpu": _cpu_model()}
+            "sample": f"numpy FP64 oracle (1 thread), the first {n_sample} events of the same list: T build "
+                      f"{t_build:.2f} s, 1 OSEM iteration ({n_sub} subsets) {t_iter:.3f} s, extrapolated linearly "
+                      f"to T build + {n_iter} iterations per event", "host_cpu": _cpu_model()}
 
 
 class _null:
@@ -105,6 +115,25 @@ def _cpu_model():
     return "unknown"
 
 
+def _prof_read(lib, tag):
+    c = ctypes.c_int(0)
+    t = ctypes.c_double(0.0)
+    if lib.jscc_prof_read(tag, ctypes.byref(c), ctypes.byref(t)) != 0:
+        raise RuntimeError("jscc_prof_read failed")
+    return c.value, t.value
+
+
+def _traffic(dtype_name):
+    """HBM bytes per subset launch from the committed rocprofv3 counter summary (None when absent)."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            rec = json.load(f)[dtype_name]
+    except (OSError, KeyError, ValueError):
+        return None
+    kib = rec["fetch_kib"] * rec["fetch_factor"] + rec["write_kib"] * rec["write_factor"]
+    return kib * 1024.0
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -115,7 +144,7 @@ def main():
         copy_calibration(args.copy_calibration)
         return
     import torch
-    from jscc_recon_spect_cylinder_amd import _lib, ops, synth
+    from jscc_recon_spect_cylinder_amd import _lib, synth
     from jscc_recon_spect_cylinder_amd.engine import ListModeStore
     from jscc_recon_spect_cylinder_amd.process_list_plane_strict import build_compton_rows
 
@@ -148,6 +177,8 @@ def main():
         for _ in range(args.warmup):
             step(sd, name)
         torch.cuda.synchronize()
+        for tag in (_lib.PROF_CONE, _lib.PROF_PROJ, _lib.PROF_PROJ_SUM):
+            _prof_read(lib, tag)  # drop pairs of the warm-up
         lib.jscc_prof_enable(1)
         t0 = time.perf_counter()
         for _ in range(args.steps):
@@ -155,58 +186,50 @@ def main():
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         lib.jscc_prof_enable(0)
-        cnt, tot = _ctypes_read(lib, _lib.PROF_CONE)
-        pcnt, ptot = _ctypes_read(lib, _lib.PROF_PROJ)
+        ccnt, ctot = _prof_read(lib, _lib.PROF_CONE)
+        pcnt, ptot = _prof_read(lib, _lib.PROF_PROJ)
+        scnt, stot = _prof_read(lib, _lib.PROF_PROJ_SUM)
         ms_step = (t1 - t0) * 1e3 / args.steps
         # the OSEM part alone (T already built): iterations per second
         s = st.backproject_ones()
+        xi = torch.ones_like(s)
         torch.cuda.synchronize()
         t2 = time.perf_counter()
-        xi = torch.ones_like(s)
         st.osem(xi, s, args.iters, args.subsets)
         torch.cuda.synchronize()
         osem_s = (time.perf_counter() - t2) / args.iters
-        bytes_per = {"bf16": 2, "fp32": 4}[name]
-        # per step: one sensitivity pass over all kept rows + iters x subsets subset passes;
-        # algorithmic bytes = every kept row read once per pass (+ image, weight, scale vectors)
-        launches = pcnt / args.steps
-        step_bytes = (args.iters + 1) * st.n_kept * (P * bytes_per + 8) + launches * 3 * P * 4
+        b = {"bf16": 2, "fp32": 4}[name]
+        # algorithmic bytes of one subset launch: its rows read once (+ x read, w written)
+        subset_bytes = st.n_kept / args.subsets * P * b + 2 * P * 4
         proj_ms = ptot / max(pcnt, 1)
-        proj_bytes = step_bytes / launches
         results[name] = {
             "ms_per_step": ms_step,
             "img": x.cpu().numpy(),
             "n_kept": st.n_kept,
-            "cone_ms": tot / max(cnt, 1),
-            "cone_Mevents_s": args.events / (tot / max(cnt, 1) * 1e-3) / 1e6,
-            "proj_launch_ms": proj_ms,
-            "proj_launches_per_step": pcnt / args.steps,
-            "proj_bytes": proj_bytes,
-            "proj_GBs": proj_bytes / (proj_ms * 1e-3) / 1e9,
+            "cone_ms": ctot / max(ccnt, 1),
+            "cone_Mevents_s": args.events / (ctot / max(ccnt, 1) * 1e-3) / 1e6,
+            "subset_launch_ms": proj_ms,
+            "subset_launches_per_step": pcnt / args.steps,
+            "subset_bytes": subset_bytes,
+            "subset_GBs": subset_bytes / (proj_ms * 1e-3) / 1e9,
+            "colsum_ms": stot / max(scnt, 1),
+            "colsum_GBs": st.n_kept * P * b / (stot / max(scnt, 1) * 1e-3) / 1e9,
             "osem_iter_s": 1.0 / osem_s,
         }
         del st, x
 
-    b = results["bf16"]
+    bf = results["bf16"]
     rel_err = None
     if "fp32" in results:
         f = results["fp32"]["img"]
-        rel_err = float(np.linalg.norm(b["img"] - f) / np.linalg.norm(f))
-    cone_total = b["cone_ms"]
-    proj_total = b["proj_launch_ms"] * b["proj_launches_per_step"]
-    if proj_total >= cone_total:
-        roof = {"kernel": "k_proj_fused<bf16> (one OSEM subset: T_s^T(1/T_s x) in one pass)", "bound": "hbm",
-                "achieved": round(b["proj_GBs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(b["proj_GBs"] / HBM_PEAK_GBS, 4), "traffic": None,
-                "launch_ms": round(b["proj_launch_ms"], 4), "bytes_per_launch": int(b["proj_bytes"])}
-    else:
-        pairs = args.events * P
-        flops = pairs * 60.0
-        tfl = flops / (b["cone_ms"] * 1e-3) / 1e12
-        roof = {"kernel": "k_cone_rows (fixed sigma, BF16 rows)", "bound": "valu", "achieved": round(tfl, 2),
-                "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(tfl / FP32_PEAK_TFLOPS, 4),
-                "traffic": None, "launch_ms": round(b["cone_ms"], 4)}
-    value = args.events / (b["ms_per_step"] * 1e-3) / 1e6
+        rel_err = float(np.linalg.norm(bf["img"] - f) / np.linalg.norm(f))
+    traffic = _traffic("bf16")
+    roof = {"kernel": "k_proj_fused<bf16,2,4,ratio>: one OSEM subset pass T_s^T(1/T_s x)", "bound": "hbm",
+            "achieved": round(bf["subset_GBs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(bf["subset_GBs"] / HBM_PEAK_GBS, 4),
+            "traffic": None if traffic is None else int(traffic),
+            "launch_ms": round(bf["subset_launch_ms"], 4), "bytes_per_launch": int(bf["subset_bytes"])}
+    value = args.events / (bf["ms_per_step"] * 1e-3) / 1e6
     comp = {}
     for k, r in results.items():
         comp[k] = {kk: (round(v, 4) if isinstance(v, float) else v) for kk, v in r.items() if kk != "img"}
@@ -217,14 +240,14 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(b["ms_per_step"], 3),
+        "ms_per_step": round(bf["ms_per_step"], 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "bf16 storage, fp32 accumulate",
         "data": "synthetic (seeded rod phantom, Klein-Nishina list; no measured factors exist for this geometry)",
-        "config": {"workload": "JSCCD list-mode: 2e5 events, 64x256 polar grid (16384 voxels), sigma 3 deg, "
-                               f"T {args.events}x{P} built on the GPU, {args.iters} iterations x {args.subsets} subsets",
+        "config": {"workload": f"JSCCD list-mode: {args.events} events, 64x256 polar grid ({P} voxels), sigma 3 deg, "
+                               f"T built on the GPU, {args.iters} iterations x {args.subsets} subsets",
                    "events": args.events, "voxels": P, "iterations": args.iters, "subsets": args.subsets},
         "components": comp,
         "rel_err_bf16_vs_fp32_image": rel_err,
@@ -233,16 +256,6 @@ def main():
     if args.cpu_sample > 0:
         out["cpu_baseline"] = cpu_baseline(cl, args.cpu_sample, args.iters, args.subsets, sigma)
     print(json.dumps(out))
-
-
-def _ctypes_read(lib, tag):
-    import ctypes
-    c = ctypes.c_int(0)
-    t = ctypes.c_double(0.0)
-    rc = lib.jscc_prof_read(tag, ctypes.byref(c), ctypes.byref(t))
-    if rc != 0:
-        raise RuntimeError("jscc_prof_read failed")
-    return c.value, t.value
 
 
 if __name__ == "__main__":

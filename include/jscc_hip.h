@@ -219,8 +219,10 @@ int jscc_sum_f64(const float* v, int64_t n, int64_t stride, double* out, void* s
 int jscc_fp8_selftest(float* out512, void* stream);
 
 /* Built-in kernel timer (no counterpart in the reference).  While enabled
- * (per calling thread), every cone-row launch (tag 0) and fused projection
- * launch (tag 1) is bracketed by a HIP event pair on its own stream;
+ * (per calling thread), the main kernel of every launch is bracketed by a HIP
+ * event pair on its own stream, by tag: 0 cone rows (jscc_cone_rows),
+ * 1 projection modes 0-2 (jscc_proj / jscc_proj_em), 2 projection mode 3
+ * (column sums), 3 no-store cone projection (jscc_cone_proj).
  * jscc_prof_read synchronises the pairs of one tag and returns their count and
  * summed milliseconds. */
 int jscc_prof_enable(int on);

@@ -62,7 +62,7 @@ SIGNATURES = {
     "jscc_prof_read": (c_int, [c_int, vp, vp]),
 }
 
-PROF_CONE, PROF_PROJ = 0, 1
+PROF_CONE, PROF_PROJ, PROF_PROJ_SUM, PROF_CONE_PROJ = 0, 1, 2, 3
 
 
 class JsccError(RuntimeError):

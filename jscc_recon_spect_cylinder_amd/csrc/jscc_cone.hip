@@ -576,7 +576,7 @@ extern "C" int jscc_cone_proj(int mode, const void* events, const int32_t* ev_id
   a.fwd_out = fwd_out; a.dsig_out = dsig_out; a.partial = (float*)workspace;
   const bool dsig = dsig_out != nullptr;
   int rc;
-  jscc_prof_begin(JSCC_TAG_CONE, s);
+  jscc_prof_begin(JSCC_TAG_CONE_PROJ, s);
   if (sigma_model == SIG_STRICT) {
     rc = sysmat ? launch_cp_size<SIG_STRICT, AF_ROW, false>(mode, a, G, s)
                 : launch_cp_size<SIG_STRICT, AF_NONE, false>(mode, a, G, s);
@@ -587,7 +587,7 @@ extern "C" int jscc_cone_proj(int mode, const void* events, const int32_t* ev_id
     rc = sysmat ? launch_cp_size<SIG_FIXED, AF_ROW, false>(mode, a, G, s)
                 : launch_cp_size<SIG_FIXED, AF_NONE, false>(mode, a, G, s);
   }
-  jscc_prof_end(JSCC_TAG_CONE, s);
+  jscc_prof_end(JSCC_TAG_CONE_PROJ, s);
   if (rc != 0) return rc;
   JSCC_CHECK_LAUNCH();
   if (mode == CP_FWD) return 0;

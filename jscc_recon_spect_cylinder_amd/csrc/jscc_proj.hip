@@ -631,11 +631,12 @@ static int proj_impl(int mode, const void* mat, int dtype, int64_t ld, int64_t n
   a.y = y; a.y_stride = y_stride; a.coef = coef; a.eps = eps; a.use_eps = use_eps; a.fwd_out = fwd_out;
   a.partial = (float*)workspace; a.err = err; a.y_by_row = (flags & 1) ? 1 : 0;
   int rc;
-  jscc_prof_begin(JSCC_TAG_PROJ, s);
+  const int tag = mode == 3 ? JSCC_TAG_PROJ_SUM : JSCC_TAG_PROJ;
+  jscc_prof_begin(tag, s);
   if (dtype == JSCC_DT_F32) rc = dispatch_nch<JSCC_DT_F32>(mode, a, G, s);
   else if (dtype == JSCC_DT_BF16) rc = dispatch_nch<JSCC_DT_BF16>(mode, a, G, s);
   else rc = dispatch_nch<JSCC_DT_FP8>(mode, a, G, s);
-  jscc_prof_end(JSCC_TAG_PROJ, s);
+  jscc_prof_end(tag, s);
   if (rc != 0) return rc;
   JSCC_CHECK_LAUNCH();
   if (mode == 1) return 0;
