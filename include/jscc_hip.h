@@ -107,7 +107,9 @@ int jscc_rows_gather_normalize(const void* src, int src_dtype, int64_t ld_src, i
 int jscc_arows_normalize(const float* sysmat, int64_t lda, int64_t n_cols, const int32_t* arow, int64_t n_out,
                          float* dst, int64_t ld_dst, void* stream);
 
-/* FP8 e4m3fn storage of rows with per-row scale (row_scale * code ~ t / S).
+/* FP8 e4m3fn storage of rows with per-row scale (row_scale * code ~ t / S,
+ * S and max from stats); stats NULL: rows of a system matrix, scaled by
+ * their own largest |value| and not normalised (row_scale * code ~ a).
  * No counterpart in the reference (precision mode of the MI355X engine). */
 int jscc_quantize_fp8_rows(const void* src, int src_dtype, int64_t ld_src, int64_t n_cols, int64_t n_rows,
                            const double* stats, uint8_t* dst, int64_t ld_dst, float* row_scale, void* stream);
@@ -162,18 +164,21 @@ int jscc_proj(int mode, const void* mat, int dtype, int64_t ld, int64_t n_cols, 
               void* stream);
 
 /* One OSEM subset step in one call: the mode-0 back-projection of jscc_proj
- * into w, then, in the same reduction pass, em_x[j] = update(em_x[j], w[j],
- * em_s[j]) with the modes of jscc_em_update (c1 = 1, no second weight) and w
- * cleared afterwards when zero_w != 0.  Replaces the pair
- * get_weight_compton / get_weight_single + "img = img * weight / s_map" of
- * osem_bin_mode / osem_list_mode (recon_osem_plane.py:45-75) for one subset;
- * em_x may be the projected image x. */
+ * into w, then, in the same reduction pass, em_x[j] = update(em_x[j],
+ * em_c1*w[j] (+ em_c2*em_w2[j]), em_s[j]) with the modes of jscc_em_update,
+ * and w (and em_w2) cleared afterwards when zero_w != 0.  Replaces, for one
+ * subset, get_weight_compton / get_weight_single + "img = img * weight /
+ * s_map" of osem_bin_mode / osem_list_mode (recon_osem_plane.py:45-80) and,
+ * with em_w2 = the single-photon weight, the "(2 - alpha) * weight_compton +
+ * alpha * weight_single" update of osem_joint_mode (:117-118).  em_x may be
+ * the projected image x; em_w2 may be NULL. */
 int jscc_proj_em(const void* mat, int dtype, int64_t ld, int64_t n_cols, const int32_t* rows, int64_t n_rows,
                  int64_t row_base, const float* row_scale, const float* x, const int64_t* x_perm,
                  int64_t x_perm_stride, int64_t x_len, const float* y, int64_t y_stride, float eps, int use_eps,
                  const int64_t* inv_perm, int64_t inv_perm_stride, float out_scale, int accumulate, int flags,
-                 float* w, int64_t w_len, float* em_x, const float* em_s, int em_mode, float em_eps, int zero_w,
-                 void* workspace, int64_t workspace_bytes, int32_t* err, void* stream);
+                 float* w, int64_t w_len, float* em_x, const float* em_s, int em_mode, float em_eps, float em_c1,
+                 float* em_w2, float em_c2, int zero_w, void* workspace, int64_t workspace_bytes, int32_t* err,
+                 void* stream);
 
 /* EM update with w = c1*w1 (+ c2*w2):
  *   mode 0: x = x * w / s                        (recon_osem_plane.py:54, 73, 117-118)

@@ -158,14 +158,17 @@ __device__ __forceinline__ int jscc_wave_id() { return threadIdx.x >> 6; }
 int jscc_num_cus();
 
 // Optional EM update applied where a subset weight is finished:
-// x[j] <- em(x[j], w[j], s[j]) (modes of jscc_em_update); w[j] is zeroed after
-// when zero_w is set.  x == nullptr: no update.
+// x[j] <- em(x[j], c1*w[j] (+ c2*w2[j]), s[j]) (modes of jscc_em_update); w[j]
+// (and w2[j]) are zeroed after when zero_w is set.  x == nullptr: no update.
 struct JsccEm {
   float* x;
   const float* s;
   int mode;
   float eps;
   int zero_w;
+  float c1;
+  float* w2;
+  float c2;
 };
 
 // Deterministic reduction of per-work-group partial back-projections [G, P]

@@ -403,8 +403,17 @@ __device__ __forceinline__ void finish_weight(float sum, int64_t j, float scale,
   const float add = scale == 1.0f ? sum : scale * sum;
   const float wj = accumulate ? w[j] + add : add;
   if (em.x) {
-    em.x[j] = em_value(em.x[j], wj, em.s[j], em.mode, em.eps);
+    // the weight combination of k_em_update: c1*w (+ c2*w2), same order
+    float wc = wj;
+    if (em.w2) {
+      wc = em.c1 * wc;
+      wc = wc + em.c2 * em.w2[j];
+    } else if (em.c1 != 1.0f) {
+      wc = em.c1 * wc;
+    }
+    em.x[j] = em_value(em.x[j], wc, em.s[j], em.mode, em.eps);
     w[j] = em.zero_w ? 0.0f : wj;
+    if (em.w2 && em.zero_w) em.w2[j] = 0.0f;
   } else {
     w[j] = wj;
   }
@@ -652,27 +661,28 @@ extern "C" int jscc_proj(int mode, const void* mat, int dtype, int64_t ld, int64
                          const int64_t* inv_perm, int64_t inv_perm_stride, float out_scale, int accumulate,
                          int flags, float* w, int64_t w_len, void* workspace, int64_t workspace_bytes,
                          int32_t* err, void* stream) {
-  const JsccEm none = {nullptr, nullptr, 0, 0.0f, 0};
+  const JsccEm none = {nullptr, nullptr, 0, 0.0f, 0, 1.0f, nullptr, 0.0f};
   return proj_impl(mode, mat, dtype, ld, n_cols, rows, n_rows, row_base, row_scale, x, x_perm, x_perm_stride, x_len,
                    y, y_stride, coef, eps, use_eps, fwd_out, inv_perm, inv_perm_stride, out_scale, accumulate, flags,
                    w, w_len, workspace, workspace_bytes, err, stream, none);
 }
 
 // One OSEM subset step: jscc_proj's ratio back-projection into w followed, in
-// the same reduction pass, by the EM update of jscc_em_update (c1 = 1, no
-// second weight) on em_x with sensitivity em_s; w is zeroed afterwards when
-// zero_w is set.  em_x may be the projected image x itself.
+// the same reduction pass, by the EM update of jscc_em_update on em_x with the
+// weight em_c1*w (+ em_c2*em_w2) and sensitivity em_s; w (and em_w2) are
+// zeroed afterwards when zero_w is set.  em_x may be the projected image x.
 extern "C" int jscc_proj_em(const void* mat, int dtype, int64_t ld, int64_t n_cols, const int32_t* rows,
                             int64_t n_rows, int64_t row_base, const float* row_scale, const float* x,
                             const int64_t* x_perm, int64_t x_perm_stride, int64_t x_len, const float* y,
                             int64_t y_stride, float eps, int use_eps, const int64_t* inv_perm,
                             int64_t inv_perm_stride, float out_scale, int accumulate, int flags, float* w,
-                            int64_t w_len, float* em_x, const float* em_s, int em_mode, float em_eps, int zero_w,
-                            void* workspace, int64_t workspace_bytes, int32_t* err, void* stream) {
+                            int64_t w_len, float* em_x, const float* em_s, int em_mode, float em_eps, float em_c1,
+                            float* em_w2, float em_c2, int zero_w, void* workspace, int64_t workspace_bytes,
+                            int32_t* err, void* stream) {
   JSCC_REQUIRE(em_x != nullptr && em_s != nullptr, "jscc_proj_em: null image or sensitivity");
   JSCC_REQUIRE(em_mode >= 0 && em_mode <= 2, "jscc_proj_em: bad EM mode %d", em_mode);
   JSCC_REQUIRE(n_rows > 0, "jscc_proj_em: empty subset (use jscc_em_update)");
-  const JsccEm em = {em_x, em_s, em_mode, em_eps, zero_w};
+  const JsccEm em = {em_x, em_s, em_mode, em_eps, zero_w, em_c1, em_w2, em_c2};
   return proj_impl(0, mat, dtype, ld, n_cols, rows, n_rows, row_base, row_scale, x, x_perm, x_perm_stride, x_len, y,
                    y_stride, nullptr, eps, use_eps, nullptr, inv_perm, inv_perm_stride, out_scale, accumulate, flags,
                    w, w_len, workspace, workspace_bytes, err, stream, em);

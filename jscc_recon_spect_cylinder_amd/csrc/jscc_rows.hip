@@ -64,19 +64,33 @@ __global__ void __launch_bounds__(kRowThreads) k_arows_normalize(const float* __
 }
 
 // FP8 e4m3fn rows with a per-row scale: stored = fp8(t / (max/448)),
-// row_scale = (max/448) / S so that row_scale * stored ~ t / S.
+// row_scale = (max/448) / S so that row_scale * stored ~ t / S.  Without
+// statistics (a system-matrix row): max = the row's largest |value|, S = 1.
 template <typename T>
 __global__ void __launch_bounds__(kRowThreads) k_quantize_fp8(const T* __restrict__ src, int64_t lds, int64_t P,
                                                               int64_t n, const double* __restrict__ stats,
                                                               uint8_t* __restrict__ dst, int64_t ldd,
                                                               float* __restrict__ row_scale) {
+  __shared__ float red[kRowThreads / JSCC_WAVE];
   const int64_t r = blockIdx.x;
   if (r >= n) return;
-  const double S = stats[r * 4 + 0];
-  const double mx = stats[r * 4 + 3];
+  const T* s = src + r * lds;
+  double S = 1.0, mx;
+  if (stats) {
+    S = stats[r * 4 + 0];
+    mx = stats[r * 4 + 3];
+  } else {  // a matrix row kept unnormalised: scale by its own largest magnitude
+    float m = 0.0f;
+    for (int64_t p = threadIdx.x; p < P; p += kRowThreads) m = fmaxf(m, fabsf(load_as_f32<T>(s, p)));
+    m = jscc_wave_max(m);
+    if (jscc_lane() == 0) red[jscc_wave_id()] = m;
+    __syncthreads();
+    m = red[0];
+    for (int w = 1; w < kRowThreads / JSCC_WAVE; ++w) m = fmaxf(m, red[w]);
+    mx = m;
+  }
   const double q = mx > 0.0 ? mx / 448.0 : 1.0;
   const float inv_q = (float)(1.0 / q);
-  const T* s = src + r * lds;
   uint8_t* d = dst + r * ldd;
   for (int64_t p = threadIdx.x; p < ldd; p += kRowThreads) {
     const float v = p < P ? load_as_f32<T>(s, p) : 0.0f;
@@ -184,7 +198,7 @@ extern "C" int jscc_quantize_fp8_rows(const void* src, int src_dtype, int64_t ld
                                       void* stream) {
   JSCC_REQUIRE(n_rows >= 0 && ld_dst >= n_cols, "jscc_quantize_fp8_rows: bad shape");
   if (n_rows == 0) return 0;
-  JSCC_REQUIRE(src && stats && dst && row_scale, "jscc_quantize_fp8_rows: null pointer");
+  JSCC_REQUIRE(src && dst && row_scale, "jscc_quantize_fp8_rows: null pointer");
   hipStream_t s = (hipStream_t)stream;
   if (src_dtype == JSCC_DT_F32) {
     hipLaunchKernelGGL(k_quantize_fp8<float>, dim3((unsigned)n_rows), dim3(kRowThreads), 0, s, (const float*)src,

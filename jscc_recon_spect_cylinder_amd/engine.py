@@ -85,6 +85,97 @@ class ListModeStore:
 
 
 @dataclass
+class SystemStore:
+    """A single-photon system matrix A [M, P] (detector bins x pixels) held for
+    the fused projection in FP32, BF16 or FP8-e4m3 (per-row scale = the row's
+    largest value / 448), always accumulated in FP32.  The reference keeps A in
+    FP32 (main_plane.py:174-196); the low-precision forms are the engine's
+    precision modes.  Subsets are contiguous bin ranges with split_tensor_even
+    bounds; the data y is indexed by bin, so no copy of A or y is made."""
+    mat: torch.Tensor
+    dtype_c: int
+    n_vox: int
+    n_rows: int
+    row_scale: torch.Tensor | None
+
+    @classmethod
+    def from_dense(cls, A: torch.Tensor, dtype="bf16") -> "SystemStore":
+        ops.require_gpu(A, "sysmat")
+        code = ops.dtype_code(dtype)
+        M, P = A.shape
+        if A.dtype != torch.float32:
+            A = A.float()
+        ld = ops.round_up(P, 64)
+        if code == ops.DT_FP8:
+            A32 = A if (A.stride(1) == 1 and A.stride(0) % 4 == 0) else A.contiguous()
+            codes, scale = ops.quantize_fp8(A32, P, M, None)
+            return cls(codes, code, P, M, scale)
+        tdt = torch.float32 if code == ops.DT_F32 else torch.bfloat16
+        mat = torch.zeros((M, ld), dtype=tdt, device=A.device)
+        mat[:, :P] = A
+        return cls(mat, code, P, M, None)
+
+    def subset_bounds(self, k: int):
+        return [(self.n_rows * i // k, self.n_rows * (i + 1) // k) for i in range(k)]
+
+    def column_sums(self) -> torch.Tensor:
+        """s = A^T 1 (the sensitivity of main_plane.py:320-323 for explicitly stacked rotations)."""
+        w = torch.zeros(self.n_vox, dtype=torch.float32, device=self.mat.device)
+        ops.proj(ops.MODE_ONES, self.mat, dtype_c=self.dtype_c, n_cols=self.n_vox, n_rows=self.n_rows,
+                 row_scale=self.row_scale, w=w)
+        return w
+
+    def forward(self, x: torch.Tensor) -> torch.Tensor:
+        f = torch.empty(self.n_rows, dtype=torch.float32, device=self.mat.device)
+        ops.proj(ops.MODE_FWD, self.mat, dtype_c=self.dtype_c, n_cols=self.n_vox, n_rows=self.n_rows,
+                 row_scale=self.row_scale, x=x, fwd_out=f)
+        return f
+
+    def subset_weight(self, x, w, lo, hi, y, eps=None, scale=1.0):
+        """w (+)= scale * A_s^T (y_s / A_s x) for bins [lo, hi) (get_weight_single, recon_osem_plane.py:28-30)."""
+        if hi <= lo:
+            return
+        ops.proj(ops.MODE_RATIO, self.mat, dtype_c=self.dtype_c, n_cols=self.n_vox, row_base=lo, n_rows=hi - lo,
+                 row_scale=self.row_scale, x=x, y=y, y_by_row=True, eps=eps, out_scale=scale, accumulate=True, w=w)
+
+    def osem(self, x, y, s, n_iter, n_subsets, eps=None, em_mode: int = ops.EM_PLAIN) -> torch.Tensor:
+        """SC OSEM over bin subsets (osem_bin_mode with one rotation block per subset); x updated in place."""
+        w = torch.zeros_like(x)
+        for _ in range(n_iter):
+            for lo, hi in self.subset_bounds(n_subsets):
+                if hi > lo:
+                    ops.proj(ops.MODE_RATIO, self.mat, dtype_c=self.dtype_c, n_cols=self.n_vox, row_base=lo,
+                             n_rows=hi - lo, row_scale=self.row_scale, x=x, y=y, y_by_row=True, eps=eps,
+                             accumulate=True, w=w, em=(x, s, em_mode, 1e-12, True))
+        return x
+
+
+def joint_osem(a: SystemStore, y: torch.Tensor, t: "ListModeStore", x: torch.Tensor, s_joint: torch.Tensor,
+               alpha: float, n_iter: int, n_subsets: int, eps=None, em_mode: int = ops.EM_PLAIN) -> torch.Tensor:
+    """JSCC-SD OSEM (osem_joint_mode, recon_osem_plane.py:100-125): per subset
+    w = (2 - alpha) * T_s^T(1/T_s x) + alpha * A_s^T(y_s/A_s x), x <- x * w / s_joint.
+    Subset k takes the k-th split_tensor_even range of the bins and of the
+    kept events.  Two launches per subset: the A pass accumulates the
+    single-photon weight, the T pass finishes the Compton weight and applies the
+    combined update in its reduction (jscc_proj_em); x is updated in place."""
+    ws = torch.zeros_like(x)
+    wc = torch.zeros_like(x)
+    a_b = a.subset_bounds(n_subsets)
+    t_b = t.subset_bounds(n_subsets)
+    for _ in range(n_iter):
+        for (alo, ahi), (tlo, thi) in zip(a_b, t_b):
+            a.subset_weight(x, ws, alo, ahi, y, eps=eps)
+            if thi > tlo:
+                ops.proj(ops.MODE_RATIO, t.mat, dtype_c=t.dtype_c, n_cols=t.n_vox, rows=t.kept, row_base=tlo,
+                         n_rows=thi - tlo, row_scale=t.row_scale, x=x, eps=eps, accumulate=True, w=wc,
+                         em=(x, s_joint, em_mode, 1e-12, True, 2.0 - alpha, ws, alpha))
+            else:
+                ops.em_update(x, wc, s_joint, mode=em_mode, c1=2.0 - alpha, w2=ws, c2=alpha, zero_after=wc)
+                ws.zero_()
+    return x
+
+
+@dataclass
 class LazyListStore:
     """List-mode data kept as per-event cone parameters (64 B per event) instead of
     T: every projection re-evaluates the rows in registers (jscc_cone_proj).

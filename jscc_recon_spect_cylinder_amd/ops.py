@@ -278,14 +278,14 @@ def arows_normalize(A: torch.Tensor, arow: torch.Tensor, n_out: int) -> torch.Te
     return dst
 
 
-def quantize_fp8(src: torch.Tensor, P: int, n_rows: int, stats: torch.Tensor):
-    """FP8 e4m3fn copy of the rows plus per-row scales (scale * code ~ t / S)."""
+def quantize_fp8(src: torch.Tensor, P: int, n_rows: int, stats: torch.Tensor | None):
+    """FP8 e4m3fn copy of the rows plus per-row scales (scale * code ~ t / S; stats None: scale * code ~ row)."""
     dev = src.device
     ld = round_up(P, 64)
     dst = torch.empty((max(n_rows, 1), ld), dtype=torch.uint8, device=dev)
     scale = torch.empty(max(n_rows, 1), dtype=torch.float32, device=dev)
     code = DT_F32 if src.dtype == torch.float32 else DT_BF16
-    rc = _lib.lib().jscc_quantize_fp8_rows(src.data_ptr(), code, src.stride(0), P, n_rows, stats.data_ptr(),
+    rc = _lib.lib().jscc_quantize_fp8_rows(src.data_ptr(), code, src.stride(0), P, n_rows, _p(stats),
                                            dst.data_ptr(), ld, scale.data_ptr(), stream_of(dev))
     _lib.check(rc, "jscc_quantize_fp8_rows")
     return dst, scale
@@ -387,9 +387,10 @@ def proj(mode: int, mat: torch.Tensor, *, dtype_c: int | None = None, n_cols: in
          sanitize: bool = False, em: tuple | None = None) -> None:
     """Launch jscc_proj on the current stream (see include/jscc_hip.h).
 
-    em = (x_em, s, em_mode, em_eps, zero_w) (mode MODE_RATIO only) runs
-    jscc_proj_em instead: the EM update of x_em with sensitivity s is applied
-    in the same reduction pass that finishes w."""
+    em = (x_em, s, em_mode, em_eps, zero_w[, c1, w2, c2]) (mode MODE_RATIO
+    only) runs jscc_proj_em instead: the EM update of x_em with the weight
+    c1*w (+ c2*w2) and sensitivity s is applied in the same reduction pass
+    that finishes w."""
     dev = mat.device
     if dtype_c is None:
         dtype_c = dtype_code(mat.dtype) if mat.dtype != torch.uint8 else DT_FP8
@@ -431,10 +432,11 @@ def proj(mode: int, mat: torch.Tensor, *, dtype_c: int | None = None, n_cols: in
     xl = x.shape[0] if x is not None else 0
     flags = (1 if y_by_row else 0) | (2 if sanitize else 0)
     if em is not None:
-        x_em, s_em, em_mode, em_eps, zero_w = em
+        x_em, s_em, em_mode, em_eps, zero_w = em[:5]
+        c1, w2, c2 = em[5:] if len(em) > 5 else (1.0, None, 0.0)
         if mode != MODE_RATIO:
             raise ValueError("the EM epilogue follows the ratio back-projection only")
-        for t in (x_em, s_em):
+        for t in (x_em, s_em) + ((w2,) if w2 is not None else ()):
             if (t.dtype != torch.float32 or not t.is_contiguous() or t.device != dev or w is None
                     or t.numel() < w.shape[0]):
                 raise ValueError("EM operands must be contiguous float32 covering the weight, on the matrix's device")
@@ -443,7 +445,8 @@ def proj(mode: int, mat: torch.Tensor, *, dtype_c: int | None = None, n_cols: in
             x_perm.stride(0) if x_perm is not None else 0, xl, _p(y), y.stride(0) if y is not None else 0,
             float(eps) if eps is not None else 0.0, 1 if eps is not None else 0, _p(inv_perm),
             inv_perm.stride(0) if inv_perm is not None else 0, float(out_scale), 1 if accumulate else 0, flags,
-            _p(w), wl, x_em.data_ptr(), s_em.data_ptr(), int(em_mode), float(em_eps), 1 if zero_w else 0,
+            _p(w), wl, x_em.data_ptr(), s_em.data_ptr(), int(em_mode), float(em_eps), float(c1), _p(w2), float(c2),
+            1 if zero_w else 0,
             ws.data_ptr(), ws.numel(), err_word(dev).data_ptr(), stream_of(dev))
         _lib.check(rc, "jscc_proj_em")
         return
