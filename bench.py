@@ -41,6 +41,10 @@ def parse():
     ap.add_argument("--subsets", type=int, default=8)
     ap.add_argument("--cpu-sample", type=int, default=12000, help="events in the CPU-baseline sample (0: skip)")
     ap.add_argument("--skip-fp32", action="store_true")
+    ap.add_argument("--workload", default="jsccd", choices=["jsccd", "sc", "jsccsd", "grad"],
+                    help="jsccd (default, the reported metric): JSCCD list mode, 2e5 events; sc: single-photon "
+                         "OSEM, A 5120x4096; jsccsd: joint mode, A 81920x16384 + 1e6 events, FP32/BF16/FP8; "
+                         "grad: Poisson log-likelihood gradient through the no-store cone operator")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (HBM counter calibration) and exit")
     return ap.parse_args()
@@ -97,6 +101,272 @@ def cpu_baseline(cl, n_sample, n_iter, n_sub, sigma, chunk=1000):
                       f"to T build + {n_iter} iterations per event", "host_cpu": _cpu_model()}
 
 
+# --------------------------------------------------------------------------- other workloads
+def _timed(fn, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        out = fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps, out
+
+
+def _rel(a, b):
+    a = np.asarray(a, np.float64).ravel()
+    b = np.asarray(b, np.float64).ravel()
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def cpu_baseline_sc(pb, n_iter):
+    """The numpy oracle (FP64, one thread): the full SC OSEM run; also the reference image."""
+    sys.path.insert(0, HERE)
+    from threadpoolctl import threadpool_limits
+    from oracle import osem
+    A = pb.A.astype(np.float64)
+    s = A.sum(0)
+    x = np.ones(A.shape[1])
+    with threadpool_limits(limits=1):
+        t0 = time.perf_counter()
+        for _ in range(n_iter):
+            for a, b in pb.subsets:
+                x = osem.em_plain(x, osem.weight_single(A[a:b], pb.y[a:b].astype(np.float64), x), s)
+        dt = time.perf_counter() - t0
+    return {"value": n_iter / dt, "unit": "iter/s", "cores": 1, "kind": "port",
+            "sample": f"numpy FP64 oracle (1 thread), the full {n_iter}-iteration x {len(pb.subsets)}-subset run",
+            "host_cpu": _cpu_model()}, x
+
+
+def run_sc(args):
+    """Single-photon OSEM (osem_bin_mode, one subset per rotation block): A 5120 x 4096 (32 x 128 polar grid,
+    5 rotation blocks x 1024 bins, density 0.3, values exp(N(-9, 1.5))), 10 iterations x 5 subsets."""
+    import torch
+    from jscc_recon_spect_cylinder_amd import _lib, synth
+    from jscc_recon_spect_cylinder_amd.engine import SystemStore
+    dev = torch.device("cuda:0")
+    lib = _lib.load()
+    pb = synth.sc_osem()
+    M, P = pb.A.shape
+    n_iter, K = 10, len(pb.subsets)
+    A = torch.from_numpy(pb.A).to(dev)
+    y = torch.from_numpy(pb.y).to(dev)
+    res = {}
+    imgs = {}
+    for name in ("fp32", "bf16", "fp8"):
+        a = SystemStore.from_dense(A, name)
+        s = a.column_sums()
+        _prof_read(lib, _lib.PROF_PROJ)
+        lib.jscc_prof_enable(1)
+        dt, x = _timed(lambda: a.osem(torch.ones(P, device=dev), y, s, n_iter, K), args.steps, args.warmup)
+        lib.jscc_prof_enable(0)
+        c, t = _prof_read(lib, _lib.PROF_PROJ)
+        b = {"fp32": 4, "bf16": 2, "fp8": 1}[name]
+        sub_bytes = M / K * P * b
+        res[name] = {"iter_s": round(n_iter / dt, 2), "ms_per_run": round(dt * 1e3, 3),
+                     "subset_launch_us": round(t / max(c, 1) * 1e3, 2),
+                     "subset_GBs": round(sub_bytes / (t / max(c, 1) * 1e-3) / 1e9, 1)}
+        imgs[name] = x.cpu().numpy()
+    cpu, xref = cpu_baseline_sc(pb, n_iter)
+    for name in res:
+        res[name]["rel_err_vs_fp64_oracle"] = _rel(imgs[name], xref)
+    return {"metric": "SC OSEM iter/s (osem_bin_mode, A 5120x4096 FP32, 10 iterations x 5 subsets)",
+            "value": res["fp32"]["iter_s"], "unit": "iter/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": res["fp32"]["ms_per_run"], "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic (seeded)",
+            "config": {"workload": "SC OSEM: 32x128 polar grid, A 5120x4096 (5 rotation blocks x 1024 bins), "
+                                   "10 iterations x 5 subsets; A (84 MB FP32) is Infinity-Cache resident"},
+            "components": res, "cpu_baseline": cpu}
+
+
+def _big_sysmat(M, P, seed, dev, density=0.3, mu=-9.0, sd=1.5, chunk=4096):
+    """Dense A [M, P] with the single-photon value law, drawn on the device in row chunks (synthetic data)."""
+    import torch
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    A = torch.empty((M, P), dtype=torch.float32, device=dev)
+    for lo in range(0, M, chunk):
+        hi = min(lo + chunk, M)
+        v = torch.randn((hi - lo, P), generator=g, device=dev).mul_(sd).add_(mu).exp_()
+        v.mul_(torch.rand((hi - lo, P), generator=g, device=dev) < density)
+        A[lo:hi] = v
+    return A
+
+
+def _noise(n):
+    """The smearing draws (host generator, like the reference's CPU torch.randn calls)."""
+    import torch
+    g = torch.Generator().manual_seed(7)
+    return torch.randn(n, generator=g), torch.randn(n, generator=g)
+
+
+def cpu_baseline_jsccsd(A, y, cl, keep_idx, n_iter, K, alpha, sigma, noise):
+    """The numpy oracle (FP64, one thread) of osem_joint_mode on the reduced instance: T of the kept events
+    built by the oracle in chunks, then n_iter joint iterations.  Returns the timing and the FP64 image."""
+    sys.path.insert(0, HERE)
+    from threadpoolctl import threadpool_limits
+    from oracle import cone, osem
+    from jscc_recon_spect_cylinder_amd import synth
+    tmax, tmin, tsum = synth.energy_thresholds(cl.e0)
+    with threadpool_limits(limits=1):
+        t0 = time.perf_counter()
+        ev = cone.prepare_events(cl.list_origin, cl.detector, cl.e0, 0.02, tmax, tmin, tsum, noise[0], noise[1])
+        sel = np.flatnonzero(np.isin(ev["idx"], keep_idx))
+        blocks = []
+        for lo in range(0, sel.size, 1000):
+            ii = sel[lo:lo + 1000]
+            th = cone.theta_from_e1(ev["e1"][ii], cl.e0)
+            kn = cone.klein_nishina(ev["e1"][ii], cl.e0)
+            tc = cone.cone_tc(ev["pos1"][ii], ev["pos2"][ii], None, None, th, kn, None, None, cl.coor,
+                              fixed_sigma=sigma)
+            blocks.append(tc / tc.sum(1, keepdims=True))
+        T = np.concatenate(blocks)
+        t_build = time.perf_counter() - t0
+        s_j = alpha * A.sum(0) + (2 - alpha) * T.sum(0)
+        x = np.ones(A.shape[1])
+        ab = [(A.shape[0] * i // K, A.shape[0] * (i + 1) // K) for i in range(K)]
+        tb = [(T.shape[0] * i // K, T.shape[0] * (i + 1) // K) for i in range(K)]
+        t0 = time.perf_counter()
+        for _ in range(n_iter):
+            for (alo, ahi), (tlo, thi) in zip(ab, tb):
+                ws = osem.weight_single(A[alo:ahi], y[alo:ahi], x)
+                wc = osem.weight_compton(T[tlo:thi], x)
+                x = x * ((2 - alpha) * wc + alpha * ws) / s_j
+        t_iter = (time.perf_counter() - t0) / n_iter
+    return {"value": 1.0 / t_iter, "unit": "iter/s", "cores": 1, "kind": "port",
+            "sample": f"numpy FP64 oracle (1 thread) on the reduced instance (A {A.shape[0]}x{A.shape[1]}, "
+                      f"{T.shape[0]} kept events): {n_iter} joint iterations x {K} subsets, {t_iter:.2f} s per "
+                      f"iteration (T build {t_build:.1f} s not counted)", "host_cpu": _cpu_model()}, x
+
+
+def run_jsccsd(args):
+    """Joint JSCCSD OSEM with A 81920 x 16384 (20 rotations x 4096 bins, single-photon value law) and a 1e6-event
+    list (64 x 256 grid, sigma 3 deg), alpha = 0.5, A and T stored in FP32 / BF16 / FP8 (per-row scales), FP32
+    accumulation; one iteration = 10 subsets (8192 bins + 1/10 of the kept events each).  The image error is
+    measured against the FP64 oracle on a reduced instance (A 8192 x 16384, 1e4 events, 10 iterations x 4
+    subsets) computed in the CPU-baseline leg."""
+    import torch
+    from jscc_recon_spect_cylinder_amd import _lib, synth
+    from jscc_recon_spect_cylinder_amd.engine import ListModeStore, SystemStore, joint_osem
+    from jscc_recon_spect_cylinder_amd.process_list_plane_strict import build_compton_rows
+    dev = torch.device("cuda:0")
+    lib = _lib.load()
+    alpha, sigma = 0.5, float(np.deg2rad(3.0))
+
+    def problem(M, n_ev, seed):
+        cl = synth.compton_list(n_events=n_ev, n_rings=64, n_angles=256, seed=seed)
+        P = cl.coor.shape[0]
+        A = _big_sysmat(M, P, seed, dev)
+        x_true = torch.from_numpy(cl.x_true.astype(np.float32)).to(dev)
+        a32 = SystemStore.from_dense(A, "fp32")
+        f = a32.forward(x_true)
+        y = torch.poisson(f * (1e6 * M / 81920 / float(f.sum())))
+        return cl, A, y
+
+    def rows_of(cl, dt):
+        tmax, tmin, tsum = synth.energy_thresholds(cl.e0)
+        n1, n2 = _noise(cl.list_origin.shape[0])
+        return build_compton_rows(None, torch.from_numpy(cl.detector).to(dev), torch.from_numpy(cl.coor).to(dev),
+                                  torch.from_numpy(cl.list_origin).to(dev), 0.0, 0.0, cl.e0, 0.02, tmax, tmin, tsum,
+                                  dev, noise=(n1.to(dev), n2.to(dev)), sigma_model="fixed", fixed_sigma=sigma,
+                                  store_dtype=dt)
+
+    def stores(A, rows32, rows16, name):
+        a = SystemStore.from_dense(A, name)
+        t = ListModeStore.from_rows(rows32 if name == "fp32" else rows16, name)
+        return a, t
+
+    # full size: timing
+    n_ev = args.events if args.events != 200_000 else 1_000_000
+    cl, A, y = problem(81920, n_ev, 20260333)
+    P = A.shape[1]
+    res = {}
+    for name in ("bf16", "fp8", "fp32"):
+        rows = rows_of(cl, torch.float32 if name == "fp32" else torch.bfloat16)
+        a, t = stores(A, rows, rows, name)
+        del rows
+        s_j = (alpha * a.column_sums() + (2 - alpha) * t.backproject_ones()).contiguous()
+        K = 10
+        dt, _ = _timed(lambda: joint_osem(a, y, t, torch.ones(P, device=dev), s_j, alpha, 1, K), args.steps,
+                       args.warmup)
+        b = {"fp32": 4, "bf16": 2, "fp8": 1}[name]
+        nbytes = (a.n_rows + t.n_kept) * P * b
+        res[name] = {"iter_s": round(1.0 / dt, 3), "ms_per_iter": round(dt * 1e3, 3), "kept_events": t.n_kept,
+                     "bytes_per_iter": nbytes, "GBs": round(nbytes / dt / 1e9, 1),
+                     "hbm_frac": round(nbytes / dt / 1e9 / HBM_PEAK_GBS, 4)}
+        del a, t
+        torch.cuda.empty_cache()
+    del A, y
+    torch.cuda.empty_cache()
+    # reduced instance: error against the FP64 oracle
+    cl_r, A_r, y_r = problem(8192, 10_000, 20260334)
+    rows32, rows16 = rows_of(cl_r, torch.float32), rows_of(cl_r, torch.bfloat16)
+    imgs = {}
+    for name in ("fp32", "bf16", "fp8"):
+        a, t = stores(A_r, rows32, rows16, name)
+        s_j = (alpha * a.column_sums() + (2 - alpha) * t.backproject_ones()).contiguous()
+        imgs[name] = joint_osem(a, y_r, t, torch.ones(P, device=dev), s_j, alpha, 10, 4).cpu().numpy()
+    cpu, xref = cpu_baseline_jsccsd(A_r.cpu().numpy().astype(np.float64), y_r.cpu().numpy().astype(np.float64), cl_r,
+                                    rows32.kept_event_index().cpu().numpy(), 10, 4, alpha, sigma,
+                                    [v.numpy() for v in _noise(cl_r.list_origin.shape[0])])
+    for name in imgs:
+        res[name]["rel_err_vs_fp64_oracle_reduced"] = _rel(imgs[name], xref)
+    return {"metric": "JSCCSD joint OSEM iter/s (A 81920x16384 + 1e6-event T, BF16 / FP8 / FP32 storage)",
+            "value": res["bf16"]["iter_s"], "unit": "iter/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": res["bf16"]["ms_per_iter"], "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "bf16 storage, fp32 accumulate",
+            "data": "synthetic (seeded)",
+            "config": {"workload": f"JSCCSD joint: A 81920x16384, {n_ev} events on 64x256, alpha 0.5, 1 iteration "
+                                   "= 10 subsets"},
+            "components": res, "cpu_baseline": cpu}
+
+
+def run_grad(args):
+    """Gradient of the list-mode Poisson log-likelihood w.r.t. the image and the angular blur sigma through the
+    no-store cone operator (T recomputed in the backward), on the 2e5-event list at sigma = 3 deg."""
+    import torch
+    from jscc_recon_spect_cylinder_amd import _lib, ops, synth
+    from jscc_recon_spect_cylinder_amd.differentiable import cone_project, list_loglik
+    from jscc_recon_spect_cylinder_amd.engine import LazyListStore
+    from jscc_recon_spect_cylinder_amd.process_list_plane_strict import build_compton_rows
+    dev = torch.device("cuda:0")
+    lib = _lib.load()
+    sigma0 = float(np.deg2rad(3.0))
+    cl = synth.compton_list(n_events=args.events, n_rings=64, n_angles=256)
+    tmax, tmin, tsum = synth.energy_thresholds(cl.e0)
+    rows = build_compton_rows(None, torch.from_numpy(cl.detector).to(dev), torch.from_numpy(cl.coor).to(dev),
+                              torch.from_numpy(cl.list_origin).to(dev), 0.0, 0.0, cl.e0, 0.02, tmax, tmin, tsum, dev,
+                              noise="none", sigma_model="fixed", fixed_sigma=sigma0, store_dtype=None)
+    lz = LazyListStore.from_rows(rows, sigma_model=ops.SIG_FIXED, fixed_sigma=sigma0)
+    x0 = torch.from_numpy((cl.x_true + 0.5).astype(np.float32)).to(dev)
+
+    def grad():
+        x = x0.clone().requires_grad_(True)
+        sig = torch.tensor(sigma0, dtype=torch.float64, device=dev, requires_grad=True)
+        L = list_loglik(cone_project(x, sig, lz))
+        L.backward()
+        return x.grad, sig.grad
+
+    for _ in range(args.warmup):
+        grad()
+    torch.cuda.synchronize()
+    _prof_read(lib, _lib.PROF_CONE_PROJ)
+    lib.jscc_prof_enable(1)  # the timer is per thread: it sees the forward passes (autograd runs the backward
+    dt, (gx, gs) = _timed(grad, args.steps, 0)  # on its own device thread)
+    lib.jscc_prof_enable(0)
+    c, t = _prof_read(lib, _lib.PROF_CONE_PROJ)
+    return {"metric": "list-mode log-likelihood gradient (d/dx, d/dsigma) per second, T recomputed",
+            "value": round(1.0 / dt, 3), "unit": "gradients/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "fp32", "data": "synthetic (seeded)",
+            "config": {"workload": f"{lz.n_kept} kept events x {cl.coor.shape[0]} voxels, sigma 3 deg, forward + "
+                                   "backward (x and sigma) with T recomputed"},
+            "components": {"forward_cone_proj_ms_avg": round(t / max(c, 1), 3),
+                           "pairs_per_s_per_pass": round(lz.n_kept * cl.coor.shape[0] / (t / max(c, 1) * 1e-3), 1),
+                           "grad_sigma": float(gs), "grad_x_norm": float(gx.norm())}}
+
+
 class _null:
     def __enter__(self):
         return self
@@ -142,6 +412,9 @@ def main():
         sys.exit(2)
     if args.copy_calibration > 0:
         copy_calibration(args.copy_calibration)
+        return
+    if args.workload != "jsccd":
+        print(json.dumps({"sc": run_sc, "jsccsd": run_jsccsd, "grad": run_grad}[args.workload](args)))
         return
     import torch
     from jscc_recon_spect_cylinder_amd import _lib, synth
