@@ -40,7 +40,7 @@ def parse():
     ap.add_argument("--iters", type=int, default=10)
     ap.add_argument("--subsets", type=int, default=8)
     ap.add_argument("--cpu-sample", type=int, default=12000, help="events in the CPU-baseline sample (0: skip)")
-    ap.add_argument("--skip-fp32", action="store_true")
+    ap.add_argument("--skip-fp32", action="store_true", help="time the BF16 path only")
     ap.add_argument("--workload", default="jsccd", choices=["jsccd", "sc", "jsccsd", "grad"],
                     help="jsccd (default, the reported metric): JSCCD list mode, 2e5 events; sc: single-photon "
                          "OSEM, A 5120x4096; jsccsd: joint mode, A 81920x16384 + 1e6 events, FP32/BF16/FP8; "
@@ -154,7 +154,7 @@ def run_sc(args):
     y = torch.from_numpy(pb.y).to(dev)
     res = {}
     imgs = {}
-    for name in ("fp32", "bf16", "fp8"):
+    for name in ("fp32", "bf16", "fp16", "fp8"):
         a = SystemStore.from_dense(A, name)
         s = a.column_sums()
         _prof_read(lib, _lib.PROF_PROJ)
@@ -162,7 +162,7 @@ def run_sc(args):
         dt, x = _timed(lambda: a.osem(torch.ones(P, device=dev), y, s, n_iter, K), args.steps, args.warmup)
         lib.jscc_prof_enable(0)
         c, t = _prof_read(lib, _lib.PROF_PROJ)
-        b = {"fp32": 4, "bf16": 2, "fp8": 1}[name]
+        b = {"fp32": 4, "bf16": 2, "fp16": 2, "fp8": 1}[name]
         sub_bytes = M / K * P * b
         res[name] = {"iter_s": round(n_iter / dt, 2), "ms_per_run": round(dt * 1e3, 3),
                      "subset_launch_us": round(t / max(c, 1) * 1e3, 2),
@@ -274,7 +274,7 @@ def run_jsccsd(args):
 
     def stores(A, rows32, rows16, name):
         a = SystemStore.from_dense(A, name)
-        t = ListModeStore.from_rows(rows32 if name == "fp32" else rows16, name)
+        t = ListModeStore.from_rows(rows16 if name == "bf16" else rows32, name)
         return a, t
 
     # full size: timing
@@ -282,15 +282,15 @@ def run_jsccsd(args):
     cl, A, y = problem(81920, n_ev, 20260333)
     P = A.shape[1]
     res = {}
-    for name in ("bf16", "fp8", "fp32"):
-        rows = rows_of(cl, torch.float32 if name == "fp32" else torch.bfloat16)
+    for name in ("bf16", "fp16", "fp8", "fp32"):
+        rows = rows_of(cl, torch.bfloat16 if name == "bf16" else torch.float32)
         a, t = stores(A, rows, rows, name)
         del rows
         s_j = (alpha * a.column_sums() + (2 - alpha) * t.backproject_ones()).contiguous()
         K = 10
         dt, _ = _timed(lambda: joint_osem(a, y, t, torch.ones(P, device=dev), s_j, alpha, 1, K), args.steps,
                        args.warmup)
-        b = {"fp32": 4, "bf16": 2, "fp8": 1}[name]
+        b = {"fp32": 4, "bf16": 2, "fp16": 2, "fp8": 1}[name]
         nbytes = (a.n_rows + t.n_kept) * P * b
         res[name] = {"iter_s": round(1.0 / dt, 3), "ms_per_iter": round(dt * 1e3, 3), "kept_events": t.n_kept,
                      "bytes_per_iter": nbytes, "GBs": round(nbytes / dt / 1e9, 1),
@@ -303,7 +303,7 @@ def run_jsccsd(args):
     cl_r, A_r, y_r = problem(8192, 10_000, 20260334)
     rows32, rows16 = rows_of(cl_r, torch.float32), rows_of(cl_r, torch.bfloat16)
     imgs = {}
-    for name in ("fp32", "bf16", "fp8"):
+    for name in ("fp32", "bf16", "fp16", "fp8"):
         a, t = stores(A_r, rows32, rows16, name)
         s_j = (alpha * a.column_sums() + (2 - alpha) * t.backproject_ones()).contiguous()
         imgs[name] = joint_osem(a, y_r, t, torch.ones(P, device=dev), s_j, alpha, 10, 4).cpu().numpy()
@@ -312,7 +312,7 @@ def run_jsccsd(args):
                                     [v.numpy() for v in _noise(cl_r.list_origin.shape[0])])
     for name in imgs:
         res[name]["rel_err_vs_fp64_oracle_reduced"] = _rel(imgs[name], xref)
-    return {"metric": "JSCCSD joint OSEM iter/s (A 81920x16384 + 1e6-event T, BF16 / FP8 / FP32 storage)",
+    return {"metric": "JSCCSD joint OSEM iter/s (A 81920x16384 + 1e6-event T, BF16 / FP16 / FP8 / FP32 storage)",
             "value": res["bf16"]["iter_s"], "unit": "iter/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": res["bf16"]["ms_per_iter"], "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None, "dtype": "bf16 storage, fp32 accumulate",
@@ -444,8 +444,8 @@ def main():
         return st, x
 
     results = {}
-    for name, sd in (("bf16", torch.bfloat16), ("fp32", torch.float32)):
-        if name == "fp32" and args.skip_fp32:
+    for name, sd in (("bf16", torch.bfloat16), ("fp32", torch.float32), ("fp16", torch.float32)):
+        if name != "bf16" and args.skip_fp32:
             continue
         for _ in range(args.warmup):
             step(sd, name)
@@ -471,7 +471,7 @@ def main():
         st.osem(xi, s, args.iters, args.subsets)
         torch.cuda.synchronize()
         osem_s = (time.perf_counter() - t2) / args.iters
-        b = {"bf16": 2, "fp32": 4}[name]
+        b = {"bf16": 2, "fp32": 4, "fp16": 2}[name]
         # algorithmic bytes of one subset launch: its rows read once (+ x read, w written)
         subset_bytes = st.n_kept / args.subsets * P * b + 2 * P * 4
         proj_ms = ptot / max(pcnt, 1)
@@ -496,6 +496,7 @@ def main():
     if "fp32" in results:
         f = results["fp32"]["img"]
         rel_err = float(np.linalg.norm(bf["img"] - f) / np.linalg.norm(f))
+        results["fp16"]["rel_err_vs_fp32_image"] = float(np.linalg.norm(results["fp16"]["img"] - f) / np.linalg.norm(f))
     traffic = _traffic("bf16")
     roof = {"kernel": "k_proj_fused<bf16,2,4,ratio>: one OSEM subset pass T_s^T(1/T_s x)", "bound": "hbm",
             "achieved": round(bf["subset_GBs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -505,7 +506,7 @@ def main():
     value = args.events / (bf["ms_per_step"] * 1e-3) / 1e6
     comp = {}
     for k, r in results.items():
-        comp[k] = {kk: (round(v, 4) if isinstance(v, float) else v) for kk, v in r.items() if kk != "img"}
+        comp[k] = {kk: (float(f"{v:.5g}") if isinstance(v, float) else v) for kk, v in r.items() if kk != "img"}
     out = {
         "metric": "Compton backproj Mevents/s + OSEM iter/s, BF16 vs FP32 path, rel err<=1e-3",
         "value": round(value, 4),

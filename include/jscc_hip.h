@@ -16,7 +16,8 @@
  *   - "err" words are device int32 flags OR-ed by kernels when an index is out
  *     of range (1: crystal number, 2: image gather, 4: weight gather,
  *     8: sensitivity gather).  The host raises IndexError when set.
- *   - storage dtypes: 0 = FP32, 1 = BF16, 2 = FP8 e4m3fn (per-row scale).
+ *   - storage dtypes: 0 = FP32, 1 = BF16, 2 = FP8 e4m3fn (per-row scale),
+ *     3 = FP16 (IEEE half, per-row scale).
  *
  * Each entry point names the reference interface (file:line in
  * lpz7777777/JSCC_Recon_SPECT_Cylinder) whose work it performs.
@@ -113,6 +114,13 @@ int jscc_arows_normalize(const float* sysmat, int64_t lda, int64_t n_cols, const
  * No counterpart in the reference (precision mode of the MI355X engine). */
 int jscc_quantize_fp8_rows(const void* src, int src_dtype, int64_t ld_src, int64_t n_cols, int64_t n_rows,
                            const double* stats, uint8_t* dst, int64_t ld_dst, float* row_scale, void* stream);
+
+/* The same for dst_dtype 2 (FP8 e4m3fn, row max coded as 448) or 3 (IEEE half,
+ * row max coded as 2^14, so values down to 2^-28 of the row max stay normal).
+ * No counterpart in the reference (precision modes of the MI355X engine). */
+int jscc_quantize_rows(const void* src, int src_dtype, int64_t ld_src, int64_t n_cols, int64_t n_rows,
+                       const double* stats, void* dst, int dst_dtype, int64_t ld_dst, float* row_scale,
+                       void* stream);
 
 /* ---- coarse/fine polar-grid operators ----------------------------------- */
 

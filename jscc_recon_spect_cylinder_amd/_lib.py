@@ -37,6 +37,7 @@ SIGNATURES = {
     "jscc_rows_gather_normalize": (c_int, [vp, c_int, c_i64, c_i64, vp, c_i64, vp, c_int, vp, vp, c_i64, c_i64, vp]),
     "jscc_arows_normalize": (c_int, [vp, c_i64, c_i64, vp, c_i64, vp, c_i64, vp]),
     "jscc_quantize_fp8_rows": (c_int, [vp, c_int, c_i64, c_i64, c_i64, vp, vp, c_i64, vp, vp]),
+    "jscc_quantize_rows": (c_int, [vp, c_int, c_i64, c_i64, c_i64, vp, vp, c_int, c_i64, vp, vp]),
     "jscc_rows_csr_map": (c_int, [vp, c_i64, c_i64, vp, vp, vp, c_i64, vp, c_i64, vp]),
     "jscc_materialize_rows": (c_int, [vp, c_i64, c_i64, c_i64, vp, vp, vp, c_i64, vp, c_i64, c_i64, vp, c_i64, c_int,
                                       vp, vp]),

@@ -51,6 +51,7 @@ enum {
   JSCC_DT_F32 = 0,
   JSCC_DT_BF16 = 1,
   JSCC_DT_FP8 = 2,  // OCP e4m3fn, per-row scale carried separately
+  JSCC_DT_FP16 = 3, // IEEE half, per-row scale carried separately (row max at 2^14)
 };
 
 // ---------------------------------------------------------------------------

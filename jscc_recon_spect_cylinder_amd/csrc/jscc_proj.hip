@@ -31,6 +31,7 @@ template <int DT> struct Vec;
 template <> struct Vec<JSCC_DT_F32> { static constexpr int N = 4; using T = jscc_f4v; };
 template <> struct Vec<JSCC_DT_BF16> { static constexpr int N = 8; using T = jscc_u4v; };
 template <> struct Vec<JSCC_DT_FP8> { static constexpr int N = 16; using T = jscc_u4v; };
+template <> struct Vec<JSCC_DT_FP16> { static constexpr int N = 8; using T = jscc_u4v; };
 
 template <int DT>
 __device__ __forceinline__ void decode(const typename Vec<DT>::T& raw, float* v);
@@ -45,6 +46,16 @@ __device__ __forceinline__ void decode<JSCC_DT_BF16>(const jscc_u4v& r, float* v
   for (int k = 0; k < 4; ++k) {
     v[2 * k] = __uint_as_float(r[k] << 16);
     v[2 * k + 1] = __uint_as_float(r[k] & 0xffff0000u);
+  }
+}
+// IEEE half: the conversion folds into the consuming FMA (v_fma_mix_f32 selects
+// either half of the register), so an FP16 element costs one instruction
+template <>
+__device__ __forceinline__ void decode<JSCC_DT_FP16>(const jscc_u4v& r, float* v) {
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    v[2 * k] = (float)__builtin_bit_cast(_Float16, (uint16_t)(r[k] & 0xffffu));
+    v[2 * k + 1] = (float)__builtin_bit_cast(_Float16, (uint16_t)(r[k] >> 16));
   }
 }
 // gfx950 converts OCP e4m3fn in hardware, two bytes per instruction
@@ -110,7 +121,7 @@ template <int DT, int NCH, int R>
 __device__ __forceinline__ void fetch_rows(const ProjArgs& a, const RowMeta& meta, int64_t i0, int64_t c0,
                                            int64_t c1, int nvec, int tid, RowBuf<DT, NCH, R>& b) {
   using VT = typename Vec<DT>::T;
-  constexpr int EB = DT == JSCC_DT_F32 ? 4 : (DT == JSCC_DT_BF16 ? 2 : 1);
+  constexpr int EB = DT == JSCC_DT_F32 ? 4 : (DT == JSCC_DT_FP8 ? 1 : 2);
   // The load offsets pass through an empty volatile asm, which stays ordered
   // after the one closing the previous block's accumulation: the next
   // buffer's loads cannot be hoisted above the current rows' arithmetic (which
@@ -566,12 +577,12 @@ int dispatch_nch(int mode, const ProjArgs& a, int G, hipStream_t s) {
 }  // namespace
 
 extern "C" int jscc_proj_vec_width(int dtype) {
-  return dtype == JSCC_DT_F32 ? 4 : (dtype == JSCC_DT_BF16 ? 8 : (dtype == JSCC_DT_FP8 ? 16 : -1));
+  return dtype == JSCC_DT_F32 ? 4 : ((dtype == JSCC_DT_BF16 || dtype == JSCC_DT_FP16) ? 8 : (dtype == JSCC_DT_FP8 ? 16 : -1));
 }
 
 extern "C" int jscc_proj_max_cols(int dtype) {
   // register-resident accumulators: up to 8 (FP8: 4) vectors per thread
-  return dtype == JSCC_DT_F32 ? 8 * kProjThreads * 4 : (dtype == JSCC_DT_BF16 ? 8 * kProjThreads * 8 : 4 * kProjThreads * 16);
+  return dtype == JSCC_DT_F32 ? 8 * kProjThreads * 4 : (dtype == JSCC_DT_FP8 ? 4 * kProjThreads * 16 : 8 * kProjThreads * 8);
 }
 
 extern "C" int jscc_proj_grid(int64_t n_rows) {
@@ -644,6 +655,7 @@ static int proj_impl(int mode, const void* mat, int dtype, int64_t ld, int64_t n
   jscc_prof_begin(tag, s);
   if (dtype == JSCC_DT_F32) rc = dispatch_nch<JSCC_DT_F32>(mode, a, G, s);
   else if (dtype == JSCC_DT_BF16) rc = dispatch_nch<JSCC_DT_BF16>(mode, a, G, s);
+  else if (dtype == JSCC_DT_FP16) rc = dispatch_nch<JSCC_DT_FP16>(mode, a, G, s);
   else rc = dispatch_nch<JSCC_DT_FP8>(mode, a, G, s);
   jscc_prof_end(tag, s);
   if (rc != 0) return rc;

@@ -63,14 +63,15 @@ __global__ void __launch_bounds__(kRowThreads) k_arows_normalize(const float* __
   for (int64_t p = threadIdx.x; p < P; p += kRowThreads) d[p] = (float)((double)a[p] / tot);
 }
 
-// FP8 e4m3fn rows with a per-row scale: stored = fp8(t / (max/448)),
-// row_scale = (max/448) / S so that row_scale * stored ~ t / S.  Without
-// statistics (a system-matrix row): max = the row's largest |value|, S = 1.
-template <typename T>
-__global__ void __launch_bounds__(kRowThreads) k_quantize_fp8(const T* __restrict__ src, int64_t lds, int64_t P,
-                                                              int64_t n, const double* __restrict__ stats,
-                                                              uint8_t* __restrict__ dst, int64_t ldd,
-                                                              float* __restrict__ row_scale) {
+// Low-precision rows with a per-row scale: stored = code(t / q) with
+// q = max / TOP (TOP = 448 for e4m3fn, 2^14 for IEEE half), row_scale = q / S
+// so that row_scale * stored ~ t / S.  Without statistics (a system-matrix
+// row): max = the row's largest |value|, S = 1.
+template <typename T, int OUT>
+__global__ void __launch_bounds__(kRowThreads) k_quantize(const T* __restrict__ src, int64_t lds, int64_t P,
+                                                          int64_t n, const double* __restrict__ stats,
+                                                          void* __restrict__ dst, int64_t ldd,
+                                                          float* __restrict__ row_scale) {
   __shared__ float red[kRowThreads / JSCC_WAVE];
   const int64_t r = blockIdx.x;
   if (r >= n) return;
@@ -79,7 +80,7 @@ __global__ void __launch_bounds__(kRowThreads) k_quantize_fp8(const T* __restric
   if (stats) {
     S = stats[r * 4 + 0];
     mx = stats[r * 4 + 3];
-  } else {  // a matrix row kept unnormalised: scale by its own largest magnitude
+  } else {
     float m = 0.0f;
     for (int64_t p = threadIdx.x; p < P; p += kRowThreads) m = fmaxf(m, fabsf(load_as_f32<T>(s, p)));
     m = jscc_wave_max(m);
@@ -89,14 +90,35 @@ __global__ void __launch_bounds__(kRowThreads) k_quantize_fp8(const T* __restric
     for (int w = 1; w < kRowThreads / JSCC_WAVE; ++w) m = fmaxf(m, red[w]);
     mx = m;
   }
-  const double q = mx > 0.0 ? mx / 448.0 : 1.0;
+  constexpr double kTop = OUT == JSCC_DT_FP8 ? 448.0 : 16384.0;
+  const double q = mx > 0.0 ? mx / kTop : 1.0;
   const float inv_q = (float)(1.0 / q);
-  uint8_t* d = dst + r * ldd;
   for (int64_t p = threadIdx.x; p < ldd; p += kRowThreads) {
-    const float v = p < P ? load_as_f32<T>(s, p) : 0.0f;
-    d[p] = (uint8_t)jscc_f32_to_fp8(v * inv_q);
+    const float v = p < P ? load_as_f32<T>(s, p) * inv_q : 0.0f;
+    if (OUT == JSCC_DT_FP8) {
+      reinterpret_cast<uint8_t*>(dst)[r * ldd + p] = (uint8_t)jscc_f32_to_fp8(v);
+    } else {
+      reinterpret_cast<_Float16*>(dst)[r * ldd + p] = (_Float16)v;  // round to nearest even
+    }
   }
   if (threadIdx.x == 0) row_scale[r] = (float)(q / S);
+}
+
+template <int OUT>
+int launch_quantize(const void* src, int src_dtype, int64_t ld_src, int64_t n_cols, int64_t n_rows,
+                    const double* stats, void* dst, int64_t ld_dst, float* row_scale, hipStream_t s) {
+  if (src_dtype == JSCC_DT_F32) {
+    hipLaunchKernelGGL((k_quantize<float, OUT>), dim3((unsigned)n_rows), dim3(kRowThreads), 0, s, (const float*)src,
+                       ld_src, n_cols, n_rows, stats, dst, ld_dst, row_scale);
+  } else if (src_dtype == JSCC_DT_BF16) {
+    hipLaunchKernelGGL((k_quantize<uint16_t, OUT>), dim3((unsigned)n_rows), dim3(kRowThreads), 0, s,
+                       (const uint16_t*)src, ld_src, n_cols, n_rows, stats, dst, ld_dst, row_scale);
+  } else {
+    jscc_set_error("jscc_quantize_rows: unsupported source dtype %d", src_dtype);
+    return -2;
+  }
+  JSCC_CHECK_LAUNCH();
+  return 0;
 }
 
 // out[r, j] = sum_k val[k] * in[r, col[k]], k in [ptr[j], ptr[j+1])   (row-wise sparse linear map)
@@ -193,25 +215,27 @@ extern "C" int jscc_arows_normalize(const float* A, int64_t lda, int64_t n_cols,
   return 0;
 }
 
+extern "C" int jscc_quantize_rows(const void* src, int src_dtype, int64_t ld_src, int64_t n_cols, int64_t n_rows,
+                                  const double* stats, void* dst, int dst_dtype, int64_t ld_dst, float* row_scale,
+                                  void* stream) {
+  JSCC_REQUIRE(n_rows >= 0 && ld_dst >= n_cols, "jscc_quantize_rows: bad shape");
+  if (n_rows == 0) return 0;
+  JSCC_REQUIRE(src && dst && row_scale, "jscc_quantize_rows: null pointer");
+  JSCC_REQUIRE(n_rows <= 0x7fffffffLL, "jscc_quantize_rows: too many rows");
+  hipStream_t s = (hipStream_t)stream;
+  if (dst_dtype == JSCC_DT_FP8)
+    return launch_quantize<JSCC_DT_FP8>(src, src_dtype, ld_src, n_cols, n_rows, stats, dst, ld_dst, row_scale, s);
+  if (dst_dtype == JSCC_DT_FP16)
+    return launch_quantize<JSCC_DT_FP16>(src, src_dtype, ld_src, n_cols, n_rows, stats, dst, ld_dst, row_scale, s);
+  jscc_set_error("jscc_quantize_rows: unsupported storage dtype %d", dst_dtype);
+  return -2;
+}
+
 extern "C" int jscc_quantize_fp8_rows(const void* src, int src_dtype, int64_t ld_src, int64_t n_cols, int64_t n_rows,
                                       const double* stats, uint8_t* dst, int64_t ld_dst, float* row_scale,
                                       void* stream) {
-  JSCC_REQUIRE(n_rows >= 0 && ld_dst >= n_cols, "jscc_quantize_fp8_rows: bad shape");
-  if (n_rows == 0) return 0;
-  JSCC_REQUIRE(src && dst && row_scale, "jscc_quantize_fp8_rows: null pointer");
-  hipStream_t s = (hipStream_t)stream;
-  if (src_dtype == JSCC_DT_F32) {
-    hipLaunchKernelGGL(k_quantize_fp8<float>, dim3((unsigned)n_rows), dim3(kRowThreads), 0, s, (const float*)src,
-                       ld_src, n_cols, n_rows, stats, dst, ld_dst, row_scale);
-  } else if (src_dtype == JSCC_DT_BF16) {
-    hipLaunchKernelGGL(k_quantize_fp8<uint16_t>, dim3((unsigned)n_rows), dim3(kRowThreads), 0, s,
-                       (const uint16_t*)src, ld_src, n_cols, n_rows, stats, dst, ld_dst, row_scale);
-  } else {
-    jscc_set_error("jscc_quantize_fp8_rows: unsupported dtype %d", src_dtype);
-    return -2;
-  }
-  JSCC_CHECK_LAUNCH();
-  return 0;
+  return jscc_quantize_rows(src, src_dtype, ld_src, n_cols, n_rows, stats, dst, JSCC_DT_FP8, ld_dst, row_scale,
+                            stream);
 }
 
 extern "C" int jscc_rows_csr_map(const float* in, int64_t ld_in, int64_t n_rows, const int32_t* ptr,

@@ -32,15 +32,15 @@ class ListModeStore:
     @classmethod
     def from_rows(cls, rows: ComptonRows, dtype="bf16") -> "ListModeStore":
         code = ops.dtype_code(dtype)
-        if code == ops.DT_FP8:
-            codes, scale = ops.quantize_fp8(rows.store, rows.n_vox, rows.n_sel, rows.stats)
-            return cls(codes, ops.DT_FP8, rows.n_vox, rows.kept, rows.n_kept, scale)
+        if code in (ops.DT_FP8, ops.DT_FP16):
+            codes, scale = ops.quantize_rows(rows.store, rows.n_vox, rows.n_sel, rows.stats, code)
+            return cls(codes, code, rows.n_vox, rows.kept, rows.n_kept, scale)
         if ops.dtype_code(rows.store.dtype) != code:
             raise ValueError("the rows were built in another storage dtype")
         return cls(rows.store, code, rows.n_vox, rows.kept, rows.n_kept, rows.inv_sum)
 
     def nbytes_kept(self) -> int:
-        per = {ops.DT_F32: 4, ops.DT_BF16: 2, ops.DT_FP8: 1}[self.dtype_c]
+        per = {ops.DT_F32: 4, ops.DT_BF16: 2, ops.DT_FP8: 1, ops.DT_FP16: 2}[self.dtype_c]
         return self.n_kept * self.n_vox * per
 
     def subset_bounds(self, k: int):
@@ -87,8 +87,9 @@ class ListModeStore:
 @dataclass
 class SystemStore:
     """A single-photon system matrix A [M, P] (detector bins x pixels) held for
-    the fused projection in FP32, BF16 or FP8-e4m3 (per-row scale = the row's
-    largest value / 448), always accumulated in FP32.  The reference keeps A in
+    the fused projection in FP32, BF16, FP16 (per-row scale = the row's largest
+    value / 2^14) or FP8-e4m3 (per-row scale = largest value / 448), always
+    accumulated in FP32.  The reference keeps A in
     FP32 (main_plane.py:174-196); the low-precision forms are the engine's
     precision modes.  Subsets are contiguous bin ranges with split_tensor_even
     bounds; the data y is indexed by bin, so no copy of A or y is made."""
@@ -106,9 +107,9 @@ class SystemStore:
         if A.dtype != torch.float32:
             A = A.float()
         ld = ops.round_up(P, 64)
-        if code == ops.DT_FP8:
+        if code in (ops.DT_FP8, ops.DT_FP16):
             A32 = A if (A.stride(1) == 1 and A.stride(0) % 4 == 0) else A.contiguous()
-            codes, scale = ops.quantize_fp8(A32, P, M, None)
+            codes, scale = ops.quantize_rows(A32, P, M, None, code)
             return cls(codes, code, P, M, scale)
         tdt = torch.float32 if code == ops.DT_F32 else torch.bfloat16
         mat = torch.zeros((M, ld), dtype=tdt, device=A.device)

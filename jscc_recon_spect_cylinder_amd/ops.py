@@ -14,12 +14,12 @@ import torch
 
 from . import _lib
 
-DT_F32, DT_BF16, DT_FP8 = 0, 1, 2
+DT_F32, DT_BF16, DT_FP8, DT_FP16 = 0, 1, 2, 3
 SIG_STRICT, SIG_FIXED = 0, 1
 EVENT_RECORD_FLOATS = 16  # 64-byte per-event record
 
-_DTYPE_CODE = {torch.float32: DT_F32, torch.bfloat16: DT_BF16, torch.float8_e4m3fn: DT_FP8}
-_CODE_TORCH = {DT_F32: torch.float32, DT_BF16: torch.bfloat16, DT_FP8: torch.uint8}
+_DTYPE_CODE = {torch.float32: DT_F32, torch.bfloat16: DT_BF16, torch.float8_e4m3fn: DT_FP8, torch.float16: DT_FP16}
+_CODE_TORCH = {DT_F32: torch.float32, DT_BF16: torch.bfloat16, DT_FP8: torch.uint8, DT_FP16: torch.float16}
 
 
 def dtype_code(dtype) -> int:
@@ -27,7 +27,7 @@ def dtype_code(dtype) -> int:
         return dtype
     if isinstance(dtype, str):
         return {"fp32": DT_F32, "f32": DT_F32, "float32": DT_F32, "bf16": DT_BF16, "bfloat16": DT_BF16,
-                "fp8": DT_FP8, "e4m3": DT_FP8}[dtype.lower()]
+                "fp8": DT_FP8, "e4m3": DT_FP8, "fp16": DT_FP16, "float16": DT_FP16, "half": DT_FP16}[dtype.lower()]
     return _DTYPE_CODE[dtype]
 
 
@@ -278,17 +278,26 @@ def arows_normalize(A: torch.Tensor, arow: torch.Tensor, n_out: int) -> torch.Te
     return dst
 
 
-def quantize_fp8(src: torch.Tensor, P: int, n_rows: int, stats: torch.Tensor | None):
-    """FP8 e4m3fn copy of the rows plus per-row scales (scale * code ~ t / S; stats None: scale * code ~ row)."""
+def quantize_rows(src: torch.Tensor, P: int, n_rows: int, stats: torch.Tensor | None, dtype="fp8"):
+    """FP8 e4m3fn (uint8 codes) or FP16 copy of the rows plus per-row scales
+    (scale * code ~ t / S from the row statistics; stats None: scale * code ~ row)."""
     dev = src.device
+    code_out = dtype_code(dtype)
+    if code_out not in (DT_FP8, DT_FP16):
+        raise ValueError("quantised storage is FP8 or FP16")
     ld = round_up(P, 64)
-    dst = torch.empty((max(n_rows, 1), ld), dtype=torch.uint8, device=dev)
+    dst = torch.empty((max(n_rows, 1), ld), dtype=_CODE_TORCH[code_out], device=dev)
     scale = torch.empty(max(n_rows, 1), dtype=torch.float32, device=dev)
     code = DT_F32 if src.dtype == torch.float32 else DT_BF16
-    rc = _lib.lib().jscc_quantize_fp8_rows(src.data_ptr(), code, src.stride(0), P, n_rows, _p(stats),
-                                           dst.data_ptr(), ld, scale.data_ptr(), stream_of(dev))
-    _lib.check(rc, "jscc_quantize_fp8_rows")
+    rc = _lib.lib().jscc_quantize_rows(src.data_ptr(), code, src.stride(0), P, n_rows, _p(stats), dst.data_ptr(),
+                                       code_out, ld, scale.data_ptr(), stream_of(dev))
+    _lib.check(rc, "jscc_quantize_rows")
     return dst, scale
+
+
+def quantize_fp8(src: torch.Tensor, P: int, n_rows: int, stats: torch.Tensor | None):
+    """FP8 e4m3fn copy of the rows plus per-row scales (scale * code ~ t / S; stats None: scale * code ~ row)."""
+    return quantize_rows(src, P, n_rows, stats, "fp8")
 
 
 # ----------------------------------------------------------------------------- CSR row maps

@@ -219,7 +219,7 @@ def test_rotation_index_error_raises():
                          torch.ones((P, 1), device=DEV), 2)
 
 
-@pytest.mark.parametrize("dtype,tol", [("bf16", 2e-3), ("fp8", 3e-2)])
+@pytest.mark.parametrize("dtype,tol", [("bf16", 2e-3), ("fp16", 5e-4), ("fp8", 3e-2)])
 def test_list_mode_store_low_precision_vs_fp32(dtype, tol):
     """Engine path: T built into HBM (FP32 / BF16 / FP8) and reconstructed there."""
     from jscc_recon_spect_cylinder_amd import synth
@@ -277,7 +277,7 @@ def _oracle_joint(A, y, T, s_j, alpha, n_iter, k):
     return x
 
 
-@pytest.mark.parametrize("dtype,tol", [("fp32", 2e-5), ("bf16", 3e-3), ("fp8", 3e-2)])
+@pytest.mark.parametrize("dtype,tol", [("fp32", 2e-5), ("bf16", 3e-3), ("fp16", 1e-3), ("fp8", 3e-2)])
 def test_joint_osem_stores_vs_fp64_oracle(dtype, tol):
     """JSCCSD with A and T stored in FP32 / BF16 / FP8 (per-row scales), FP32
     accumulation, against the FP64 oracle of osem_joint_mode (3 iterations x 4 subsets)."""
@@ -288,7 +288,7 @@ def test_joint_osem_stores_vs_fp64_oracle(dtype, tol):
     s_j = alpha * A64.sum(0) + (2 - alpha) * T.sum(0)
     ref = _oracle_joint(A64, sc.y.astype(np.float64), T, s_j, alpha, 3, 4)
     a = SystemStore.from_dense(torch.from_numpy(sc.A).to(DEV), dtype)
-    t = ListModeStore.from_rows(rows[torch.float32 if dtype == "fp32" else torch.bfloat16], dtype)
+    t = ListModeStore.from_rows(rows[torch.bfloat16 if dtype == "bf16" else torch.float32], dtype)
     assert t.n_kept == T.shape[0]
     s = torch.from_numpy(s_j.astype(np.float32)).to(DEV)
     x = torch.ones(A64.shape[1], device=DEV)
@@ -296,7 +296,7 @@ def test_joint_osem_stores_vs_fp64_oracle(dtype, tol):
     assert rel(x.cpu().numpy(), ref) < tol
 
 
-@pytest.mark.parametrize("dtype,tol", [("fp32", 1e-5), ("bf16", 3e-3), ("fp8", 3e-2)])
+@pytest.mark.parametrize("dtype,tol", [("fp32", 1e-5), ("bf16", 3e-3), ("fp16", 1e-3), ("fp8", 3e-2)])
 def test_system_store_osem_and_column_sums_vs_oracle(dtype, tol):
     from jscc_recon_spect_cylinder_amd.engine import SystemStore
     from jscc_recon_spect_cylinder_amd import synth
@@ -340,3 +340,15 @@ def test_fused_joint_em_step_equals_projection_then_update():
     ops.proj(ops.MODE_RATIO, T, x=x, eps=1e-12, accumulate=True, w=wc, em=(x, s, 0, 1e-12, True, 1.5, ws, 0.5))
     assert torch.equal(x, x_ref)
     assert not wc.any() and not ws.any()
+
+
+def test_fp16_rows_scaled_to_two_pow_14():
+    from jscc_recon_spect_cylinder_amd import ops
+    rng = np.random.default_rng(4)
+    A = (rng.random((64, 512)) * np.exp(rng.normal(-9, 1.5, (64, 1)))).astype(np.float32)
+    codes, scale = ops.quantize_rows(torch.from_numpy(A).to(DEV), 512, 64, None, "fp16")
+    assert codes.dtype == torch.float16
+    np.testing.assert_allclose(scale.cpu().numpy(), A.max(1) / 16384.0, rtol=1e-6)
+    dec = codes[:, :512].float().cpu().numpy() * scale.cpu().numpy()[:, None]
+    # half precision: 11 significant bits for values within 2^-28 of the row max
+    assert np.all(np.abs(dec - A) <= 2.0 ** -11 * np.abs(A) + 2.0 ** -24 * scale.cpu().numpy()[:, None])
