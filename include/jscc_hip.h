@@ -223,6 +223,30 @@ int jscc_cone_proj(int mode, const void* events, const int32_t* ev_idx, int64_t 
                    int normalize, float* fwd_out, float* dsig_out, float out_scale, int accumulate, float* w,
                    int64_t w_len, void* workspace, int64_t workspace_bytes, int32_t* err, void* stream);
 
+/* Batched no-store cone projection: B reconstructions (each with its own
+ * image and events) in one launch.  ev_idx lists, reconstruction by
+ * reconstruction, the event records of the current subset's rows:
+ * reconstruction b owns ev_idx[row_off[b] .. row_off[b+1]) (row_off: device
+ * int64 [B+1], non-decreasing, row_off[0] = 0, row_off[B] = n_rows).  The
+ * work-groups split the n_rows rows evenly whatever the list lengths.
+ *   mode 0 (OSEM step): x[b, :] <- update(x[b, :], T_b^T (1 / T_b x_b),
+ *           sens[b, :]) with the modes of jscc_em_update, for every b at once
+ *           (osem_list_mode's subset update, recon_osem_plane.py:63-80, for B
+ *           independent reconstructions); w (optional) receives the weights;
+ *   mode 1 (column sums): w[b, :] = sum of the normalised rows of b (the
+ *           list-mode sensitivity of main_plane.py:348-355).
+ * x, sens, w: FP32 [B, ld*] with 16-byte aligned rows.  groups_per_cu: work-
+ * groups per CU (0: 4), the tile choice an autotuner may vary.  No
+ * counterpart in the reference (it reconstructs one image per process). */
+int jscc_cone_batch_groups(int64_t n_rows, int groups_per_cu);
+int64_t jscc_cone_batch_workspace_bytes(int64_t n_rows, int n_recon, int64_t n_vox, int groups_per_cu);
+int jscc_cone_proj_batched(int mode, const void* events, const int32_t* ev_idx, int64_t n_rows, const int64_t* row_off,
+                           int n_recon, const float* coor_x, const float* coor_y, const float* coor_z, int64_t n_vox,
+                           const float* sysmat, int64_t ld_sysmat, int sigma_model, double fixed_sigma, float* x,
+                           int64_t ldx, float eps, int use_eps, int normalize, const float* sens, int64_t ld_sens,
+                           int em_mode, float em_eps, float* w, int64_t ldw, int groups_per_cu, void* workspace,
+                           int64_t workspace_bytes, int32_t* err, void* stream);
+
 /* FP64 sum of n strided FP32 values into *out (device), fixed order: the
  * per-iteration count / likelihood reductions (main_plane.py:312-313). */
 int jscc_sum_f64(const float* v, int64_t n, int64_t stride, double* out, void* stream);

@@ -158,6 +158,29 @@ __device__ __forceinline__ int jscc_wave_id() { return threadIdx.x >> 6; }
 // number of compute units of the current device (cached per process)
 int jscc_num_cus();
 
+// EM updates: 0 plain x*w/s (recon_osem_plane.py:54); 1 safe_em_update
+// (recon_osem_plane_sparse.py:60-66); 2 apply_em_update
+// (distributed/python/recon_osem_dist.py:46-51).
+__device__ __forceinline__ float jscc_nan_to_num(float v, float nan_v, float pinf, float ninf) {
+  if (v != v) return nan_v;
+  if (v == __builtin_inff()) return pinf;
+  if (v == -__builtin_inff()) return ninf;
+  return v;
+}
+
+__device__ __forceinline__ float jscc_em_value(float xv, float w, float sv, int mode, float eps) {
+  if (mode == 0) return (xv * w) / sv;
+  if (mode == 1) {
+    const float ws = jscc_nan_to_num(w, 0.0f, 0.0f, 0.0f);
+    const float ss = jscc_nan_to_num(sv, 0.0f, 0.0f, 0.0f);
+    return ss > eps ? (xv * fmaxf(ws, 0.0f)) / ss : 0.0f;
+  }
+  const float ss = (sv != sv) ? sv : fmaxf(sv, eps);
+  float u = w / ss;
+  u = jscc_nan_to_num(u, 1.0f, 1e6f, 0.0f);
+  return jscc_nan_to_num(xv * u, 0.0f, 1e12f, 0.0f);
+}
+
 // Optional EM update applied where a subset weight is finished:
 // x[j] <- em(x[j], c1*w[j] (+ c2*w2[j]), s[j]) (modes of jscc_em_update); w[j]
 // (and w2[j]) are zeroed after when zero_w is set.  x == nullptr: no update.

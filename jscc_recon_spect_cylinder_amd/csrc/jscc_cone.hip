@@ -519,7 +519,7 @@ int launch_cp_size(int mode, const ConeProjArgs& a, int G, hipStream_t s) {
   // the row lives in registers: 4 * NCH voxels per thread
   if (a.P <= 4 * 256 * 4) launch_cp_mode<SIG, AF, 256, 4, DSIG>(mode, a, G, s);
   else if (a.P <= 4 * 512 * 4) launch_cp_mode<SIG, AF, 512, 4, DSIG>(mode, a, G, s);
-  else if (a.P <= 4 * 512 * 8) launch_cp_mode<SIG, AF, 512, 8, DSIG>(mode, a, G, s);
+  else if (a.P <= 4 * 1024 * 4) launch_cp_mode<SIG, AF, 1024, 4, DSIG>(mode, a, G, s);
   else if (a.P <= 4 * 1024 * 8) launch_cp_mode<SIG, AF, 1024, 8, DSIG>(mode, a, G, s);
   else {
     jscc_set_error("jscc_cone_proj: %d voxels exceed the register-resident row limit (32768)", a.P);
@@ -594,4 +594,280 @@ extern "C" int jscc_cone_proj(int mode, const void* events, const int32_t* ev_id
   const JsccEm none = {nullptr, nullptr, 0, 0.0f, 0, 1.0f, nullptr, 0.0f};
   return jscc_reduce_partials((float*)workspace, G, (int)n_vox, nullptr, 0, out_scale, accumulate, 0, w, w_len, err, s,
                               none);
+}
+
+// ============================================================================
+// Batched no-store projection: many reconstructions (each its own image and
+// event list) in one launch.  The rows of all reconstructions for the current
+// subset are concatenated reconstruction by reconstruction (row_off[b] ..
+// row_off[b+1]); the work-groups split the concatenation evenly, so lists of
+// very different lengths share the chip without idle work-groups.  A
+// work-group flushes its accumulators whenever it crosses into the next
+// reconstruction, into partial slot g + b (unique, since both the work-group
+// ranges and the reconstructions are contiguous and ordered).  The column
+// reduction then sums, for reconstruction b, the slots of the work-groups
+// whose range meets [row_off[b], row_off[b+1]) and applies either the EM
+// update of image b (OSEM step) or writes the weight (column sums).
+// ============================================================================
+namespace {
+
+enum { BP_OSEM = 0, BP_SUM = 1 };
+
+struct ConeBatchArgs {
+  const JsccEvent* ev;
+  const int32_t* ev_idx;
+  int64_t n_rows;
+  const int64_t* row_off;  // [n_recon + 1]
+  int n_recon;
+  const float* cx;
+  const float* cy;
+  const float* cz;
+  int P;
+  const float* A;
+  int64_t lda;
+  ConeConsts cc;
+  const float* x;  // [n_recon, ldx]
+  int64_t ldx;
+  float eps;
+  int use_eps;
+  int normalize;
+  float* partial;  // [G + n_recon - 1, P]
+};
+
+// rows of work-group g under the even split of n rows over G groups
+__device__ __forceinline__ void wg_rows(int64_t n, int G, int g, int64_t* b, int64_t* e) {
+  const int64_t per = n / G, rem = n % G;
+  *b = g * per + (g < rem ? g : rem);
+  *e = *b + per + (g < rem ? 1 : 0);
+}
+
+__device__ __forceinline__ int wg_of_row(int64_t n, int G, int64_t r) {
+  const int64_t per = n / G, rem = n % G;
+  const int64_t big = rem * (per + 1);
+  if (r < big) return (int)(r / (per + 1));
+  return (int)(rem + (r - big) / per);
+}
+
+template <int SIG, int AF, int BLOCK, int NCH, int MODE>
+__global__ void __launch_bounds__(BLOCK) k_cone_proj_batched(ConeBatchArgs a) {
+  constexpr int NW = BLOCK / JSCC_WAVE;
+  const int tid = threadIdx.x;
+  const int g = blockIdx.x;
+  int64_t r_begin, r_end;
+  wg_rows(a.n_rows, gridDim.x, g, &r_begin, &r_end);
+  if (r_begin >= r_end) return;
+  // reconstruction of the first row: last b with row_off[b] <= r_begin
+  int lo = 0, hi = a.n_recon;  // invariant row_off[lo] <= r_begin < row_off[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (a.row_off[mid] <= r_begin) lo = mid; else hi = mid;
+  }
+  int b = lo;
+  __shared__ double red[2][2][NW];
+  int buf = 0;
+  float tr[NCH][4];
+  float wr[NCH][4];
+  int64_t i = r_begin;
+  while (i < r_end) {
+    while (a.row_off[b + 1] <= i) ++b;  // skip reconstructions without rows here
+    const int64_t seg_end = a.row_off[b + 1] < r_end ? a.row_off[b + 1] : r_end;
+    const float* xb = MODE == BP_OSEM ? a.x + (int64_t)b * a.ldx : nullptr;
+#pragma unroll
+    for (int c = 0; c < NCH; ++c)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) wr[c][k] = 0.0f;
+    for (; i < seg_end; ++i) {
+      const ConeEv e = load_event(a.ev[a.ev_idx[i]], a.lda);
+      float S = 0.0f, D = 0.0f;
+#pragma unroll
+      for (int c = 0; c < NCH; ++c) {
+        const int base = (c * BLOCK + tid) * 4;
+        if (base < a.P) {
+          const float4 vx4 = *reinterpret_cast<const float4*>(a.cx + base);
+          const float4 vy4 = *reinterpret_cast<const float4*>(a.cy + base);
+          const float4 vz4 = *reinterpret_cast<const float4*>(a.cz + base);
+          const float vcx[4] = {vx4.x, vx4.y, vx4.z, vx4.w};
+          const float vcy[4] = {vy4.x, vy4.y, vy4.z, vy4.w};
+          const float vcz[4] = {vz4.x, vz4.y, vz4.z, vz4.w};
+          float av[4];
+          afactor4<AF>(e, a.A, base, nullptr, nullptr, nullptr, nullptr, av);
+          float xv[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+          if (MODE == BP_OSEM) {
+            const float4 x4 = *reinterpret_cast<const float4*>(xb + base);
+            xv[0] = x4.x; xv[1] = x4.y; xv[2] = x4.z; xv[3] = x4.w;
+          }
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            const float t = cone_tc<SIG, false>(e, vcx[k], vcy[k], vcz[k], a.cc, nullptr) * av[k];
+            tr[c][k] = t;
+            S += t;
+            if (MODE == BP_OSEM) D = fmaf(t, xv[k], D);
+          }
+        } else {
+#pragma unroll
+          for (int k = 0; k < 4; ++k) tr[c][k] = 0.0f;
+        }
+      }
+      const double vS = jscc_wave_sum((double)S);
+      const double vD = MODE == BP_OSEM ? jscc_wave_sum((double)D) : 0.0;
+      if (jscc_lane() == 0) {
+        red[buf][0][jscc_wave_id()] = vS;
+        red[buf][1][jscc_wave_id()] = vD;
+      }
+      __syncthreads();
+      double tS = 0.0, tD = 0.0;
+#pragma unroll
+      for (int w = 0; w < NW; ++w) {
+        tS += red[buf][0][w];
+        if (MODE == BP_OSEM) tD += red[buf][1][w];
+      }
+      buf ^= 1;
+      const double inv = a.normalize ? 1.0 / tS : 1.0;
+      float coef;
+      if (MODE == BP_OSEM) {
+        const double f = tD * inv;
+        const double den = a.use_eps ? (f > (double)a.eps ? f : (double)a.eps) : f;
+        coef = (float)(inv / den);
+      } else {
+        coef = (float)inv;
+      }
+#pragma unroll
+      for (int c = 0; c < NCH; ++c)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) wr[c][k] = fmaf(coef, tr[c][k], wr[c][k]);
+    }
+    float* dst = a.partial + (int64_t)(g + b) * a.P;
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+      const int base = (c * BLOCK + tid) * 4;
+      if (base < a.P) *reinterpret_cast<float4*>(dst + base) = make_float4(wr[c][0], wr[c][1], wr[c][2], wr[c][3]);
+    }
+  }
+}
+
+// Column reduction per reconstruction (blockIdx.y = b), 64 columns per block:
+// 16 groups of 16 threads sum every 16th contributing slot, then combine in
+// LDS in a fixed order.  OSEM: x_b <- em(x_b, w, s_b); SUM: w_out_b <- w.
+__global__ void __launch_bounds__(256) k_reduce_batched(const float* __restrict__ part, int G, int64_t n_rows,
+                                                        const int64_t* __restrict__ row_off, int P, int mode,
+                                                        float* __restrict__ x, int64_t ldx,
+                                                        const float* __restrict__ sens, int64_t lds, int em_mode,
+                                                        float em_eps, float* __restrict__ w, int64_t ldw) {
+  __shared__ float4 acc[16][16];
+  const int b = blockIdx.y;
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  const int col = blockIdx.x * 64 + tx * 4;
+  const int64_t lo = row_off[b], hi = row_off[b + 1];
+  float4 sum = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (hi > lo && col < P) {
+    const int g0 = wg_of_row(n_rows, G, lo), g1 = wg_of_row(n_rows, G, hi - 1);
+    for (int gg = g0 + ty; gg <= g1; gg += 16) {
+      const float4 v = *reinterpret_cast<const float4*>(part + (int64_t)(gg + b) * P + col);
+      sum.x += v.x; sum.y += v.y; sum.z += v.z; sum.w += v.w;
+    }
+  }
+  acc[ty][tx] = sum;
+  __syncthreads();
+  if (ty == 0 && col < P) {
+    float4 t = acc[0][tx];
+#pragma unroll
+    for (int k = 1; k < 16; ++k) {
+      t.x += acc[k][tx].x; t.y += acc[k][tx].y; t.z += acc[k][tx].z; t.w += acc[k][tx].w;
+    }
+    const float tv[4] = {t.x, t.y, t.z, t.w};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int64_t j = col + k;
+      if (w) w[(int64_t)b * ldw + j] = tv[k];
+      if (mode == BP_OSEM) {
+        float* xp = x + (int64_t)b * ldx + j;
+        *xp = jscc_em_value(*xp, tv[k], sens[(int64_t)b * lds + j], em_mode, em_eps);
+      }
+    }
+  }
+}
+
+template <int SIG, int AF, int BLOCK, int NCH>
+void launch_bp_mode(int mode, const ConeBatchArgs& a, int G, hipStream_t s) {
+  if (mode == BP_OSEM)
+    hipLaunchKernelGGL((k_cone_proj_batched<SIG, AF, BLOCK, NCH, BP_OSEM>), dim3(G), dim3(BLOCK), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_cone_proj_batched<SIG, AF, BLOCK, NCH, BP_SUM>), dim3(G), dim3(BLOCK), 0, s, a);
+}
+
+template <int SIG, int AF>
+int launch_bp_size(int mode, const ConeBatchArgs& a, int G, hipStream_t s) {
+  if (a.P <= 4 * 256 * 4) launch_bp_mode<SIG, AF, 256, 4>(mode, a, G, s);
+  else if (a.P <= 4 * 512 * 4) launch_bp_mode<SIG, AF, 512, 4>(mode, a, G, s);
+  else if (a.P <= 4 * 1024 * 4) launch_bp_mode<SIG, AF, 1024, 4>(mode, a, G, s);
+  else if (a.P <= 4 * 1024 * 8) launch_bp_mode<SIG, AF, 1024, 8>(mode, a, G, s);
+  else {
+    jscc_set_error("jscc_cone_proj_batched: %d voxels exceed the register-resident row limit (32768)", a.P);
+    return -3;
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int jscc_cone_batch_groups(int64_t n_rows, int groups_per_cu) {
+  int64_t g = (int64_t)(groups_per_cu > 0 ? groups_per_cu : 4) * jscc_num_cus();
+  if (g > n_rows) g = n_rows;
+  return (int)(g < 1 ? 1 : g);
+}
+
+extern "C" int64_t jscc_cone_batch_workspace_bytes(int64_t n_rows, int n_recon, int64_t n_vox, int groups_per_cu) {
+  const int64_t G = jscc_cone_batch_groups(n_rows, groups_per_cu);
+  return (G + (n_recon > 0 ? n_recon : 1) - 1) * n_vox * (int64_t)sizeof(float);
+}
+
+extern "C" int jscc_cone_proj_batched(int mode, const void* events, const int32_t* ev_idx, int64_t n_rows,
+                                      const int64_t* row_off, int n_recon, const float* coor_x, const float* coor_y,
+                                      const float* coor_z, int64_t n_vox, const float* sysmat, int64_t ld_sysmat,
+                                      int sigma_model, double fixed_sigma, float* x, int64_t ldx, float eps,
+                                      int use_eps, int normalize, const float* sens, int64_t ld_sens, int em_mode,
+                                      float em_eps, float* w, int64_t ldw, int groups_per_cu, void* workspace,
+                                      int64_t workspace_bytes, int32_t* err, void* stream) {
+  JSCC_REQUIRE(mode == BP_OSEM || mode == BP_SUM, "jscc_cone_proj_batched: bad mode");
+  JSCC_REQUIRE(n_recon >= 1 && n_recon <= 65535, "jscc_cone_proj_batched: bad reconstruction count");
+  JSCC_REQUIRE(n_rows >= 0 && n_rows < ((int64_t)1 << 31) && n_vox > 0 && n_vox % 4 == 0,
+               "jscc_cone_proj_batched: voxel count must be a positive multiple of 4");
+  JSCC_REQUIRE(sigma_model == SIG_STRICT || sigma_model == SIG_FIXED, "jscc_cone_proj_batched: bad sigma model");
+  if (sigma_model == SIG_FIXED) JSCC_REQUIRE(fixed_sigma > 0.0, "jscc_cone_proj_batched: fixed sigma must be positive");
+  JSCC_REQUIRE(events && ev_idx && row_off && coor_x && coor_y && coor_z && err, "jscc_cone_proj_batched: null pointer");
+  JSCC_REQUIRE(((uintptr_t)coor_x | (uintptr_t)coor_y | (uintptr_t)coor_z) % 16 == 0,
+               "jscc_cone_proj_batched: coordinates must be 16-byte aligned");
+  if (sysmat) JSCC_REQUIRE(ld_sysmat % 4 == 0 && (uintptr_t)sysmat % 16 == 0, "jscc_cone_proj_batched: sysmat alignment");
+  if (mode == BP_OSEM) {
+    JSCC_REQUIRE(x && sens && ldx >= n_vox && ld_sens >= n_vox && ldx % 4 == 0 && (uintptr_t)x % 16 == 0,
+                 "jscc_cone_proj_batched: images [n_recon, ldx] (16-byte aligned rows) and sensitivities needed");
+    JSCC_REQUIRE(em_mode >= 0 && em_mode <= 2, "jscc_cone_proj_batched: bad EM mode");
+  } else {
+    JSCC_REQUIRE(w != nullptr && ldw >= n_vox, "jscc_cone_proj_batched: null weight output");
+  }
+  if (n_rows == 0) return 0;
+  const int G = jscc_cone_batch_groups(n_rows, groups_per_cu);
+  JSCC_REQUIRE(workspace && workspace_bytes >= jscc_cone_batch_workspace_bytes(n_rows, n_recon, n_vox, groups_per_cu),
+               "jscc_cone_proj_batched: workspace too small");
+  hipStream_t s = (hipStream_t)stream;
+  ConeBatchArgs a;
+  a.ev = (const JsccEvent*)events; a.ev_idx = ev_idx; a.n_rows = n_rows; a.row_off = row_off; a.n_recon = n_recon;
+  a.cx = coor_x; a.cy = coor_y; a.cz = coor_z; a.P = (int)n_vox; a.A = sysmat; a.lda = ld_sysmat;
+  a.cc = make_consts(sigma_model, fixed_sigma);
+  a.x = x; a.ldx = ldx; a.eps = eps; a.use_eps = use_eps; a.normalize = normalize; a.partial = (float*)workspace;
+  int rc;
+  jscc_prof_begin(JSCC_TAG_CONE_PROJ, s);
+  if (sigma_model == SIG_STRICT) {
+    rc = sysmat ? launch_bp_size<SIG_STRICT, AF_ROW>(mode, a, G, s) : launch_bp_size<SIG_STRICT, AF_NONE>(mode, a, G, s);
+  } else {
+    rc = sysmat ? launch_bp_size<SIG_FIXED, AF_ROW>(mode, a, G, s) : launch_bp_size<SIG_FIXED, AF_NONE>(mode, a, G, s);
+  }
+  jscc_prof_end(JSCC_TAG_CONE_PROJ, s);
+  if (rc != 0) return rc;
+  JSCC_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_reduce_batched, dim3((unsigned)((n_vox + 63) / 64), (unsigned)n_recon), dim3(256), 0, s,
+                     (const float*)workspace, G, n_rows, row_off, (int)n_vox, mode, x, ldx, sens, ld_sens, em_mode,
+                     em_eps, w, ldw);
+  JSCC_CHECK_LAUNCH();
+  return 0;
 }

@@ -385,27 +385,6 @@ __global__ void __launch_bounds__(256) k_reduce_stage1(const float* __restrict__
   *reinterpret_cast<float4*>(out + (int64_t)blockIdx.y * P + col) = s;
 }
 
-// EM updates: 0 plain x*w/s ; 1 safe_em_update ; 2 apply_em_update.
-__device__ __forceinline__ float nan_to_num_f(float v, float nan_v, float pinf, float ninf) {
-  if (v != v) return nan_v;
-  if (v == __builtin_inff()) return pinf;
-  if (v == -__builtin_inff()) return ninf;
-  return v;
-}
-
-__device__ __forceinline__ float em_value(float xv, float w, float sv, int mode, float eps) {
-  if (mode == 0) return (xv * w) / sv;
-  if (mode == 1) {
-    const float ws = nan_to_num_f(w, 0.0f, 0.0f, 0.0f);
-    const float ss = nan_to_num_f(sv, 0.0f, 0.0f, 0.0f);
-    return ss > eps ? (xv * fmaxf(ws, 0.0f)) / ss : 0.0f;
-  }
-  const float ss = (sv != sv) ? sv : fmaxf(sv, eps);
-  float u = w / ss;
-  u = nan_to_num_f(u, 1.0f, 1e6f, 0.0f);
-  return nan_to_num_f(xv * u, 0.0f, 1e12f, 0.0f);
-}
-
 typedef JsccEm EmEpilogue;
 
 __device__ __forceinline__ void finish_weight(float sum, int64_t j, float scale, int accumulate, int sanitize,
@@ -422,7 +401,7 @@ __device__ __forceinline__ void finish_weight(float sum, int64_t j, float scale,
     } else if (em.c1 != 1.0f) {
       wc = em.c1 * wc;
     }
-    em.x[j] = em_value(em.x[j], wc, em.s[j], em.mode, em.eps);
+    em.x[j] = jscc_em_value(em.x[j], wc, em.s[j], em.mode, em.eps);
     w[j] = em.zero_w ? 0.0f : wj;
     if (em.w2 && em.zero_w) em.w2[j] = 0.0f;
   } else {
@@ -495,7 +474,7 @@ __global__ void __launch_bounds__(256) k_em_update(float* __restrict__ x, const 
   } else if (c1 != 1.0f) {
     w = c1 * w;
   }
-  x[j] = em_value(x[j], w, s[j], mode, eps);
+  x[j] = jscc_em_value(x[j], w, s[j], mode, eps);
   if (zero_after) zero_after[j] = 0.0f;
 }
 

@@ -238,3 +238,143 @@ class LazyListStore:
                 self.subset_weight(x, w, lo, hi, eps=eps)
                 ops.em_update(x, w, s, mode=em_mode, zero_after=w)
         return x
+
+
+@dataclass
+class BatchedLazyStore:
+    """B independent list-mode reconstructions on one GPU, T never stored: every
+    subset step evaluates the cone rows of all B lists in registers in ONE
+    launch (jscc_cone_proj_batched) whose work-groups split the concatenated
+    rows evenly, so lists of 1e4 and 2e6 events share the chip, followed by one
+    reduction launch that applies the B EM updates.  One iteration (2 launches
+    per subset) can be captured as a HIP graph and replayed.  Kept sets and row
+    normalisation are those of the stored form (process_list_plane_strict.py:
+    137-148, 241-248); subsets are split_tensor_even ranges of each list."""
+    events: torch.Tensor          # concatenated event records of all reconstructions
+    kept: list                    # per reconstruction: int32 record indices of its kept events, in order
+    vox: ops.VoxelSoA
+    sysmat: torch.Tensor | None
+    sigma_model: int
+    fixed_sigma: float
+    groups_per_cu: int = 0
+
+    @classmethod
+    def from_rows(cls, rows_list, sysmat=None, sigma_model=ops.SIG_FIXED, fixed_sigma=0.0):
+        """Concatenate the event records of per-reconstruction ComptonRows (built with store_dtype=None:
+        statistics and kept sets only)."""
+        evs, kept, base = [], [], 0
+        for r in rows_list:
+            k = r.sel_idx[r.kept[: r.n_kept].long()].to(torch.int32) + base
+            evs.append(r.events)
+            kept.append(k.contiguous())
+            base += r.events.shape[0]
+        return cls(torch.cat(evs).contiguous(), kept, rows_list[0].vox, sysmat, sigma_model, fixed_sigma)
+
+    @property
+    def n_recon(self) -> int:
+        return len(self.kept)
+
+    @property
+    def n_kept(self) -> list:
+        return [int(k.numel()) for k in self.kept]
+
+    def tables(self, n_subsets: int):
+        """Per subset: (concatenated int32 record list, int64 row offsets [B+1]) on the device."""
+        dev = self.events.device
+        out = []
+        for i in range(n_subsets):
+            parts, off = [], [0]
+            for k in self.kept:
+                n = k.numel()
+                lo, hi = n * i // n_subsets, n * (i + 1) // n_subsets
+                parts.append(k[lo:hi])
+                off.append(off[-1] + hi - lo)
+            out.append((torch.cat(parts).contiguous(), torch.tensor(off, dtype=torch.int64, device=dev)))
+        return out
+
+    def _call(self, mode, ev_idx, row_off, **kw):
+        ops.cone_proj_batched(mode, self.events, ev_idx, row_off, self.vox, sysmat=self.sysmat,
+                              sigma_model=self.sigma_model, fixed_sigma=self.fixed_sigma,
+                              groups_per_cu=kw.pop("groups_per_cu", self.groups_per_cu), **kw)
+
+    def sensitivity(self) -> torch.Tensor:
+        """[B, P] list-mode sensitivities: column sums of each reconstruction's normalised rows."""
+        (ev_idx, off), = self.tables(1)
+        w = torch.zeros((self.n_recon, ops.round_up(self.vox.n, 4)), dtype=torch.float32,
+                        device=self.events.device)
+        self._call(ops.BP_SUM, ev_idx, off, w=w)
+        return w
+
+    def autotune(self, x: torch.Tensor, s: torch.Tensor, n_subsets: int, candidates=(1, 2, 4, 8)) -> int:
+        """Pick the work-groups per CU of the projection launch by timing one subset step of each
+        candidate on a copy of the images; the choice is kept in groups_per_cu and returned."""
+        ev_idx, off = self.tables(n_subsets)[0]
+        best, best_t = candidates[0], float("inf")
+        for c in candidates:
+            xs = x.clone()
+            ws = ops.cone_batch_workspace(x.device, ev_idx.numel(), self.n_recon, self.vox.n, c)
+            self._call(ops.BP_OSEM, ev_idx, off, x=xs, sens=s, workspace=ws, groups_per_cu=c)  # warm
+            torch.cuda.synchronize()
+            t0 = torch.cuda.Event(enable_timing=True)
+            t1 = torch.cuda.Event(enable_timing=True)
+            t0.record()
+            self._call(ops.BP_OSEM, ev_idx, off, x=xs, sens=s, workspace=ws, groups_per_cu=c)
+            t1.record()
+            t1.synchronize()
+            t = t0.elapsed_time(t1)
+            if t < best_t:
+                best, best_t = c, t
+        self.groups_per_cu = best
+        return best
+
+    def _iteration(self, x, s, n_subsets, eps, em_mode):
+        """The launches of one OSEM iteration and the buffers they read (tables, workspace)."""
+        tabs = self.tables(n_subsets)
+        n_max = max(t[0].numel() for t in tabs)
+        ws = ops.cone_batch_workspace(x.device, n_max, self.n_recon, self.vox.n, self.groups_per_cu)
+
+        def run():
+            for ev_idx, off in tabs:
+                if ev_idx.numel():
+                    self._call(ops.BP_OSEM, ev_idx, off, x=x, sens=s, eps=eps, em_mode=em_mode, workspace=ws)
+        return run, (tabs, ws)
+
+    def capture(self, x: torch.Tensor, s: torch.Tensor, n_subsets: int, eps=None,
+                em_mode: int = ops.EM_PLAIN) -> "GraphedIteration":
+        """One OSEM iteration of all B images captured as a HIP graph (x updated in place at each
+        replay).  The returned object owns every buffer the graph reads; keep it while replaying."""
+        run, bufs = self._iteration(x, s, n_subsets, eps, em_mode)
+        x_keep = x.clone()
+        run()  # warm-up on the real buffers (library and allocator state), then restore the images
+        x.copy_(x_keep)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            run()
+        return GraphedIteration(g, bufs, x, s)
+
+    def osem(self, x: torch.Tensor, s: torch.Tensor, n_iter: int, n_subsets: int, eps=None,
+             em_mode: int = ops.EM_PLAIN, graph: bool = False) -> torch.Tensor:
+        """OSEM of all B images x [B, >=P] (updated in place) with sensitivities s [B, >=P].
+        graph=True captures one iteration (2 launches per subset) as a HIP graph and replays it."""
+        if graph:
+            gi = self.capture(x, s, n_subsets, eps, em_mode)
+            for _ in range(n_iter):
+                gi.replay()
+            return x
+        run, _bufs = self._iteration(x, s, n_subsets, eps, em_mode)
+        for _ in range(n_iter):
+            run()
+        return x
+
+
+@dataclass
+class GraphedIteration:
+    """A captured OSEM iteration together with the device buffers its kernels read."""
+    graph: torch.cuda.CUDAGraph
+    buffers: tuple
+    x: torch.Tensor
+    s: torch.Tensor
+
+    def replay(self) -> None:
+        self.graph.replay()

@@ -547,6 +547,50 @@ def cone_proj(mode: int, events: torch.Tensor, ev_idx: torch.Tensor, n_rows: int
     _lib.check(rc, "jscc_cone_proj")
 
 
+BP_OSEM, BP_SUM = 0, 1
+
+
+def cone_proj_batched(mode: int, events: torch.Tensor, ev_idx: torch.Tensor, row_off: torch.Tensor, vox: VoxelSoA,
+                      *, x: torch.Tensor | None = None, sens: torch.Tensor | None = None, w: torch.Tensor | None = None,
+                      sysmat=None, sigma_model: int = SIG_FIXED, fixed_sigma: float = 0.0, eps=None,
+                      normalize: bool = True, em_mode: int = 0, em_eps: float = 1e-12, groups_per_cu: int = 0,
+                      workspace: torch.Tensor | None = None) -> None:
+    """Batched no-store cone projection over B reconstructions (see include/jscc_hip.h):
+    BP_OSEM updates the images x [B, ldx] in place (one OSEM subset step each),
+    BP_SUM writes the column sums of the normalised rows to w [B, ldw]."""
+    dev = events.device
+    P = vox.n
+    B = row_off.numel() - 1
+    n_rows = ev_idx.numel()
+    if row_off.dtype != torch.int64 or row_off.device != dev or ev_idx.dtype != torch.int32 or B < 1:
+        raise ValueError("row offsets must be int64 [B+1] and the row list int32, on the events' device")
+    for t in (x, sens, w):
+        if t is not None and (t.ndim != 2 or t.shape[0] != B or t.shape[1] < P or t.dtype != torch.float32
+                              or t.stride(1) != 1 or t.device != dev):
+            raise ValueError("images / sensitivities / weights must be float32 [B, >= P] with unit column stride")
+    A = None
+    if sysmat is not None:
+        A = aligned_rows(sysmat, 4, dev)
+    nbytes = _lib.lib().jscc_cone_batch_workspace_bytes(max(n_rows, 1), B, P, int(groups_per_cu))
+    ws = workspace if workspace is not None else _workspace(dev, nbytes)
+    if ws.numel() < nbytes:
+        raise ValueError("workspace too small")
+    rc = _lib.lib().jscc_cone_proj_batched(
+        mode, events.data_ptr(), ev_idx.data_ptr(), n_rows, row_off.data_ptr(), B, vox.x.data_ptr(),
+        vox.y.data_ptr(), vox.z.data_ptr(), P, _p(A), A.stride(0) if A is not None else 0, int(sigma_model),
+        float(fixed_sigma), _p(x), x.stride(0) if x is not None else 0, float(eps) if eps is not None else 0.0,
+        1 if eps is not None else 0, 1 if normalize else 0, _p(sens), sens.stride(0) if sens is not None else 0,
+        int(em_mode), float(em_eps), _p(w), w.stride(0) if w is not None else 0, int(groups_per_cu), ws.data_ptr(),
+        ws.numel(), err_word(dev).data_ptr(), stream_of(dev))
+    _lib.check(rc, "jscc_cone_proj_batched")
+
+
+def cone_batch_workspace(dev, n_rows: int, n_recon: int, P: int, groups_per_cu: int = 0) -> torch.Tensor:
+    """A dedicated workspace for the batched projection (stable address, e.g. for graph capture)."""
+    nbytes = _lib.lib().jscc_cone_batch_workspace_bytes(max(n_rows, 1), n_recon, P, int(groups_per_cu))
+    return torch.empty(nbytes, dtype=torch.uint8, device=dev)
+
+
 def sum_f64(v: torch.Tensor) -> torch.Tensor:
     """FP64 sum of a float32 vector as a device scalar (deterministic order)."""
     if v.dtype != torch.float32 or v.ndim != 1:

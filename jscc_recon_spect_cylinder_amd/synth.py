@@ -146,3 +146,75 @@ def compton_list(n_events=200_000, n_rings=64, n_angles=256, r_max=50.0, e0=E0_5
 def energy_thresholds(e0: float):
     """(max, min, sum) windows of main_plane.py:157-159 with the 511 keV sum window."""
     return 2 * e0 ** 2 / (0.511 + 2 * e0) - 0.001, 0.05, 0.46
+
+
+def random_rods(rng, n_min=3, n_max=12, r_max=50.0, rad=(2.0, 8.0), contrast=(2.0, 8.0)):
+    """3-12 hot rods of random radius and contrast inside the disk (batched-reconstruction phantoms)."""
+    n = int(rng.integers(n_min, n_max + 1))
+    rods = []
+    for _ in range(n):
+        rr = float(rng.uniform(rad[0], rad[1]))
+        d = float(rng.uniform(0.0, r_max - rr))
+        a = float(rng.uniform(0.0, 2 * np.pi))
+        rods.append((d * math.cos(a), d * math.sin(a), rr, float(rng.uniform(contrast[0], contrast[1]))))
+    return rods
+
+
+def compton_list_device(n_events: int, coor: np.ndarray, x_true: np.ndarray, n_rings: int, n_angles: int,
+                        device, seed: int, r_max=50.0, e0=E0_511, cyl_r=100.0, cyl_half_z=20.0, d_min=10.0,
+                        d_max=40.0):
+    """The generator of compton_list with torch on `device` (FP64), for lists of millions of events.
+    Returns (detector [2N, 3] float32, list [N, 4] float32) device tensors, crystal numbers 1..2N."""
+    import torch
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    f64 = dict(dtype=torch.float64, device=device)
+    cxy = torch.from_numpy(coor[:, :2].astype(np.float64)).to(device)
+    prob = torch.from_numpy((x_true / x_true.sum()).astype(np.float64)).to(device)
+    srcs, dirs, have = [], [], 0
+    while have < n_events:
+        m = int((n_events - have) * 2.5) + 64
+        vox = torch.multinomial(prob, m, replacement=True, generator=g)
+        rr = torch.hypot(cxy[vox, 0], cxy[vox, 1]) + (torch.rand(m, generator=g, **f64) - 0.5) * (r_max / n_rings)
+        aa = torch.atan2(cxy[vox, 1], cxy[vox, 0]) + (torch.rand(m, generator=g, **f64) - 0.5) * (
+            2 * np.pi / n_angles)
+        s = torch.stack([rr * torch.cos(aa), rr * torch.sin(aa), torch.zeros_like(rr)], dim=1)
+        u = torch.randn((m, 3), generator=g, **f64)
+        u = u / u.norm(dim=1, keepdim=True)
+        a = u[:, 0] ** 2 + u[:, 1] ** 2
+        b = 2 * (s[:, 0] * u[:, 0] + s[:, 1] * u[:, 1])
+        c = s[:, 0] ** 2 + s[:, 1] ** 2 - cyl_r ** 2
+        t = (-b + torch.sqrt(torch.clamp(b * b - 4 * a * c, min=0.0))) / torch.clamp(2 * a, min=1e-300)
+        hit = s + t[:, None] * u
+        ok = (a > 1e-12) & (hit[:, 2].abs() <= cyl_half_z)
+        srcs.append(hit[ok])
+        dirs.append(u[ok])
+        have += int(ok.sum())
+    pos1 = torch.cat(srcs)[:n_events]
+    u = torch.cat(dirs)[:n_events]
+    k = e0 / ME_C2
+    cts, have = [], 0
+    while have < n_events:
+        m = int((n_events - have) * 2.2) + 64
+        cc = torch.rand(m, generator=g, **f64) * 2 - 1
+        p = 1.0 / (1.0 + k * (1.0 - cc))
+        f = 0.5 * p * p * (p + 1.0 / p - (1.0 - cc * cc))
+        acc = torch.rand(m, generator=g, **f64) < f
+        cts.append(cc[acc])
+        have += int(acc.sum())
+    ct = torch.cat(cts)[:n_events]
+    st = torch.sqrt(torch.clamp(1 - ct * ct, min=0.0))
+    phi = torch.rand(n_events, generator=g, **f64) * (2 * np.pi)
+    helper = torch.where(u[:, 2:3].abs() < 0.9, torch.tensor([[0.0, 0.0, 1.0]], **f64),
+                         torch.tensor([[1.0, 0.0, 0.0]], **f64))
+    e1v = torch.linalg.cross(u, helper)
+    e1v = e1v / e1v.norm(dim=1, keepdim=True)
+    e2v = torch.linalg.cross(u, e1v)
+    v = ct[:, None] * u + st[:, None] * (torch.cos(phi)[:, None] * e1v + torch.sin(phi)[:, None] * e2v)
+    d = d_min + (d_max - d_min) * torch.rand(n_events, generator=g, **f64)
+    pos2 = pos1 + d[:, None] * v
+    e_scat = e0 / (1.0 + (e0 / ME_C2) * (1.0 - ct))
+    det = torch.cat([pos1, pos2]).float()
+    idx = torch.arange(1, n_events + 1, **f64)
+    lst = torch.stack([idx, e0 - e_scat, idx + n_events, e_scat], dim=1).float()
+    return det, lst
