@@ -450,10 +450,11 @@ __global__ void __launch_bounds__(BLOCK) k_cone_proj(ConeProjArgs a) {
         for (int k = 0; k < 4; ++k) tr[c][k] = 0.0f;
       }
     }
-    double vS = jscc_wave_sum((double)S);
-    double vD = jscc_wave_sum((double)D);
-    double vdS = DSIG ? jscc_wave_sum((double)dS) : 0.0;
-    double vdD = DSIG ? jscc_wave_sum((double)dD) : 0.0;
+    // wave sums in FP32 by DPP (64 partials of <= 32 values each), across waves in FP64
+    double vS = (double)jscc_wave_sum_dpp(S);
+    double vD = (double)jscc_wave_sum_dpp(D);
+    double vdS = DSIG ? (double)jscc_wave_sum_dpp(dS) : 0.0;
+    double vdD = DSIG ? (double)jscc_wave_sum_dpp(dD) : 0.0;
     if (jscc_lane() == 0) {
       red[buf][0][jscc_wave_id()] = vS;
       red[buf][1][jscc_wave_id()] = vD;
@@ -648,10 +649,23 @@ __device__ __forceinline__ int wg_of_row(int64_t n, int G, int64_t r) {
   return (int)(rem + (r - big) / per);
 }
 
-template <int SIG, int AF, int BLOCK, int NCH, int MODE>
-__global__ void __launch_bounds__(BLOCK) k_cone_proj_batched(ConeBatchArgs a) {
+// R rows per work-group step: the rows are evaluated into registers, their
+// sums reduced by DPP within each wave, and one barrier serves all R rows;
+// lane q of every wave then finishes row q (cross-wave FP64 sum and
+// coefficient), which is read back as a scalar.
+// (the pointers are __restrict__ parameters so that the wave-uniform event
+// records and offsets are read with scalar loads into scalar registers)
+template <int SIG, int AF, int BLOCK, int NCH, int R, int MODE>
+__global__ void __launch_bounds__(BLOCK)
+    k_cone_proj_batched(const JsccEvent* __restrict__ ev, const int32_t* __restrict__ ev_idx,
+                        const int64_t* __restrict__ row_off, const float* __restrict__ cx,
+                        const float* __restrict__ cy, const float* __restrict__ cz, const float* __restrict__ A,
+                        const float* __restrict__ x, float* __restrict__ partial, ConeBatchArgs a) {
+  a.ev = ev; a.ev_idx = ev_idx; a.row_off = row_off; a.cx = cx; a.cy = cy; a.cz = cz; a.A = A; a.x = x;
+  a.partial = partial;
   constexpr int NW = BLOCK / JSCC_WAVE;
   const int tid = threadIdx.x;
+  const int lane = jscc_lane();
   const int g = blockIdx.x;
   int64_t r_begin, r_end;
   wg_rows(a.n_rows, gridDim.x, g, &r_begin, &r_end);
@@ -663,9 +677,8 @@ __global__ void __launch_bounds__(BLOCK) k_cone_proj_batched(ConeBatchArgs a) {
     if (a.row_off[mid] <= r_begin) lo = mid; else hi = mid;
   }
   int b = lo;
-  __shared__ double red[2][2][NW];
+  __shared__ float red[2][2][R][NW];
   int buf = 0;
-  float tr[NCH][4];
   float wr[NCH][4];
   int64_t i = r_begin;
   while (i < r_end) {
@@ -676,12 +689,23 @@ __global__ void __launch_bounds__(BLOCK) k_cone_proj_batched(ConeBatchArgs a) {
     for (int c = 0; c < NCH; ++c)
 #pragma unroll
       for (int k = 0; k < 4; ++k) wr[c][k] = 0.0f;
-    for (; i < seg_end; ++i) {
-      const ConeEv e = load_event(a.ev[a.ev_idx[i]], a.lda);
-      float S = 0.0f, D = 0.0f;
+    for (; i < seg_end; i += R) {
+      float tr[R][NCH][4];
+      float S[R], D[R];
+      ConeEv e[R];
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        const int64_t iq = i + q < seg_end ? i + q : i;  // a slot past the segment repeats row i (weight 0)
+        e[q] = load_event(a.ev[a.ev_idx[iq]], a.lda);
+        S[q] = 0.0f;
+        D[q] = 0.0f;
+      }
 #pragma unroll
       for (int c = 0; c < NCH; ++c) {
-        const int base = (c * BLOCK + tid) * 4;
+        int base = (c * BLOCK + tid) * 4;
+        // reloaded for every step (L2 hits) instead of hoisted out of the row loop, which
+        // would pin 12 registers per chunk for the whole kernel
+        asm volatile("" : "+v"(base));
         if (base < a.P) {
           const float4 vx4 = *reinterpret_cast<const float4*>(a.cx + base);
           const float4 vy4 = *reinterpret_cast<const float4*>(a.cy + base);
@@ -689,52 +713,70 @@ __global__ void __launch_bounds__(BLOCK) k_cone_proj_batched(ConeBatchArgs a) {
           const float vcx[4] = {vx4.x, vx4.y, vx4.z, vx4.w};
           const float vcy[4] = {vy4.x, vy4.y, vy4.z, vy4.w};
           const float vcz[4] = {vz4.x, vz4.y, vz4.z, vz4.w};
-          float av[4];
-          afactor4<AF>(e, a.A, base, nullptr, nullptr, nullptr, nullptr, av);
           float xv[4] = {0.0f, 0.0f, 0.0f, 0.0f};
           if (MODE == BP_OSEM) {
             const float4 x4 = *reinterpret_cast<const float4*>(xb + base);
             xv[0] = x4.x; xv[1] = x4.y; xv[2] = x4.z; xv[3] = x4.w;
           }
 #pragma unroll
-          for (int k = 0; k < 4; ++k) {
-            const float t = cone_tc<SIG, false>(e, vcx[k], vcy[k], vcz[k], a.cc, nullptr) * av[k];
-            tr[c][k] = t;
-            S += t;
-            if (MODE == BP_OSEM) D = fmaf(t, xv[k], D);
+          for (int q = 0; q < R; ++q) {
+            float av[4];
+            afactor4<AF>(e[q], a.A, base, nullptr, nullptr, nullptr, nullptr, av);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+              const float t = cone_tc<SIG, false>(e[q], vcx[k], vcy[k], vcz[k], a.cc, nullptr) * av[k];
+              tr[q][c][k] = t;
+              S[q] += t;
+              if (MODE == BP_OSEM) D[q] = fmaf(t, xv[k], D[q]);
+            }
           }
         } else {
 #pragma unroll
-          for (int k = 0; k < 4; ++k) tr[c][k] = 0.0f;
+          for (int q = 0; q < R; ++q)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) tr[q][c][k] = 0.0f;
         }
       }
-      const double vS = jscc_wave_sum((double)S);
-      const double vD = MODE == BP_OSEM ? jscc_wave_sum((double)D) : 0.0;
-      if (jscc_lane() == 0) {
-        red[buf][0][jscc_wave_id()] = vS;
-        red[buf][1][jscc_wave_id()] = vD;
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        S[q] = jscc_wave_sum_dpp(S[q]);
+        if (MODE == BP_OSEM) D[q] = jscc_wave_sum_dpp(D[q]);
+      }
+      if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+          red[buf][0][q][jscc_wave_id()] = S[q];
+          red[buf][1][q][jscc_wave_id()] = D[q];
+        }
       }
       __syncthreads();
-      double tS = 0.0, tD = 0.0;
+      float cl = 0.0f;
+      if (lane < R) {
+        double tS = 0.0, tD = 0.0;
 #pragma unroll
-      for (int w = 0; w < NW; ++w) {
-        tS += red[buf][0][w];
-        if (MODE == BP_OSEM) tD += red[buf][1][w];
+        for (int w = 0; w < NW; ++w) {
+          tS += (double)red[buf][0][lane][w];
+          if (MODE == BP_OSEM) tD += (double)red[buf][1][lane][w];
+        }
+        const double inv = a.normalize ? 1.0 / tS : 1.0;
+        if (MODE == BP_OSEM) {
+          const double f = tD * inv;
+          const double den = a.use_eps ? (f > (double)a.eps ? f : (double)a.eps) : f;
+          cl = (float)(inv / den);
+        } else {
+          cl = (float)inv;
+        }
+        if (i + lane >= seg_end) cl = 0.0f;
       }
       buf ^= 1;
-      const double inv = a.normalize ? 1.0 / tS : 1.0;
-      float coef;
-      if (MODE == BP_OSEM) {
-        const double f = tD * inv;
-        const double den = a.use_eps ? (f > (double)a.eps ? f : (double)a.eps) : f;
-        coef = (float)(inv / den);
-      } else {
-        coef = (float)inv;
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        const float coef = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cl), q));
+#pragma unroll
+        for (int c = 0; c < NCH; ++c)
+#pragma unroll
+          for (int k = 0; k < 4; ++k) wr[c][k] = fmaf(coef, tr[q][c][k], wr[c][k]);
       }
-#pragma unroll
-      for (int c = 0; c < NCH; ++c)
-#pragma unroll
-        for (int k = 0; k < 4; ++k) wr[c][k] = fmaf(coef, tr[c][k], wr[c][k]);
     }
     float* dst = a.partial + (int64_t)(g + b) * a.P;
 #pragma unroll
@@ -742,6 +784,7 @@ __global__ void __launch_bounds__(BLOCK) k_cone_proj_batched(ConeBatchArgs a) {
       const int base = (c * BLOCK + tid) * 4;
       if (base < a.P) *reinterpret_cast<float4*>(dst + base) = make_float4(wr[c][0], wr[c][1], wr[c][2], wr[c][3]);
     }
+    i = seg_end;
   }
 }
 
@@ -789,10 +832,13 @@ __global__ void __launch_bounds__(256) k_reduce_batched(const float* __restrict_
 
 template <int SIG, int AF, int BLOCK, int NCH>
 void launch_bp_mode(int mode, const ConeBatchArgs& a, int G, hipStream_t s) {
+  constexpr int R = NCH <= 4 ? 2 : 1;  // rows per step: tr[R][NCH][4] next to the accumulators
   if (mode == BP_OSEM)
-    hipLaunchKernelGGL((k_cone_proj_batched<SIG, AF, BLOCK, NCH, BP_OSEM>), dim3(G), dim3(BLOCK), 0, s, a);
+    hipLaunchKernelGGL((k_cone_proj_batched<SIG, AF, BLOCK, NCH, R, BP_OSEM>), dim3(G), dim3(BLOCK), 0, s, a.ev,
+                       a.ev_idx, a.row_off, a.cx, a.cy, a.cz, a.A, a.x, a.partial, a);
   else
-    hipLaunchKernelGGL((k_cone_proj_batched<SIG, AF, BLOCK, NCH, BP_SUM>), dim3(G), dim3(BLOCK), 0, s, a);
+    hipLaunchKernelGGL((k_cone_proj_batched<SIG, AF, BLOCK, NCH, R, BP_SUM>), dim3(G), dim3(BLOCK), 0, s, a.ev,
+                       a.ev_idx, a.row_off, a.cx, a.cy, a.cz, a.A, a.x, a.partial, a);
 }
 
 template <int SIG, int AF>
