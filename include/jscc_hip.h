@@ -15,7 +15,8 @@
  *     column of an [P, R] int64 tensor;
  *   - "err" words are device int32 flags OR-ed by kernels when an index is out
  *     of range (1: crystal number, 2: image gather, 4: weight gather,
- *     8: sensitivity gather).  The host raises IndexError when set.
+ *     8: sensitivity gather, 16: thinning position).  The host raises
+ *     IndexError when set.
  *   - storage dtypes: 0 = FP32, 1 = BF16, 2 = FP8 e4m3fn (per-row scale),
  *     3 = FP16 (IEEE half, per-row scale).
  *
@@ -246,6 +247,16 @@ int jscc_cone_proj_batched(int mode, const void* events, const int32_t* ev_idx, 
                            int64_t ldx, float eps, int use_eps, int normalize, const float* sens, int64_t ld_sens,
                            int em_mode, float em_eps, float* w, int64_t ldw, int groups_per_cu, void* workspace,
                            int64_t workspace_bytes, int32_t* err, void* stream);
+
+/* Count thinning: out[j * out_stride] = number of positions[0..n_pos) that fall
+ * in bin j, where bin j owns the copy positions [cum[j-1], cum[j]) (cum: the
+ * inclusive prefix sums of round(count)).  With positions = the first k
+ * entries of a random permutation of all copies this is the equal-count
+ * projection of main_plane.py:300-310 and the count downsampling of
+ * sparse_main_utils.py:130-150.  workspace: int32 [n_bins]; err bit 16: a
+ * position outside [0, cum[n_bins-1]). */
+int jscc_thin_counts(const int64_t* cum_counts, int64_t n_bins, const int64_t* positions, int64_t n_pos, float* out,
+                     int64_t out_stride, int32_t* workspace, int32_t* err, void* stream);
 
 /* FP64 sum of n strided FP32 values into *out (device), fixed order: the
  * per-iteration count / likelihood reductions (main_plane.py:312-313). */

@@ -59,6 +59,7 @@ SIGNATURES = {
     "jscc_cone_proj": (c_int, [c_int, vp, vp, c_i64, vp, vp, vp, c_i64, vp, c_i64, c_int, c_f64, vp, vp, c_f32, c_int,
                                c_int, vp, vp, c_f32, c_int, vp, c_i64, vp, c_i64, vp, vp]),
     "jscc_sum_f64": (c_int, [vp, c_i64, c_i64, vp, vp]),
+    "jscc_thin_counts": (c_int, [vp, c_i64, vp, c_i64, vp, c_i64, vp, vp, vp]),
     "jscc_cone_batch_groups": (c_int, [c_i64, c_int]),
     "jscc_cone_batch_workspace_bytes": (c_i64, [c_i64, c_int, c_i64, c_int]),
     "jscc_cone_proj_batched": (c_int, [c_int, vp, vp, c_i64, vp, c_int, vp, vp, vp, c_i64, vp, c_i64, c_int, c_f64,

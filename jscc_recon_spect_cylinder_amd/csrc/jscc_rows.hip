@@ -285,3 +285,63 @@ extern "C" int jscc_fp8_selftest(float* out512, void* stream) {
   JSCC_CHECK_LAUNCH();
   return 0;
 }
+
+// ---------------------------------------------------------------------------
+// Count thinning (main_plane.py:300-310; sparse_main_utils.py:130-150): the
+// reference expands bin j into round(count_j) copies of j, draws a random
+// permutation of the copies and keeps the first k.  Here the expansion is
+// implicit: copy position p belongs to the bin whose cumulative count range
+// contains p (upper bound in the inclusive prefix sums), and the kept copies
+// are histogrammed with integer atomics (exact, order independent).
+// ---------------------------------------------------------------------------
+namespace {
+
+__global__ void __launch_bounds__(256) k_fill_i32(int32_t* __restrict__ v, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) v[i] = 0;
+}
+
+__global__ void __launch_bounds__(256) k_thin_hist(const int64_t* __restrict__ cum, int64_t n_bins,
+                                                   const int64_t* __restrict__ pos, int64_t n_pos,
+                                                   int32_t* __restrict__ hist, int32_t* __restrict__ err) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n_pos) return;
+  const int64_t p = pos[i];
+  if (p < 0 || n_bins == 0 || p >= cum[n_bins - 1]) {
+    atomicOr(err, 16);
+    return;
+  }
+  int64_t lo = 0, hi = n_bins - 1;  // first j with cum[j] > p
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (cum[mid] > p) hi = mid; else lo = mid + 1;
+  }
+  atomicAdd(hist + lo, 1);
+}
+
+__global__ void __launch_bounds__(256) k_i32_to_f32(const int32_t* __restrict__ h, int64_t n, float* __restrict__ out,
+                                                    int64_t stride) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) out[i * stride] = (float)h[i];
+}
+
+}  // namespace
+
+extern "C" int jscc_thin_counts(const int64_t* cum_counts, int64_t n_bins, const int64_t* positions, int64_t n_pos,
+                                float* out, int64_t out_stride, int32_t* workspace, int32_t* err, void* stream) {
+  JSCC_REQUIRE(n_bins >= 0 && n_pos >= 0, "jscc_thin_counts: bad sizes");
+  if (n_bins == 0) return 0;
+  JSCC_REQUIRE(cum_counts && out && workspace && err && (n_pos == 0 || positions), "jscc_thin_counts: null pointer");
+  hipStream_t s = (hipStream_t)stream;
+  const unsigned gb = (unsigned)((n_bins + 255) / 256);
+  hipLaunchKernelGGL(k_fill_i32, dim3(gb), dim3(256), 0, s, workspace, n_bins);
+  JSCC_CHECK_LAUNCH();
+  if (n_pos > 0) {
+    hipLaunchKernelGGL(k_thin_hist, dim3((unsigned)((n_pos + 255) / 256)), dim3(256), 0, s, cum_counts, n_bins,
+                       positions, n_pos, workspace, err);
+    JSCC_CHECK_LAUNCH();
+  }
+  hipLaunchKernelGGL(k_i32_to_f32, dim3(gb), dim3(256), 0, s, workspace, n_bins, out, out_stride > 0 ? out_stride : 1);
+  JSCC_CHECK_LAUNCH();
+  return 0;
+}

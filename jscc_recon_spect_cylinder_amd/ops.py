@@ -54,7 +54,8 @@ _err_words: dict[int, torch.Tensor] = {}
 _ERR_TEXT = {1: "crystal number out of range of the detector table",
              2: "image gather index (RotMat) out of range",
              4: "weight gather index (RotMatInv) out of range",
-             8: "sensitivity gather index (RotMatInv) out of range"}
+             8: "sensitivity gather index (RotMatInv) out of range",
+             16: "thinning position out of range of the expanded counts"}
 
 
 def err_word(device: torch.device) -> torch.Tensor:
@@ -589,6 +590,24 @@ def cone_batch_workspace(dev, n_rows: int, n_recon: int, P: int, groups_per_cu: 
     """A dedicated workspace for the batched projection (stable address, e.g. for graph capture)."""
     nbytes = _lib.lib().jscc_cone_batch_workspace_bytes(max(n_rows, 1), n_recon, P, int(groups_per_cu))
     return torch.empty(nbytes, dtype=torch.uint8, device=dev)
+
+
+def thin_counts(counts: torch.Tensor, positions: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+    """Histogram over the bins of the copy positions `positions` (int64, device) in the implicit expansion
+    of round(counts) (bin j repeated round(counts[j]) times); FP32 [n_bins] (see jscc_thin_counts)."""
+    dev = counts.device
+    require_gpu(counts, "counts")
+    cnt = torch.round(counts.double()).to(torch.int64)
+    cum = torch.cumsum(cnt, 0)
+    n = counts.numel()
+    if out is None:
+        out = torch.empty(n, dtype=torch.float32, device=dev)
+    pos = positions.to(device=dev, dtype=torch.int64).contiguous()
+    ws = torch.empty(max(n, 1), dtype=torch.int32, device=dev)
+    rc = _lib.lib().jscc_thin_counts(cum.data_ptr(), n, pos.data_ptr(), pos.numel(), out.data_ptr(), out.stride(0),
+                                     ws.data_ptr(), err_word(dev).data_ptr(), stream_of(dev))
+    _lib.check(rc, "jscc_thin_counts")
+    return out
 
 
 def sum_f64(v: torch.Tensor) -> torch.Tensor:

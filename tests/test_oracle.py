@@ -230,3 +230,32 @@ def test_sensitivity_closed_form():
     s = osem.sensitivity(A, inv)
     cs = A.sum(0)
     np.testing.assert_allclose(s, (cs + cs[[1, 0, 3, 2]]) / 2)
+
+
+def test_scd_thinning_oracle_equals_the_literal_expansion():
+    """The implicit-expansion thinning equals the reference's literal expand/compare algorithm."""
+    import torch
+    from oracle import scd
+    rng = np.random.default_rng(12)
+    for _ in range(20):
+        counts = rng.integers(0, 6, 17).astype(np.float32)
+        counts[rng.random(17) < 0.2] += 0.5  # half counts round to even, like Python's round
+        n = int(sum(round(float(c)) for c in counts))
+        if n == 0:
+            continue
+        k = int(rng.integers(0, n + 1))
+        perm = torch.randperm(n).numpy()[:k]
+        np.testing.assert_array_equal(scd.thin(counts, perm), scd.thin_literal(counts, perm))
+
+
+def test_scd_equal_count_projection_properties():
+    import torch
+    from oracle import scd
+    rng = np.random.default_rng(13)
+    proj = rng.poisson(3.0, (40, 3)).astype(np.float32)
+    torch.manual_seed(5)
+    k = [10, int(proj[:, 1].sum()), 0]
+    out = scd.equal_count_projection(proj, k)
+    assert out[:, 0].sum() == 10 and np.all(out <= proj)
+    np.testing.assert_array_equal(out[:, 1], proj[:, 1])  # all counts kept
+    assert not out[:, 2].any()
