@@ -537,8 +537,11 @@ def main():
         torch.cuda.synchronize()
         osem_s = (time.perf_counter() - t2) / args.iters
         b = {"bf16": 2, "fp32": 4, "fp16": 2}[name]
-        # algorithmic bytes of one subset launch: its rows read once (+ x read, w written)
-        subset_bytes = st.n_kept / args.subsets * P * b + 2 * P * 4
+        # algorithmic bytes of the OSEM launches of one step: every kept row read once per iteration
+        # (+ the image read and the weight written per subset), divided over the subset launches
+        osem_bytes = args.iters * (st.n_kept * P * b + args.subsets * 2 * P * 4)
+        launches = pcnt / args.steps
+        subset_bytes = osem_bytes / launches
         proj_ms = ptot / max(pcnt, 1)
         results[name] = {
             "ms_per_step": ms_step,
@@ -546,10 +549,10 @@ def main():
             "n_kept": st.n_kept,
             "cone_ms": ctot / max(ccnt, 1),
             "cone_Mevents_s": args.events / (ctot / max(ccnt, 1) * 1e-3) / 1e6,
-            "subset_launch_ms": proj_ms,
-            "subset_launches_per_step": pcnt / args.steps,
-            "subset_bytes": subset_bytes,
-            "subset_GBs": subset_bytes / (proj_ms * 1e-3) / 1e9,
+            "osem_launch_ms": proj_ms,
+            "osem_launches_per_step": launches,
+            "osem_bytes_per_launch": subset_bytes,
+            "osem_GBs": subset_bytes / (proj_ms * 1e-3) / 1e9,
             "colsum_ms": stot / max(scnt, 1),
             "colsum_GBs": st.n_kept * P * b / (stot / max(scnt, 1) * 1e-3) / 1e9,
             "osem_iter_s": 1.0 / osem_s,
@@ -564,10 +567,10 @@ def main():
         results["fp16"]["rel_err_vs_fp32_image"] = float(np.linalg.norm(results["fp16"]["img"] - f) / np.linalg.norm(f))
     traffic = _traffic("bf16")
     roof = {"kernel": "k_proj_fused<bf16,2,4,ratio>: one OSEM subset pass T_s^T(1/T_s x)", "bound": "hbm",
-            "achieved": round(bf["subset_GBs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(bf["subset_GBs"] / HBM_PEAK_GBS, 4),
+            "achieved": round(bf["osem_GBs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(bf["osem_GBs"] / HBM_PEAK_GBS, 4),
             "traffic": None if traffic is None else int(traffic),
-            "launch_ms": round(bf["subset_launch_ms"], 4), "bytes_per_launch": int(bf["subset_bytes"])}
+            "launch_ms": round(bf["osem_launch_ms"], 4), "bytes_per_launch": int(bf["osem_bytes_per_launch"])}
     value = args.events / (bf["ms_per_step"] * 1e-3) / 1e6
     comp = {}
     for k, r in results.items():

@@ -15,8 +15,8 @@
  *     column of an [P, R] int64 tensor;
  *   - "err" words are device int32 flags OR-ed by kernels when an index is out
  *     of range (1: crystal number, 2: image gather, 4: weight gather,
- *     8: sensitivity gather, 16: thinning position).  The host raises
- *     IndexError when set.
+ *     8: sensitivity gather, 16: thinning position, 32: grid barrier time-out
+ *     of the persistent OSEM kernel).  The host raises IndexError when set.
  *   - storage dtypes: 0 = FP32, 1 = BF16, 2 = FP8 e4m3fn (per-row scale),
  *     3 = FP16 (IEEE half, per-row scale).
  *
@@ -188,6 +188,23 @@ int jscc_proj_em(const void* mat, int dtype, int64_t ld, int64_t n_cols, const i
                  float* w, int64_t w_len, float* em_x, const float* em_s, int em_mode, float em_eps, float em_c1,
                  float* em_w2, float em_c2, int zero_w, void* workspace, int64_t workspace_bytes, int32_t* err,
                  void* stream);
+
+/* Every subset of n_iter list-mode OSEM iterations in ONE cooperative launch
+ * (one work-group per CU): per subset the rows rows[sub_off[k] ..
+ * sub_off[k+1]) are streamed once (mode 0 of jscc_proj with y = 1, no
+ * rotation), the work-groups meet at a grid barrier, each sums the partial
+ * back-projections of its own pixels and applies the EM update (modes of
+ * jscc_em_update, c1 = 1) to x, and after a second barrier all reload x.  The
+ * first rows of the next subset are requested before the barriers.
+ * Equivalent to n_iter x K calls of jscc_proj_em (osem_list_mode,
+ * recon_osem_plane.py:63-80, one rotation and one energy).  sub_off: device
+ * int64 [n_sub + 1].  A grid barrier that does not complete within its
+ * polling bound sets err bit 32 and ends the launch (results invalid). */
+int64_t jscc_osem_persistent_workspace_bytes(int64_t n_cols);
+int jscc_osem_persistent(const void* mat, int dtype, int64_t ld, int64_t n_cols, const int32_t* rows,
+                         const float* row_scale, const int64_t* sub_off, int n_sub, int n_iter, float* x,
+                         const float* sens, float eps, int use_eps, int em_mode, float em_eps, void* workspace,
+                         int64_t workspace_bytes, int32_t* err, void* stream);
 
 /* EM update with w = c1*w1 (+ c2*w2):
  *   mode 0: x = x * w / s                        (recon_osem_plane.py:54, 73, 117-118)

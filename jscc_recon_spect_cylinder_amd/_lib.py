@@ -60,6 +60,9 @@ SIGNATURES = {
                                c_int, vp, vp, c_f32, c_int, vp, c_i64, vp, c_i64, vp, vp]),
     "jscc_sum_f64": (c_int, [vp, c_i64, c_i64, vp, vp]),
     "jscc_thin_counts": (c_int, [vp, c_i64, vp, c_i64, vp, c_i64, vp, vp, vp]),
+    "jscc_osem_persistent_workspace_bytes": (c_i64, [c_i64]),
+    "jscc_osem_persistent": (c_int, [vp, c_int, c_i64, c_i64, vp, vp, vp, c_int, c_int, vp, vp, c_f32, c_int, c_int,
+                                     c_f32, vp, c_i64, vp, vp]),
     "jscc_cone_batch_groups": (c_int, [c_i64, c_int]),
     "jscc_cone_batch_workspace_bytes": (c_i64, [c_i64, c_int, c_i64, c_int]),
     "jscc_cone_proj_batched": (c_int, [c_int, vp, vp, c_i64, vp, c_int, vp, vp, vp, c_i64, vp, c_i64, c_int, c_f64,

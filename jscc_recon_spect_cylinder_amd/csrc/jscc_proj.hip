@@ -250,6 +250,44 @@ __global__ void __launch_bounds__(kProjThreads)
   const int tid = threadIdx.x;
   const int nvec = a.P / VN;
 
+  // this work-group's contiguous share of the rows
+  const int64_t per = a.n_rows / gridDim.x, rem = a.n_rows % gridDim.x;
+  const int64_t r_begin = blockIdx.x * per + (blockIdx.x < rem ? blockIdx.x : rem);
+  const int64_t r_end = r_begin + per + (blockIdx.x < rem ? 1 : 0);
+
+  __shared__ RowMeta meta;
+  __shared__ __align__(16) float red[2][R][kProjWaves];
+  int buf = 0;
+  // Row metadata of a chunk into LDS (ids, scales, numerators).
+  auto stage = [&](int64_t c0, int64_t c1) {
+    __syncthreads();  // the previous chunk's metadata is no longer read
+    if (tid < c1 - c0) {
+      const int64_t i = c0 + tid;
+      const int64_t sid = rows ? (int64_t)rows[a.row_base + i] : a.row_base + i;
+      meta.sid[tid] = sid;
+      meta.sc[tid] = row_scale ? row_scale[sid] : 1.0f;
+      float num = 1.0f;
+      if (MODE == MODE_COEF) {
+        num = coef[i];
+      } else if (MODE == MODE_RATIO && y) {
+        num = y[(a.y_by_row ? sid : i) * a.y_stride];
+      }
+      meta.num[tid] = num;
+    }
+    __syncthreads();
+  };
+
+  // Two register buffers in ping-pong: the next R rows are in flight while
+  // the current R rows are reduced and accumulated.  Every fetch is issued
+  // (slots past the chunk read nothing), so the loop body is straight-line.
+  // The first rows are requested before the image is loaded into LDS, so that
+  // their HBM latency overlaps the image load.
+  RowBuf<DT, NCH, R> ba, bb;
+  if (r_begin < r_end) {
+    stage(r_begin, r_begin + kStage < r_end ? r_begin + kStage : r_end);
+    fetch_rows<DT, NCH, R>(a, meta, r_begin, r_begin, r_begin + kStage < r_end ? r_begin + kStage : r_end, nvec,
+                           tid, ba);
+  }
   // The image (gathered through the rotation) is kept in LDS as per-thread
   // storage: each thread only reads back the positions it wrote, in the same
   // chunk layout as the rows, so no barrier is needed and the registers go to
@@ -305,39 +343,15 @@ __global__ void __launch_bounds__(kProjThreads)
     }
   }
 
-  // this work-group's contiguous share of the rows
-  const int64_t per = a.n_rows / gridDim.x, rem = a.n_rows % gridDim.x;
-  const int64_t r_begin = blockIdx.x * per + (blockIdx.x < rem ? blockIdx.x : rem);
-  const int64_t r_end = r_begin + per + (blockIdx.x < rem ? 1 : 0);
 
-  __shared__ RowMeta meta;
-  __shared__ __align__(16) float red[2][R][kProjWaves];
-  int buf = 0;
   for (int64_t c0 = r_begin; c0 < r_end; c0 += kStage) {
     const int64_t c1 = c0 + kStage < r_end ? c0 + kStage : r_end;
-    __syncthreads();  // the previous chunk's metadata is no longer read
-    if (tid < c1 - c0) {
-      const int64_t i = c0 + tid;
-      const int64_t sid = rows ? (int64_t)rows[a.row_base + i] : a.row_base + i;
-      meta.sid[tid] = sid;
-      meta.sc[tid] = row_scale ? row_scale[sid] : 1.0f;
-      float num = 1.0f;
-      if (MODE == MODE_COEF) {
-        num = coef[i];
-      } else if (MODE == MODE_RATIO && y) {
-        num = y[(a.y_by_row ? sid : i) * a.y_stride];
-      }
-      meta.num[tid] = num;
+    if (c0 != r_begin) {
+      stage(c0, c1);
+      fetch_rows<DT, NCH, R>(a, meta, c0, c0, c1, nvec, tid, ba);
     }
-    __syncthreads();
-
-    // Two register buffers in ping-pong: the next R rows are in flight while
-    // the current R rows are reduced and accumulated.  Every fetch is issued
-    // (slots past the chunk read nothing), so the loop body is straight-line.
-    RowBuf<DT, NCH, R> ba, bb;
     const int64_t n_steps = (c1 - c0 + R - 1) / R;
     int64_t i0 = c0;
-    fetch_rows<DT, NCH, R>(a, meta, i0, c0, c1, nvec, tid, ba);
     for (int64_t st = 0; st + 2 <= n_steps; st += 2) {
       fetch_rows<DT, NCH, R>(a, meta, i0 + R, c0, c1, nvec, tid, bb);
       process_rows<DT, NCH, R, MODE>(a, meta, i0, c0, c1, tid, ba, xs4, red, buf, wr);
@@ -360,6 +374,201 @@ __global__ void __launch_bounds__(kProjThreads)
               make_float4(wr[c][k], wr[c][k + 1], wr[c][k + 2], wr[c][k + 3]);
       }
     }
+  }
+}
+
+// ============================================================================
+// Persistent list-mode OSEM: every subset of every iteration in ONE
+// cooperative launch (one work-group per CU, all resident).  Per subset:
+//   A. the work-groups stream their share of the subset's rows exactly as
+//      k_proj_fused does (ratio mode) and write their partial back-projection;
+//      each then requests the first rows of the NEXT subset (rows do not depend
+//      on the image), so their HBM latency hides behind the two barriers;
+//   B. grid barrier; work-group g sums the partials of its own columns in a
+//      fixed order and applies the EM update to those pixels of x;
+//   C. grid barrier; every work-group reloads the updated image into LDS.
+// The launch-to-launch gaps, per-launch ramp and the separate reduction
+// launch of the one-launch-per-subset path disappear.  The grid barrier is a
+// counter + generation pair with agent-scope release/acquire (the fences
+// write back / invalidate the per-XCD L2 caches), and a bounded wait: a
+// barrier that does not complete within spin_limit polls sets error bit 32
+// and every work-group leaves the kernel.
+// ============================================================================
+__device__ __forceinline__ bool grid_sync(unsigned* bar, unsigned n_groups, int64_t spin_limit, int32_t* err) {
+  __shared__ int ok;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    ok = 1;
+    __threadfence();  // release this group's writes device-wide
+    const unsigned gen = __hip_atomic_load(bar + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned arrived = __hip_atomic_fetch_add(bar, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1u;
+    if (arrived == n_groups) {
+      __hip_atomic_store(bar, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(bar + 1, gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      int64_t spins = 0;
+      // relaxed polling: an acquire load per poll would invalidate this XCD's L2 every time;
+      // the fence below is the one acquire
+      while (__hip_atomic_load(bar + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+        __builtin_amdgcn_s_sleep(2);
+        if (++spins > spin_limit) {
+          atomicOr(err, 32);
+          ok = 0;
+          break;
+        }
+      }
+    }
+    __threadfence();  // acquire: what the other groups wrote is visible from here on
+  }
+  __syncthreads();
+  return ok != 0;
+}
+
+struct OsemArgs {
+  int64_t ld;
+  int P;
+  int n_sub;
+  int n_iter;
+  float eps;
+  int use_eps;
+  int em_mode;
+  float em_eps;
+  int64_t spin_limit;
+};
+
+// rows [lo, hi) of the kept list split evenly over the work-groups
+__device__ __forceinline__ void share_of(int64_t lo, int64_t hi, int g, int G, int64_t* rb, int64_t* re) {
+  const int64_t n = hi - lo, per = n / G, rem = n % G;
+  *rb = lo + g * per + (g < rem ? g : rem);
+  *re = *rb + per + (g < rem ? 1 : 0);
+}
+
+template <int DT, int NCH, int R>
+__global__ void __launch_bounds__(kProjThreads)
+    k_osem_persistent(const void* __restrict__ M, const int32_t* __restrict__ rows,
+                      const float* __restrict__ row_scale, float* __restrict__ x, const float* __restrict__ sens,
+                      float* __restrict__ partial, const int64_t* __restrict__ sub_off, unsigned* __restrict__ bar,
+                      int32_t* __restrict__ err, OsemArgs o) {
+  constexpr int VN = Vec<DT>::N;
+  const int tid = threadIdx.x;
+  const int g = blockIdx.x;
+  const int G = gridDim.x;
+  const int nvec = o.P / VN;
+  ProjArgs a;
+  a.M = M; a.ld = o.ld; a.P = o.P; a.rows = rows; a.n_rows = 0; a.row_base = 0; a.row_scale = row_scale;
+  a.x = x; a.x_perm = nullptr; a.x_perm_stride = 0; a.x_len = o.P; a.y = nullptr; a.y_stride = 0; a.coef = nullptr;
+  a.eps = o.eps; a.use_eps = o.use_eps; a.fwd_out = nullptr; a.partial = partial; a.err = err; a.y_by_row = 0;
+
+  extern __shared__ jscc_f4v xs4[];
+  __shared__ RowMeta meta;
+  __shared__ __align__(16) float red[2][R][kProjWaves];
+  int buf = 0;
+  auto stage = [&](int64_t c0, int64_t c1) {
+    __syncthreads();
+    if (tid < c1 - c0) {
+      const int64_t sid = (int64_t)rows[c0 + tid];
+      meta.sid[tid] = sid;
+      meta.sc[tid] = row_scale ? row_scale[sid] : 1.0f;
+      meta.num[tid] = 1.0f;
+    }
+    __syncthreads();
+  };
+  auto load_image = [&]() {
+    const jscc_f4v* x4 = reinterpret_cast<const jscc_f4v*>(x);
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+      const int vi = c * kProjThreads + tid;
+#pragma unroll
+      for (int kk = 0; kk < VN / 4; ++kk)
+        xs4[(c * (VN / 4) + kk) * kProjThreads + tid] = vi < nvec ? x4[vi * (VN / 4) + kk] : jscc_f4v(0.0f);
+    }
+  };
+
+  RowBuf<DT, NCH, R> ba, bb;
+  float wr[NCH][VN];
+  const int total = o.n_sub * o.n_iter;
+  int64_t rb, re;
+  share_of(sub_off[0], sub_off[1], g, G, &rb, &re);
+  if (rb < re) {
+    stage(rb, rb + kStage < re ? rb + kStage : re);
+    fetch_rows<DT, NCH, R>(a, meta, rb, rb, rb + kStage < re ? rb + kStage : re, nvec, tid, ba);
+  }
+  load_image();
+  for (int step = 0; step < total; ++step) {
+    const int k = step % o.n_sub;
+#pragma unroll
+    for (int c = 0; c < NCH; ++c)
+#pragma unroll
+      for (int j = 0; j < VN; ++j) wr[c][j] = 0.0f;
+    // ---- A: this group's rows of subset k (the first R are already in flight)
+    for (int64_t c0 = rb; c0 < re; c0 += kStage) {
+      const int64_t c1 = c0 + kStage < re ? c0 + kStage : re;
+      if (c0 != rb) {
+        stage(c0, c1);
+        fetch_rows<DT, NCH, R>(a, meta, c0, c0, c1, nvec, tid, ba);
+      }
+      const int64_t n_steps = (c1 - c0 + R - 1) / R;
+      int64_t i0 = c0;
+      for (int64_t st = 0; st + 2 <= n_steps; st += 2) {
+        fetch_rows<DT, NCH, R>(a, meta, i0 + R, c0, c1, nvec, tid, bb);
+        process_rows<DT, NCH, R, MODE_RATIO>(a, meta, i0, c0, c1, tid, ba, xs4, red, buf, wr);
+        fetch_rows<DT, NCH, R>(a, meta, i0 + 2 * R, c0, c1, nvec, tid, ba);
+        process_rows<DT, NCH, R, MODE_RATIO>(a, meta, i0 + R, c0, c1, tid, bb, xs4, red, buf, wr);
+        i0 += 2 * R;
+      }
+      if (n_steps & 1) process_rows<DT, NCH, R, MODE_RATIO>(a, meta, i0, c0, c1, tid, ba, xs4, red, buf, wr);
+    }
+    {
+      float* dst = partial + (int64_t)g * o.P;
+#pragma unroll
+      for (int c = 0; c < NCH; ++c) {
+        const int vi = c * kProjThreads + tid;
+        if (vi < nvec) {
+#pragma unroll
+          for (int j = 0; j < VN; j += 4)
+            *reinterpret_cast<float4*>(dst + (int64_t)vi * VN + j) =
+                make_float4(wr[c][j], wr[c][j + 1], wr[c][j + 2], wr[c][j + 3]);
+        }
+      }
+    }
+    // the next subset's first rows go out now
+    if (step + 1 < total) {
+      const int kn = (step + 1) % o.n_sub;
+      share_of(sub_off[kn], sub_off[kn + 1], g, G, &rb, &re);
+      if (rb < re) {
+        stage(rb, rb + kStage < re ? rb + kStage : re);
+        fetch_rows<DT, NCH, R>(a, meta, rb, rb, rb + kStage < re ? rb + kStage : re, nvec, tid, ba);
+      }
+    }
+    if (!grid_sync(bar, (unsigned)G, o.spin_limit, err)) return;
+    // ---- B: column sums of this group's pixels over all partials, then the EM update.
+    // 1024 threads = 64 groups x 16 float4 columns; the image LDS is the scratch (reloaded after).
+    {
+      const int tx = tid & 15, ty = tid >> 4;
+      const int cpw = ((o.P + G - 1) / G + 3) / 4 * 4;  // columns per work-group, multiple of 4
+      const int col_begin = g * cpw;
+      jscc_f4v* acc = xs4;  // [64][16]
+      for (int cb = col_begin; cb < col_begin + cpw && cb < o.P; cb += 64) {
+        const int col = cb + tx * 4;
+        jscc_f4v sum = jscc_f4v(0.0f);
+        if (col < o.P && col < col_begin + cpw) {
+          for (int gg = ty; gg < G; gg += 64)
+            sum += *reinterpret_cast<const jscc_f4v*>(partial + (int64_t)gg * o.P + col);
+        }
+        __syncthreads();
+        acc[ty * 16 + tx] = sum;
+        __syncthreads();
+        if (ty == 0 && col < o.P && col < col_begin + cpw) {
+          jscc_f4v t = acc[tx];
+          for (int q = 1; q < 64; ++q) t += acc[q * 16 + tx];
+          const float tv[4] = {t.x, t.y, t.z, t.w};
+#pragma unroll
+          for (int j = 0; j < 4; ++j) x[col + j] = jscc_em_value(x[col + j], tv[j], sens[col + j], o.em_mode, o.em_eps);
+        }
+      }
+    }
+    if (!grid_sync(bar, (unsigned)G, o.spin_limit, err)) return;
+    load_image();
   }
 }
 
@@ -550,6 +759,45 @@ int dispatch_nch(int mode, const ProjArgs& a, int G, hipStream_t s) {
   if (nch <= 4) return launch_fused<DT, 4, rows_per_buf(VN, 4)>(mode, a, G, s);
   if (DT != JSCC_DT_FP8 && nch <= 8) return launch_fused<DT, (DT == JSCC_DT_FP8 ? 4 : 8), 1>(mode, a, G, s);
   jscc_set_error("jscc_proj: %d columns exceed the register-resident limit for this dtype", a.P);
+  return -3;
+}
+
+__global__ void __launch_bounds__(64) k_zero_u32(unsigned* __restrict__ v, int n) {
+  if ((int)threadIdx.x < n) v[threadIdx.x] = 0u;
+}
+
+template <int DT, int NCH, int R>
+int launch_persistent(const void* M, const int32_t* rows, const float* row_scale, float* x, const float* sens,
+                      float* partial, const int64_t* sub_off, unsigned* bar, int32_t* err, OsemArgs o, int G,
+                      hipStream_t s) {
+  const size_t img = (size_t)NCH * kProjThreads * Vec<DT>::N * 4;
+  const size_t lds = img > 16384 ? img : 16384;  // the column-sum scratch of phase B needs 16 KB
+  const void* fn = reinterpret_cast<const void*>(&k_osem_persistent<DT, NCH, R>);
+  if (lds > 65536) JSCC_CHECK_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  int per_cu = 0;
+  JSCC_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kProjThreads, lds));
+  JSCC_REQUIRE(per_cu >= 1, "jscc_osem_persistent: the kernel does not fit on a compute unit");
+  void* args[] = {(void*)&M, (void*)&rows, (void*)&row_scale, (void*)&x, (void*)&sens, (void*)&partial,
+                  (void*)&sub_off, (void*)&bar, (void*)&err, (void*)&o};
+  JSCC_CHECK_HIP(hipLaunchCooperativeKernel(fn, dim3(G), dim3(kProjThreads), args, (unsigned)lds, s));
+  return 0;
+}
+
+template <int DT>
+int dispatch_persistent(const void* M, const int32_t* rows, const float* row_scale, float* x, const float* sens,
+                        float* partial, const int64_t* sub_off, unsigned* bar, int32_t* err, OsemArgs o, int G,
+                        hipStream_t s) {
+  constexpr int VN = Vec<DT>::N;
+  const int nch = (o.P / VN + kProjThreads - 1) / kProjThreads;
+  // half the rows per buffer of k_proj_fused: the prefetched buffer stays live across the barriers
+#define JSCC_PERSIST(N)                                                                                      \
+  return launch_persistent<DT, N, (rows_per_buf(VN, N) > 1 ? rows_per_buf(VN, N) / 2 : 1)>(                  \
+      M, rows, row_scale, x, sens, partial, sub_off, bar, err, o, G, s)
+  if (nch <= 1) JSCC_PERSIST(1);
+  if (nch <= 2) JSCC_PERSIST(2);
+  if (nch <= 4) JSCC_PERSIST(4);
+#undef JSCC_PERSIST
+  jscc_set_error("jscc_osem_persistent: %d columns exceed the persistent kernel's limit for this dtype", o.P);
   return -3;
 }
 
@@ -772,4 +1020,42 @@ extern "C" int jscc_sensitivity(const float* A, int64_t M, int64_t lda, int64_t 
                      (int)G, P, inv, inv_row_stride, inv_col_stride, R, out, err);
   JSCC_CHECK_LAUNCH();
   return 0;
+}
+
+extern "C" int64_t jscc_osem_persistent_workspace_bytes(int64_t n_cols) {
+  return 256 + (int64_t)jscc_num_cus() * n_cols * (int64_t)sizeof(float);
+}
+
+extern "C" int jscc_osem_persistent(const void* mat, int dtype, int64_t ld, int64_t n_cols, const int32_t* rows,
+                                    const float* row_scale, const int64_t* sub_off, int n_sub, int n_iter, float* x,
+                                    const float* sens, float eps, int use_eps, int em_mode, float em_eps,
+                                    void* workspace, int64_t workspace_bytes, int32_t* err, void* stream) {
+  JSCC_REQUIRE(n_sub >= 1 && n_iter >= 0, "jscc_osem_persistent: bad subset / iteration count");
+  JSCC_REQUIRE(mat && rows && sub_off && x && sens && err && workspace, "jscc_osem_persistent: null pointer");
+  const int vw = jscc_proj_vec_width(dtype);
+  JSCC_REQUIRE(vw > 0, "jscc_osem_persistent: bad dtype %d", dtype);
+  JSCC_REQUIRE(n_cols > 0 && n_cols % vw == 0 && ld % vw == 0 && ld >= n_cols,
+               "jscc_osem_persistent: columns and row stride must be multiples of %d", vw);
+  JSCC_REQUIRE(dtype != JSCC_DT_FP8 || row_scale != nullptr, "jscc_osem_persistent: FP8 rows need a per-row scale");
+  JSCC_REQUIRE(((uintptr_t)mat | (uintptr_t)x) % 16 == 0, "jscc_osem_persistent: matrix and image must be 16-byte aligned");
+  JSCC_REQUIRE(em_mode >= 0 && em_mode <= 2, "jscc_osem_persistent: bad EM mode");
+  JSCC_REQUIRE(workspace_bytes >= jscc_osem_persistent_workspace_bytes(n_cols), "jscc_osem_persistent: workspace too small");
+  if (n_iter == 0) return 0;
+  hipStream_t s = (hipStream_t)stream;
+  unsigned* bar = (unsigned*)workspace;
+  float* partial = (float*)((char*)workspace + 256);
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, bar, 2);
+  JSCC_CHECK_LAUNCH();
+  OsemArgs o;
+  o.ld = ld; o.P = (int)n_cols; o.n_sub = n_sub; o.n_iter = n_iter; o.eps = eps; o.use_eps = use_eps;
+  o.em_mode = em_mode; o.em_eps = em_eps; o.spin_limit = 4000000;  // ~0.1-1 s of polling: far above any phase
+  const int G = jscc_num_cus();
+  int rc;
+  jscc_prof_begin(JSCC_TAG_PROJ, s);
+  if (dtype == JSCC_DT_F32) rc = dispatch_persistent<JSCC_DT_F32>(mat, rows, row_scale, x, sens, partial, sub_off, bar, err, o, G, s);
+  else if (dtype == JSCC_DT_BF16) rc = dispatch_persistent<JSCC_DT_BF16>(mat, rows, row_scale, x, sens, partial, sub_off, bar, err, o, G, s);
+  else if (dtype == JSCC_DT_FP16) rc = dispatch_persistent<JSCC_DT_FP16>(mat, rows, row_scale, x, sens, partial, sub_off, bar, err, o, G, s);
+  else rc = dispatch_persistent<JSCC_DT_FP8>(mat, rows, row_scale, x, sens, partial, sub_off, bar, err, o, G, s);
+  jscc_prof_end(JSCC_TAG_PROJ, s);
+  return rc;
 }

@@ -69,8 +69,19 @@ class ListModeStore:
                  out_scale=scale, accumulate=True, w=w)
 
     def osem(self, x: torch.Tensor, s: torch.Tensor, n_iter: int, n_subsets: int, eps=None,
-             em_mode: int = ops.EM_PLAIN) -> torch.Tensor:
-        """List-mode OSEM (osem_list_mode semantics, one rotation, one energy); x updated in place."""
+             em_mode: int = ops.EM_PLAIN, persistent: bool = False) -> torch.Tensor:
+        """List-mode OSEM (osem_list_mode semantics, one rotation, one energy); x updated in place:
+        one fused projection + one reduction/update launch per subset (jscc_proj_em).
+        persistent=True runs all subsets and iterations in one cooperative launch
+        (jscc_osem_persistent) instead; on MI355X its two grid barriers per subset cost more
+        (~100 us per subset at 256 work-groups) than the launch they replace, so it is not the default."""
+        if persistent and self.n_kept > 0:
+            off = torch.tensor([b[0] for b in self.subset_bounds(n_subsets)] + [self.n_kept], dtype=torch.int64,
+                               device=self.mat.device)
+            ops.osem_persistent(self.mat, self.dtype_c, self.n_vox, self.kept, self.row_scale, off, n_iter, x, s,
+                                eps=eps, em_mode=em_mode)
+            ops.raise_on_index_error(self.mat.device)
+            return x
         w = torch.zeros_like(x)
         bounds = self.subset_bounds(n_subsets)
         for _ in range(n_iter):

@@ -55,7 +55,8 @@ _ERR_TEXT = {1: "crystal number out of range of the detector table",
              2: "image gather index (RotMat) out of range",
              4: "weight gather index (RotMatInv) out of range",
              8: "sensitivity gather index (RotMatInv) out of range",
-             16: "thinning position out of range of the expanded counts"}
+             16: "thinning position out of range of the expanded counts",
+             32: "grid barrier of the persistent OSEM kernel timed out (results invalid)"}
 
 
 def err_word(device: torch.device) -> torch.Tensor:
@@ -590,6 +591,35 @@ def cone_batch_workspace(dev, n_rows: int, n_recon: int, P: int, groups_per_cu: 
     """A dedicated workspace for the batched projection (stable address, e.g. for graph capture)."""
     nbytes = _lib.lib().jscc_cone_batch_workspace_bytes(max(n_rows, 1), n_recon, P, int(groups_per_cu))
     return torch.empty(nbytes, dtype=torch.uint8, device=dev)
+
+
+_persist_ws: dict = {}
+
+
+def osem_persistent(mat: torch.Tensor, dtype_c: int, n_cols: int, rows: torch.Tensor, row_scale, sub_off: torch.Tensor,
+                    n_iter: int, x: torch.Tensor, sens: torch.Tensor, eps=None, em_mode: int = 0,
+                    em_eps: float = 1e-12) -> None:
+    """All n_iter x K list-mode OSEM subset steps in one cooperative launch (jscc_osem_persistent);
+    sub_off: int64 [K+1] device offsets into the kept-row list `rows`.  x is updated in place."""
+    dev = mat.device
+    for t in (x, sens):
+        if t.dtype != torch.float32 or t.numel() < n_cols or not t.is_contiguous() or t.device != dev:
+            raise ValueError("image and sensitivity must be contiguous float32 with one value per pixel")
+    if rows.dtype != torch.int32 or sub_off.dtype != torch.int64 or sub_off.device != dev:
+        raise ValueError("row list int32 and subset offsets int64 on the matrix's device")
+    K = sub_off.numel() - 1
+    nbytes = _lib.lib().jscc_osem_persistent_workspace_bytes(n_cols)
+    key = (str(dev), nbytes)
+    ws = _persist_ws.get(key)
+    if ws is None:
+        _persist_ws.clear()
+        ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+        _persist_ws[key] = ws
+    rc = _lib.lib().jscc_osem_persistent(
+        mat.data_ptr(), dtype_c, mat.stride(0), n_cols, rows.data_ptr(), _p(row_scale), sub_off.data_ptr(), K,
+        n_iter, x.data_ptr(), sens.data_ptr(), float(eps) if eps is not None else 0.0, 1 if eps is not None else 0,
+        int(em_mode), float(em_eps), ws.data_ptr(), ws.numel(), err_word(dev).data_ptr(), stream_of(dev))
+    _lib.check(rc, "jscc_osem_persistent")
 
 
 def thin_counts(counts: torch.Tensor, positions: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:

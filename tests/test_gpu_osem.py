@@ -352,3 +352,37 @@ def test_fp16_rows_scaled_to_two_pow_14():
     dec = codes[:, :512].float().cpu().numpy() * scale.cpu().numpy()[:, None]
     # half precision: 11 significant bits for values within 2^-28 of the row max
     assert np.all(np.abs(dec - A) <= 2.0 ** -11 * np.abs(A) + 2.0 ** -24 * scale.cpu().numpy()[:, None])
+
+
+@pytest.mark.parametrize("dtype", ["fp32", "bf16", "fp16", "fp8"])
+def test_persistent_osem_equals_per_subset_launches(dtype):
+    """All subsets of all iterations in one cooperative launch (grid barriers between the
+    phases) reproduce the one-launch-per-subset path up to the column-sum order."""
+    from jscc_recon_spect_cylinder_amd import ops, synth
+    from jscc_recon_spect_cylinder_amd.engine import ListModeStore
+    from jscc_recon_spect_cylinder_amd.process_list_plane_strict import build_compton_rows
+    cl = synth.compton_list(n_events=6000, n_rings=32, n_angles=128, seed=17)
+    tmax, tmin, tsum = synth.energy_thresholds(cl.e0)
+    rows = build_compton_rows(None, torch.from_numpy(cl.detector), torch.from_numpy(cl.coor),
+                              torch.from_numpy(cl.list_origin.copy()), 0.0, 0.0, cl.e0, 0.02, tmax, tmin, tsum, DEV,
+                              noise="none", sigma_model="fixed", fixed_sigma=np.deg2rad(3.0),
+                              store_dtype=torch.bfloat16 if dtype == "bf16" else torch.float32)
+    st = ListModeStore.from_rows(rows, dtype)
+    s = st.backproject_ones()
+    x1 = st.osem(torch.ones_like(s), s, 3, 5, persistent=False)
+    x2 = st.osem(torch.ones_like(s), s, 3, 5, persistent=True)
+    assert rel(x2.cpu().numpy(), x1.cpu().numpy()) < 2e-6
+    assert int(ops.err_word(torch.device(DEV)).item()) == 0
+
+
+def test_persistent_osem_with_fewer_rows_than_work_groups():
+    """Subsets smaller than the grid (idle work-groups still meet every barrier)."""
+    from jscc_recon_spect_cylinder_amd.engine import ListModeStore
+    rng = np.random.default_rng(30)
+    T = torch.from_numpy((rng.random((90, 1024)) ** 4).astype(np.float32)).to(DEV)
+    kept = torch.arange(90, dtype=torch.int32, device=DEV)
+    st = ListModeStore(T, 0, 1024, kept, 90, torch.ones(90, device=DEV))
+    s = st.backproject_ones()
+    x1 = st.osem(torch.ones_like(s), s, 4, 7, persistent=False)
+    x2 = st.osem(torch.ones_like(s), s, 4, 7, persistent=True)
+    assert rel(x2.cpu().numpy(), x1.cpu().numpy()) < 2e-6
