@@ -142,6 +142,30 @@ int jscc_materialize_rows(const float* packed, int64_t ld_packed, int64_t n_rows
                           const float* sysmat, int64_t lda, int64_t M, float* out, int64_t ld_out, int normalize,
                           int32_t* flags, void* stream);
 
+/* Fused list-mode weight of packed coarse rows [cpnum1, t_c(n_zc*n_lc)], the
+ * sparse production mode, with no fine row written:
+ *   t = expand(t_c) * sysmat[clamp(round(cpnum1)-1, 0, n_arows-1)], expand =
+ *   z_expand (x) layer_expand given as two taps per fine index: z_idx/z_w
+ *   [n_zf][2] (coarse z index, weight), l_idx/l_w [n_lf][2] (coarse layer index,
+ *   weight; l_idx = NULL: identity layer map);
+ *   normalize != 0: compton_sparse_ops.py:380-400 then
+ *     recon_osem_local_sparse_jsccsd_only.py:13-19 (element-wise nan_to_num, rows
+ *     with a non-finite or non-positive sum dropped, t/sum, clamped ratio);
+ *   normalize == 0: get_weight_compton_sparse of recon_osem_plane_sparse.py:22-57
+ *     (unnormalised rows, clamped ratio).
+ * w[j] (+)= out_scale * nan_to_num(sum_i t_i[src(j)] / max(t_i . x_rot, eps)),
+ * x_rot / src as in jscc_proj.  Fine pixels n_zf*n_lf <= 32768 (a multiple
+ * of 4), coarse pixels <= 16384.  Error word bits: 2 x_perm, 4 inv_perm,
+ * 64 tap table index out of range. */
+int64_t jscc_coarse_weight_workspace_bytes(int64_t n_rows, int64_t n_fine);
+int jscc_coarse_weight(const float* packed, int64_t ld_packed, int64_t n_rows, const int32_t* z_idx,
+                       const float* z_w, int n_zf, int n_zc, const int32_t* l_idx, const float* l_w, int n_lf,
+                       int n_lc, const float* sysmat, int64_t lda, int64_t n_arows, const float* x,
+                       const int64_t* x_perm, int64_t x_perm_stride, int64_t x_len, float eps, int use_eps,
+                       int normalize, const int64_t* inv_perm, int64_t inv_perm_stride, float out_scale,
+                       int accumulate, float* w, int64_t w_len, void* workspace, int64_t workspace_bytes,
+                       int32_t* err, void* stream);
+
 /* ---- OSEM projections --------------------------------------------------- */
 
 /* Fused one-pass projection of a block of rows (A or T), replacing
@@ -287,7 +311,8 @@ int jscc_fp8_selftest(float* out512, void* stream);
  * (per calling thread), the main kernel of every launch is bracketed by a HIP
  * event pair on its own stream, by tag: 0 cone rows (jscc_cone_rows),
  * 1 projection modes 0-2 (jscc_proj / jscc_proj_em), 2 projection mode 3
- * (column sums), 3 no-store cone projection (jscc_cone_proj).
+ * (column sums), 3 no-store cone projection (jscc_cone_proj), 4 fused coarse
+ * row weight (jscc_coarse_weight).
  * jscc_prof_read synchronises the pairs of one tag and returns their count and
  * summed milliseconds. */
 int jscc_prof_enable(int on);

@@ -25,6 +25,7 @@ SOURCES = [
     ("jscc_cone.hip", []),
     ("jscc_proj.hip", []),
     ("jscc_rows.hip", []),
+    ("jscc_coarse.hip", []),
 ]
 
 COMMON_FLAGS = [

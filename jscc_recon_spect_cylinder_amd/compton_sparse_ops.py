@@ -34,13 +34,14 @@ class SparseTables:
     adjoint: ops.CSR       # fine -> coarse, exact adjoint of expand
     reduce_t: ops.CSR      # coarse -> fine, transpose of reduce (upsample_coarse_weight_to_fine)
     taps: ops.ATaps | None  # the <=2-tap reduce as cone-kernel tables
+    axis: ops.AxisTaps | None = None  # the separable expand as per-axis taps (fused coarse-row weight)
 
     def to(self, device):
         taps = None
         if self.taps is not None:
             taps = ops.ATaps(*(t.to(device) for t in (self.taps.i0, self.taps.i1, self.taps.w0, self.taps.w1)))
         return SparseTables(self.reduce.to(device), self.expand.to(device), self.adjoint.to(device),
-                            self.reduce_t.to(device), taps)
+                            self.reduce_t.to(device), taps, self.axis.to(device) if self.axis is not None else None)
 
 
 @dataclass
@@ -271,7 +272,8 @@ def build_compton_sparse_projector(coor_full, theta_stride=2, z_stride=2, rotate
                                        dim=1) for zv in coarse_z], dim=0)
     reduce = _csr_kron(z_reduce, layer_reduce)
     expand = _csr_kron(z_expand, layer_expand)
-    tables = SparseTables(reduce, expand, _csr_transpose(expand), _csr_transpose(reduce), _taps_of(reduce))
+    tables = SparseTables(reduce, expand, _csr_transpose(expand), _csr_transpose(reduce), _taps_of(reduce),
+                          ops.axis_taps(z_expand, layer_expand))
     return ComptonSparseProjector(
         theta_stride=theta_stride, z_stride=z_stride, ring_strides=tuple(strides), fine_layer_num=fine_layer_num,
         fine_z_num=fine_z_num, coarse_layer_num=coarse_layer.size(0), coarse_z_num=coarse_z.numel(),

@@ -39,7 +39,7 @@ struct ProfPair {
 thread_local bool g_prof_on = false;
 thread_local std::vector<ProfPair> g_prof_pending;
 thread_local std::vector<hipEvent_t> g_prof_pool;
-thread_local hipEvent_t g_prof_open[JSCC_NUM_TAGS] = {nullptr, nullptr};
+thread_local hipEvent_t g_prof_open[JSCC_NUM_TAGS] = {};
 
 hipEvent_t prof_event() {
   if (!g_prof_pool.empty()) {

@@ -205,7 +205,14 @@ int64_t jscc_partials_bytes(int G, int64_t P);
 
 // Built-in kernel timer: when enabled, hot kernels are bracketed by HIP event
 // pairs recorded on their launch stream (tag = kernel family).
-enum { JSCC_TAG_CONE = 0, JSCC_TAG_PROJ = 1, JSCC_TAG_PROJ_SUM = 2, JSCC_TAG_CONE_PROJ = 3, JSCC_NUM_TAGS = 4 };
+enum {
+  JSCC_TAG_CONE = 0,
+  JSCC_TAG_PROJ = 1,
+  JSCC_TAG_PROJ_SUM = 2,
+  JSCC_TAG_CONE_PROJ = 3,
+  JSCC_TAG_COARSE = 4,
+  JSCC_NUM_TAGS = 5
+};
 bool jscc_prof_on();
 void jscc_prof_begin(int tag, hipStream_t s);
 void jscc_prof_end(int tag, hipStream_t s);

@@ -142,6 +142,10 @@ __device__ __forceinline__ float zero_nonfinite(float v) {
 
 // compton_sparse_ops.py:380-400 : t_fine = nan_to_num(expand(t_c) * A[cp]) ; row-normalise ; valid
 // Rows are written normalised (in place, by the same threads) together with a flag.
+// A non-finite coarse value makes the whole fine row non-finite in the
+// reference's dense expansion (0 * inf = NaN): the row is written as zeros and
+// flagged invalid (normalize = 1) or written as NaN (normalize = 0, the
+// unnormalised rows of the grouped weight, whose products are not filtered).
 __global__ void __launch_bounds__(kRowThreads) k_materialize(const float* __restrict__ packed, int64_t ldp,
                                                              int64_t n, int64_t Pc, const int32_t* __restrict__ ptr,
                                                              const int32_t* __restrict__ col,
@@ -158,11 +162,19 @@ __global__ void __launch_bounds__(kRowThreads) k_materialize(const float* __rest
   const float* tc = row + 1;
   const float* a = A + cp * lda;
   float* o = out + r * ldo;
+  int bad = 0;
+  for (int64_t k = threadIdx.x; k < Pc; k += kRowThreads) {
+    const float v = tc[k];
+    bad |= (v == v && fabsf(v) != __builtin_inff()) ? 0 : 1;
+  }
+  bad = __syncthreads_or(bad);
   double s = 0.0;
   for (int64_t p = threadIdx.x; p < Pf; p += kRowThreads) {
     float u = 0.0f;
     for (int k = ptr[p]; k < ptr[p + 1]; ++k) u = fmaf(val[k], tc[col[k]], u);
-    const float t = zero_nonfinite(u * a[p]);
+    float t = u * a[p];
+    if (bad) t = normalize ? 0.0f : __builtin_nanf("");
+    else if (normalize) t = zero_nonfinite(t);
     o[p] = t;
     s += (double)t;
   }
@@ -171,7 +183,7 @@ __global__ void __launch_bounds__(kRowThreads) k_materialize(const float* __rest
   __syncthreads();
   double tot = 0.0;
   for (int w = 0; w < kRowThreads / JSCC_WAVE; ++w) tot += red[w];
-  const bool valid = (tot == tot) && fabs(tot) != __builtin_inf() && tot > 0.0;
+  const bool valid = !bad && (tot == tot) && fabs(tot) != __builtin_inf() && tot > 0.0;
   if (valid && normalize) {
     for (int64_t p = threadIdx.x; p < Pf; p += kRowThreads) o[p] = (float)((double)o[p] / tot);
   }

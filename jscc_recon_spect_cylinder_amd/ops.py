@@ -56,7 +56,8 @@ _ERR_TEXT = {1: "crystal number out of range of the detector table",
              4: "weight gather index (RotMatInv) out of range",
              8: "sensitivity gather index (RotMatInv) out of range",
              16: "thinning position out of range of the expanded counts",
-             32: "grid barrier of the persistent OSEM kernel timed out (results invalid)"}
+             32: "grid barrier of the persistent OSEM kernel timed out (results invalid)",
+             64: "expand tap table index out of range of the coarse grid"}
 
 
 def err_word(device: torch.device) -> torch.Tensor:
@@ -357,6 +358,116 @@ def materialize(packed: torch.Tensor, expand: CSR, sysmat: torch.Tensor, normali
                                           1 if normalize else 0, flags.data_ptr(), stream_of(dev))
     _lib.check(rc, "jscc_materialize_rows")
     return out[:n], flags[:n]
+
+
+@dataclass
+class AxisTaps:
+    """z_expand (x) layer_expand as at most two (index, weight) taps per fine z
+    slice and per fine layer pixel: the form jscc_coarse_weight expands coarse
+    rows with.  l_idx / l_w are None for an identity layer map."""
+    z_idx: torch.Tensor          # int32 [n_zf, 2] coarse z indices
+    z_w: torch.Tensor            # float32 [n_zf, 2]
+    n_zc: int
+    l_idx: torch.Tensor | None   # int32 [n_lf, 2] coarse layer indices
+    l_w: torch.Tensor | None     # float32 [n_lf, 2]
+    n_lf: int
+    n_lc: int
+
+    @property
+    def n_zf(self) -> int:
+        return self.z_idx.shape[0]
+
+    @property
+    def n_fine(self) -> int:
+        return self.n_zf * self.n_lf
+
+    @property
+    def n_coarse(self) -> int:
+        return self.n_zc * self.n_lc
+
+    def to(self, device):
+        mv = lambda t: None if t is None else t.to(device)  # noqa: E731
+        return AxisTaps(mv(self.z_idx), mv(self.z_w), self.n_zc, mv(self.l_idx), mv(self.l_w), self.n_lf, self.n_lc)
+
+
+def _two_taps(m: np.ndarray, what: str):
+    """(idx [n, 2] int32, w [n, 2] float32) of a dense interpolation matrix whose rows
+    hold one or two nonzeros (a single tap repeats its index with weight 0)."""
+    n = m.shape[0]
+    idx = np.zeros((n, 2), np.int32)
+    w = np.zeros((n, 2), np.float32)
+    for f in range(n):
+        nz = np.flatnonzero(m[f])
+        if nz.size == 0 or nz.size > 2:
+            raise ValueError(f"{what} row {f} has {nz.size} taps (1 or 2 expected)")
+        idx[f, 0], w[f, 0] = nz[0], m[f, nz[0]]
+        idx[f, 1] = nz[-1]
+        if nz.size == 2:
+            w[f, 1] = m[f, nz[1]]
+    return idx, w
+
+
+def axis_taps(z_expand, layer_expand) -> AxisTaps:
+    """Tap tables of the separable expand kron(z_expand, layer_expand) (dense [fine, coarse])."""
+    ze = np.asarray(torch.as_tensor(z_expand).detach().cpu().float().numpy())
+    le = np.asarray(torch.as_tensor(layer_expand).detach().cpu().float().numpy())
+    zi, zw = _two_taps(ze, "z_expand")
+    identity = le.shape[0] == le.shape[1] and np.array_equal(le, np.eye(le.shape[0], dtype=le.dtype))
+    li = lw = None
+    if not identity:
+        a, b = _two_taps(le, "layer_expand")
+        li, lw = torch.from_numpy(a), torch.from_numpy(b)
+    return AxisTaps(torch.from_numpy(zi), torch.from_numpy(zw), ze.shape[1], li, lw, le.shape[0], le.shape[1])
+
+
+_COARSE_MAX_FINE = 32 * 1024
+_COARSE_MAX_COARSE = 16 * 1024
+_COARSE_LDS = 160 * 1024
+
+
+def coarse_weight_fits(taps: AxisTaps) -> bool:
+    """Whether jscc_coarse_weight takes this grid: fine pixels <= 32768 (a multiple
+    of 4), coarse pixels <= 16384, and both coarse row buffers plus the tap tables
+    within the 160 KB of LDS of a work-group."""
+    P, Pc = taps.n_fine, taps.n_coarse
+    lds = 2 * round_up(Pc, 4) * 4 + 16 * taps.n_zf + (0 if taps.l_idx is None else 16 * taps.n_lf) + 3 * 16 * 4
+    return P <= _COARSE_MAX_FINE and P % 4 == 0 and Pc <= _COARSE_MAX_COARSE and lds <= _COARSE_LDS
+
+
+def coarse_weight(packed: torch.Tensor, taps: AxisTaps, sysmat: torch.Tensor, x: torch.Tensor, w: torch.Tensor, *,
+                  normalize: bool, eps: float | None = 1e-12, x_perm: torch.Tensor | None = None,
+                  inv_perm: torch.Tensor | None = None, out_scale: float = 1.0, accumulate: bool = True) -> None:
+    """w (+)= out_scale * nan_to_num(T^T (1 / max(T x_rot, eps))) for packed coarse rows
+    [cpnum1, t_c] expanded on the fly (jscc_coarse_weight, include/jscc_hip.h)."""
+    dev = packed.device
+    if packed.ndim != 2 or packed.shape[1] != taps.n_coarse + 1:
+        raise ValueError("packed rows must have shape [n, 1 + coarse_pixel_num]")
+    if packed.dtype != torch.float32 or packed.stride(1) != 1:
+        packed = packed.float().contiguous()
+    A = sysmat if (sysmat.dtype == torch.float32 and sysmat.stride(1) == 1) else sysmat.float().contiguous()
+    if A.ndim != 2 or A.shape[1] != taps.n_fine:
+        raise ValueError("sysmat columns must equal the fine pixel count")
+    if x_perm is None and x.shape[0] != taps.n_fine:
+        raise ValueError(f"image has {x.shape[0]} pixels, the fine grid {taps.n_fine}")
+    if x_perm is not None and x_perm.shape[0] < taps.n_fine:
+        raise ValueError(f"RotMat column has {x_perm.shape[0]} entries for {taps.n_fine} pixels")
+    if inv_perm is not None and inv_perm.shape[0] < w.shape[0]:
+        raise ValueError("RotMatInv column shorter than the weight vector")
+    for t in (A, x, w, x_perm, inv_perm, taps.z_idx):
+        if t is not None and t.device != dev:
+            raise ValueError("all operands must be on the rows' device")
+    if x.dtype != torch.float32 or not x.is_contiguous() or w.dtype != torch.float32 or not w.is_contiguous():
+        raise ValueError("image and weight must be contiguous float32")
+    n = packed.shape[0]
+    ws = _workspace(dev, _lib.lib().jscc_coarse_weight_workspace_bytes(max(n, 1), taps.n_fine))
+    rc = _lib.lib().jscc_coarse_weight(
+        packed.data_ptr() if n else None, packed.stride(0), n, taps.z_idx.data_ptr(), taps.z_w.data_ptr(),
+        taps.n_zf, taps.n_zc, _p(taps.l_idx), _p(taps.l_w), taps.n_lf, taps.n_lc, A.data_ptr(), A.stride(0),
+        A.shape[0], x.data_ptr(), _p(x_perm), x_perm.stride(0) if x_perm is not None else 0, x.shape[0],
+        float(eps) if eps is not None else 0.0, 1 if eps is not None else 0, 1 if normalize else 0, _p(inv_perm),
+        inv_perm.stride(0) if inv_perm is not None else 0, float(out_scale), 1 if accumulate else 0, w.data_ptr(),
+        w.shape[0], ws.data_ptr(), ws.numel(), err_word(dev).data_ptr(), stream_of(dev))
+    _lib.check(rc, "jscc_coarse_weight")
 
 
 # ----------------------------------------------------------------------------- projections

@@ -2,9 +2,11 @@
 (recon_osem_local_sparse_jsccsd_only.py).
 
 Same names and signatures as recon_osem_local_sparse_jsccsd_only.py:8-202.
-Per T block and iteration the coarse rows are materialised on the fine grid
+Per T block and iteration the coarse rows are expanded on the fine grid
 (normalised, invalid rows dropped) and T^T(1/max(Tx, 1e-12)) is formed with
-nan_to_num (:13-19); the system-matrix term uses the clamped weight, each
+nan_to_num (:13-19) in one gfx950 pass over the packed rows
+(jscc_coarse_weight; grids beyond its limits materialise the fine rows and run
+the fused projection); the system-matrix term uses the clamped weight, each
 term is scaled by alpha or 2-alpha before accumulation (:58-100) and the
 update is safe_em_update.  Invalid rows are kept in place with a zero row
 scale instead of being compacted: they contribute nothing, as in the
@@ -36,6 +38,11 @@ def _materialized_weight(w, event_block, sysmat_full, x, projector, *, x_perm=No
         return
     dev = x.device
     t = _tables(projector, dev)
+    if t.axis is not None and ops.coarse_weight_fits(t.axis):
+        # materialise + normalise + ratio back-projection in one pass over the packed rows
+        ops.coarse_weight(event_block.to(dev), t.axis, sysmat_full, x, w, normalize=True, eps=1e-12, x_perm=x_perm,
+                          inv_perm=inv_perm, out_scale=scale, accumulate=True)
+        return
     rows, flags = ops.materialize(event_block.to(dev), t.expand, sysmat_full, normalize=True)
     row_scale = flags.to(torch.float32)
     core.add_weight(w, ops.proj_ready(rows, ops.DT_F32), x, eps=1e-12, row_scale=row_scale, x_perm=x_perm,

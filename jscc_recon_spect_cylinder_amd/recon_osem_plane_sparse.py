@@ -3,9 +3,11 @@
 Same names and signatures as recon_osem_plane_sparse.py:22-348.  The grouped
 list-mode weight (:22-57) equals T_f^T (1 / max(T_f x, 1e-12)) with the
 unnormalised fine rows T_f = expand(t_c) * A[cpnum1] followed by nan_to_num;
-here the fine rows of a block are materialised by the gfx950 expand kernel and
-consumed by the fused projection kernel in one pass, instead of a Python loop
-over first-hit crystals.  safe_em_update is the kernel's mode 1.
+here one gfx950 kernel streams the packed coarse rows of a block, expands each
+row with index arithmetic, applies its A-row factor and accumulates the ratio
+back-projection (jscc_coarse_weight), instead of a Python loop over first-hit
+crystals.  Grids beyond that kernel's limits materialise the fine rows first
+and run the fused projection over them.  safe_em_update is the kernel's mode 1.
 """
 from __future__ import annotations
 
@@ -16,7 +18,7 @@ import torch
 
 from . import _osem_core as core
 from . import ops
-from .compton_sparse_ops import materialize_sparse_event_rows_to_fine
+from .compton_sparse_ops import _tables, materialize_sparse_event_rows_to_fine
 from .recon_osem_plane import (get_gpu_memory_usage, get_t_budget_bytes_per_gpu, get_weight_single,  # noqa: F401
                                osem_bin_mode, save_img, split_tensor_even)
 
@@ -29,6 +31,12 @@ def _sparse_block_weight(w, event_block, sysmat_full, x, sparse_projector, *, x_
     rows_blk = event_block.to(dev, non_blocking=True)
     A = sysmat_full if sysmat_full.device == dev else sysmat_full.to(dev, non_blocking=True)
     proj = sparse_projector if sparse_projector.coor_coarse.device == dev else sparse_projector.to(dev)
+    axis = _tables(proj, dev).axis
+    if axis is not None and ops.coarse_weight_fits(axis):
+        # one pass over the packed rows: expand, A-row factor and the ratio back-projection fused
+        ops.coarse_weight(rows_blk, axis, A, x, w, normalize=False, eps=1e-12, x_perm=x_perm, inv_perm=inv_perm,
+                          out_scale=scale, accumulate=True)
+        return
     t_fine, _ = materialize_sparse_event_rows_to_fine(rows_blk, A.float(), proj, normalize=False)
     core.add_weight(w, ops.proj_ready(t_fine, ops.DT_F32), x, eps=1e-12, x_perm=x_perm, inv_perm=inv_perm,
                     scale=scale, sanitize=True)
