@@ -41,11 +41,12 @@ def parse():
     ap.add_argument("--subsets", type=int, default=8)
     ap.add_argument("--cpu-sample", type=int, default=12000, help="events in the CPU-baseline sample (0: skip)")
     ap.add_argument("--skip-fp32", action="store_true", help="time the BF16 path only")
-    ap.add_argument("--workload", default="jsccd", choices=["jsccd", "sc", "jsccsd", "grad", "batched"],
+    ap.add_argument("--workload", default="jsccd", choices=["jsccd", "sc", "jsccsd", "grad", "batched", "sparse"],
                     help="jsccd (default, the reported metric): JSCCD list mode, 2e5 events; sc: single-photon "
                          "OSEM, A 5120x4096; jsccsd: joint mode, A 81920x16384 + 1e6 events, FP32/BF16/FP8; "
                          "grad: Poisson log-likelihood gradient through the no-store cone operator; batched: 48 "
-                         "reconstructions of 1e4-2e6 events in one launch per subset step")
+                         "reconstructions of 1e4-2e6 events in one launch per subset step; sparse: coarse-T "
+                         "list-mode OSEM with on-the-fly expansion, 2e5 packed rows on the 25600-voxel grid")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (HBM counter calibration) and exit")
     return ap.parse_args()
@@ -323,6 +324,114 @@ def run_jsccsd(args):
             "components": res, "cpu_baseline": cpu}
 
 
+def cpu_baseline_sparse(coor, packed, A, n_iter, K, eps=1e-12):
+    """The numpy oracle (FP64, one thread) on a bounded sample of the packed rows: the materialised-variant
+    weight (dense expansion, A-row factor, nan_to_num, row normalisation, clamped ratio back-projection) and
+    the safe EM update for n_iter iterations x K subsets; returns the baseline record and its image."""
+    sys.path.insert(0, HERE)
+    from threadpoolctl import threadpool_limits
+    from oracle import osem, sparse
+    pj = sparse.build(coor, 1, 2, 20)
+    P = coor.shape[0]
+    s = np.ones(P)
+    x = np.ones(P)
+    bounds = osem.chunk_bounds(packed.shape[0], K)
+    with threadpool_limits(limits=1):
+        t0 = time.perf_counter()
+        for _ in range(n_iter):
+            for lo, hi in bounds:
+                rows, _ = sparse.materialize(packed[lo:hi], A, pj)
+                w = osem.nan_to_num(osem.weight_compton(rows, x, eps=eps))
+                x = osem.em_safe(x, w, s)
+        dt = time.perf_counter() - t0
+    per_row_iter = dt / (n_iter * packed.shape[0])
+    return {"value": 1e-6 / per_row_iter, "unit": "Mevent-iterations/s", "cores": 1, "kind": "port",
+            "sample": f"numpy FP64 oracle (1 thread): {packed.shape[0]} packed rows, {n_iter} iterations x {K} "
+                      f"subsets in {dt:.2f} s (expand, A-row factor, normalisation, weight, safe EM update)",
+            "host_cpu": _cpu_model()}, x
+
+
+def run_sparse(args):
+    """Sparse (coarse-T) list-mode OSEM, the reference's production mode (recon_osem_local_sparse_jsccsd_only):
+    the 25600-voxel multi-slice grid of the 511keV_RotateNum20 factors (25 rings, 20 slices), theta stride 1,
+    z stride 2 (12800 coarse pixels), A 10496 x 25600 FP32, 2e5 packed coarse rows [cpnum1, t_c] expanded on the
+    fly by the fused weight kernel (materialised variant: normalised rows, clamped ratio, safe EM update),
+    10 iterations x 8 subsets; rows ordered by first-hit crystal within each subset."""
+    import torch
+    from jscc_recon_spect_cylinder_amd import _lib, ops, synth
+    from jscc_recon_spect_cylinder_amd.compton_sparse_ops import build_compton_sparse_projector
+    dev = torch.device("cuda:0")
+    lib = _lib.load()
+    coor = synth.polar_grid_3d(np.arange(6, 151, 6), [20, 40, 60, 80], np.arange(-28.5, 28.6, 3.0))
+    proj = build_compton_sparse_projector(torch.from_numpy(coor.astype(np.float32)), 1, 2, 20).to(dev)
+    ax = proj.tables.axis
+    P, Pc, M = proj.fine_pixel_num, proj.coarse_pixel_num, 10496
+    n = args.events
+    n_iter, K = args.iters, args.subsets
+    A = _big_sysmat(M, P, 20260335, dev)
+    packed = synth.packed_coarse_rows(n, Pc, M, dev, seed=20260335)
+    bounds = [(lo, min(n, lo + -(-n // K))) for lo in range(0, n, -(-n // K))]  # torch.chunk sizes
+    for lo, hi in bounds:  # first-hit crystal order within each subset (the weight is a sum over rows)
+        order = torch.argsort(packed[lo:hi, 0], stable=True)
+        packed[lo:hi] = packed[lo:hi][order]
+    s = torch.ones(P, device=dev)
+    w = torch.zeros(P, device=dev)
+
+    def run(x):
+        for _ in range(n_iter):
+            for lo, hi in bounds:
+                ops.coarse_weight(packed[lo:hi], ax, A, x, w, normalize=True, eps=1e-12, accumulate=True)
+                ops.em_update(x, w, s, mode=ops.EM_SAFE, zero_after=w)
+        return x
+
+    _prof_read(lib, _lib.PROF_COARSE)
+    lib.jscc_prof_enable(1)
+    dt, x = _timed(lambda: run(torch.ones(P, device=dev)), args.steps, args.warmup)
+    lib.jscc_prof_enable(0)
+    ops.raise_on_index_error(dev)
+    cnt, tot = _prof_read(lib, _lib.PROF_COARSE)
+    launches = n_iter * len(bounds)
+    cnt_step = cnt / (args.steps + args.warmup)
+    assert abs(cnt_step - launches) < 1e-6, (cnt, launches)
+    launch_ms = tot / cnt
+    # algorithmic bytes of one launch: its packed rows once, and each distinct A row it touches once
+    distinct = [int(torch.unique(packed[lo:hi, 0]).numel()) for lo, hi in bounds]
+    nbytes = sum((hi - lo) * (Pc + 1) * 4 + d * P * 4 for (lo, hi), d in zip(bounds, distinct)) / len(bounds)
+    achieved = nbytes / (launch_ms * 1e-3) / 1e9
+    # parity and CPU baseline on a bounded sample (the first 1000 rows)
+    ns = 1000
+    sample = packed[:ns].contiguous()
+    bs = [(lo, min(ns, lo + -(-ns // 2))) for lo in range(0, ns, -(-ns // 2))]
+    xg = torch.ones(P, device=dev)
+    for _ in range(3):
+        for lo, hi in bs:
+            ops.coarse_weight(sample[lo:hi], ax, A, xg, w, normalize=True, eps=1e-12, accumulate=True)
+            ops.em_update(xg, w, s, mode=ops.EM_SAFE, zero_after=w)
+    # the oracle gets only the A rows the sample touches (crystal numbers remapped to them)
+    cps, inv_idx = torch.unique(sample[:, 0], return_inverse=True)
+    sample_c = sample.clone()
+    sample_c[:, 0] = inv_idx.float() + 1.0
+    cpu, xref = cpu_baseline_sparse(coor.astype(np.float32), sample_c.cpu().numpy(),
+                                    A[cps.long() - 1].cpu().numpy(), 3, 2)
+    return {"metric": "sparse list-mode OSEM Mevent-iterations/s (packed coarse rows expanded in the weight pass)",
+            "value": round(n * n_iter / dt / 1e6, 3), "unit": "Mevent-iterations/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 3),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic (seeded): random row-normalised coarse rows, cpnum1 uniform over 10496 crystals, "
+                    "A with the single-photon value law",
+            "config": {"workload": f"sparse OSEM: {n} packed rows, 25600-voxel 511keV_RotateNum20 grid, theta 1 / "
+                                   f"z 2 coarse grid ({Pc} pixels), A {M}x{P} FP32, {n_iter} iterations x {K} "
+                                   "subsets"},
+            "iter_s": round(n_iter / dt, 3),
+            "roofline": {"kernel": "k_coarse_weight<25, identity layer, uniform z, normalised>: one subset",
+                         "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                         "launch_us": round(launch_ms * 1e3, 2), "bytes_per_launch": int(nbytes),
+                         "distinct_arows_per_subset": int(np.mean(distinct))},
+            "rel_err_vs_fp64_oracle_sample": _rel(xg.cpu().numpy(), xref),
+            "cpu_baseline": cpu}
+
+
 def run_grad(args):
     """Gradient of the list-mode Poisson log-likelihood w.r.t. the image and the angular blur sigma through the
     no-store cone operator (T recomputed in the backward), on the 2e5-event list at sigma = 3 deg."""
@@ -478,8 +587,8 @@ def main():
         copy_calibration(args.copy_calibration)
         return
     if args.workload != "jsccd":
-        print(json.dumps({"sc": run_sc, "jsccsd": run_jsccsd, "grad": run_grad,
-                          "batched": run_batched}[args.workload](args)))
+        print(json.dumps({"sc": run_sc, "jsccsd": run_jsccsd, "grad": run_grad, "batched": run_batched,
+                          "sparse": run_sparse}[args.workload](args)))
         return
     import torch
     from jscc_recon_spect_cylinder_amd import _lib, synth

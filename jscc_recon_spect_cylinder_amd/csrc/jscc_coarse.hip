@@ -40,6 +40,7 @@ constexpr int kMaxCoarse = kRowSlots * kCoarseThreads;
 // gfx9 raw buffer resource word 3
 constexpr int kBufWord3 = 0x00020000;
 typedef __attribute__((address_space(3))) void* lds_vp;
+typedef float jscc_f4v __attribute__((ext_vector_type(4)));
 
 struct AxisTap {
   int i0, i1;    // coarse indices (z: already multiplied by the coarse layer count)
@@ -63,7 +64,7 @@ __global__ void __launch_bounds__(kCoarseThreads)
   float* rowbuf = lds;                                                    // [2][pc_pad]
   AxisTap* zt = reinterpret_cast<AxisTap*>(lds + 2 * pc_pad);             // [Zf]
   AxisTap* lt = zt + Zf;                                                  // [Lf] (general layer map)
-  float* red = reinterpret_cast<float*>(LID ? lt : lt + Lf);              // [3][kCoarseWaves]
+  float* red = reinterpret_cast<float*>(LID ? lt : lt + Lf);              // [3][kCoarseWaves], 16-byte aligned
   (void)kMaxCoarse;
 
   // ---- tap tables (indices checked once; a bad table sets bit 64 and reads index 0)
@@ -158,9 +159,12 @@ __global__ void __launch_bounds__(kCoarseThreads)
     // a non-finite coarse value anywhere in the row
     int bad = 0;
 #pragma unroll 1
-    for (int m = 0; m < slots; ++m) {
-      const int cc = tid + m * kCoarseThreads;
-      if (cc < Pc) bad |= finite_f(c[cc]) ? 0 : 1;
+    for (int q = tid; q * 4 < Pc; q += kCoarseThreads) {
+      const jscc_f4v v = reinterpret_cast<const jscc_f4v*>(c)[q];  // the padding past Pc is not checked
+      bad |= finite_f(v.x) ? 0 : 1;
+      bad |= (q * 4 + 1 >= Pc || finite_f(v.y)) ? 0 : 1;
+      bad |= (q * 4 + 2 >= Pc || finite_f(v.z)) ? 0 : 1;
+      bad |= (q * 4 + 3 >= Pc || finite_f(v.w)) ? 0 : 1;
     }
     if (r + 1 < re) fetch(r + 1, buf ^ 1);
 
@@ -251,11 +255,13 @@ __global__ void __launch_bounds__(kCoarseThreads)
     }
     __syncthreads();
     float St = 0.0f, Dt = 0.0f, Bt = 0.0f;
+    const jscc_f4v* red4 = reinterpret_cast<const jscc_f4v*>(red);  // 16-byte reads, waves in order
 #pragma unroll
-    for (int q = 0; q < kCoarseWaves; ++q) {
-      St += red[q];
-      Dt += red[kCoarseWaves + q];
-      Bt += red[2 * kCoarseWaves + q];
+    for (int q = 0; q < kCoarseWaves / 4; ++q) {
+      const jscc_f4v a = red4[q], b = red4[kCoarseWaves / 4 + q], f = red4[kCoarseWaves / 2 + q];
+      St += a.x; St += a.y; St += a.z; St += a.w;
+      Dt += b.x; Dt += b.y; Dt += b.z; Dt += b.w;
+      Bt += f.x + f.y + f.z + f.w;
     }
 
     float cf;

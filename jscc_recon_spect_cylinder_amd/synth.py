@@ -9,6 +9,9 @@ shapes:
   compton_list(...)   two-hit Compton list: sources drawn from a rod phantom,
                       isotropic emission onto a cylinder, Klein-Nishina
                       scattering, second hit along the scattered ray
+  polar_grid_3d(...), packed_coarse_rows(...)
+                      the multi-slice factor grid and packed coarse event
+                      rows of the sparse (coarse-T) mode
 """
 from __future__ import annotations
 
@@ -27,6 +30,37 @@ def polar_grid(n_rings: int, n_angles: int, r_max: float, z: float = 0.0) -> np.
     a = np.arange(n_angles) * 2.0 * np.pi / n_angles
     rr, aa = np.meshgrid(r, a, indexing="ij")
     return np.stack([rr * np.cos(aa), rr * np.sin(aa), np.full(rr.shape, z)], axis=-1).reshape(-1, 3)
+
+
+def polar_grid_3d(radii, angle_groups, z_values) -> np.ndarray:
+    """[L*Z, 3] multi-slice polar grid in the reference's factor layout: ring k (1-based)
+    takes angle_groups[ceil(k / (n_rings / n_groups)) - 1] equally spaced angles from 0,
+    rings in order within a slice, slices in order (genSysMatPolar_3D.m:84-108)."""
+    radii = np.asarray(radii, np.float64)
+    n, g = len(radii), len(angle_groups)
+    layer = []
+    for k, r in enumerate(radii, start=1):
+        m = int(angle_groups[math.ceil(k / (n / g)) - 1])
+        a = np.arange(m) * 2.0 * np.pi / m
+        layer.append(np.stack([r * np.cos(a), r * np.sin(a)], axis=1))
+    layer = np.concatenate(layer)
+    return np.concatenate([np.hstack([layer, np.full((layer.shape[0], 1), z)]) for z in z_values])
+
+
+def packed_coarse_rows(n_rows: int, n_coarse: int, n_crystals: int, device, seed: int = 0, chunk: int = 8192):
+    """[n_rows, 1 + n_coarse] packed sparse event rows [float(cpnum1), t_c] with cpnum1
+    uniform over 1..n_crystals and row-normalised t_c ~ U(0,1)^3 (synthetic, on `device`)."""
+    import torch
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    out = torch.empty((n_rows, n_coarse + 1), dtype=torch.float32, device=device)
+    for lo in range(0, n_rows, chunk):
+        hi = min(lo + chunk, n_rows)
+        t = torch.rand((hi - lo, n_coarse), generator=g, device=device).pow_(3)
+        t.div_(t.sum(1, keepdim=True))
+        out[lo:hi, 1:] = t
+        out[lo:hi, 0] = torch.randint(1, n_crystals + 1, (hi - lo,), generator=g, device=device).float()
+    return out
 
 
 def rod_phantom(coor: np.ndarray, r_max: float, rods=None, background: float = 1.0) -> np.ndarray:

@@ -34,7 +34,8 @@ def grid(request):
 def oracle_weight(ev, A, x_rot, pj, normalize):
     """nan_to_num(T^T (1 / max(T x_rot, 1e-12))) of the packed rows, FP64."""
     if normalize:
-        rows, _ = sparse.materialize(ev, A, pj)
+        with np.errstate(invalid="ignore"):
+            rows, _ = sparse.materialize(ev, A, pj)
     else:
         cp = np.clip(np.round(ev[:, 0]).astype(np.int64) - 1, 0, A.shape[0] - 1)
         with np.errstate(invalid="ignore"):
