@@ -425,7 +425,9 @@ def run_sparse(args):
             "iter_s": round(n_iter / dt, 3),
             "roofline": {"kernel": "k_coarse_weight<25, identity layer, uniform z, normalised>: one subset",
                          "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                         "frac": round(achieved / HBM_PEAK_GBS, 4),
+                         "traffic": _traffic("f32", os.path.join(HERE, "profiles",
+                                                                 "k_coarse_weight_sparse_2e5x25600_traffic.json")),
                          "launch_us": round(launch_ms * 1e3, 2), "bytes_per_launch": int(nbytes),
                          "distinct_arows_per_subset": int(np.mean(distinct))},
             "rel_err_vs_fp64_oracle_sample": _rel(xg.cpu().numpy(), xref),
@@ -566,10 +568,10 @@ def _prof_read(lib, tag):
     return c.value, t.value
 
 
-def _traffic(dtype_name):
+def _traffic(dtype_name, path=TRAFFIC_FILE):
     """HBM bytes per subset launch from the committed rocprofv3 counter summary (None when absent)."""
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(path) as f:
             rec = json.load(f)[dtype_name]
     except (OSError, KeyError, ValueError):
         return None
