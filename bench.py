@@ -166,7 +166,19 @@ def run_sc(args):
         c, t = _prof_read(lib, _lib.PROF_PROJ)
         b = {"fp32": 4, "bf16": 2, "fp16": 2, "fp8": 1}[name]
         sub_bytes = M / K * P * b
+        # the same run with each iteration replayed from a HIP graph captured once
+        xg = torch.ones(P, device=dev)
+        gi = a.capture(xg, y, s, K)
+
+        def replayed():
+            xg.fill_(1.0)
+            for _ in range(n_iter):
+                gi.replay()
+            return xg
+        dtg, xg = _timed(replayed, args.steps, args.warmup)
         res[name] = {"iter_s": round(n_iter / dt, 2), "ms_per_run": round(dt * 1e3, 3),
+                     "graph_iter_s": round(n_iter / dtg, 2), "graph_ms_per_run": round(dtg * 1e3, 3),
+                     "graph_equals_eager": bool(torch.equal(xg, x)),
                      "subset_launch_us": round(t / max(c, 1) * 1e3, 2),
                      "subset_GBs": round(sub_bytes / (t / max(c, 1) * 1e-3) / 1e9, 1)}
         imgs[name] = x.cpu().numpy()
@@ -174,11 +186,12 @@ def run_sc(args):
     for name in res:
         res[name]["rel_err_vs_fp64_oracle"] = _rel(imgs[name], xref)
     return {"metric": "SC OSEM iter/s (osem_bin_mode, A 5120x4096 FP32, 10 iterations x 5 subsets)",
-            "value": res["fp32"]["iter_s"], "unit": "iter/s", "n_gpus": 1, "steps": args.steps,
-            "warmup": args.warmup, "ms_per_step": res["fp32"]["ms_per_run"], "higher_is_better": True,
+            "value": res["fp32"]["graph_iter_s"], "unit": "iter/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": res["fp32"]["graph_ms_per_run"], "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic (seeded)",
             "config": {"workload": "SC OSEM: 32x128 polar grid, A 5120x4096 (5 rotation blocks x 1024 bins), "
-                                   "10 iterations x 5 subsets; A (84 MB FP32) is Infinity-Cache resident"},
+                                   "10 iterations x 5 subsets, each iteration replayed from a captured HIP graph "
+                                   "(eager launches in components); A (84 MB FP32) is Infinity-Cache resident"},
             "components": res, "cpu_baseline": cpu}
 
 

@@ -6,7 +6,7 @@ import torch
 
 from . import ops
 
-_perm_cache: dict = {}
+_perm_cache = ops.SourceCache(64)
 
 
 def gpu_device(t: torch.Tensor) -> torch.device:
@@ -19,13 +19,9 @@ def perm_on(perm: torch.Tensor, dev: torch.device) -> torch.Tensor:
     """1-based [P, R] rotation table as int64 on `dev` (cached per source tensor)."""
     if perm.device == dev and perm.dtype == torch.int64:
         return perm
-    key = (perm.data_ptr(), perm._version, tuple(perm.shape), tuple(perm.stride()), str(perm.device), str(dev))
-    hit = _perm_cache.get(key)
+    hit = _perm_cache.get(perm, (str(dev),))
     if hit is None:
-        hit = perm.to(device=dev, dtype=torch.int64)
-        if len(_perm_cache) > 64:
-            _perm_cache.clear()
-        _perm_cache[key] = hit
+        hit = _perm_cache.put(perm, (str(dev),), perm.to(device=dev, dtype=torch.int64))
     return hit
 
 

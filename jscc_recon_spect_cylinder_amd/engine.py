@@ -150,15 +150,47 @@ class SystemStore:
         ops.proj(ops.MODE_RATIO, self.mat, dtype_c=self.dtype_c, n_cols=self.n_vox, row_base=lo, n_rows=hi - lo,
                  row_scale=self.row_scale, x=x, y=y, y_by_row=True, eps=eps, out_scale=scale, accumulate=True, w=w)
 
-    def osem(self, x, y, s, n_iter, n_subsets, eps=None, em_mode: int = ops.EM_PLAIN) -> torch.Tensor:
-        """SC OSEM over bin subsets (osem_bin_mode with one rotation block per subset); x updated in place."""
+    def _iteration(self, x, y, s, n_subsets, eps, em_mode):
+        """The launches of one OSEM iteration (one fused projection + EM launch pair per subset) and the
+        buffers they read: the weight and a workspace owned by this iteration."""
+        bounds = [(lo, hi) for lo, hi in self.subset_bounds(n_subsets) if hi > lo]
         w = torch.zeros_like(x)
+        n_max = max((hi - lo for lo, hi in bounds), default=1)
+        ws = torch.empty(ops.lib_proj_workspace_bytes(n_max, self.n_vox), dtype=torch.uint8, device=x.device)
+
+        def run():
+            for lo, hi in bounds:
+                ops.proj(ops.MODE_RATIO, self.mat, dtype_c=self.dtype_c, n_cols=self.n_vox, row_base=lo,
+                         n_rows=hi - lo, row_scale=self.row_scale, x=x, y=y, y_by_row=True, eps=eps,
+                         accumulate=True, w=w, em=(x, s, em_mode, 1e-12, True), workspace=ws)
+        return run, (w, ws, y, s, self.mat, self.row_scale)
+
+    def capture(self, x, y, s, n_subsets, eps=None, em_mode: int = ops.EM_PLAIN) -> "GraphedIteration":
+        """One SC OSEM iteration captured as a HIP graph (x updated in place at each replay).  The
+        returned object owns every buffer the graph reads; keep it while replaying."""
+        run, bufs = self._iteration(x, y, s, n_subsets, eps, em_mode)
+        x_keep = x.clone()
+        run()  # warm-up on the real buffers, then restore the image
+        x.copy_(x_keep)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            run()
+        return GraphedIteration(g, bufs, x, s)
+
+    def osem(self, x, y, s, n_iter, n_subsets, eps=None, em_mode: int = ops.EM_PLAIN,
+             graph: bool = False) -> torch.Tensor:
+        """SC OSEM over bin subsets (osem_bin_mode with one rotation block per subset); x updated in place.
+        graph=True captures one iteration as a HIP graph and replays it (the launches of a small A are
+        short enough for the launch overhead to matter)."""
+        if graph:
+            gi = self.capture(x, y, s, n_subsets, eps, em_mode)
+            for _ in range(n_iter):
+                gi.replay()
+            return x
+        run, _bufs = self._iteration(x, y, s, n_subsets, eps, em_mode)
         for _ in range(n_iter):
-            for lo, hi in self.subset_bounds(n_subsets):
-                if hi > lo:
-                    ops.proj(ops.MODE_RATIO, self.mat, dtype_c=self.dtype_c, n_cols=self.n_vox, row_base=lo,
-                             n_rows=hi - lo, row_scale=self.row_scale, x=x, y=y, y_by_row=True, eps=eps,
-                             accumulate=True, w=w, em=(x, s, em_mode, 1e-12, True))
+            run()
         return x
 
 

@@ -88,13 +88,35 @@ class VoxelSoA:
     n: int
 
 
-_voxel_cache: dict = {}
+class SourceCache:
+    """Results derived from a source tensor, reused only for that same tensor object at
+    the same version.  Each entry holds a reference to its source, so the source's memory
+    cannot be freed and handed to another tensor while the entry lives (a key made of the
+    data pointer alone would then return a stale result)."""
+
+    def __init__(self, capacity: int):
+        self.capacity = capacity
+        self.entries: dict = {}
+
+    def get(self, src: torch.Tensor, extra: tuple = ()):
+        e = self.entries.get((id(src),) + extra)
+        if e is not None and e[0] is src and e[1] == src._version:
+            return e[2]
+        return None
+
+    def put(self, src: torch.Tensor, extra: tuple, value):
+        if len(self.entries) >= self.capacity:
+            self.entries.clear()
+        self.entries[(id(src),) + extra] = (src, src._version, value)
+        return value
+
+
+_voxel_cache = SourceCache(16)
 
 
 def voxel_soa(coor: torch.Tensor, device: torch.device) -> VoxelSoA:
     """[P,3] voxel centres -> three FP32 arrays padded to a multiple of 4 on `device`."""
-    key = (coor.data_ptr(), coor._version, tuple(coor.shape), tuple(coor.stride()), str(coor.device), str(device))
-    hit = _voxel_cache.get(key)
+    hit = _voxel_cache.get(coor, (str(device),))
     if hit is not None:
         return hit
     if coor.ndim != 2 or coor.shape[1] < 3:
@@ -104,32 +126,23 @@ def voxel_soa(coor: torch.Tensor, device: torch.device) -> VoxelSoA:
     c = coor[:, :3].to(device=device, dtype=torch.float32)
     soa = torch.zeros((3, Pp), dtype=torch.float32, device=device)
     soa[:, :P] = c.t()
-    v = VoxelSoA(soa[0], soa[1], soa[2], P)
-    if len(_voxel_cache) > 16:
-        _voxel_cache.clear()
-    _voxel_cache[key] = v
-    return v
+    return _voxel_cache.put(coor, (str(device),), VoxelSoA(soa[0], soa[1], soa[2], P))
 
 
-_det_cache: dict = {}
+_det_cache = SourceCache(16)
 
 
 def detector_layers(detector: torch.Tensor) -> tuple[int, float]:
     """(four |y| layers?, rear-layer |y|) of a crystal table (process_list_plane_strict.py:59-71)."""
-    key = (detector.data_ptr(), detector._version, tuple(detector.shape), str(detector.device))
-    hit = _det_cache.get(key)
+    hit = _det_cache.get(detector)
     if hit is not None:
         return hit
     y = np.abs(detector[:, 1].detach().to("cpu", torch.float32).numpy())
     layers = np.unique(y)
-    res = (1, float(layers[-1])) if layers.size == 4 else (0, 0.0)
-    if len(_det_cache) > 16:
-        _det_cache.clear()
-    _det_cache[key] = res
-    return res
+    return _det_cache.put(detector, (), (1, float(layers[-1])) if layers.size == 4 else (0, 0.0))
 
 
-_aligned_cache: dict = {}
+_aligned_cache = SourceCache(2)
 
 
 def aligned_rows(A: torch.Tensor, vec: int = 4, device: torch.device | None = None) -> torch.Tensor:
@@ -144,18 +157,13 @@ def aligned_rows(A: torch.Tensor, vec: int = 4, device: torch.device | None = No
     if A.device == dev and A.dtype == torch.float32 and A.stride(1) == 1 and A.stride(0) % vec == 0 \
             and A.stride(0) >= A.shape[1] and A.data_ptr() % 16 == 0:
         return A
-    key = (A.data_ptr(), A._version, tuple(A.shape), tuple(A.stride()), A.dtype, str(A.device), str(dev), vec)
-    hit = _aligned_cache.get(key)
+    hit = _aligned_cache.get(A, (str(dev), vec))
     if hit is not None:
         return hit
     M, P = A.shape
     buf = torch.zeros((M, round_up(P, vec)), dtype=torch.float32, device=dev)
     buf[:, :P] = A.to(dev)
-    out = buf[:, :P]
-    if len(_aligned_cache) >= 2:
-        _aligned_cache.clear()
-    _aligned_cache[key] = out
-    return out
+    return _aligned_cache.put(A, (str(dev), vec), buf[:, :P])
 
 
 # ----------------------------------------------------------------------------- events
@@ -476,6 +484,11 @@ MODE_RATIO, MODE_FWD, MODE_COEF, MODE_ONES = 0, 1, 2, 3
 _ws_cache: dict = {}
 
 
+def lib_proj_workspace_bytes(n_rows: int, n_cols: int) -> int:
+    """Workspace bytes of one projection launch over n_rows rows of n_cols pixels."""
+    return int(_lib.lib().jscc_proj_workspace_bytes(max(n_rows, 1), n_cols))
+
+
 def _workspace(dev: torch.device, nbytes: int) -> torch.Tensor:
     key = dev.index if dev.index is not None else torch.cuda.current_device()
     ws = _ws_cache.get(key)
@@ -506,13 +519,14 @@ def proj(mode: int, mat: torch.Tensor, *, dtype_c: int | None = None, n_cols: in
          y: torch.Tensor | None = None, coef: torch.Tensor | None = None, eps: float | None = None,
          fwd_out: torch.Tensor | None = None, inv_perm: torch.Tensor | None = None, out_scale: float = 1.0,
          accumulate: bool = False, w: torch.Tensor | None = None, y_by_row: bool = False,
-         sanitize: bool = False, em: tuple | None = None) -> None:
+         sanitize: bool = False, em: tuple | None = None, workspace: torch.Tensor | None = None) -> None:
     """Launch jscc_proj on the current stream (see include/jscc_hip.h).
 
     em = (x_em, s, em_mode, em_eps, zero_w[, c1, w2, c2]) (mode MODE_RATIO
     only) runs jscc_proj_em instead: the EM update of x_em with the weight
     c1*w (+ c2*w2) and sensitivity s is applied in the same reduction pass
-    that finishes w."""
+    that finishes w.  workspace: a caller-owned buffer of at least
+    jscc_proj_workspace_bytes (graph capture); else the per-device cache."""
     dev = mat.device
     if dtype_c is None:
         dtype_c = dtype_code(mat.dtype) if mat.dtype != torch.uint8 else DT_FP8
@@ -550,7 +564,13 @@ def proj(mode: int, mat: torch.Tensor, *, dtype_c: int | None = None, n_cols: in
     for t in (x, x_perm, y, coef, fwd_out, inv_perm, w, row_scale, rows):
         if t is not None and t.device != dev:
             raise ValueError("all operands must be on the matrix's device")
-    ws = _workspace(dev, _lib.lib().jscc_proj_workspace_bytes(max(n_rows, 1), P))
+    need = _lib.lib().jscc_proj_workspace_bytes(max(n_rows, 1), P)
+    if workspace is not None:
+        if workspace.numel() * workspace.element_size() < need or workspace.device != dev:
+            raise ValueError(f"projection workspace needs {need} bytes on the matrix's device")
+        ws = workspace
+    else:
+        ws = _workspace(dev, need)
     xl = x.shape[0] if x is not None else 0
     flags = (1 if y_by_row else 0) | (2 if sanitize else 0)
     if em is not None:

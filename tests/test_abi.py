@@ -48,3 +48,31 @@ def test_product_package_does_not_import_the_oracle():
         if fn.endswith(".py"):
             src = open(os.path.join(pkg, fn)).read()
             assert "import oracle" not in src and "from oracle" not in src, fn
+
+
+def test_source_cache_hits_only_the_same_tensor_at_the_same_version():
+    """Derived-data caches (rotation tables, voxel arrays, aligned matrices) must not
+    return a result for a different tensor that happens to reuse the same memory."""
+    import gc
+    import weakref
+    import numpy as np
+    import torch
+    from jscc_recon_spect_cylinder_amd.ops import SourceCache
+    c = SourceCache(4)
+    a = torch.from_numpy(np.arange(12, dtype=np.int64).reshape(6, 2))
+    c.put(a, ("d",), "A")
+    assert c.get(a, ("d",)) == "A"
+    assert c.get(a, ("other",)) is None
+    assert c.get(a.view(6, 2), ("d",)) is None           # same memory, another tensor
+    assert c.get(torch.from_numpy(a.numpy()), ("d",)) is None
+    a.add_(1)
+    assert c.get(a, ("d",)) is None                       # modified in place
+    ref = weakref.ref(a)
+    c.put(a, ("d",), "A2")
+    del a
+    gc.collect()
+    assert ref() is not None                              # the entry keeps its source alive
+    for k in range(4):
+        c.put(torch.zeros(1), (k,), k)                    # capacity reached: entries dropped
+    gc.collect()
+    assert ref() is None

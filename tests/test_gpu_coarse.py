@@ -20,14 +20,42 @@ def rel(a, b):
     return np.linalg.norm(a - b) / np.linalg.norm(b)
 
 
-@pytest.fixture(scope="module", params=[(1, 2), (2, 2), (2, 4)], ids=["theta1_z2", "theta2_z2", "theta2_z4"])
+def _odd_grid(n_z):
+    """A layer of 300 pixels (10 rings x 30 angles, not a multiple of 64) over n_z slices, with
+    identity rotation maps (every pixel its own source)."""
+    from jscc_recon_spect_cylinder_amd import synth
+    coor = synth.polar_grid_3d(np.arange(1, 11) * 5.0, [30], np.arange(n_z) * 3.0).astype(np.float32)
+    ident = np.arange(1, coor.shape[0] + 1, dtype=np.int64)[:, None].repeat(8, axis=1)
+    return coor, ident, ident
+
+
+def _deep_grid():
+    """The 1280-pixel layer of the 511keV factors over 24 slices: 30720 voxels (the kernel's
+    32-slot register layout)."""
+    from jscc_recon_spect_cylinder_amd import synth
+    coor = synth.polar_grid_3d(np.arange(6, 151, 6), [20, 40, 60, 80], np.arange(24) * 3.0).astype(np.float32)
+    ident = np.arange(1, coor.shape[0] + 1, dtype=np.int64)[:, None].repeat(8, axis=1)
+    return coor, ident, ident
+
+
+@pytest.fixture(scope="module",
+                params=[("ref", 1, 2), ("ref", 2, 2), ("ref", 2, 4), ("odd", 1, 2), ("odd", 2, 2), ("deep", 1, 2)],
+                ids=["theta1_z2", "theta2_z2", "theta2_z4", "odd_theta1_z2", "odd_theta2_z2", "deep_theta1_z2"])
 def grid(request):
-    ts, zs = request.param
-    coor, rot, inv = layout.reference_grid("511keV_RotateNum20")
+    which, ts, zs = request.param
+    if which == "ref":
+        coor, rot, inv = layout.reference_grid("511keV_RotateNum20")
+        rnum = 20
+    elif which == "odd":
+        coor, rot, inv = _odd_grid(6)
+        rnum = None
+    else:
+        coor, rot, inv = _deep_grid()
+        rnum = None
     coor = coor.astype(np.float32)
     from jscc_recon_spect_cylinder_amd.compton_sparse_ops import build_compton_sparse_projector
-    p = build_compton_sparse_projector(torch.from_numpy(coor), ts, zs, 20)
-    pj = sparse.build(coor, ts, zs, 20)
+    p = build_compton_sparse_projector(torch.from_numpy(coor), ts, zs, rnum)
+    pj = sparse.build(coor, ts, zs, rnum)
     return rot, inv, p, pj
 
 
