@@ -490,7 +490,9 @@ def lib_proj_workspace_bytes(n_rows: int, n_cols: int) -> int:
 
 
 def _workspace(dev: torch.device, nbytes: int) -> torch.Tensor:
-    key = dev.index if dev.index is not None else torch.cuda.current_device()
+    """Scratch of the launches issued on the device's current stream (one buffer per
+    stream: launches on different streams may overlap, launches on one stream are ordered)."""
+    key = (dev.index if dev.index is not None else torch.cuda.current_device(), stream_of(dev))
     ws = _ws_cache.get(key)
     if ws is None or ws.numel() < nbytes:
         ws = torch.empty(max(nbytes, 256), dtype=torch.uint8, device=dev)
@@ -740,7 +742,7 @@ def osem_persistent(mat: torch.Tensor, dtype_c: int, n_cols: int, rows: torch.Te
         raise ValueError("row list int32 and subset offsets int64 on the matrix's device")
     K = sub_off.numel() - 1
     nbytes = _lib.lib().jscc_osem_persistent_workspace_bytes(n_cols)
-    key = (str(dev), nbytes)
+    key = (str(dev), stream_of(dev), nbytes)
     ws = _persist_ws.get(key)
     if ws is None:
         _persist_ws.clear()
