@@ -39,8 +39,9 @@ def build_sparse_rows(sysmat, detector, sparse_projector, list_origin, delta_r1,
 
 def get_compton_backproj_list_single_sparse(sysmat, detector, sparse_projector, list_origin, delta_r1, delta_r2, e0,
                                             ene_resolution, ene_threshold_max, ene_threshold_min, ene_threshold_sum,
-                                            device, model_compton_generator=None):
-    """Packed coarse rows (event_rows, None, None) on the CPU (process_list_plane_sparse.py:25-123)."""
+                                            device, model_compton_generator=None, *, keep_on_device=False):
+    """Packed coarse rows (event_rows, None, None) on the CPU (process_list_plane_sparse.py:25-123);
+    keep_on_device=True (engine option) returns them on `device` without the host copy."""
     if model_compton_generator is not None:
         raise NotImplementedError("model_compton_generator is not supported in sparse Compton mode.")
     dev = _device_of(device)
@@ -48,9 +49,9 @@ def get_compton_backproj_list_single_sparse(sysmat, detector, sparse_projector, 
     rows = build_sparse_rows(sysmat, detector, sparse_projector, list_origin, delta_r1, delta_r2, e0,
                              ene_resolution, ene_threshold_max, ene_threshold_min, ene_threshold_sum, dev)
     if rows.n_kept == 0:
-        return torch.empty((0, pc + 1), dtype=torch.float32), None, None
+        return torch.empty((0, pc + 1), dtype=torch.float32, device=dev if keep_on_device else "cpu"), None, None
     lst = list_origin.to(dev)
     src = rows.events.view(torch.int32)[rows.sel_idx[:rows.n_sel].long(), 15].long()
     lead = lst[src, 0].to(torch.float32).int().to(torch.float32).contiguous()  # float(cpnum1)
     packed = ops.gather_normalize(rows.tc_store, pc, rows.kept, rows.n_kept, rows.stats, 2, lead=lead)
-    return packed.cpu(), None, None
+    return (packed if keep_on_device else packed.cpu()), None, None

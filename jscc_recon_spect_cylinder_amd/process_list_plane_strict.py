@@ -174,9 +174,10 @@ def build_compton_rows(sysmat, detector, coor, list_origin, delta_r1, delta_r2, 
 
 def get_compton_backproj_list_single(sysmat, detector, coor, list_origin, delta_r1, delta_r2, e0, ene_resolution,
                                      ene_threshold_max, ene_threshold_min, ene_threshold_sum, device,
-                                     model_compton_generator=None):
+                                     model_compton_generator=None, *, keep_on_device=False):
     """Row-normalised (t, t_compton, t_single) of the kept events, FP32 on the CPU
-    (process_list_plane_strict.py:151-253)."""
+    (process_list_plane_strict.py:151-253); keep_on_device=True (engine option)
+    returns them on `device` without the host copy."""
     if model_compton_generator is not None:
         raise NotImplementedError("model_compton_generator hooks are not supported by the GPU cone kernels")
     dev = _device_of(device)
@@ -187,4 +188,6 @@ def get_compton_backproj_list_single(sysmat, detector, coor, list_origin, delta_
     t = rows.normalized_t()
     t_compton = rows.normalized_tc()
     t_single = rows.normalized_single(sysmat_dev)
+    if keep_on_device:
+        return t, t_compton, t_single
     return t.cpu(), t_compton.cpu(), t_single.cpu()

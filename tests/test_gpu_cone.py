@@ -57,6 +57,14 @@ def test_strict_dropin_draws_cpu_noise_and_mutates_list():
     assert _rel_rows(t.numpy(), ref["t"]) < 1e-4
     assert _rel_rows(tc.numpy(), ref["t_compton"]) < 1e-4
     np.testing.assert_allclose(ts.numpy(), ref["t_single"], rtol=1e-5, atol=1e-9)
+    # keep_on_device: the same rows, left on the GPU (same draws again)
+    lst2 = torch.from_numpy(g["list_in"].copy()).to(DEV)
+    torch.manual_seed(1234)
+    t2, tc2, ts2 = get_compton_backproj_list_single(torch.from_numpy(A).to(DEV), torch.from_numpy(det).to(DEV),
+                                                    torch.from_numpy(coor).to(DEV), lst2, dr1, dr2, e0, res, tmax,
+                                                    tmin, tsum, torch.device(DEV), keep_on_device=True)
+    assert t2.device.type == "cuda" and tc2.device.type == "cuda" and ts2.device.type == "cuda"
+    assert torch.equal(t2.cpu(), t) and torch.equal(tc2.cpu(), tc) and torch.equal(ts2.cpu(), ts)
 
 
 def test_strict_kernels_match_reference_golden_vectors():
