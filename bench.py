@@ -27,6 +27,7 @@ import time
 import numpy as np
 
 HBM_PEAK_GBS = 8000.0
+FP32_LANE_PEAK = 157.3e12 / 2  # MI355X FP32 vector peak in lane instructions per second (one FMA = 2 flops)
 HERE = os.path.dirname(os.path.abspath(__file__))
 TRAFFIC_FILE = os.path.join(HERE, "profiles", "k_proj_fused_jsccd_2e5x16384_traffic.json")
 
@@ -573,6 +574,25 @@ def _cpu_model():
     return "unknown"
 
 
+def _copy_bandwidth(dev, mib=2048, reps=5):
+    """Measured HBM bandwidth of a plain device copy of `mib` MiB (bytes read + written per second, GB/s),
+    reported beside the nominal 8 TB/s."""
+    import torch
+    src = torch.ones(mib * (1 << 20) // 4, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    b.record()
+    b.synchronize()
+    gbs = 2 * src.numel() * 4 * reps / (a.elapsed_time(b) * 1e-3) / 1e9
+    del src, dst
+    torch.cuda.empty_cache()
+    return round(gbs, 1)
+
+
 def _prof_read(lib, tag):
     c = ctypes.c_int(0)
     t = ctypes.c_double(0.0)
@@ -623,9 +643,12 @@ def main():
     n2 = torch.randn(args.events, generator=g).to(dev)
     P = cl.coor.shape[0]
 
+    n_sel = [0]
+
     def step(store_dtype, name):
         rows = build_compton_rows(None, det, coor, lst0.clone(), 0.0, 0.0, cl.e0, 0.02, tmax, tmin, tsum, dev,
                                   noise=(n1, n2), sigma_model="fixed", fixed_sigma=sigma, store_dtype=store_dtype)
+        n_sel[0] = rows.n_sel
         st = ListModeStore.from_rows(rows, name)
         s = st.backproject_ones()
         x = torch.ones_like(s)
@@ -673,6 +696,8 @@ def main():
             "n_kept": st.n_kept,
             "cone_ms": ctot / max(ccnt, 1),
             "cone_Mevents_s": args.events / (ctot / max(ccnt, 1) * 1e-3) / 1e6,
+            "cone_rows": n_sel[0],
+            "cone_pairs_s": n_sel[0] * P / (ctot / max(ccnt, 1) * 1e-3),
             "osem_launch_ms": proj_ms,
             "osem_launches_per_step": launches,
             "osem_bytes_per_launch": subset_bytes,
@@ -696,6 +721,14 @@ def main():
             "traffic": None if traffic is None else int(traffic),
             "launch_ms": round(bf["osem_launch_ms"], 4), "bytes_per_launch": int(bf["osem_bytes_per_launch"])}
     value = args.events / (bf["ms_per_step"] * 1e-3) / 1e6
+    # the cone kernel is VALU-bound: its instruction count per (event, voxel) pair, counted in the gfx950
+    # ISA of k_cone_rows<fixed sigma, no A factor, BF16 out> (920 VALU instructions, 64 of them exp / rcp /
+    # sqrt, per loop iteration of 4 events x 4 voxels), times the measured pair rate
+    cone_roof = {"kernel": "k_cone_rows<fixed sigma, BF16>", "bound": "valu", "valu_instr_per_pair": 57.5,
+                 "transcendental_per_pair": 4, "pairs_s": round(bf["cone_pairs_s"], -8),
+                 "valu_lane_instr_s": round(bf["cone_pairs_s"] * 57.5, -10),
+                 "peak_lane_instr_s": FP32_LANE_PEAK,
+                 "frac": round(bf["cone_pairs_s"] * 57.5 / FP32_LANE_PEAK, 4)}
     comp = {}
     for k, r in results.items():
         comp[k] = {kk: (float(f"{v:.5g}") if isinstance(v, float) else v) for kk, v in r.items() if kk != "img"}
@@ -718,6 +751,8 @@ def main():
         "components": comp,
         "rel_err_bf16_vs_fp32_image": rel_err,
         "roofline": roof,
+        "cone_roofline": cone_roof,
+        "hbm_copy_GBs": _copy_bandwidth(dev),
     }
     if args.cpu_sample > 0:
         out["cpu_baseline"] = cpu_baseline(cl, args.cpu_sample, args.iters, args.subsets, sigma)
