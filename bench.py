@@ -27,7 +27,10 @@ import time
 import numpy as np
 
 HBM_PEAK_GBS = 8000.0
-FP32_LANE_PEAK = 157.3e12 / 2  # MI355X FP32 vector peak in lane instructions per second (one FMA = 2 flops)
+# MI355X VALU issue rate without packed FP32 (kept out of every kernel): one 64-lane instruction per 4 cycles
+# per SIMD, 1024 SIMDs x 16 lanes x 2.4 GHz = 3.93e13 lane instructions/s (the 157.3 TFLOP/s FP32 figure
+# counts packed FMAs: 2 lanes x 2 flops per lane instruction)
+FP32_LANE_PEAK = 157.3e12 / 4
 HERE = os.path.dirname(os.path.abspath(__file__))
 TRAFFIC_FILE = os.path.join(HERE, "profiles", "k_proj_fused_jsccd_2e5x16384_traffic.json")
 
@@ -723,7 +726,8 @@ def main():
     value = args.events / (bf["ms_per_step"] * 1e-3) / 1e6
     # the cone kernel is VALU-bound: its instruction count per (event, voxel) pair, counted in the gfx950
     # ISA of k_cone_rows<fixed sigma, no A factor, BF16 out> (920 VALU instructions, 64 of them exp / rcp /
-    # sqrt, per loop iteration of 4 events x 4 voxels), times the measured pair rate
+    # sqrt, per loop iteration of 4 events x 4 voxels; SQ_INSTS_VALU of the whole kernel gives 58.7),
+    # times the measured pair rate, against the non-packed issue rate
     cone_roof = {"kernel": "k_cone_rows<fixed sigma, BF16>", "bound": "valu", "valu_instr_per_pair": 57.5,
                  "transcendental_per_pair": 4, "pairs_s": round(bf["cone_pairs_s"], -8),
                  "valu_lane_instr_s": round(bf["cone_pairs_s"] * 57.5, -10),
