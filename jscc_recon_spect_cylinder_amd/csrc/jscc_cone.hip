@@ -233,12 +233,16 @@ __global__ void __launch_bounds__(kConeThreads) k_cone_rows(
       for (int k = 0; k < 4; ++k) {
         accS[e] += tv[k];
         accQ[e] = fma((double)tv[k], (double)tv[k], accQ[e]);
-        accC[e] += tcv[k];
+        if (AF != AF_NONE) accC[e] += tcv[k];  // without an A factor t = t_c: Sc = S (set below)
         accM[e] = fmaxf(accM[e], tv[k]);  // fmaxf drops NaN; NaN rows are caught by S
       }
     }
   }
 
+  if (AF == AF_NONE) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) accC[e] = accS[e];
+  }
   // block reduction of the E x 4 statistics
   __shared__ double red[4][kConeThreads / JSCC_WAVE][E];
 #pragma unroll

@@ -153,6 +153,11 @@ __global__ void __launch_bounds__(256) k_events_prepare(
           ia = i1;
         }
         ev.arow = (int32_t)ia;
+        // a non-finite hit position makes every weight of the row NaN in the reference
+        // (:201-228), which then drops the row (:241-246): drop the event here
+        const float fin = ev.p1x * 0.0f + ev.p1y * 0.0f + ev.p1z * 0.0f + ev.ax * 0.0f + ev.ay * 0.0f +
+                          ev.az * 0.0f + ev.d12sq * 0.0f;
+        if (!(fin == 0.0f)) keep = false;
       }
     }
   }

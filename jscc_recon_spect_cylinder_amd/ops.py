@@ -121,6 +121,9 @@ def voxel_soa(coor: torch.Tensor, device: torch.device) -> VoxelSoA:
         return hit
     if coor.ndim != 2 or coor.shape[1] < 3:
         raise ValueError("coor must have shape [pixel_num, 3]")
+    if not bool(torch.isfinite(coor[:, :3]).all()):
+        # (the reference's rows would all hold NaN and be dropped)
+        raise ValueError("voxel coordinates must be finite")
     P = coor.shape[0]
     Pp = round_up(max(P, 1), 4)
     c = coor[:, :3].to(device=device, dtype=torch.float32)

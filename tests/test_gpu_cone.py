@@ -87,6 +87,47 @@ def test_strict_kernels_match_reference_golden_vectors():
     assert _rel_rows(rows.normalized_tc().cpu().numpy(), g["t_compton"]) < 1e-4
 
 
+def test_non_finite_hit_positions_drop_their_events():
+    """A crystal with a non-finite coordinate makes the reference's rows NaN, which its
+    row filter drops: the events using it are not kept, the others are unchanged."""
+    from jscc_recon_spect_cylinder_amd.process_list_plane_strict import build_compton_rows
+    g = np.load(STRICT)
+    det = np.load(FIX)["detector"].copy()
+    coor = g["coor"]
+    A = _golden_sysmat(det, coor)
+    dr1, dr2, e0, res, tmax, tmin, tsum = g["params"]
+    lst = g["list_in"]
+    base = cone.backproj_list_single(A, det, coor, lst, dr1, dr2, e0, res, tmax, tmin, tsum, g["noise1"],
+                                     g["noise2"])["kept"]
+    hit1 = int(lst[base[0], 0]) - 1   # the first-hit crystal of the first kept event: NaN x
+    hit2 = int(lst[base[1], 2]) - 1   # the second-hit crystal of the second kept event: infinite z
+    det[hit1, 0] = np.nan
+    det[hit2, 2] = np.inf
+    with np.errstate(invalid="ignore", over="ignore"):
+        ref = cone.backproj_list_single(A, det, coor, lst, dr1, dr2, e0, res, tmax, tmin, tsum, g["noise1"],
+                                        g["noise2"])
+    rows = build_compton_rows(torch.from_numpy(A), torch.from_numpy(det), torch.from_numpy(coor),
+                              torch.from_numpy(lst.copy()), dr1, dr2, e0, res, tmax, tmin, tsum, DEV,
+                              noise=(torch.from_numpy(g["noise1"]), torch.from_numpy(g["noise2"])), want_tc=True)
+    kept = rows.kept_event_index().cpu().numpy()
+    uses = (np.round(lst[:, 0]).astype(int) - 1 == hit1) | (np.round(lst[:, 2]).astype(int) - 1 == hit2)
+    assert np.isin(np.nonzero(uses)[0], base).any() and not np.isin(np.nonzero(uses)[0], kept).any()
+    assert np.array_equal(kept, ref["kept"]) and len(kept) < len(base)
+    assert _rel_rows(rows.normalized_t().cpu().numpy(), ref["t"]) < 1e-4
+
+
+def test_non_finite_voxel_coordinates_raise():
+    from jscc_recon_spect_cylinder_amd.process_list_plane_strict import build_compton_rows
+    g = np.load(STRICT)
+    det = np.load(FIX)["detector"]
+    coor = g["coor"].copy()
+    coor[3, 1] = np.nan
+    dr1, dr2, e0, res, tmax, tmin, tsum = g["params"]
+    with pytest.raises(ValueError):
+        build_compton_rows(None, torch.from_numpy(det), torch.from_numpy(coor), torch.from_numpy(g["list_in"].copy()),
+                           dr1, dr2, e0, res, tmax, tmin, tsum, DEV, noise="none")
+
+
 def test_strict_rows_match_fp64_oracle_and_kept_indices_bit_exact():
     from jscc_recon_spect_cylinder_amd.process_list_plane_strict import build_compton_rows
     g = np.load(STRICT)
