@@ -109,7 +109,9 @@ __device__ __forceinline__ float cone_tc(const ConeEv& e, float cxv, float cyv, 
   const float msq = fmaf(mx, mx, fmaf(my, my, mz * mz));
   // hardware sqrt / exp2 (about 1 ulp): no range-reduction branches in the loop
   float beta = atan2_ypos(__builtin_amdgcn_sqrtf(msq), dot);
-  beta = fminf(fmaxf(beta, cc.beta_min), cc.beta_max);
+  // clamp as one median-of-three (beta is finite: events with non-finite hits are
+  // dropped before this kernel and voxel coordinates are checked finite)
+  beta = __builtin_amdgcn_fmed3f(beta, cc.beta_min, cc.beta_max);
   const float inv_d01sq = __builtin_amdgcn_rcpf(d01sq);
   // sin^2(beta) of the clamped beta: never below the clamp's floor
   const float s2 = fmaxf(msq * inv_d01sq * e.inv_d12sq, cc.s2_min);
