@@ -14,7 +14,8 @@
 //     float4 (one 16-byte load per component per thread, contiguous across the
 //     wave); the first-hit system-matrix row is read the same way.
 //   * beta = atan2(|v01 x v12|, v01 . v12) with a hand-fitted polynomial: well
-//     conditioned at the cone axis, unlike acos of an FP32 cosine.
+//     conditioned at the cone axis, unlike acos of an FP32 cosine; |v01 x v12|^2
+//     by Lagrange's identity away from the axis, by the cross product near it.
 //   * rows are written unnormalised (FP32 or BF16) and the row statistics
 //     S = sum t, Q = sum t^2, Sc = sum t_c and max t are reduced in FP64; the
 //     normalisation 1/S is applied by whoever consumes the row.
@@ -47,7 +48,7 @@ __device__ __forceinline__ float atan_01(float r) {
 // atan2(y, x) for y >= 0
 __device__ __forceinline__ float atan2_ypos(float y, float x) {
   const float ax = fabsf(x);
-  const float mn = fminf(ax, y);
+  const float mn = __builtin_amdgcn_fmed3f(ax, y, -1.0f);
   const float mx = fmaxf(fmaxf(ax, y), 1e-30f);
   float a = atan_01(mn * __builtin_amdgcn_rcpf(mx));
   a = (y > ax) ? (1.57079632679489662f - a) : a;
@@ -76,7 +77,7 @@ ConeConsts make_consts(int sigma_model, double fixed_sigma) {
 
 // wave-uniform parameters of one event (scalar registers)
 struct ConeEv {
-  float p1x, p1y, p1z, ax, ay, az, d12sq, inv_d12sq, th, kn, sp, sm, wx, wy, wz;
+  float p1x, p1y, p1z, ax, ay, az, d12sq, th, kn, sp, sm, wx, wy, wz;
   int64_t arow_off;
 };
 
@@ -85,7 +86,6 @@ __device__ __forceinline__ ConeEv load_event(const JsccEvent& q, int64_t lda) {
   e.p1x = q.p1x; e.p1y = q.p1y; e.p1z = q.p1z;
   e.ax = q.ax; e.ay = q.ay; e.az = q.az;
   e.d12sq = q.d12sq;
-  e.inv_d12sq = 1.0f / q.d12sq;
   e.th = q.theta; e.kn = q.kn;
   e.sp = q.sig_plus; e.sm = q.sig_minus;
   e.wx = q.wx; e.wy = q.wy; e.wz = q.wz;
@@ -93,46 +93,77 @@ __device__ __forceinline__ ConeEv load_event(const JsccEvent& q, int64_t lda) {
   return e;
 }
 
-// t_c of one (event, voxel) pair.  For the fixed-sigma model and DSIG,
-// *dlog = d log(t_c) / d sigma = delta^2 / sigma^3.
+// sin^2(beta) below which |v01 x v12|^2 is taken from the cross product: above
+// it, d01^2 d12^2 - (v01 . v12)^2 loses at most ~3 eps cot(beta) rad of beta
+// (7e-7 rad at the bound), below it the difference cancels toward the cone axis.
+constexpr float kLagrangeMinS2 = 0.0625f;
+
+// t_c of one event at 4 voxels.  For the fixed-sigma model and DSIG,
+// dlog[k] = d log(t_c) / d sigma = delta^2 / sigma^3.
+//
+// |v01 x v12|^2 = d01^2 d12^2 - (v01 . v12)^2 (Lagrange's identity: 2 VALU
+// instead of 9 for the cross product and its norm).  A lane whose 4 voxels
+// include one with sin^2(beta) < kLagrangeMinS2 (or a negative difference)
+// recomputes all 4 from the cross product; a wave skips that branch unless one
+// of its lanes takes it (about 1 wave in 7 on a 64 x 256 polar grid).  The
+// choice depends only on the lane's own 4 voxels, so every kernel that tiles
+// voxels in aligned quads gets the same bits for the same pair.
 template <int SIG, bool DSIG>
-__device__ __forceinline__ float cone_tc(const ConeEv& e, float cxv, float cyv, float czv, const ConeConsts& cc,
-                                         float* dlog) {
-  const float vx = e.p1x - cxv;
-  const float vy = e.p1y - cyv;
-  const float vz = e.p1z - czv;
-  const float d01sq = fmaf(vx, vx, fmaf(vy, vy, vz * vz));
-  const float dot = fmaf(vx, e.ax, fmaf(vy, e.ay, vz * e.az));
-  const float mx = fmaf(vy, e.az, -vz * e.ay);
-  const float my = fmaf(vz, e.ax, -vx * e.az);
-  const float mz = fmaf(vx, e.ay, -vy * e.ax);
-  const float msq = fmaf(mx, mx, fmaf(my, my, mz * mz));
-  // hardware sqrt / exp2 (about 1 ulp): no range-reduction branches in the loop
-  float beta = atan2_ypos(__builtin_amdgcn_sqrtf(msq), dot);
-  // clamp as one median-of-three (beta is finite: events with non-finite hits are
-  // dropped before this kernel and voxel coordinates are checked finite)
-  beta = __builtin_amdgcn_fmed3f(beta, cc.beta_min, cc.beta_max);
-  const float inv_d01sq = __builtin_amdgcn_rcpf(d01sq);
-  // sin^2(beta) of the clamped beta: never below the clamp's floor
-  const float s2 = fmaxf(msq * inv_d01sq * e.inv_d12sq, cc.s2_min);
-  const float delta = beta - e.th;
-  float hl2;  // log2(e) / (2 sigma^2)
-  if (SIG == SIG_STRICT) {
-    const float sige = delta >= 0.0f ? e.sp : e.sm;
-    const float qx = fmaf(e.d12sq, vx, -dot * e.ax);
-    const float qy = fmaf(e.d12sq, vy, -dot * e.ay);
-    const float qz = fmaf(e.d12sq, vz, -dot * e.az);
-    const float num = fmaf(qx * qx, e.wx, fmaf(qy * qy, e.wy, qz * qz * e.wz));
-    const float den = e.d12sq * e.d12sq * fmaxf(msq, 1e-12f * d01sq * e.d12sq);
-    const float spos2 = fmaxf(num * __builtin_amdgcn_rcpf(den), 1e-12f);
-    const float sig2 = fmaxf(fmaf(sige, sige, spos2), 1e-12f);
-    hl2 = 0.72134752044448170368f * __builtin_amdgcn_rcpf(sig2);
-  } else {
-    hl2 = cc.fixed_hl2;
+__device__ __forceinline__ void cone_tc4(const ConeEv& e, const float* cxv, const float* cyv, const float* czv,
+                                         const ConeConsts& cc, float* tc, float* dlog) {
+  float vx[4], vy[4], vz[4], d01sq[4], dot[4], msq[4], rp[4], s2r[4];
+  bool near_axis = false;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    vx[k] = e.p1x - cxv[k];
+    vy[k] = e.p1y - cyv[k];
+    vz[k] = e.p1z - czv[k];
+    d01sq[k] = fmaf(vx[k], vx[k], fmaf(vy[k], vy[k], vz[k] * vz[k]));
+    dot[k] = fmaf(vx[k], e.ax, fmaf(vy[k], e.ay, vz[k] * e.az));
+    const float pr = d01sq[k] * e.d12sq;
+    msq[k] = fmaf(-dot[k], dot[k], pr);
+    rp[k] = __builtin_amdgcn_rcpf(pr);
+    s2r[k] = msq[k] * rp[k];
+    near_axis |= s2r[k] < kLagrangeMinS2;
   }
-  const float d2 = delta * delta;
-  if (DSIG) *dlog = d2 * cc.inv_sig3;
-  return __builtin_amdgcn_exp2f(-d2 * hl2) * (e.kn - s2);
+  if (near_axis) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const float mx = fmaf(vy[k], e.az, -vz[k] * e.ay);
+      const float my = fmaf(vz[k], e.ax, -vx[k] * e.az);
+      const float mz = fmaf(vx[k], e.ay, -vy[k] * e.ax);
+      msq[k] = fmaf(mx, mx, fmaf(my, my, mz * mz));
+      s2r[k] = msq[k] * rp[k];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    // hardware sqrt / exp2 (about 1 ulp): no range-reduction branches in the loop
+    float beta = atan2_ypos(__builtin_amdgcn_sqrtf(msq[k]), dot[k]);
+    // clamp as one median-of-three (beta is finite: events with non-finite hits are
+    // dropped before this kernel and voxel coordinates are checked finite)
+    beta = __builtin_amdgcn_fmed3f(beta, cc.beta_min, cc.beta_max);
+    // sin^2(beta) of the clamped beta: never below the clamp's floor
+    const float s2 = __builtin_amdgcn_fmed3f(s2r[k], cc.s2_min, 2.0f);
+    const float delta = beta - e.th;
+    float hl2;  // log2(e) / (2 sigma^2)
+    if (SIG == SIG_STRICT) {
+      const float sige = delta >= 0.0f ? e.sp : e.sm;
+      const float qx = fmaf(e.d12sq, vx[k], -dot[k] * e.ax);
+      const float qy = fmaf(e.d12sq, vy[k], -dot[k] * e.ay);
+      const float qz = fmaf(e.d12sq, vz[k], -dot[k] * e.az);
+      const float num = fmaf(qx * qx, e.wx, fmaf(qy * qy, e.wy, qz * qz * e.wz));
+      const float den = e.d12sq * e.d12sq * fmaxf(msq[k], 1e-12f * d01sq[k] * e.d12sq);
+      const float spos2 = fmaxf(num * __builtin_amdgcn_rcpf(den), 1e-12f);
+      const float sig2 = fmaxf(fmaf(sige, sige, spos2), 1e-12f);
+      hl2 = 0.72134752044448170368f * __builtin_amdgcn_rcpf(sig2);
+    } else {
+      hl2 = cc.fixed_hl2;
+    }
+    const float d2 = delta * delta;
+    if (DSIG) dlog[k] = d2 * cc.inv_sig3;
+    tc[k] = __builtin_amdgcn_exp2f(-d2 * hl2) * (e.kn - s2);
+  }
 }
 
 template <int AF>
@@ -206,11 +237,9 @@ __global__ void __launch_bounds__(kConeThreads) k_cone_rows(
       float av[4];
       afactor4<AF>(evs[e], A, base, ti0, ti1, tw0, tw1, av);
       float tv[4], tcv[4];
+      cone_tc4<SIG, false>(evs[e], vcx, vcy, vcz, cc, tcv, nullptr);
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        tcv[k] = cone_tc<SIG, false>(evs[e], vcx[k], vcy[k], vcz[k], cc, nullptr);
-        tv[k] = tcv[k] * av[k];
-      }
+      for (int k = 0; k < 4; ++k) tv[k] = tcv[k] * av[k];
       if (base + 4 > P) {  // the padded tail of the last tile (P not a multiple of 4)
 #pragma unroll
         for (int k = 0; k < 4; ++k)
@@ -438,15 +467,16 @@ __global__ void __launch_bounds__(BLOCK) k_cone_proj(ConeProjArgs a) {
         const float xv[4] = {x4.x, x4.y, x4.z, x4.w};
         float av[4];
         afactor4<AF>(e, a.A, base, nullptr, nullptr, nullptr, nullptr, av);
+        float tcv[4], dl[4];
+        cone_tc4<SIG, DSIG>(e, vcx, vcy, vcz, a.cc, tcv, dl);
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-          float dl = 0.0f;
-          const float t = cone_tc<SIG, DSIG>(e, vcx[k], vcy[k], vcz[k], a.cc, &dl) * av[k];
+          const float t = tcv[k] * av[k];
           tr[c][k] = t;
           S += t;
           D = fmaf(t, xv[k], D);
           if (DSIG) {
-            const float dt = t * dl;
+            const float dt = t * dl[k];
             dS += dt;
             dD = fmaf(dt, xv[k], dD);
           }
@@ -728,9 +758,11 @@ __global__ void __launch_bounds__(BLOCK)
           for (int q = 0; q < R; ++q) {
             float av[4];
             afactor4<AF>(e[q], a.A, base, nullptr, nullptr, nullptr, nullptr, av);
+            float tcv[4];
+            cone_tc4<SIG, false>(e[q], vcx, vcy, vcz, a.cc, tcv, nullptr);
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-              const float t = cone_tc<SIG, false>(e[q], vcx[k], vcy[k], vcz[k], a.cc, nullptr) * av[k];
+              const float t = tcv[k] * av[k];
               tr[q][c][k] = t;
               S[q] += t;
               if (MODE == BP_OSEM) D[q] = fmaf(t, xv[k], D[q]);

@@ -116,6 +116,61 @@ def test_non_finite_hit_positions_drop_their_events():
     assert _rel_rows(rows.normalized_t().cpu().numpy(), ref["t"]) < 1e-4
 
 
+def _axis_geometry(u_dir):
+    """Hits p1 and p2 = p1 + 30 u and voxels placed at known angles beta from the axis:
+    60 angles from 1e-5 to 0.6 rad (dense around sin^2 beta = 1/16) next to beta = 0 and
+    beta = pi, at two distances, plus 96 voxels on each of the cone surfaces beta = 0.522
+    and beta = 2.537 (the scattering angles of the two test events)."""
+    p1 = np.array([90.0, 10.0, 3.0])
+    u = np.asarray(u_dir, np.float64)
+    u /= np.linalg.norm(u)
+    p2 = p1 + 30.0 * u
+    h = np.array([0.0, 0.0, 1.0]) if abs(u[2]) < 0.9 else np.array([1.0, 0.0, 0.0])
+    w1 = np.cross(u, h)
+    w1 /= np.linalg.norm(w1)
+    w2 = np.cross(u, w1)
+    betas = np.unique(np.concatenate([np.logspace(-5, np.log10(0.6), 48), np.linspace(0.24, 0.27, 12)]))
+    pts = []
+    for s in (20.0, 80.0):
+        for b in betas:
+            pts.append(p1 - s * (np.cos(b) * u + np.sin(b) * w1))  # angle b between p1 - c and p2 - p1
+            pts.append(p1 + s * (np.cos(b) * u - np.sin(b) * w2))  # angle pi - b
+    for th in (0.522, 2.537):  # the two events' cone surfaces
+        for ph in np.linspace(0.0, 2.0 * np.pi, 96, endpoint=False):
+            pts.append(p1 - 50.0 * (np.cos(th) * u + np.sin(th) * (np.cos(ph) * w1 + np.sin(ph) * w2)))
+    det = np.stack([p1, p2]).astype(np.float32)
+    return det, np.asarray(pts, np.float32), betas
+
+
+@pytest.mark.parametrize("u_dir", [(-0.6, -0.75, 0.1), (0.2, 0.9, -0.4)])
+def test_cone_rows_near_the_cone_axis(u_dir):
+    """Voxels within 1e-5..0.6 rad of the cone axis (both ends) against the FP64 oracle,
+    element by element.  The kernel forms |v01 x v12|^2 as d01^2 d12^2 - (v01 . v12)^2
+    away from the axis and from the cross product near it; the difference form alone
+    cancels near the axis (beta off by up to 1.3e-4 rad here, t by 0.7 %)."""
+    from jscc_recon_spect_cylinder_amd.process_list_plane_strict import build_compton_rows
+    det, coor, betas = _axis_geometry(u_dir)
+    assert betas.min() < 1e-4 and (np.abs(np.sin(betas) ** 2 - 0.0625) < 2e-3).sum() >= 2
+    e0, tmax, tmin, tsum = 0.511, 0.3397, 0.05, 0.46
+    # theta = 0.522 rad (voxels near beta = 0 carry weight) and 2.537 rad (near beta = pi)
+    lst = np.array([[1.0, 0.06, 2.0, 0.451], [1.0, 0.33, 2.0, 0.181]], np.float32)
+    sig = 0.1  # steep enough that an angle error near the axis (delta ~ 0.52) shows: d t / t = delta / sig^2 * d beta
+    ref = cone.backproj_list_single(None, det, coor, lst, 0.0, 0.0, e0, 0.02, tmax, tmin, tsum, None, None,
+                                    fixed_sigma=sig)
+    assert ref["kept"].tolist() == [0, 1]
+    rows = build_compton_rows(None, torch.from_numpy(det), torch.from_numpy(coor), torch.from_numpy(lst.copy()),
+                              0.0, 0.0, e0, 0.02, tmax, tmin, tsum, DEV, noise="none", sigma_model="fixed",
+                              fixed_sigma=sig, want_tc=True)
+    assert rows.kept_event_index().cpu().tolist() == [0, 1]
+    t = rows.normalized_tc().cpu().numpy().astype(np.float64)
+    r = np.asarray(ref["t_compton"], np.float64)
+    for i in range(2):
+        m = r[i] > 1e-30
+        assert m.sum() > 200
+        np.testing.assert_allclose(t[i][m], r[i][m], rtol=1e-4, atol=0)
+        assert np.all(t[i][~m] < 1e-30)
+
+
 def test_non_finite_voxel_coordinates_raise():
     from jscc_recon_spect_cylinder_amd.process_list_plane_strict import build_compton_rows
     g = np.load(STRICT)
