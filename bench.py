@@ -725,16 +725,17 @@ def main():
             "launch_ms": round(bf["osem_launch_ms"], 4), "bytes_per_launch": int(bf["osem_bytes_per_launch"])}
     value = args.events / (bf["ms_per_step"] * 1e-3) / 1e6
     # the cone kernel is VALU-bound: its instruction count per (event, voxel) pair on this workload, from
-    # SQ_INSTS_VALU x 64 / pairs (50.6, 4 of them exp / rcp / sqrt; the gfx950 ISA of k_cone_rows<fixed sigma,
-    # no A factor, BF16 out> has 791 per loop iteration of 4 events x 4 voxels on the common path plus 40 per
-    # event and lane quad near the cone axis), times the measured pair rate, against the non-packed issue
-    # rate (exp / rcp / sqrt issue to the transcendental unit beside the main VALU: the fraction counts the
-    # 46.6 other instructions per pair)
-    cone_roof = {"kernel": "k_cone_rows<fixed sigma, BF16>", "bound": "valu", "valu_instr_per_pair": 50.6,
-                 "transcendental_per_pair": 4, "pairs_s": round(bf["cone_pairs_s"], -8),
+    # SQ_INSTS_VALU x 64 / pairs (46.6, 5 of them sqrt / rsq / rcp / exp; the gfx950 ISA of k_cone_rows<fixed
+    # sigma, no A factor, BF16 out> has 727 per loop iteration of 4 events x 4 voxels on the common path plus
+    # 40 per event and lane quad near the cone axis), times the measured pair rate, against the non-packed
+    # issue rate at the 2.4 GHz engine clock (one wave64 instruction per 4 cycles per SIMD).  Under load the
+    # kernel is power-limited to ~2.15-2.3 GHz (GRBM_GUI_ACTIVE per dispatch), which caps the fraction below 1.
+    cone_roof = {"kernel": "k_cone_rows<fixed sigma, BF16>", "bound": "valu", "valu_instr_per_pair": 46.6,
+                 "transcendental_per_pair": 5, "pairs_s": round(bf["cone_pairs_s"], -8),
                  "valu_lane_instr_s": round(bf["cone_pairs_s"] * 46.6, -10),
                  "peak_lane_instr_s": FP32_LANE_PEAK,
-                 "frac": round(bf["cone_pairs_s"] * 46.6 / FP32_LANE_PEAK, 4)}
+                 "frac": round(bf["cone_pairs_s"] * 46.6 / FP32_LANE_PEAK, 4),
+                 "clock_GHz_profiled": 2.15}
     comp = {}
     for k, r in results.items():
         comp[k] = {kk: (float(f"{v:.5g}") if isinstance(v, float) else v) for kk, v in r.items() if kk != "img"}

@@ -13,9 +13,10 @@
 //   * voxel coordinates are SoA float arrays padded to a multiple of 4, read as
 //     float4 (one 16-byte load per component per thread, contiguous across the
 //     wave); the first-hit system-matrix row is read the same way.
-//   * beta = atan2(|v01 x v12|, v01 . v12) with a hand-fitted polynomial: well
-//     conditioned at the cone axis, unlike acos of an FP32 cosine; |v01 x v12|^2
-//     by Lagrange's identity away from the axis, by the cross product near it.
+//   * beta from |v01 x v12| and v01 . v12 (the half-angle form of atan2, a
+//     hand-fitted polynomial): well conditioned at the cone axis, unlike acos
+//     of an FP32 cosine; |v01 x v12|^2 by Lagrange's identity away from the
+//     axis, by the cross product near it.
 //   * rows are written unnormalised (FP32 or BF16) and the row statistics
 //     S = sum t, Q = sum t^2, Sc = sum t_c and max t are reduced in FP64; the
 //     normalisation 1/S is applied by whoever consumes the row.
@@ -30,34 +31,24 @@ enum { AF_NONE = 0, AF_ROW = 1, AF_TAPS = 2 };
 constexpr int kConeThreads = 256;
 constexpr int kConeVec = 4;
 
-// atan(r) on [0,1] as r*P(r^2); coefficients fitted to minimise relative
-// error (max abs error 1.9e-7 rad in FP32).
-__device__ __forceinline__ float atan_01(float r) {
+// 2 atan(r) on [0, 1] as r*P(r^2); coefficients fitted to minimise the
+// absolute error (2.3e-7 rad in FP32 Horner form, the FP32 rounding floor of
+// values up to pi/2).
+__device__ __forceinline__ float two_atan_01(float r) {
   const float s = r * r;
-  float p = -4.687702868e-03f;
-  p = fmaf(p, s, 2.423363684e-02f);
-  p = fmaf(p, s, -5.946168906e-02f);
-  p = fmaf(p, s, 9.912689883e-02f);
-  p = fmaf(p, s, -1.401895101e-01f);
-  p = fmaf(p, s, 1.996964129e-01f);
-  p = fmaf(p, s, -3.333198598e-01f);
-  p = fmaf(p, s, 9.999999005e-01f);
+  float p = -8.105187127e-03f;
+  p = fmaf(p, s, 4.371275015e-02f);
+  p = fmaf(p, s, -1.118076552e-01f);
+  p = fmaf(p, s, 1.928333648e-01f);
+  p = fmaf(p, s, -2.781694243e-01f);
+  p = fmaf(p, s, 3.989309530e-01f);
+  p = fmaf(p, s, -6.665972215e-01f);
+  p = fmaf(p, s, 1.999998673e+00f);
   return r * p;
 }
 
-// atan2(y, x) for y >= 0
-__device__ __forceinline__ float atan2_ypos(float y, float x) {
-  const float ax = fabsf(x);
-  const float mn = __builtin_amdgcn_fmed3f(ax, y, -1.0f);
-  const float mx = fmaxf(fmaxf(ax, y), 1e-30f);
-  float a = atan_01(mn * __builtin_amdgcn_rcpf(mx));
-  a = (y > ax) ? (1.57079632679489662f - a) : a;
-  a = (x < 0.0f) ? (3.14159265358979324f - a) : a;
-  return a;
-}
-
 struct ConeConsts {
-  float beta_min, beta_max;  // acos of the reference's FP32 clamp bounds
+  float beta_min;            // acos of the reference's FP32 clamp bound (beta_max = pi - beta_min)
   float s2_min;              // 1 - cos_max^2 : sin^2 floor implied by the clamp
   float fixed_hl2;           // log2(e) / (2 sigma^2) for the fixed-sigma model
   float inv_sig3;            // 1 / sigma^3 (d log t / d sigma = delta^2 / sigma^3)
@@ -67,7 +58,6 @@ ConeConsts make_consts(int sigma_model, double fixed_sigma) {
   ConeConsts cc;
   const double cmax = (double)(float)(1.0 - 1e-7);  // the reference's FP32 clamp of cos(beta)
   cc.beta_min = (float)acos(cmax);
-  cc.beta_max = (float)(M_PI - acos(cmax));
   cc.s2_min = (float)(1.0 - cmax * cmax);
   const bool fixed = sigma_model == SIG_FIXED && fixed_sigma > 0.0;
   cc.fixed_hl2 = fixed ? (float)(0.5 * 1.4426950408889634 / (fixed_sigma * fixed_sigma)) : 0.0f;
@@ -77,7 +67,7 @@ ConeConsts make_consts(int sigma_model, double fixed_sigma) {
 
 // wave-uniform parameters of one event (scalar registers)
 struct ConeEv {
-  float p1x, p1y, p1z, ax, ay, az, d12sq, th, kn, sp, sm, wx, wy, wz;
+  float p1x, p1y, p1z, ax, ay, az, d12sq, th, pi_m_th, kn, sp, sm, wx, wy, wz;
   int64_t arow_off;
 };
 
@@ -87,6 +77,7 @@ __device__ __forceinline__ ConeEv load_event(const JsccEvent& q, int64_t lda) {
   e.ax = q.ax; e.ay = q.ay; e.az = q.az;
   e.d12sq = q.d12sq;
   e.th = q.theta; e.kn = q.kn;
+  e.pi_m_th = 3.14159265358979324f - q.theta;
   e.sp = q.sig_plus; e.sm = q.sig_minus;
   e.wx = q.wx; e.wy = q.wy; e.wz = q.wz;
   e.arow_off = (int64_t)q.arow * lda;
@@ -138,14 +129,25 @@ __device__ __forceinline__ void cone_tc4(const ConeEv& e, const float* cxv, cons
   }
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    // hardware sqrt / exp2 (about 1 ulp): no range-reduction branches in the loop
-    float beta = atan2_ypos(__builtin_amdgcn_sqrtf(msq[k]), dot[k]);
-    // clamp as one median-of-three (beta is finite: events with non-finite hits are
-    // dropped before this kernel and voxel coordinates are checked finite)
-    beta = __builtin_amdgcn_fmed3f(beta, cc.beta_min, cc.beta_max);
+    // Half-angle form of beta = atan2(|v01 x v12|, v01 . v12) with h = d01 d12:
+    // b = 2 atan(|v01 x v12| / (h + |v01 . v12|)) is beta for a non-negative dot
+    // product and pi - beta otherwise, and the ratio lies in [0, 1] without a
+    // min/max swap or quadrant fix-up.  Hardware sqrt / rsq / rcp / exp2 (about
+    // 1 ulp): no range-reduction branches in the loop.
+    const float y = __builtin_amdgcn_sqrtf(msq[k]);
+    const float h = __builtin_amdgcn_rsqf(rp[k]);
+    float b = two_atan_01(y * __builtin_amdgcn_rcpf(h + fabsf(dot[k])));
+    // the clamp of beta to [beta_min, pi - beta_min] is the clamp of b to
+    // [beta_min, pi/2] (b <= pi/2 already); one median-of-three (b is finite:
+    // events with non-finite hits are dropped before this kernel and voxel
+    // coordinates are checked finite)
+    b = __builtin_amdgcn_fmed3f(b, cc.beta_min, 2.0f);
+    const bool fwd = dot[k] >= 0.0f;
     // sin^2(beta) of the clamped beta: never below the clamp's floor
     const float s2 = __builtin_amdgcn_fmed3f(s2r[k], cc.s2_min, 2.0f);
-    const float delta = beta - e.th;
+    // delta = beta - theta; the fixed-sigma model needs only delta^2 = (b - theta')^2
+    // with theta' = theta or pi - theta
+    const float delta = SIG == SIG_STRICT ? (fwd ? b - e.th : e.pi_m_th - b) : b - (fwd ? e.th : e.pi_m_th);
     float hl2;  // log2(e) / (2 sigma^2)
     if (SIG == SIG_STRICT) {
       const float sige = delta >= 0.0f ? e.sp : e.sm;
