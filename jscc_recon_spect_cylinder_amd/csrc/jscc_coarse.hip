@@ -64,7 +64,7 @@ __global__ void __launch_bounds__(kCoarseThreads)
   float* rowbuf = lds;                                                    // [2][pc_pad]
   AxisTap* zt = reinterpret_cast<AxisTap*>(lds + 2 * pc_pad);             // [Zf]
   AxisTap* lt = zt + Zf;                                                  // [Lf] (general layer map)
-  float* red = reinterpret_cast<float*>(LID ? lt : lt + Lf);              // [3][kCoarseWaves], 16-byte aligned
+  float* red = reinterpret_cast<float*>(LID ? lt : lt + Lf);              // [2][3][kCoarseWaves], 16-byte aligned
   (void)kMaxCoarse;
 
   // ---- tap tables (indices checked once; a bad table sets bit 64 and reads index 0)
@@ -156,6 +156,8 @@ __global__ void __launch_bounds__(kCoarseThreads)
   for (int64_t r = rb; r < re; ++r) {
     const int buf = (int)((r - rb) & 1);
     const float* c = rowbuf + buf * pc_pad;
+    float* rd = red + buf * 3 * kCoarseWaves;  // wave sums, double-buffered: a fast wave may write the
+                                               // next row's while a slow one still reads this row's
     // a non-finite coarse value anywhere in the row
     int bad = 0;
 #pragma unroll 1
@@ -249,13 +251,17 @@ __global__ void __launch_bounds__(kCoarseThreads)
     D = jscc_wave_sum_dpp(D);
     const int wbad = __builtin_amdgcn_ballot_w64(bad != 0) != 0;
     if (jscc_lane() == 0) {
-      red[jscc_wave_id()] = S;
-      red[kCoarseWaves + jscc_wave_id()] = D;
-      red[2 * kCoarseWaves + jscc_wave_id()] = wbad ? 1.0f : 0.0f;
+      rd[jscc_wave_id()] = S;
+      rd[kCoarseWaves + jscc_wave_id()] = D;
+      rd[2 * kCoarseWaves + jscc_wave_id()] = wbad ? 1.0f : 0.0f;
     }
+    // one barrier per row: after it the wave sums are complete, the next row
+    // has landed in LDS (every wave waited for its own copies) and no wave
+    // reads this row's buffer any more (the copy into it is issued next row)
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
     float St = 0.0f, Dt = 0.0f, Bt = 0.0f;
-    const jscc_f4v* red4 = reinterpret_cast<const jscc_f4v*>(red);  // 16-byte reads, waves in order
+    const jscc_f4v* red4 = reinterpret_cast<const jscc_f4v*>(rd);  // 16-byte reads, waves in order
 #pragma unroll
     for (int q = 0; q < kCoarseWaves / 4; ++q) {
       const jscc_f4v a = red4[q], b = red4[kCoarseWaves / 4 + q], f = red4[kCoarseWaves / 2 + q];
@@ -287,8 +293,6 @@ __global__ void __launch_bounds__(kCoarseThreads)
 #pragma unroll
       for (int k = 0; k < K; ++k) wr[k] = fmaf(t[k], cf, wr[k]);
     }
-    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // the next row has landed in LDS
-    __syncthreads();
   }
 
   float* out = partial + (int64_t)g * P;
@@ -308,7 +312,7 @@ int coarse_grid(int64_t n_rows) {
 size_t coarse_lds(int Pc, int Zf, int Lf, bool lid) {
   const size_t pc_pad = (size_t)((Pc + 3) & ~3);
   return 2 * pc_pad * 4 + (size_t)Zf * sizeof(AxisTap) + (lid ? 0 : (size_t)Lf * sizeof(AxisTap)) +
-         3 * kCoarseWaves * sizeof(float);
+         2 * 3 * kCoarseWaves * sizeof(float);
 }
 
 template <int K, bool LID, bool ZU, bool NORM>
