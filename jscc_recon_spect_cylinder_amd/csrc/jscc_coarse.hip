@@ -64,7 +64,7 @@ __global__ void __launch_bounds__(kCoarseThreads)
   float* rowbuf = lds;                                                    // [2][pc_pad]
   AxisTap* zt = reinterpret_cast<AxisTap*>(lds + 2 * pc_pad);             // [Zf]
   AxisTap* lt = zt + Zf;                                                  // [Lf] (general layer map)
-  float* red = reinterpret_cast<float*>(LID ? lt : lt + Lf);              // [2][3][kCoarseWaves], 16-byte aligned
+  float* red = reinterpret_cast<float*>(LID ? lt : lt + Lf);              // [3][3][kCoarseWaves], 16-byte aligned
   (void)kMaxCoarse;
 
   // ---- tap tables (indices checked once; a bad table sets bit 64 and reads index 0)
@@ -188,10 +188,12 @@ __global__ void __launch_bounds__(kCoarseThreads)
 
     // phase 1: expand, weight by A, row sum and forward projection.  Branch
     // free: pixels past P read a clamped tap and have t = 0 (buffer range) and x = 0.
+    // The normalised variant's element-wise nan_to_num is deferred: a
+    // non-finite product makes the row sum non-finite, and only then (rare)
+    // are the products filtered and summed again.
     float S = 0.0f, D = 0.0f;
     auto consume = [&](int k, float u) {
-      float v = u * t[k];
-      if (NORM) v = finite_f(v) ? v : 0.0f;
+      const float v = u * t[k];
       t[k] = v;
       S += v;
       D = fmaf(v, xr[k], D);
@@ -261,13 +263,42 @@ __global__ void __launch_bounds__(kCoarseThreads)
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
     float St = 0.0f, Dt = 0.0f, Bt = 0.0f;
-    const jscc_f4v* red4 = reinterpret_cast<const jscc_f4v*>(rd);  // 16-byte reads, waves in order
+    auto block_sums = [&](const float* rs) {
+      St = 0.0f; Dt = 0.0f; Bt = 0.0f;
+      const jscc_f4v* red4 = reinterpret_cast<const jscc_f4v*>(rs);  // 16-byte reads, waves in order
 #pragma unroll
-    for (int q = 0; q < kCoarseWaves / 4; ++q) {
-      const jscc_f4v a = red4[q], b = red4[kCoarseWaves / 4 + q], f = red4[kCoarseWaves / 2 + q];
-      St += a.x; St += a.y; St += a.z; St += a.w;
-      Dt += b.x; Dt += b.y; Dt += b.z; Dt += b.w;
-      Bt += f.x + f.y + f.z + f.w;
+      for (int q = 0; q < kCoarseWaves / 4; ++q) {
+        const jscc_f4v a = red4[q], b = red4[kCoarseWaves / 4 + q], f = red4[kCoarseWaves / 2 + q];
+        St += a.x; St += a.y; St += a.z; St += a.w;
+        Dt += b.x; Dt += b.y; Dt += b.z; Dt += b.w;
+        Bt += f.x + f.y + f.z + f.w;
+      }
+    };
+    block_sums(rd);
+    if (NORM && Bt == 0.0f && !finite_f(St)) {
+      // some product is not finite (or the sum overflowed): nan_to_num the
+      // products and sum again (the same decision in every thread: St is read
+      // in the same order everywhere).  The third sums buffer is not the
+      // next row's, which a fast wave may already be writing.
+      float* rf = red + 2 * 3 * kCoarseWaves;
+      S = 0.0f;
+      D = 0.0f;
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const float v = finite_f(t[k]) ? t[k] : 0.0f;
+        t[k] = v;
+        S += v;
+        D = fmaf(v, xr[k], D);
+      }
+      S = jscc_wave_sum_dpp(S);
+      D = jscc_wave_sum_dpp(D);
+      if (jscc_lane() == 0) {
+        rf[jscc_wave_id()] = S;
+        rf[kCoarseWaves + jscc_wave_id()] = D;
+        rf[2 * kCoarseWaves + jscc_wave_id()] = 0.0f;
+      }
+      __syncthreads();
+      block_sums(rf);
     }
 
     float cf;
@@ -296,9 +327,13 @@ __global__ void __launch_bounds__(kCoarseThreads)
   }
 
   float* out = partial + (int64_t)g * P;
+  // the output offsets are formed here: left to the compiler, they are shared
+  // with the prologue's and spilled across the row loop
+  int jb = tid;
+  asm volatile("" : "+v"(jb));
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    const int j = tid + k * kCoarseThreads;
+    const int j = jb + k * kCoarseThreads;
     if (j < P) out[j] = wr[k];
   }
 }
@@ -312,7 +347,7 @@ int coarse_grid(int64_t n_rows) {
 size_t coarse_lds(int Pc, int Zf, int Lf, bool lid) {
   const size_t pc_pad = (size_t)((Pc + 3) & ~3);
   return 2 * pc_pad * 4 + (size_t)Zf * sizeof(AxisTap) + (lid ? 0 : (size_t)Lf * sizeof(AxisTap)) +
-         2 * 3 * kCoarseWaves * sizeof(float);
+         3 * 3 * kCoarseWaves * sizeof(float);
 }
 
 template <int K, bool LID, bool ZU, bool NORM>

@@ -441,7 +441,7 @@ def coarse_weight_fits(taps: AxisTaps) -> bool:
     of 4), coarse pixels <= 16384, and both coarse row buffers plus the tap tables
     within the 160 KB of LDS of a work-group."""
     P, Pc = taps.n_fine, taps.n_coarse
-    lds = 2 * round_up(Pc, 4) * 4 + 16 * taps.n_zf + (0 if taps.l_idx is None else 16 * taps.n_lf) + 2 * 3 * 16 * 4
+    lds = 2 * round_up(Pc, 4) * 4 + 16 * taps.n_zf + (0 if taps.l_idx is None else 16 * taps.n_lf) + 3 * 3 * 16 * 4
     return P <= _COARSE_MAX_FINE and P % 4 == 0 and Pc <= _COARSE_MAX_COARSE and lds <= _COARSE_LDS
 
 

@@ -137,6 +137,38 @@ def test_fused_weight_non_finite_rows(grid, normalize):
         assert not np.any(ref) and not np.any(got)
 
 
+@pytest.mark.parametrize("normalize", [True, False], ids=["materialised", "grouped"])
+def test_fused_weight_non_finite_system_matrix(grid, normalize):
+    """Infinite and NaN entries in the first-hit A rows with finite coarse values: the
+    materialised variant's element-wise nan_to_num zeroes those products and keeps the
+    rows (the kernel filters only when a row's sum comes out non-finite), the grouped
+    variant has no element-wise filter."""
+    from jscc_recon_spect_cylinder_amd import ops
+    rot, inv, p, pj = grid
+    ax = p.tables.axis.to(DEV)
+    n, M = 300, 16
+    ev, rng = make_rows(p, n, M, 13)
+    A = (rng.random((M, p.fine_pixel_num)) + 0.05).astype(np.float32)
+    A[3, 11] = np.inf
+    A[3, p.fine_pixel_num - 2] = -np.inf
+    A[9, p.fine_pixel_num // 2] = np.nan
+    used = np.isin(np.round(ev[:, 0]).astype(int) - 1, [3, 9])
+    assert used.sum() >= 10
+    x = (rng.random(p.fine_pixel_num) + 0.2).astype(np.float32)
+    w = torch.zeros(p.fine_pixel_num, device=DEV)
+    ops.coarse_weight(torch.from_numpy(ev).to(DEV), ax, torch.from_numpy(A).to(DEV), torch.from_numpy(x).to(DEV), w,
+                      normalize=normalize, eps=1e-12)
+    ref = oracle_weight(ev, A, x, pj, normalize)
+    got = w.cpu().numpy()
+    if normalize:
+        assert np.all(np.isfinite(got)) and rel(got, ref) < 2e-6
+        # the rows through the poisoned A rows contribute (nan_to_num per element, not a dropped row)
+        assert rel(got, oracle_weight(ev[~used], A, x, pj, True)) > 1e-3
+    else:
+        # a NaN (or inf - inf) row sum makes every pixel of the block weight NaN: zero after nan_to_num
+        assert not np.any(ref) and not np.any(got)
+
+
 def test_fused_weight_equals_materialise_then_project(grid):
     """The fused pass and the two-kernel path (materialise fine rows, fused projection)
     give the same weight."""
