@@ -193,6 +193,7 @@ __global__ void __launch_bounds__(kCoarseThreads)
     // are the products filtered and summed again.
     float S = 0.0f, D = 0.0f;
     auto consume = [&](int k, float u) {
+#pragma clang fp contract(off)  // S sums the rounded products, as the filtered pass does
       const float v = u * t[k];
       t[k] = v;
       S += v;
