@@ -389,33 +389,50 @@ __global__ void __launch_bounds__(kProjThreads)
 //   C. grid barrier; every work-group reloads the updated image into LDS.
 // The launch-to-launch gaps, per-launch ramp and the separate reduction
 // launch of the one-launch-per-subset path disappear.  The grid barrier is a
-// counter + generation pair with agent-scope release/acquire (the fences
-// write back / invalidate the per-XCD L2 caches), and a bounded wait: a
-// barrier that does not complete within spin_limit polls sets error bit 32
-// and every work-group leaves the kernel.
+// two-level counter tree plus a generation word with agent-scope
+// release/acquire (the fences write back / invalidate the per-XCD L2 caches),
+// and a bounded wait: a barrier that does not complete within spin_limit polls
+// sets error bit 32 and every work-group leaves the kernel.
 // ============================================================================
+// Barrier words (unsigned, each on its own 128-byte line): [kBarGroups]
+// group counters at i * kBarStride, the group-leader counter and the
+// generation after them.  Work-group g arrives on counter g % kBarGroups; the
+// last arrival of a group arrives on the leader counter, and the last leader
+// advances the generation: at most G / kBarGroups + kBarGroups serialised
+// atomics on any one word instead of G.
+constexpr int kBarGroups = 8;
+constexpr int kBarStride = 32;
+constexpr int kBarLead = kBarGroups * kBarStride;
+constexpr int kBarGen = kBarLead + kBarStride;
+constexpr int kBarWords = kBarGen + kBarStride;
+
 __device__ __forceinline__ bool grid_sync(unsigned* bar, unsigned n_groups, int64_t spin_limit, int32_t* err) {
   __shared__ int ok;
   __syncthreads();
   if (threadIdx.x == 0) {
     ok = 1;
     __threadfence();  // release this group's writes device-wide
-    const unsigned gen = __hip_atomic_load(bar + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned arrived = __hip_atomic_fetch_add(bar, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1u;
-    if (arrived == n_groups) {
-      __hip_atomic_store(bar, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(bar + 1, gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-      int64_t spins = 0;
-      // relaxed polling: an acquire load per poll would invalidate this XCD's L2 every time;
-      // the fence below is the one acquire
-      while (__hip_atomic_load(bar + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
-        __builtin_amdgcn_s_sleep(2);
-        if (++spins > spin_limit) {
-          atomicOr(err, 32);
-          ok = 0;
-          break;
-        }
+    const unsigned gen = __hip_atomic_load(bar + kBarGen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned grp = blockIdx.x % kBarGroups;
+    const unsigned in_grp = (n_groups - grp + kBarGroups - 1) / kBarGroups;  // groups g with g % kBarGroups == grp
+    const unsigned leaders = n_groups < (unsigned)kBarGroups ? n_groups : (unsigned)kBarGroups;
+    unsigned* cg = bar + grp * kBarStride;
+    if (__hip_atomic_fetch_add(cg, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1u == in_grp) {
+      __hip_atomic_store(cg, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (__hip_atomic_fetch_add(bar + kBarLead, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1u == leaders) {
+        __hip_atomic_store(bar + kBarLead, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(bar + kBarGen, gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+    int64_t spins = 0;
+    // relaxed polling: an acquire load per poll would invalidate this XCD's L2 every time;
+    // the fence below is the one acquire
+    while (__hip_atomic_load(bar + kBarGen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+      __builtin_amdgcn_s_sleep(2);
+      if (++spins > spin_limit) {
+        atomicOr(err, 32);
+        ok = 0;
+        break;
       }
     }
     __threadfence();  // acquire: what the other groups wrote is visible from here on
@@ -762,8 +779,8 @@ int dispatch_nch(int mode, const ProjArgs& a, int G, hipStream_t s) {
   return -3;
 }
 
-__global__ void __launch_bounds__(64) k_zero_u32(unsigned* __restrict__ v, int n) {
-  if ((int)threadIdx.x < n) v[threadIdx.x] = 0u;
+__global__ void __launch_bounds__(256) k_zero_u32(unsigned* __restrict__ v, int n) {
+  for (int i = threadIdx.x; i < n; i += 256) v[i] = 0u;
 }
 
 template <int DT, int NCH, int R>
@@ -1023,7 +1040,7 @@ extern "C" int jscc_sensitivity(const float* A, int64_t M, int64_t lda, int64_t 
 }
 
 extern "C" int64_t jscc_osem_persistent_workspace_bytes(int64_t n_cols) {
-  return 256 + (int64_t)jscc_num_cus() * n_cols * (int64_t)sizeof(float);
+  return (int64_t)kBarWords * 4 + (int64_t)jscc_num_cus() * n_cols * (int64_t)sizeof(float);
 }
 
 extern "C" int jscc_osem_persistent(const void* mat, int dtype, int64_t ld, int64_t n_cols, const int32_t* rows,
@@ -1043,8 +1060,8 @@ extern "C" int jscc_osem_persistent(const void* mat, int dtype, int64_t ld, int6
   if (n_iter == 0) return 0;
   hipStream_t s = (hipStream_t)stream;
   unsigned* bar = (unsigned*)workspace;
-  float* partial = (float*)((char*)workspace + 256);
-  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, bar, 2);
+  float* partial = (float*)((char*)workspace + kBarWords * 4);
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, s, bar, kBarWords);
   JSCC_CHECK_LAUNCH();
   OsemArgs o;
   o.ld = ld; o.P = (int)n_cols; o.n_sub = n_sub; o.n_iter = n_iter; o.eps = eps; o.use_eps = use_eps;

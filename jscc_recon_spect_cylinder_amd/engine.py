@@ -74,7 +74,8 @@ class ListModeStore:
         one fused projection + one reduction/update launch per subset (jscc_proj_em).
         persistent=True runs all subsets and iterations in one cooperative launch
         (jscc_osem_persistent) instead; on MI355X its two grid barriers per subset cost more
-        (~100 us per subset at 256 work-groups) than the launch they replace, so it is not the default."""
+        than the launches they replace (153 vs 117 us per subset at 2e4 BF16 rows x 16384),
+        so it is not the default."""
         if persistent and self.n_kept > 0:
             off = torch.tensor([b[0] for b in self.subset_bounds(n_subsets)] + [self.n_kept], dtype=torch.int64,
                                device=self.mat.device)
