@@ -104,10 +104,107 @@ __global__ void __launch_bounds__(kRowThreads) k_quantize(const T* __restrict__ 
   if (threadIdx.x == 0) row_scale[r] = (float)(q / S);
 }
 
+// 8 consecutive source values (16-byte aligned) as FP32
+template <typename T>
+__device__ __forceinline__ void load8_as_f32(const T* p, float* v);
+template <>
+__device__ __forceinline__ void load8_as_f32<float>(const float* p, float* v) {
+  const float4 a = reinterpret_cast<const float4*>(p)[0], b = reinterpret_cast<const float4*>(p)[1];
+  v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+}
+template <>
+__device__ __forceinline__ void load8_as_f32<uint16_t>(const uint16_t* p, float* v) {
+  const uint4 a = *reinterpret_cast<const uint4*>(p);
+  const uint32_t w[4] = {a.x, a.y, a.z, a.w};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    v[2 * k] = __uint_as_float(w[k] << 16);
+    v[2 * k + 1] = __uint_as_float(w[k] & 0xffff0000u);
+  }
+}
+
+// k_quantize with 8 values per thread and step: 16- or 32-byte loads, 8- (FP8)
+// or 16-byte (FP16) stores.  Needs 16-byte aligned rows on both sides
+// (checked by the launcher); values past P are written as zeros, as there.
+template <typename T, int OUT>
+__global__ void __launch_bounds__(kRowThreads) k_quantize_v8(const T* __restrict__ src, int64_t lds, int64_t P,
+                                                             int64_t n, const double* __restrict__ stats,
+                                                             void* __restrict__ dst, int64_t ldd,
+                                                             float* __restrict__ row_scale) {
+  __shared__ float red[kRowThreads / JSCC_WAVE];
+  const int64_t r = blockIdx.x;
+  if (r >= n) return;
+  const T* s = src + r * lds;
+  double S = 1.0, mx;
+  if (stats) {
+    S = stats[r * 4 + 0];
+    mx = stats[r * 4 + 3];
+  } else {
+    float m = 0.0f;
+    for (int64_t p = threadIdx.x * 8; p < P; p += kRowThreads * 8) {
+      float v[8];
+      load8_as_f32<T>(s + p, v);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) m = (p + k < P) ? fmaxf(m, fabsf(v[k])) : m;
+    }
+    m = jscc_wave_max(m);
+    if (jscc_lane() == 0) red[jscc_wave_id()] = m;
+    __syncthreads();
+    m = red[0];
+    for (int w = 1; w < kRowThreads / JSCC_WAVE; ++w) m = fmaxf(m, red[w]);
+    mx = m;
+  }
+  constexpr double kTop = OUT == JSCC_DT_FP8 ? 448.0 : 16384.0;
+  const double q = mx > 0.0 ? mx / kTop : 1.0;
+  const float inv_q = (float)(1.0 / q);
+#pragma unroll 2
+  for (int64_t p = threadIdx.x * 8; p < ldd; p += kRowThreads * 8) {
+    // values past P are zeroed before the scaling, so that the product feeds
+    // the conversion directly: for FP16 both round in one step
+    // (v_fma_mix), exactly as in k_quantize
+    float x[8], v[8];
+    if (p < P) {
+      load8_as_f32<T>(s + p, x);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) x[k] = p + k < P ? x[k] : 0.0f;
+    } else {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) x[k] = 0.0f;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = x[k] * inv_q;
+    if (OUT == JSCC_DT_FP8) {
+      uint2 o;
+      o.x = jscc_f32_to_fp8(v[0]) | (jscc_f32_to_fp8(v[1]) << 8) | (jscc_f32_to_fp8(v[2]) << 16) |
+            (jscc_f32_to_fp8(v[3]) << 24);
+      o.y = jscc_f32_to_fp8(v[4]) | (jscc_f32_to_fp8(v[5]) << 8) | (jscc_f32_to_fp8(v[6]) << 16) |
+            (jscc_f32_to_fp8(v[7]) << 24);
+      *reinterpret_cast<uint2*>(reinterpret_cast<uint8_t*>(dst) + r * ldd + p) = o;
+    } else {
+      uint4 o;
+      uint32_t* ow = reinterpret_cast<uint32_t*>(&o);
+#pragma unroll
+      for (int k = 0; k < 4; ++k)  // round to nearest even
+        ow[k] = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)v[2 * k]) |
+                ((uint32_t)__builtin_bit_cast(uint16_t, (_Float16)v[2 * k + 1]) << 16);
+      *reinterpret_cast<uint4*>(reinterpret_cast<_Float16*>(dst) + r * ldd + p) = o;
+    }
+  }
+  if (threadIdx.x == 0) row_scale[r] = (float)(q / S);
+}
+
 template <int OUT>
 int launch_quantize(const void* src, int src_dtype, int64_t ld_src, int64_t n_cols, int64_t n_rows,
                     const double* stats, void* dst, int64_t ld_dst, float* row_scale, hipStream_t s) {
-  if (src_dtype == JSCC_DT_F32) {
+  // 8 values per thread and step when both row layouts keep 16-byte alignment
+  const bool v8 = ld_src % 8 == 0 && ld_dst % 8 == 0 && ((uintptr_t)src % 16) == 0 && ((uintptr_t)dst % 16) == 0;
+  if (v8 && src_dtype == JSCC_DT_F32) {
+    hipLaunchKernelGGL((k_quantize_v8<float, OUT>), dim3((unsigned)n_rows), dim3(kRowThreads), 0, s,
+                       (const float*)src, ld_src, n_cols, n_rows, stats, dst, ld_dst, row_scale);
+  } else if (v8 && src_dtype == JSCC_DT_BF16) {
+    hipLaunchKernelGGL((k_quantize_v8<uint16_t, OUT>), dim3((unsigned)n_rows), dim3(kRowThreads), 0, s,
+                       (const uint16_t*)src, ld_src, n_cols, n_rows, stats, dst, ld_dst, row_scale);
+  } else if (src_dtype == JSCC_DT_F32) {
     hipLaunchKernelGGL((k_quantize<float, OUT>), dim3((unsigned)n_rows), dim3(kRowThreads), 0, s, (const float*)src,
                        ld_src, n_cols, n_rows, stats, dst, ld_dst, row_scale);
   } else if (src_dtype == JSCC_DT_BF16) {
@@ -230,7 +327,7 @@ extern "C" int jscc_arows_normalize(const float* A, int64_t lda, int64_t n_cols,
 extern "C" int jscc_quantize_rows(const void* src, int src_dtype, int64_t ld_src, int64_t n_cols, int64_t n_rows,
                                   const double* stats, void* dst, int dst_dtype, int64_t ld_dst, float* row_scale,
                                   void* stream) {
-  JSCC_REQUIRE(n_rows >= 0 && ld_dst >= n_cols, "jscc_quantize_rows: bad shape");
+  JSCC_REQUIRE(n_rows >= 0 && ld_dst >= n_cols && ld_src >= n_cols, "jscc_quantize_rows: bad shape");
   if (n_rows == 0) return 0;
   JSCC_REQUIRE(src && dst && row_scale, "jscc_quantize_rows: null pointer");
   JSCC_REQUIRE(n_rows <= 0x7fffffffLL, "jscc_quantize_rows: too many rows");

@@ -354,6 +354,37 @@ def test_fp16_rows_scaled_to_two_pow_14():
     assert np.all(np.abs(dec - A) <= 2.0 ** -11 * np.abs(A) + 2.0 ** -24 * scale.cpu().numpy()[:, None])
 
 
+@pytest.mark.parametrize("out", ["fp16", "fp8"])
+@pytest.mark.parametrize("src_dtype", [torch.float32, torch.bfloat16])
+@pytest.mark.parametrize("with_stats", [True, False], ids=["rows", "matrix"])
+def test_quantize_vector_and_scalar_layouts_agree(out, src_dtype, with_stats):
+    """Rows with a 16-byte aligned stride take the 8-values-per-thread kernel, others the
+    scalar one: the same codes bit for bit, the same scales, zeros past the last column."""
+    from jscc_recon_spect_cylinder_amd import ops
+    rng = np.random.default_rng(31)
+    n, P = 37, 515
+    base = (rng.random((n, P)) ** 3 * np.exp(rng.normal(-6, 2, (n, 1)))).astype(np.float32)
+    stats = None
+    if with_stats:
+        b64 = torch.from_numpy(base).to(src_dtype).double().numpy()
+        st = np.stack([b64.sum(1), (b64 ** 2).sum(1), b64.sum(1), b64.max(1)], axis=1)
+        stats = torch.from_numpy(st).to(DEV)
+    padded = torch.zeros((n, 520), dtype=src_dtype, device=DEV)
+    padded[:, :P] = torch.from_numpy(base).to(src_dtype).to(DEV)
+    padded[:, P:] = 7.0  # past the last column: must not leak into the codes
+    tight = padded[:, :P].contiguous()
+    assert padded.stride(0) % 8 == 0 and tight.stride(0) % 8 != 0
+    cv, sv = ops.quantize_rows(padded, P, n, stats, out)
+    cs, ss = ops.quantize_rows(tight, P, n, stats, out)
+    assert torch.equal(sv, ss)
+    assert torch.equal(cv.view(torch.uint8), cs.view(torch.uint8))
+    assert not cv.view(torch.uint8)[:, P * cv.element_size():].any()
+    ref = torch.from_numpy(base).to(src_dtype).float().numpy()
+    top = 448.0 if out == "fp8" else 16384.0
+    np.testing.assert_allclose(sv.cpu().numpy() * (stats[:, 0].cpu().numpy() if with_stats else 1.0),
+                               ref.max(1) / top, rtol=1e-6)
+
+
 @pytest.mark.parametrize("dtype", ["fp32", "bf16", "fp16", "fp8"])
 def test_persistent_osem_equals_per_subset_launches(dtype):
     """All subsets of all iterations in one cooperative launch (grid barriers between the
