@@ -187,6 +187,9 @@ int jscc_coarse_weight(const float* packed, int64_t ld_packed, int64_t n_rows, c
  * (recon_osem_local_sparse_jsccsd_only.py:13-19). */
 int jscc_proj_vec_width(int dtype);
 int jscc_proj_max_cols(int dtype);
+/* The largest number of work-groups (and partial rows in the workspace) a
+ * projection launch over n_rows rows uses: one per CU, at least 4 rows each
+ * (FP8; other dtypes take at least 8). */
 int jscc_proj_grid(int64_t n_rows);
 int64_t jscc_proj_workspace_bytes(int64_t n_rows, int64_t n_cols);
 int jscc_proj(int mode, const void* mat, int dtype, int64_t ld, int64_t n_cols, const int32_t* rows, int64_t n_rows,

@@ -829,12 +829,22 @@ extern "C" int jscc_proj_max_cols(int dtype) {
   return dtype == JSCC_DT_F32 ? 8 * kProjThreads * 4 : (dtype == JSCC_DT_FP8 ? 4 * kProjThreads * 16 : 8 * kProjThreads * 8);
 }
 
-extern "C" int jscc_proj_grid(int64_t n_rows) {
+// Work-groups of one projection launch: one per CU, each with at least
+// rows_min rows.  Below a full grid (small subsets, rows resident in the
+// Infinity Cache) a launch is latency-bound and fewer, longer work-groups
+// write fewer partials: 8 rows per group measured 10.4 vs 10.9 us per
+// 1024 x 4096 FP32 subset (FP8 rows, a quarter of the threads busy per row at
+// 4096 columns, prefer 4).
+static int proj_grid_rows(int64_t n_rows, int rows_min) {
   const int cus = jscc_num_cus();
-  int64_t g = (n_rows + 3) / 4;
+  int64_t g = (n_rows + rows_min - 1) / rows_min;
   if (g > cus) g = cus;
   if (g < 1) g = 1;
   return (int)g;
+}
+
+extern "C" int jscc_proj_grid(int64_t n_rows) {
+  return proj_grid_rows(n_rows, 4);  // the most work-groups any dtype's launch uses
 }
 
 extern "C" int64_t jscc_proj_workspace_bytes(int64_t n_rows, int64_t P) {
@@ -883,7 +893,7 @@ static int proj_impl(int mode, const void* mat, int dtype, int64_t ld, int64_t n
     return 0;
   }
   JSCC_REQUIRE(mat != nullptr && ((uintptr_t)mat % 16) == 0, "jscc_proj: matrix must be 16-byte aligned");
-  const int G = jscc_proj_grid(n_rows);
+  const int G = proj_grid_rows(n_rows, dtype == JSCC_DT_FP8 ? 4 : 8);
   const int G2 = (G + kRedRows - 1) / kRedRows;
   if (mode != 1) {
     JSCC_REQUIRE(workspace != nullptr && workspace_bytes >= jscc_proj_workspace_bytes(n_rows, n_cols),
