@@ -85,12 +85,15 @@ class ListModeStore:
             return x
         w = torch.zeros_like(x)
         bounds = self.subset_bounds(n_subsets)
+        n_max = max(hi - lo for lo, hi in bounds)
+        # back-projection and EM update in one call per subset (jscc_proj_em), validated once
+        step = ops.ProjLauncher(ops.MODE_RATIO, self.mat, n_rows_max=n_max, dtype_c=self.dtype_c,
+                                n_cols=self.n_vox, rows=self.kept, row_scale=self.row_scale, x=x, eps=eps,
+                                accumulate=True, w=w, em=(x, s, em_mode, 1e-12, True)) if n_max > 0 else None
         for _ in range(n_iter):
             for lo, hi in bounds:
-                if hi > lo:  # back-projection and EM update in one call (jscc_proj_em)
-                    ops.proj(ops.MODE_RATIO, self.mat, dtype_c=self.dtype_c, n_cols=self.n_vox, rows=self.kept,
-                             row_base=lo, n_rows=hi - lo, row_scale=self.row_scale, x=x, eps=eps, accumulate=True,
-                             w=w, em=(x, s, em_mode, 1e-12, True))
+                if hi > lo:
+                    step.launch(lo, hi - lo)
                 else:
                     ops.em_update(x, w, s, mode=em_mode, zero_after=w)
         return x
@@ -159,12 +162,16 @@ class SystemStore:
         n_max = max((hi - lo for lo, hi in bounds), default=1)
         ws = torch.empty(ops.lib_proj_workspace_bytes(n_max, self.n_vox), dtype=torch.uint8, device=x.device)
 
+        # validated once; each subset launch only sets its row range
+        step = ops.ProjLauncher(ops.MODE_RATIO, self.mat, n_rows_max=n_max, dtype_c=self.dtype_c,
+                                n_cols=self.n_vox, row_scale=self.row_scale, x=x, y=y, y_by_row=True, eps=eps,
+                                accumulate=True, w=w, em=(x, s, em_mode, 1e-12, True),
+                                workspace=ws) if bounds else None
+
         def run():
             for lo, hi in bounds:
-                ops.proj(ops.MODE_RATIO, self.mat, dtype_c=self.dtype_c, n_cols=self.n_vox, row_base=lo,
-                         n_rows=hi - lo, row_scale=self.row_scale, x=x, y=y, y_by_row=True, eps=eps,
-                         accumulate=True, w=w, em=(x, s, em_mode, 1e-12, True), workspace=ws)
-        return run, (w, ws, y, s, self.mat, self.row_scale)
+                step.launch(lo, hi - lo)
+        return run, (w, ws, y, s, self.mat, self.row_scale, step)
 
     def capture(self, x, y, s, n_subsets, eps=None, em_mode: int = ops.EM_PLAIN) -> "GraphedIteration":
         """One SC OSEM iteration captured as a HIP graph (x updated in place at each replay).  The

@@ -518,20 +518,64 @@ def proj_ready(mat: torch.Tensor, dtype_c: int) -> torch.Tensor:
     return mat.clone(memory_format=torch.contiguous_format) if mat.is_contiguous() else mat.contiguous()
 
 
-def proj(mode: int, mat: torch.Tensor, *, dtype_c: int | None = None, n_cols: int | None = None,
-         rows: torch.Tensor | None = None, n_rows: int | None = None, row_base: int = 0,
-         row_scale: torch.Tensor | None = None, x: torch.Tensor | None = None, x_perm: torch.Tensor | None = None,
-         y: torch.Tensor | None = None, coef: torch.Tensor | None = None, eps: float | None = None,
-         fwd_out: torch.Tensor | None = None, inv_perm: torch.Tensor | None = None, out_scale: float = 1.0,
-         accumulate: bool = False, w: torch.Tensor | None = None, y_by_row: bool = False,
-         sanitize: bool = False, em: tuple | None = None, workspace: torch.Tensor | None = None) -> None:
+def proj(mode: int, mat: torch.Tensor, **kw) -> None:
     """Launch jscc_proj on the current stream (see include/jscc_hip.h).
 
-    em = (x_em, s, em_mode, em_eps, zero_w[, c1, w2, c2]) (mode MODE_RATIO
-    only) runs jscc_proj_em instead: the EM update of x_em with the weight
-    c1*w (+ c2*w2) and sensitivity s is applied in the same reduction pass
-    that finishes w.  workspace: a caller-owned buffer of at least
+    Keywords: dtype_c, n_cols, rows, n_rows, row_base, row_scale, x, x_perm, y,
+    coef, eps, fwd_out, inv_perm, out_scale, accumulate, w, y_by_row, sanitize,
+    em, workspace.  em = (x_em, s, em_mode, em_eps, zero_w[, c1, w2, c2])
+    (mode MODE_RATIO only) runs jscc_proj_em instead: the EM update of x_em with
+    the weight c1*w (+ c2*w2) and sensitivity s is applied in the same reduction
+    pass that finishes w.  workspace: a caller-owned buffer of at least
     jscc_proj_workspace_bytes (graph capture); else the per-device cache."""
+    name, args, _ = _proj_call(mode, mat, **kw)
+    _lib.check(getattr(_lib.lib(), name)(*args), name)
+
+
+class ProjLauncher:
+    """One jscc_proj / jscc_proj_em call validated once and launched many times
+    over row ranges of the same operands (OSEM subset loops): the operand
+    checks, pointer and workspace lookups of `proj` happen at construction,
+    `launch(row_base, n_rows)` only replaces the two integers and takes the
+    device's current stream (so that a graph capture records it).  The
+    workspace is sized for n_rows_max rows, or is the caller's."""
+
+    def __init__(self, mode: int, mat: torch.Tensor, *, n_rows_max: int, **kw):
+        if n_rows_max < 1:
+            raise ValueError("n_rows_max must be positive")
+        kw = dict(kw)
+        kw["n_rows"] = n_rows_max
+        kw.setdefault("row_base", 0)
+        self.name, args, (self._i_rows, self._i_base) = _proj_call(mode, mat, **kw)
+        self._fn = getattr(_lib.lib(), self.name)
+        self._args = list(args)
+        self._dev = mat.device
+        self._n_max = n_rows_max
+        self._n_total = kw["rows"].shape[0] if kw.get("rows") is not None else mat.shape[0]
+        self._keep = (mat, kw)  # the operands stay alive as long as the launcher
+
+    def launch(self, row_base: int, n_rows: int) -> None:
+        if n_rows < 0 or n_rows > self._n_max or row_base < 0 or row_base + n_rows > self._n_total:
+            raise ValueError("row range outside the launcher's matrix or workspace")
+        a = self._args
+        a[self._i_rows] = n_rows
+        a[self._i_base] = row_base
+        a[-1] = stream_of(self._dev)
+        rc = self._fn(*a)
+        if rc:
+            _lib.check(rc, self.name)
+
+
+def _proj_call(mode: int, mat: torch.Tensor, *, dtype_c: int | None = None, n_cols: int | None = None,
+               rows: torch.Tensor | None = None, n_rows: int | None = None, row_base: int = 0,
+               row_scale: torch.Tensor | None = None, x: torch.Tensor | None = None,
+               x_perm: torch.Tensor | None = None, y: torch.Tensor | None = None, coef: torch.Tensor | None = None,
+               eps: float | None = None, fwd_out: torch.Tensor | None = None, inv_perm: torch.Tensor | None = None,
+               out_scale: float = 1.0, accumulate: bool = False, w: torch.Tensor | None = None,
+               y_by_row: bool = False, sanitize: bool = False, em: tuple | None = None,
+               workspace: torch.Tensor | None = None):
+    """Checks the operands of a projection launch and returns (entry point name,
+    argument list, (index of n_rows, index of row_base) in that list)."""
     dev = mat.device
     if dtype_c is None:
         dtype_c = dtype_code(mat.dtype) if mat.dtype != torch.uint8 else DT_FP8
@@ -587,24 +631,23 @@ def proj(mode: int, mat: torch.Tensor, *, dtype_c: int | None = None, n_cols: in
             if (t.dtype != torch.float32 or not t.is_contiguous() or t.device != dev or w is None
                     or t.numel() < w.shape[0]):
                 raise ValueError("EM operands must be contiguous float32 covering the weight, on the matrix's device")
-        rc = _lib.lib().jscc_proj_em(
+        args = [
             mat.data_ptr(), dtype_c, ld, P, _p(rows), n_rows, row_base, _p(row_scale), _p(x), _p(x_perm),
             x_perm.stride(0) if x_perm is not None else 0, xl, _p(y), y.stride(0) if y is not None else 0,
             float(eps) if eps is not None else 0.0, 1 if eps is not None else 0, _p(inv_perm),
             inv_perm.stride(0) if inv_perm is not None else 0, float(out_scale), 1 if accumulate else 0, flags,
             _p(w), wl, x_em.data_ptr(), s_em.data_ptr(), int(em_mode), float(em_eps), float(c1), _p(w2), float(c2),
             1 if zero_w else 0,
-            ws.data_ptr(), ws.numel(), err_word(dev).data_ptr(), stream_of(dev))
-        _lib.check(rc, "jscc_proj_em")
-        return
-    rc = _lib.lib().jscc_proj(
+            ws.data_ptr(), ws.numel(), err_word(dev).data_ptr(), stream_of(dev)]
+        return "jscc_proj_em", args, (5, 6)
+    args = [
         mode, mat.data_ptr(), dtype_c, ld, P, _p(rows), n_rows, row_base, _p(row_scale), _p(x), _p(x_perm),
         x_perm.stride(0) if x_perm is not None else 0, xl, _p(y), y.stride(0) if y is not None else 0, _p(coef),
         float(eps) if eps is not None else 0.0, 1 if eps is not None else 0, _p(fwd_out), _p(inv_perm),
         inv_perm.stride(0) if inv_perm is not None else 0, float(out_scale), 1 if accumulate else 0,
         flags, _p(w), wl,
-        ws.data_ptr(), ws.numel(), err_word(dev).data_ptr(), stream_of(dev))
-    _lib.check(rc, "jscc_proj")
+        ws.data_ptr(), ws.numel(), err_word(dev).data_ptr(), stream_of(dev)]
+    return "jscc_proj", args, (6, 7)
 
 
 EM_PLAIN, EM_SAFE, EM_APPLY = 0, 1, 2

@@ -443,3 +443,33 @@ def test_system_osem_graph_replays_match_eager(dtype):
     x_r = torch.ones(P, device=DEV)
     a.osem(x_r, y, s, 4, 3, graph=True)
     assert torch.equal(x_r, x_e)
+
+
+def test_proj_launcher_equals_proj_calls():
+    """ProjLauncher (operands checked once, row ranges per launch) gives the same bits as
+    one ops.proj call per range, with and without the EM epilogue, and rejects ranges
+    beyond its matrix or workspace."""
+    from jscc_recon_spect_cylinder_amd import ops
+    rng = np.random.default_rng(41)
+    P, M = 2048, 900
+    A = torch.from_numpy((rng.random((M, P)) ** 2).astype(np.float32)).to(DEV)
+    y = torch.from_numpy(rng.poisson(5.0, M).astype(np.float32)).to(DEV)
+    s = torch.from_numpy((rng.random(P) + 0.5).astype(np.float32)).to(DEV)
+    x0 = torch.from_numpy((rng.random(P) + 0.5).astype(np.float32)).to(DEV)
+    ranges = [(0, 300), (300, 599), (599, 899), (17, 22)]  # row ranges of at most 300 rows
+    for em in (False, True):
+        xa, wa = x0.clone(), torch.zeros(P, device=DEV)
+        xb, wb = x0.clone(), torch.zeros(P, device=DEV)
+        kw_a = dict(em=(xa, s, ops.EM_PLAIN, 1e-12, True)) if em else {}
+        kw_b = dict(em=(xb, s, ops.EM_PLAIN, 1e-12, True)) if em else {}
+        step = ops.ProjLauncher(ops.MODE_RATIO, A, n_rows_max=300, x=xa, y=y, y_by_row=True, eps=1e-12,
+                                accumulate=True, w=wa, **kw_a)
+        for lo, n in [(lo, hi - lo) for lo, hi in ranges]:
+            step.launch(lo, n)
+            ops.proj(ops.MODE_RATIO, A, row_base=lo, n_rows=n, x=xb, y=y, y_by_row=True, eps=1e-12,
+                     accumulate=True, w=wb, **kw_b)
+        assert torch.equal(xa, xb) and torch.equal(wa, wb)
+        with pytest.raises(ValueError):
+            step.launch(0, 301)
+        with pytest.raises(ValueError):
+            step.launch(850, 100)
