@@ -216,6 +216,27 @@ int jscc_proj_em(const void* mat, int dtype, int64_t ld, int64_t n_cols, const i
                  float* em_w2, float em_c2, int zero_w, void* workspace, int64_t workspace_bytes, int32_t* err,
                  void* stream);
 
+/* One JSCC-SD subset step of osem_joint_mode (recon_osem_plane.py:100-125) in
+ * ONE projection launch: the single-photon rows (mat, rows [row_base,
+ * row_base + n_rows) of the stored-row list or of the matrix when rows is
+ * NULL, numerators y, indexed by stored row when y_by_row) and the Compton rows
+ * (mat2, same dtype and n_cols, numerators y2, ones when NULL) are streamed by
+ * the same work-groups into one partial sum, then the reduction applies
+ *   w (+)= c1 * A^T(y / A x) + c2 * T^T(y2 / T x),
+ *   em_x[j] = update(em_x[j], w[j], em_s[j])      (modes of jscc_em_update)
+ * and clears w when zero_w != 0.  Replaces the pair get_weight_single +
+ * get_weight_compton and "img * ((2 - alpha) * weight_compton + alpha *
+ * weight_single) / s_map" (:117-118) with c1 = alpha, c2 = 2 - alpha.  Either
+ * segment may be empty, not both; the workspace is
+ * jscc_proj_workspace_bytes(n_rows + n_rows2, n_cols). */
+int jscc_proj_joint_em(const void* mat, int dtype, int64_t ld, int64_t n_cols, const int32_t* rows, int64_t n_rows,
+                       int64_t row_base, const float* row_scale, const float* y, int64_t y_stride, int y_by_row,
+                       float c1, const void* mat2, int64_t ld2, const int32_t* rows2, int64_t n_rows2,
+                       int64_t row_base2, const float* row_scale2, const float* y2, int64_t y2_stride, int y2_by_row,
+                       float c2, const float* x, float eps, int use_eps, int accumulate, float* w, int64_t w_len,
+                       float* em_x, const float* em_s, int em_mode, float em_eps, int zero_w, void* workspace,
+                       int64_t workspace_bytes, int32_t* err, void* stream);
+
 /* Every subset of n_iter list-mode OSEM iterations in ONE cooperative launch
  * (one work-group per CU): per subset the rows rows[sub_off[k] ..
  * sub_off[k+1]) are streamed once (mode 0 of jscc_proj with y = 1, no

@@ -69,6 +69,21 @@ __device__ __forceinline__ void decode<JSCC_DT_FP8>(const jscc_u4v& r, float* v)
   }
 }
 
+// The second row segment of a joint launch: a matrix of the same storage type
+// and width whose rows start `off` elements from the first matrix, its own
+// stored-row list, scales and numerator, and a factor on its numerators.
+struct ProjSeg {
+  int64_t off;
+  int64_t ld;
+  const int32_t* rows;
+  int64_t row_base;
+  const float* row_scale;
+  const float* y;
+  int64_t y_stride;
+  int y_by_row;
+  float num_scale;
+};
+
 struct ProjArgs {
   const void* M;             // matrix rows
   int64_t ld;                // row stride (elements)
@@ -90,6 +105,12 @@ struct ProjArgs {
   float* partial;            // [gridDim.x, P]
   int32_t* err;              // index error flag
   int y_by_row;              // numerator indexed by stored row id instead of call row
+  // Joint launches: call rows [0, n_rows1) come from the operands above with
+  // their numerators scaled by num_scale1, rows [n_rows1, n_rows) from seg2
+  // (n_rows1 == n_rows and num_scale1 == 1 otherwise).
+  int64_t n_rows1;
+  float num_scale1;
+  ProjSeg seg2;
 };
 
 // Row metadata staged in LDS for a chunk of a work-group's rows: stored row
@@ -102,7 +123,7 @@ constexpr int kBufWord3 = 0x00020000;
 constexpr int kBufNT = 2;
 
 struct RowMeta {
-  int64_t sid[kStage];
+  int64_t off[kStage];  // element offset of the row from the matrix start
   float sc[kStage];
   float num[kStage];
 };
@@ -136,8 +157,8 @@ __device__ __forceinline__ void fetch_rows(const ProjArgs& a, const RowMeta& met
   for (int q = 0; q < R; ++q) {
     const int64_t i = i0 + q;
     const bool live = i < c1;
-    const int64_t sid = meta.sid[live ? (int)(i - c0) : 0];
-    const char* rowp = reinterpret_cast<const char*>(a.M) + sid * a.ld * EB;
+    const int64_t off = meta.off[live ? (int)(i - c0) : 0];
+    const char* rowp = reinterpret_cast<const char*>(a.M) + off * EB;
     const __amdgpu_buffer_rsrc_t rs =
         __builtin_amdgcn_make_buffer_rsrc(const_cast<char*>(rowp), (short)0, live ? nvec * 16 : 0, kBufWord3);
 #pragma unroll
@@ -263,14 +284,25 @@ __global__ void __launch_bounds__(kProjThreads)
     __syncthreads();  // the previous chunk's metadata is no longer read
     if (tid < c1 - c0) {
       const int64_t i = c0 + tid;
-      const int64_t sid = rows ? (int64_t)rows[a.row_base + i] : a.row_base + i;
-      meta.sid[tid] = sid;
-      meta.sc[tid] = row_scale ? row_scale[sid] : 1.0f;
       float num = 1.0f;
-      if (MODE == MODE_COEF) {
-        num = coef[i];
-      } else if (MODE == MODE_RATIO && y) {
-        num = y[(a.y_by_row ? sid : i) * a.y_stride];
+      if (i < a.n_rows1) {
+        const int64_t sid = rows ? (int64_t)rows[a.row_base + i] : a.row_base + i;
+        meta.off[tid] = sid * a.ld;
+        meta.sc[tid] = row_scale ? row_scale[sid] : 1.0f;
+        if (MODE == MODE_COEF) {
+          num = coef[i];
+        } else if (MODE == MODE_RATIO && y) {
+          num = y[(a.y_by_row ? sid : i) * a.y_stride];
+        }
+        num *= a.num_scale1;
+      } else {  // second segment of a joint launch (ratio mode)
+        const ProjSeg& g = a.seg2;
+        const int64_t j = i - a.n_rows1;
+        const int64_t sid = g.rows ? (int64_t)g.rows[g.row_base + j] : g.row_base + j;
+        meta.off[tid] = g.off + sid * g.ld;
+        meta.sc[tid] = g.row_scale ? g.row_scale[sid] : 1.0f;
+        if (g.y) num = g.y[(g.y_by_row ? sid : j) * g.y_stride];
+        num *= g.num_scale;
       }
       meta.num[tid] = num;
     }
@@ -475,6 +507,7 @@ __global__ void __launch_bounds__(kProjThreads)
   a.M = M; a.ld = o.ld; a.P = o.P; a.rows = rows; a.n_rows = 0; a.row_base = 0; a.row_scale = row_scale;
   a.x = x; a.x_perm = nullptr; a.x_perm_stride = 0; a.x_len = o.P; a.y = nullptr; a.y_stride = 0; a.coef = nullptr;
   a.eps = o.eps; a.use_eps = o.use_eps; a.fwd_out = nullptr; a.partial = partial; a.err = err; a.y_by_row = 0;
+  a.n_rows1 = 0; a.num_scale1 = 1.0f; a.seg2 = ProjSeg{};
 
   extern __shared__ jscc_f4v xs4[];
   __shared__ RowMeta meta;
@@ -484,7 +517,7 @@ __global__ void __launch_bounds__(kProjThreads)
     __syncthreads();
     if (tid < c1 - c0) {
       const int64_t sid = (int64_t)rows[c0 + tid];
-      meta.sid[tid] = sid;
+      meta.off[tid] = sid * o.ld;
       meta.sc[tid] = row_scale ? row_scale[sid] : 1.0f;
       meta.num[tid] = 1.0f;
     }
@@ -867,7 +900,8 @@ static int proj_impl(int mode, const void* mat, int dtype, int64_t ld, int64_t n
                      int64_t y_stride, const float* coef, float eps, int use_eps, float* fwd_out,
                      const int64_t* inv_perm, int64_t inv_perm_stride, float out_scale, int accumulate,
                      int flags, float* w, int64_t w_len, void* workspace, int64_t workspace_bytes,
-                     int32_t* err, void* stream, const JsccEm& em) {
+                     int32_t* err, void* stream, const JsccEm& em, int64_t n_rows2 = 0,
+                     const ProjSeg* seg2 = nullptr, float num_scale1 = 1.0f) {
   JSCC_REQUIRE(mode >= 0 && mode <= 3, "jscc_proj: bad mode");
   JSCC_REQUIRE(n_rows >= 0 && n_cols > 0, "jscc_proj: bad shape");
   JSCC_REQUIRE(err != nullptr, "jscc_proj: null error word");
@@ -888,19 +922,21 @@ static int proj_impl(int mode, const void* mat, int dtype, int64_t ld, int64_t n
   if (mode == 1) JSCC_REQUIRE(fwd_out != nullptr, "jscc_proj: null forward output");
   if (mode == 2) JSCC_REQUIRE(coef != nullptr, "jscc_proj: null coefficients");
 
-  if (n_rows == 0) {
+  const int64_t n_total = n_rows + n_rows2;  // rows of both segments of a joint launch
+  if (n_total == 0) {
     if (mode != 1 && !accumulate) JSCC_CHECK_HIP(hipMemsetAsync(w, 0, w_len * sizeof(float), s));
     return 0;
   }
   JSCC_REQUIRE(mat != nullptr && ((uintptr_t)mat % 16) == 0, "jscc_proj: matrix must be 16-byte aligned");
-  const int G = proj_grid_rows(n_rows, dtype == JSCC_DT_FP8 ? 4 : 8);
+  const int G = proj_grid_rows(n_total, dtype == JSCC_DT_FP8 ? 4 : 8);
   const int G2 = (G + kRedRows - 1) / kRedRows;
   if (mode != 1) {
-    JSCC_REQUIRE(workspace != nullptr && workspace_bytes >= jscc_proj_workspace_bytes(n_rows, n_cols),
+    JSCC_REQUIRE(workspace != nullptr && workspace_bytes >= jscc_proj_workspace_bytes(n_total, n_cols),
                  "jscc_proj: workspace too small");
   }
   ProjArgs a;
-  a.M = mat; a.ld = ld; a.P = (int)n_cols; a.rows = rows; a.n_rows = n_rows; a.row_base = row_base;
+  a.M = mat; a.ld = ld; a.P = (int)n_cols; a.rows = rows; a.n_rows = n_total; a.row_base = row_base;
+  a.n_rows1 = n_rows; a.num_scale1 = num_scale1; a.seg2 = seg2 ? *seg2 : ProjSeg{};
   a.row_scale = row_scale; a.x = x; a.x_perm = x_perm; a.x_perm_stride = x_perm_stride; a.x_len = x_len;
   a.y = y; a.y_stride = y_stride; a.coef = coef; a.eps = eps; a.use_eps = use_eps; a.fwd_out = fwd_out;
   a.partial = (float*)workspace; a.err = err; a.y_by_row = (flags & 1) ? 1 : 0;
@@ -952,6 +988,54 @@ extern "C" int jscc_proj_em(const void* mat, int dtype, int64_t ld, int64_t n_co
   return proj_impl(0, mat, dtype, ld, n_cols, rows, n_rows, row_base, row_scale, x, x_perm, x_perm_stride, x_len, y,
                    y_stride, nullptr, eps, use_eps, nullptr, inv_perm, inv_perm_stride, out_scale, accumulate, flags,
                    w, w_len, workspace, workspace_bytes, err, stream, em);
+}
+
+// One JSCC-SD subset step in one projection launch: the rows of the first
+// matrix (numerators c1 * y1) and then those of the second (numerators
+// c2 * y2, ones when y2 is null) are back-projected with their ratios into the
+// same partial sums, so w (+)= c1 A1^T(y1 / A1 x) + c2 A2^T(y2 / A2 x), and
+// the reduction applies the EM update of jscc_em_update on em_x with that
+// weight and sensitivity em_s (w zeroed afterwards when zero_w is set).
+extern "C" int jscc_proj_joint_em(const void* mat, int dtype, int64_t ld, int64_t n_cols, const int32_t* rows,
+                                  int64_t n_rows, int64_t row_base, const float* row_scale, const float* y,
+                                  int64_t y_stride, int y_by_row, float c1, const void* mat2, int64_t ld2,
+                                  const int32_t* rows2, int64_t n_rows2, int64_t row_base2, const float* row_scale2,
+                                  const float* y2, int64_t y2_stride, int y2_by_row, float c2, const float* x,
+                                  float eps, int use_eps, int accumulate, float* w, int64_t w_len, float* em_x,
+                                  const float* em_s, int em_mode, float em_eps, int zero_w, void* workspace,
+                                  int64_t workspace_bytes, int32_t* err, void* stream) {
+  JSCC_REQUIRE(em_x != nullptr && em_s != nullptr, "jscc_proj_joint_em: null image or sensitivity");
+  JSCC_REQUIRE(em_mode >= 0 && em_mode <= 2, "jscc_proj_joint_em: bad EM mode %d", em_mode);
+  JSCC_REQUIRE(n_rows >= 0 && n_rows2 >= 0 && n_rows + n_rows2 > 0,
+               "jscc_proj_joint_em: empty subset (use jscc_em_update)");
+  const int vw = jscc_proj_vec_width(dtype);
+  JSCC_REQUIRE(vw > 0, "jscc_proj_joint_em: bad dtype %d", dtype);
+  JSCC_REQUIRE(mat != nullptr && ((uintptr_t)mat % 16) == 0, "jscc_proj_joint_em: matrix must be 16-byte aligned");
+  JSCC_REQUIRE(n_rows2 == 0 || (mat2 != nullptr && ((uintptr_t)mat2 % 16) == 0),
+               "jscc_proj_joint_em: second matrix must be 16-byte aligned");
+  JSCC_REQUIRE(n_rows2 == 0 || (ld2 % vw == 0 && ld2 >= n_cols),
+               "jscc_proj_joint_em: second row stride (%lld) must be a multiple of %d covering the columns",
+               (long long)ld2, vw);
+  JSCC_REQUIRE(n_rows2 == 0 || dtype != JSCC_DT_FP8 || row_scale2 != nullptr,
+               "jscc_proj_joint_em: FP8 rows need a per-row scale");
+  // element offset of the second matrix from the first (both 16-byte aligned,
+  // so a whole number of elements of any storage type)
+  const int eb = dtype == JSCC_DT_F32 ? 4 : (dtype == JSCC_DT_FP8 ? 1 : 2);
+  ProjSeg g{};
+  if (n_rows2 > 0) {
+    g.off = (int64_t)(reinterpret_cast<intptr_t>(mat2) - reinterpret_cast<intptr_t>(mat)) / eb;
+    g.ld = ld2; g.rows = rows2; g.row_base = row_base2; g.row_scale = row_scale2; g.y = y2; g.y_stride = y2_stride;
+    g.y_by_row = y2_by_row ? 1 : 0; g.num_scale = c2;
+  }
+  const JsccEm em = {em_x, em_s, em_mode, em_eps, zero_w, 1.0f, nullptr, 0.0f};
+  if (n_rows == 0) {  // the second segment alone
+    return proj_impl(0, mat2, dtype, ld2, n_cols, rows2, n_rows2, row_base2, row_scale2, x, nullptr, 0, n_cols, y2,
+                     y2_stride, nullptr, eps, use_eps, nullptr, nullptr, 0, 1.0f, accumulate, y2_by_row ? 1 : 0, w,
+                     w_len, workspace, workspace_bytes, err, stream, em, 0, nullptr, c2);
+  }
+  return proj_impl(0, mat, dtype, ld, n_cols, rows, n_rows, row_base, row_scale, x, nullptr, 0, n_cols, y, y_stride,
+                   nullptr, eps, use_eps, nullptr, nullptr, 0, 1.0f, accumulate, y_by_row ? 1 : 0, w, w_len,
+                   workspace, workspace_bytes, err, stream, em, n_rows2, n_rows2 > 0 ? &g : nullptr, c1);
 }
 
 // partial [G, P] (followed by room for ceil(G/16) more rows) -> w, deterministic order

@@ -207,13 +207,27 @@ def joint_osem(a: SystemStore, y: torch.Tensor, t: "ListModeStore", x: torch.Ten
     """JSCC-SD OSEM (osem_joint_mode, recon_osem_plane.py:100-125): per subset
     w = (2 - alpha) * T_s^T(1/T_s x) + alpha * A_s^T(y_s/A_s x), x <- x * w / s_joint.
     Subset k takes the k-th split_tensor_even range of the bins and of the
-    kept events.  Two launches per subset: the A pass accumulates the
-    single-photon weight, the T pass finishes the Compton weight and applies the
-    combined update in its reduction (jscc_proj_em); x is updated in place."""
+    kept events.  When A and T hold the same storage type, one projection
+    launch per subset streams the A rows (numerators alpha * y) and then the T
+    rows (numerators 2 - alpha) into the same partial sums and its reduction
+    applies the update (jscc_proj_joint_em); otherwise the A pass accumulates
+    the single-photon weight and the T pass finishes the Compton weight and
+    applies the combined update (jscc_proj_em).  x is updated in place."""
     ws = torch.zeros_like(x)
     wc = torch.zeros_like(x)
     a_b = a.subset_bounds(n_subsets)
     t_b = t.subset_bounds(n_subsets)
+    if a.mat.dtype == t.mat.dtype and a.dtype_c == t.dtype_c:
+        for _ in range(n_iter):
+            for (alo, ahi), (tlo, thi) in zip(a_b, t_b):
+                if ahi > alo or thi > tlo:
+                    ops.proj_joint_em(a.mat, t.mat, dtype_c=a.dtype_c, n_cols=a.n_vox, x=x, w=ws, s=s_joint,
+                                      c1=alpha, c2=2.0 - alpha, n_rows=ahi - alo, row_base=alo,
+                                      row_scale=a.row_scale, y=y, y_by_row=True, n_rows2=thi - tlo, rows2=t.kept,
+                                      row_base2=tlo, row_scale2=t.row_scale, eps=eps, em_mode=em_mode)
+                else:
+                    ops.em_update(x, ws, s_joint, mode=em_mode, zero_after=ws)
+        return x
     for _ in range(n_iter):
         for (alo, ahi), (tlo, thi) in zip(a_b, t_b):
             a.subset_weight(x, ws, alo, ahi, y, eps=eps)

@@ -566,6 +566,69 @@ class ProjLauncher:
             _lib.check(rc, self.name)
 
 
+def _segment_check(mat, rows, row_base, n_rows, row_scale, y, y_by_row, P, dev, what):
+    """Operand extents of one row segment of a joint projection."""
+    if n_rows < 0 or row_base < 0:
+        raise ValueError(f"{what}: negative row range")
+    if mat.stride(1) != 1 or mat.stride(0) < P:
+        raise ValueError(f"{what}: rows must be contiguous and cover the {P} columns")
+    if rows is not None:
+        if rows.dtype != torch.int32 or row_base + n_rows > rows.shape[0]:
+            raise ValueError(f"{what}: row list shorter than row_base + n_rows (int32 expected)")
+    elif row_base + n_rows > mat.shape[0]:
+        raise ValueError(f"{what}: row range exceeds the matrix")
+    if row_scale is not None and row_scale.numel() < mat.shape[0]:
+        raise ValueError(f"{what}: row_scale needs one entry per stored row")
+    if y is not None and y.shape[0] < (mat.shape[0] if y_by_row else n_rows):
+        raise ValueError(f"{what}: numerators shorter than the rows")
+    for t in (rows, row_scale, y):
+        if t is not None and t.device != dev:
+            raise ValueError(f"{what}: all operands must be on the image's device")
+
+
+def proj_joint_em(mat: torch.Tensor, mat2: torch.Tensor, *, dtype_c: int, n_cols: int, x: torch.Tensor,
+                  w: torch.Tensor, s: torch.Tensor, c1: float, c2: float, n_rows: int, n_rows2: int,
+                  rows=None, row_base: int = 0, row_scale=None, y=None, y_by_row: bool = False,
+                  rows2=None, row_base2: int = 0, row_scale2=None, y2=None, y2_by_row: bool = False,
+                  eps: float | None = None, em_mode: int = 0, em_eps: float = 1e-12, zero_w: bool = True,
+                  accumulate: bool = True, workspace: torch.Tensor | None = None) -> None:
+    """One JSCC-SD subset step in one projection launch (jscc_proj_joint_em):
+    w (+)= c1 * A^T(y / A x) + c2 * T^T(y2 / T x) over the two row segments,
+    then x <- update(x, w, s) and w cleared when zero_w.  Both matrices hold
+    the storage type dtype_c; y2 None means numerators of one."""
+    dev = x.device
+    P = int(n_cols)
+    if n_rows + n_rows2 <= 0:
+        raise ValueError("proj_joint_em: empty subset (use em_update)")
+    for m in (mat, mat2):
+        if m.device != dev:
+            raise ValueError("proj_joint_em: matrices must be on the image's device")
+    if mat.dtype != mat2.dtype:
+        raise ValueError("proj_joint_em: both matrices must hold the same storage type")
+    _segment_check(mat, rows, row_base, n_rows, row_scale, y, y_by_row, P, dev, "proj_joint_em (first segment)")
+    _segment_check(mat2, rows2, row_base2, n_rows2, row_scale2, y2, y2_by_row, P, dev,
+                   "proj_joint_em (second segment)")
+    for t in (x, w, s):
+        if t.dtype != torch.float32 or not t.is_contiguous() or t.device != dev or t.numel() != P:
+            raise ValueError(f"proj_joint_em: image, weight and sensitivity must be contiguous float32 of {P} pixels")
+    need = _lib.lib().jscc_proj_workspace_bytes(n_rows + n_rows2, P)
+    if workspace is not None:
+        if workspace.numel() * workspace.element_size() < need or workspace.device != dev:
+            raise ValueError(f"projection workspace needs {need} bytes on the image's device")
+        ws = workspace
+    else:
+        ws = _workspace(dev, need)
+    rc = _lib.lib().jscc_proj_joint_em(
+        mat.data_ptr(), int(dtype_c), mat.stride(0), P, _p(rows), int(n_rows), int(row_base), _p(row_scale), _p(y),
+        y.stride(0) if y is not None else 0, 1 if y_by_row else 0, float(c1),
+        mat2.data_ptr(), mat2.stride(0), _p(rows2), int(n_rows2), int(row_base2), _p(row_scale2), _p(y2),
+        y2.stride(0) if y2 is not None else 0, 1 if y2_by_row else 0, float(c2),
+        x.data_ptr(), float(eps) if eps is not None else 0.0, 1 if eps is not None else 0, 1 if accumulate else 0,
+        w.data_ptr(), P, x.data_ptr(), s.data_ptr(), int(em_mode), float(em_eps), 1 if zero_w else 0,
+        ws.data_ptr(), ws.numel(), err_word(dev).data_ptr(), stream_of(dev))
+    _lib.check(rc, "jscc_proj_joint_em")
+
+
 def _proj_call(mode: int, mat: torch.Tensor, *, dtype_c: int | None = None, n_cols: int | None = None,
                rows: torch.Tensor | None = None, n_rows: int | None = None, row_base: int = 0,
                row_scale: torch.Tensor | None = None, x: torch.Tensor | None = None,

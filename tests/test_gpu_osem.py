@@ -277,17 +277,21 @@ def _oracle_joint(A, y, T, s_j, alpha, n_iter, k):
     return x
 
 
-@pytest.mark.parametrize("dtype,tol", [("fp32", 2e-5), ("bf16", 3e-3), ("fp16", 1e-3), ("fp8", 3e-2)])
+@pytest.mark.parametrize("dtype,tol", [("fp32", 2e-5), ("bf16", 3e-3), ("fp16", 1e-3), ("fp8", 3e-2),
+                                       ("a-fp32", 3e-3)])
 def test_joint_osem_stores_vs_fp64_oracle(dtype, tol):
     """JSCCSD with A and T stored in FP32 / BF16 / FP8 (per-row scales), FP32
-    accumulation, against the FP64 oracle of osem_joint_mode (3 iterations x 4 subsets)."""
+    accumulation, against the FP64 oracle of osem_joint_mode (3 iterations x 4 subsets).
+    Equal storage types take the one-launch subset step; "a-fp32" (A in FP32, T in
+    BF16) the two-launch one."""
     from jscc_recon_spect_cylinder_amd.engine import ListModeStore, SystemStore, joint_osem
     sc, rows, T = _joint_problem()
     alpha = 0.5
     A64 = sc.A.astype(np.float64)
     s_j = alpha * A64.sum(0) + (2 - alpha) * T.sum(0)
     ref = _oracle_joint(A64, sc.y.astype(np.float64), T, s_j, alpha, 3, 4)
-    a = SystemStore.from_dense(torch.from_numpy(sc.A).to(DEV), dtype)
+    a = SystemStore.from_dense(torch.from_numpy(sc.A).to(DEV), "fp32" if dtype == "a-fp32" else dtype)
+    dtype = "bf16" if dtype == "a-fp32" else dtype
     t = ListModeStore.from_rows(rows[torch.bfloat16 if dtype == "bf16" else torch.float32], dtype)
     assert t.n_kept == T.shape[0]
     s = torch.from_numpy(s_j.astype(np.float32)).to(DEV)
@@ -340,6 +344,78 @@ def test_fused_joint_em_step_equals_projection_then_update():
     ops.proj(ops.MODE_RATIO, T, x=x, eps=1e-12, accumulate=True, w=wc, em=(x, s, 0, 1e-12, True, 1.5, ws, 0.5))
     assert torch.equal(x, x_ref)
     assert not wc.any() and not ws.any()
+
+
+def _two_launch_joint(A, T, x, s, y, alpha, a_rng, t_rng, eps, dtype_c, a_scale=None, t_rows=None, t_scale=None):
+    """The two-launch JSCCSD subset step: the A pass into ws, then the T pass with the combined update."""
+    from jscc_recon_spect_cylinder_amd import ops
+    P = x.numel()
+    ws, wc = torch.zeros(P, device=DEV), torch.zeros(P, device=DEV)
+    (alo, ahi), (tlo, thi) = a_rng, t_rng
+    if ahi > alo:
+        ops.proj(ops.MODE_RATIO, A, dtype_c=dtype_c, n_cols=P, row_base=alo, n_rows=ahi - alo, row_scale=a_scale,
+                 x=x, y=y, y_by_row=True, eps=eps, accumulate=True, w=ws)
+    if thi > tlo:
+        ops.proj(ops.MODE_RATIO, T, dtype_c=dtype_c, n_cols=P, rows=t_rows, row_base=tlo, n_rows=thi - tlo,
+                 row_scale=t_scale, x=x, eps=eps, accumulate=True, w=wc,
+                 em=(x, s, 0, 1e-12, True, 2.0 - alpha, ws, alpha))
+    else:
+        ops.em_update(x, wc, s, c1=2.0 - alpha, w2=ws, c2=alpha)
+
+
+@pytest.mark.parametrize("store", ["f32", "bf16", "fp8"])
+@pytest.mark.parametrize("a_rng,t_rng", [((128, 700), (40, 1900)), ((0, 0), (0, 2000)), ((0, 1024), (5, 5)),
+                                         ((1000, 1003), (1999, 2000))],
+                         ids=["both", "compton-only", "single-only", "ragged"])
+def test_joint_launch_equals_two_launches(store, a_rng, t_rng):
+    """jscc_proj_joint_em (A rows then T rows in one launch, numerators alpha*y and
+    2-alpha) equals the A pass + T pass with the combined EM update, for both
+    segments, either one empty and a few rows each; the weight is cleared."""
+    from jscc_recon_spect_cylinder_amd import ops
+    rng = np.random.default_rng(41)
+    P, alpha = 4096, 0.5
+    A = (rng.random((1024, P)) ** 3 * (rng.random((1024, P)) < 0.3)).astype(np.float32)
+    T = (rng.random((2200, P)) ** 4).astype(np.float32)
+    y = torch.from_numpy(rng.poisson(20.0, 1024).astype(np.float32)).to(DEV)
+    kept = torch.from_numpy(rng.permutation(2200)[:2000].astype(np.int32)).to(DEV)
+    a_scale = t_scale = None
+    if store == "fp8":
+        Ad, a_scale = ops.quantize_fp8(torch.from_numpy(A).to(DEV), P, 1024, None)
+        Td, t_scale = ops.quantize_fp8(torch.from_numpy(T).to(DEV), P, 2200, None)
+        dtype_c = ops.DT_FP8
+    else:
+        tdt = torch.float32 if store == "f32" else torch.bfloat16
+        Ad, Td = torch.from_numpy(A).to(DEV, tdt), torch.from_numpy(T).to(DEV, tdt)
+        t_scale = torch.from_numpy((rng.random(2200) + 0.5).astype(np.float32)).to(DEV)
+        dtype_c = ops.dtype_code(tdt)
+    x0 = torch.from_numpy((rng.random(P) + 0.5).astype(np.float32)).to(DEV)
+    s = torch.from_numpy((rng.random(P) * 50 + 10).astype(np.float32)).to(DEV)
+    x_ref = x0.clone()
+    _two_launch_joint(Ad, Td, x_ref, s, y, alpha, a_rng, t_rng, 1e-12, dtype_c, a_scale, kept, t_scale)
+    x, w = x0.clone(), torch.zeros(P, device=DEV)
+    ops.proj_joint_em(Ad, Td, dtype_c=dtype_c, n_cols=P, x=x, w=w, s=s, c1=alpha, c2=2.0 - alpha,
+                      n_rows=a_rng[1] - a_rng[0], row_base=a_rng[0], row_scale=a_scale, y=y, y_by_row=True,
+                      n_rows2=t_rng[1] - t_rng[0], rows2=kept, row_base2=t_rng[0], row_scale2=t_scale, eps=1e-12)
+    ops.raise_on_index_error(torch.device(DEV))
+    assert not w.any()
+    # same terms, other summation order and alpha folded into the numerators
+    np.testing.assert_allclose(x.cpu().numpy(), x_ref.cpu().numpy(), rtol=2e-6, atol=0)
+    assert not torch.equal(x, x0)
+
+
+def test_joint_launch_rejects_bad_operands():
+    from jscc_recon_spect_cylinder_amd import ops
+    P = 512
+    A = torch.ones((8, P), device=DEV, dtype=torch.bfloat16)
+    T = torch.ones((8, P), device=DEV, dtype=torch.float32)
+    x, w, s = torch.ones(P, device=DEV), torch.zeros(P, device=DEV), torch.ones(P, device=DEV)
+    kw = dict(dtype_c=ops.DT_BF16, n_cols=P, x=x, w=w, s=s, c1=0.5, c2=1.5)
+    with pytest.raises(ValueError, match="same storage type"):
+        ops.proj_joint_em(A, T, n_rows=8, n_rows2=8, **kw)
+    with pytest.raises(ValueError, match="empty subset"):
+        ops.proj_joint_em(A, A, n_rows=0, n_rows2=0, **kw)
+    with pytest.raises(ValueError, match="row range exceeds"):
+        ops.proj_joint_em(A, A, n_rows=4, n_rows2=8, row_base2=1, **kw)
 
 
 def test_fp16_rows_scaled_to_two_pow_14():
