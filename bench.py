@@ -668,8 +668,12 @@ def main():
         for tag in (_lib.PROF_CONE, _lib.PROF_PROJ, _lib.PROF_PROJ_SUM):
             _prof_read(lib, tag)  # drop pairs of the warm-up
         lib.jscc_prof_enable(1)
+        st = x = None
         t0 = time.perf_counter()
         for _ in range(args.steps):
+            # the previous step's T is released before the next is built (one T in HBM, as in a
+            # reconstruction), so its memory is reused instead of a second allocation of the same size
+            st = x = None
             st, x = step(sd, name)
         torch.cuda.synchronize()
         t1 = time.perf_counter()

@@ -79,6 +79,19 @@ def raise_on_index_error(device: torch.device) -> None:
         raise IndexError("; ".join(msgs))
 
 
+def read_count_checked(count: torch.Tensor) -> int:
+    """The device int32 count of a compaction read back together with the
+    device error word in one synchronisation (raises IndexError as
+    raise_on_index_error)."""
+    w = err_word(count.device)
+    count_v, code = torch.cat((count.view(1), w)).tolist()
+    if code:
+        w.zero_()
+        msgs = [txt for bit, txt in _ERR_TEXT.items() if code & bit]
+        raise IndexError("; ".join(msgs))
+    return int(count_v)
+
+
 # ----------------------------------------------------------------------------- geometry caches
 @dataclass
 class VoxelSoA:

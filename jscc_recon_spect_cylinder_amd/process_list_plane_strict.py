@@ -158,8 +158,7 @@ def build_compton_rows(sysmat, detector, coor, list_origin, delta_r1, delta_r2, 
     if copy_back is not None and n1 is not None:
         copy_back()
     sel_idx, sel_count = ops.compact(flags)
-    n_sel = int(sel_count.item())
-    ops.raise_on_index_error(dev)
+    n_sel = ops.read_count_checked(sel_count)  # one synchronisation for the count and the index check
     sig = ops.SIG_STRICT if sigma_model == "strict" else ops.SIG_FIXED
     if sig == ops.SIG_FIXED and not fixed_sigma:
         raise ValueError("sigma_model='fixed' needs fixed_sigma (rad)")
