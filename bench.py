@@ -604,6 +604,41 @@ def _prof_read(lib, tag):
     return c.value, t.value
 
 
+COS_CLAMP = float(np.float32(1.0 - 1e-7))
+
+
+def plain_fp32_cone(rows, coor, sigma, chunk=2048):
+    """The cone rows of the selected events as plain FP32 PyTorch operations on the GPU, broadcast over
+    chunks of events (the form a torch implementation of the cone backprojection takes on a GPU):
+    t = exp(-(beta - theta)^2 / (2 sigma^2)) * (kn - sin^2 beta) with beta = acos(clamp(v01 . v12 /
+    (|v01| |v12|))), written to an [n_sel, P] FP32 matrix together with the row sums.  Returns (seconds
+    for one pass over all selected events, the rows)."""
+    import torch
+    ev = rows.events[rows.sel_idx[: rows.n_sel].long()]
+    p1, a, th, kn = ev[:, 0:3], ev[:, 3:6], ev[:, 7:8], ev[:, 8:9]
+    C = coor.reshape(-1, 3).float()
+    out = torch.empty((rows.n_sel, C.shape[0]), dtype=torch.float32, device=C.device)
+    sums = torch.empty(rows.n_sel, dtype=torch.float32, device=C.device)
+
+    def run():
+        for i in range(0, rows.n_sel, chunk):
+            j = min(i + chunk, rows.n_sel)
+            v = p1[i:j, None, :] - C[None, :, :]
+            d12 = torch.linalg.vector_norm(a[i:j], dim=1, keepdim=True)
+            cosb = (v * a[i:j, None, :]).sum(2) / torch.clamp(torch.linalg.vector_norm(v, dim=2) * d12, min=1e-7)
+            beta = torch.acos(torch.clamp(cosb, -COS_CLAMP, COS_CLAMP))
+            t = torch.exp(-(beta - th[i:j]) ** 2 / (2.0 * sigma * sigma)) * (kn[i:j] - torch.sin(beta) ** 2)
+            out[i:j] = t
+            sums[i:j] = t.sum(1)
+
+    run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, out
+
+
 def _traffic(dtype_name, path=TRAFFIC_FILE):
     """HBM bytes per subset launch from the committed rocprofv3 counter summary (None when absent)."""
     try:
@@ -765,6 +800,22 @@ def main():
         "cone_roofline": cone_roof,
         "hbm_copy_GBs": _copy_bandwidth(dev),
     }
+    if not args.skip_fp32:
+        # the same rows as plain FP32 torch operations on the GPU, against the FP32 rows of the cone kernel
+        rows32 = build_compton_rows(None, det, coor, lst0.clone(), 0.0, 0.0, cl.e0, 0.02, tmax, tmin, tsum, dev,
+                                    noise=(n1, n2), sigma_model="fixed", fixed_sigma=sigma,
+                                    store_dtype=torch.float32)
+        sec, plain = plain_fp32_cone(rows32, coor, sigma)
+        k = min(512, rows32.n_sel)
+        ref = rows32.store[:k, :P]
+        rel_rows = float(torch.linalg.vector_norm(plain[:k] - ref) / torch.linalg.vector_norm(ref))
+        plain_mev = rows32.n_sel / sec / 1e6
+        out["plain_fp32_gpu"] = {
+            "form": "cone rows + row sums as plain FP32 torch operations on the GPU (chunks of 2048 events)",
+            "cone_ms": round(sec * 1e3, 2), "cone_Mevents_s": round(plain_mev, 3),
+            "bf16_kernel_speedup": round(bf["cone_Mevents_s"] / plain_mev, 2),
+            "rel_l2_rows_vs_fp32_kernel": rel_rows}
+        del plain, rows32
     if args.cpu_sample > 0:
         out["cpu_baseline"] = cpu_baseline(cl, args.cpu_sample, args.iters, args.subsets, sigma)
     print(json.dumps(out))
