@@ -697,20 +697,21 @@ constexpr int kDirCols = 64;
 __global__ void __launch_bounds__(256) k_reduce_direct(const float* __restrict__ part, int G, int P, float scale,
                                                        int accumulate, int sanitize, float* __restrict__ w,
                                                        EmEpilogue em) {
-  __shared__ jscc_f4v acc[16][kDirCols / 4];
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  constexpr int TX = kDirCols / 4, TY = 256 / TX;
+  __shared__ jscc_f4v acc[TY][TX];
+  const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
   const int col = blockIdx.x * kDirCols + tx * 4;
   jscc_f4v sum = jscc_f4v(0.0f);
   if (col < P) {
 #pragma unroll 8
-    for (int g = ty; g < G; g += 16) sum += *reinterpret_cast<const jscc_f4v*>(part + (int64_t)g * P + col);
+    for (int g = ty; g < G; g += TY) sum += *reinterpret_cast<const jscc_f4v*>(part + (int64_t)g * P + col);
   }
   acc[ty][tx] = sum;
   __syncthreads();
   if (ty == 0 && col < P) {
     jscc_f4v t = acc[0][tx];
 #pragma unroll
-    for (int k = 1; k < 16; ++k) t += acc[k][tx];
+    for (int k = 1; k < TY; ++k) t += acc[k][tx];
     finish_weight(t.x, col, scale, accumulate, sanitize, w, em);
     finish_weight(t.y, col + 1, scale, accumulate, sanitize, w, em);
     finish_weight(t.z, col + 2, scale, accumulate, sanitize, w, em);
