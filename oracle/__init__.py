@@ -15,9 +15,22 @@ Pinning (how each function is checked against something other than itself):
               equivariance of t_c under RotMat; the full strict chain vs golden
               vectors written by the reference's own function
               (tests/golden/strict_cone_small.npz)
-  osem.py     textbook GEMV (adjoint identity), EM fixed point and count
-              preservation, safe/apply update special cases
+  osem.py     brute-force loops of the GEMV weights; the rotation gather and
+              scatter against explicitly column-permuted matrices (random
+              non-involutive permutations, so a RotMat/RotMatInv swap fails);
+              alpha limits of the joint mode (alpha = 2 -> bin mode, 0 -> list
+              mode) and its count identity; EM fixed points (with rotations)
+              and count preservation; safe/apply update special cases
   sparse.py   hand-computable reduce/expand operators (fine 4, stride 2),
-              adjoint identity, reference-written golden projector
-              (tests/golden/sparse_projector_small.npz)
+              adjoint identity, reference-written golden projector digests
+              (tests/golden/sparse_projector_golden.npz); materialize against a
+              per-element loop with NaN/inf/zero/negative rows, non-finite A
+              entries, clamped and half-rounded crystal numbers
+  scd.py      thinning against the reference's literal expand-and-compare
+              algorithm; down-sampling against outputs written by the
+              reference's own function (tests/golden/scd_downsample_golden.npz)
+              and its count rules; the list-mode sensitivity against its
+              scale invariance, total and proportional-case closed forms
+All pins live in tests/test_oracle.py, tests/test_oracle_pins.py and
+tests/test_sparse_projector.py (CPU only).
 """

@@ -7,7 +7,7 @@ machines do not have):  python tests/golden/make_strict_golden.py
   get_compton_backproj_list_single (process_list_plane_strict.py:151-253), run
   on the CPU: the FP32 event filters and the smeared energies can only be pinned
   this way.  The system matrix is not stored: it is the closed-form
-  `golden_sysmat` below, recomputed by the tests.
+  `golden_sysmat` of golden_inputs.py, recomputed by the tests.
 * factors_511keV_RotateNum20.npz: the reference's detector table and checksums
   of its polar grid / RotMat / RotMatInv files (data only).
 """
@@ -18,17 +18,12 @@ import sys
 import numpy as np
 import torch
 
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+from golden_inputs import golden_sysmat  # noqa: E402
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 REF = "/root/reference"
 FACT = os.path.join(REF, "Factors", "511keV_RotateNum20")
-
-
-def golden_sysmat(detector: np.ndarray, coor: np.ndarray) -> np.ndarray:
-    """Positive smooth stand-in system matrix [D, P] (float32), a closed form of the inputs."""
-    d = detector[:, None, :3].astype(np.float64) - coor[None, :, :3].astype(np.float64)
-    dist = np.sqrt((d ** 2).sum(-1))
-    k = np.arange(detector.shape[0])[:, None]
-    return (np.exp(-dist / 150.0) * (1.25 + np.sin(0.37 * k))).astype(np.float32)
 
 
 def grid_checksum(a: np.ndarray, decimals: int) -> str:

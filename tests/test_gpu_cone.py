@@ -1,6 +1,5 @@
 """GPU parity of the cone back-projection (libjscc_hip through the drop-in API)
 against the reference's golden vectors and the FP64 oracle."""
-import importlib.util
 import os
 
 import numpy as np
@@ -9,6 +8,7 @@ import torch
 
 from oracle import cone
 from tests.conftest import GOLDEN
+from tests.golden.golden_inputs import golden_sysmat
 
 pytestmark = pytest.mark.gpu
 
@@ -18,10 +18,7 @@ DEV = "cuda:0"
 
 
 def _golden_sysmat(det, coor):
-    spec = importlib.util.spec_from_file_location("mk", os.path.join(GOLDEN, "make_strict_golden.py"))
-    mk = importlib.util.module_from_spec(spec)
-    spec.loader.exec_module(mk)
-    return mk.golden_sysmat(det, coor)
+    return golden_sysmat(det, coor)
 
 
 def _rel_rows(a, b):

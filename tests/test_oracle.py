@@ -9,17 +9,14 @@ import pytest
 
 from oracle import cone, layout, osem
 from tests.conftest import GOLDEN
+from tests.golden.golden_inputs import golden_sysmat
 
 FIX = os.path.join(GOLDEN, "factors_511keV_RotateNum20.npz")
 STRICT = os.path.join(GOLDEN, "strict_cone_small.npz")
 
 
 def _golden_sysmat(det, coor):
-    import importlib.util
-    spec = importlib.util.spec_from_file_location("mk", os.path.join(GOLDEN, "make_strict_golden.py"))
-    mk = importlib.util.module_from_spec(spec)
-    spec.loader.exec_module(mk)
-    return mk.golden_sysmat(det, coor)
+    return golden_sysmat(det, coor)
 
 
 # ------------------------------------------------------------------ layout (genSysMatPolar_3D.m)
@@ -177,11 +174,16 @@ def test_row_filters_support_definition():
 
 
 # ------------------------------------------------------------------ OSEM oracle
-def test_gemv_adjoint_identity():
+def test_weight_single_is_the_adjoint_of_the_forward_projection():
+    """With y = r * (A x) the oracle's weight_single is A^T r, so <x, w> = <A x, r>; with a
+    transpose-free mistake (A instead of A^T on the way back) the shapes or the identity fail."""
     rng = np.random.default_rng(0)
     A = rng.random((50, 40))
-    x, y = rng.random(40), rng.random(50)
-    assert math.isclose(np.dot(A @ x, y), np.dot(x, A.T @ y), rel_tol=1e-12)
+    x, r = rng.random(40) + 0.1, rng.random(50)
+    w = osem.weight_single(A, r * (A @ x), x)
+    assert w.shape == (40,)
+    assert math.isclose(float(x @ w), float((A @ x) @ r), rel_tol=1e-12)
+    np.testing.assert_allclose(w, np.einsum("ij,i->j", A, r), rtol=1e-12)
 
 
 def test_em_fixed_point_and_count_preservation():
