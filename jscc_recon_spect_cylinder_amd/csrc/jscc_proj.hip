@@ -859,8 +859,15 @@ extern "C" int jscc_proj_vec_width(int dtype) {
 }
 
 extern "C" int jscc_proj_max_cols(int dtype) {
-  // register-resident accumulators: up to 8 (FP8: 4) vectors per thread
-  return dtype == JSCC_DT_F32 ? 8 * kProjThreads * 4 : (dtype == JSCC_DT_FP8 ? 4 * kProjThreads * 16 : 8 * kProjThreads * 8);
+  const int vw = jscc_proj_vec_width(dtype);
+  if (vw < 0) return -1;
+  // the widest instantiated chunk count: register-resident accumulators of up
+  // to 8 (FP8: 4) vectors per thread, and an image of the ratio / forward
+  // modes in LDS (NCH x threads x vector x 4 bytes) within the 160 KB of a
+  // work-group -- 32768 columns for every storage type
+  int nch = dtype == JSCC_DT_FP8 ? 4 : 8;
+  while (nch > 1 && (size_t)nch * kProjThreads * vw * 4 > 160 * 1024) nch /= 2;
+  return nch * kProjThreads * vw;
 }
 
 // Work-groups of one projection launch: one per CU, each with at least

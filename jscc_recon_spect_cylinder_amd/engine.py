@@ -156,21 +156,27 @@ class SystemStore:
 
     def _iteration(self, x, y, s, n_subsets, eps, em_mode):
         """The launches of one OSEM iteration (one fused projection + EM launch pair per subset) and the
-        buffers they read: the weight and a workspace owned by this iteration."""
-        bounds = [(lo, hi) for lo, hi in self.subset_bounds(n_subsets) if hi > lo]
+        buffers they read: the weight and a workspace owned by this iteration.  An empty subset (more
+        subsets than bins) applies the update with a zero weight, x <- x * 0 / s, as the reference's
+        osem_bin_mode does for an empty block of rows (recon_osem_plane.py:45-54)."""
+        bounds = self.subset_bounds(n_subsets)
         w = torch.zeros_like(x)
-        n_max = max((hi - lo for lo, hi in bounds), default=1)
-        ws = torch.empty(ops.lib_proj_workspace_bytes(n_max, self.n_vox), dtype=torch.uint8, device=x.device)
+        n_max = max((hi - lo for lo, hi in bounds), default=0)
+        ws = torch.empty(ops.lib_proj_workspace_bytes(max(n_max, 1), self.n_vox), dtype=torch.uint8,
+                         device=x.device)
 
         # validated once; each subset launch only sets its row range
         step = ops.ProjLauncher(ops.MODE_RATIO, self.mat, n_rows_max=n_max, dtype_c=self.dtype_c,
                                 n_cols=self.n_vox, row_scale=self.row_scale, x=x, y=y, y_by_row=True, eps=eps,
                                 accumulate=True, w=w, em=(x, s, em_mode, 1e-12, True),
-                                workspace=ws) if bounds else None
+                                workspace=ws) if n_max > 0 else None
 
         def run():
             for lo, hi in bounds:
-                step.launch(lo, hi - lo)
+                if hi > lo:
+                    step.launch(lo, hi - lo)
+                else:
+                    ops.em_update(x, w, s, mode=em_mode, zero_after=w)
         return run, (w, ws, y, s, self.mat, self.row_scale, step)
 
     def capture(self, x, y, s, n_subsets, eps=None, em_mode: int = ops.EM_PLAIN) -> "GraphedIteration":

@@ -551,7 +551,10 @@ class ProjLauncher:
     checks, pointer and workspace lookups of `proj` happen at construction,
     `launch(row_base, n_rows)` only replaces the two integers and takes the
     device's current stream (so that a graph capture records it).  The
-    workspace is sized for n_rows_max rows, or is the caller's."""
+    workspace is the caller's, or a buffer the launcher allocates for
+    n_rows_max rows and owns for its lifetime; with its own buffer the
+    launcher only launches on the stream it was built on (launches on other
+    streams could overlap and share that buffer)."""
 
     def __init__(self, mode: int, mat: torch.Tensor, *, n_rows_max: int, **kw):
         if n_rows_max < 1:
@@ -559,21 +562,31 @@ class ProjLauncher:
         kw = dict(kw)
         kw["n_rows"] = n_rows_max
         kw.setdefault("row_base", 0)
+        self._own_ws = kw.get("workspace") is None
+        if self._own_ws:
+            P = kw.get("n_cols") if kw.get("n_cols") is not None else mat.shape[1]
+            kw["workspace"] = torch.empty(max(lib_proj_workspace_bytes(n_rows_max, P), 256), dtype=torch.uint8,
+                                          device=mat.device)
         self.name, args, (self._i_rows, self._i_base) = _proj_call(mode, mat, **kw)
         self._fn = getattr(_lib.lib(), self.name)
         self._args = list(args)
         self._dev = mat.device
+        self._stream = stream_of(mat.device)
         self._n_max = n_rows_max
         self._n_total = kw["rows"].shape[0] if kw.get("rows") is not None else mat.shape[0]
-        self._keep = (mat, kw)  # the operands stay alive as long as the launcher
+        self._keep = (mat, kw)  # the operands (and the workspace) stay alive as long as the launcher
 
     def launch(self, row_base: int, n_rows: int) -> None:
         if n_rows < 0 or n_rows > self._n_max or row_base < 0 or row_base + n_rows > self._n_total:
             raise ValueError("row range outside the launcher's matrix or workspace")
+        s = stream_of(self._dev)
+        if self._own_ws and s != self._stream:
+            raise ValueError("this launcher owns its workspace and launches only on the stream it was built on "
+                             "(pass workspace= to launch on others)")
         a = self._args
         a[self._i_rows] = n_rows
         a[self._i_base] = row_base
-        a[-1] = stream_of(self._dev)
+        a[-1] = s
         rc = self._fn(*a)
         if rc:
             _lib.check(rc, self.name)

@@ -76,3 +76,17 @@ def test_source_cache_hits_only_the_same_tensor_at_the_same_version():
         c.put(torch.zeros(1), (k,), k)                    # capacity reached: entries dropped
     gc.collect()
     assert ref() is None
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libjscc_hip.so not built")
+def test_projection_column_limit_covers_registers_and_lds():
+    """jscc_proj_max_cols is the smaller of the register and the LDS limit (32768 columns for every storage
+    type); a wider matrix is rejected by the host checks before any launch."""
+    from jscc_recon_spect_cylinder_amd import _lib
+    lib = _lib.load()
+    for dt in (0, 1, 2, 3):
+        assert lib.jscc_proj_max_cols(dt) == 32768
+    err = ctypes.c_int32(0)
+    rc = lib.jscc_proj(0, None, 1, 40960, 40960, None, 8, 0, None, None, None, 0, 40960, None, 0, None, 1e-12, 1,
+                       None, None, 0, 1.0, 0, 0, None, 40960, None, 0, ctypes.addressof(err), None)
+    assert rc != 0 and b"too many columns" in lib.jscc_last_error()
