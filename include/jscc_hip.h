@@ -9,7 +9,8 @@
  *   - every entry point returns 0 on success and a negative code on failure,
  *     with a message available from jscc_last_error() (thread-local);
  *   - the library never allocates device memory: scratch space is passed in
- *     as (workspace, workspace_bytes), sized by the *_workspace_bytes queries;
+ *     as (workspace, workspace_bytes), sized by the *_workspace_bytes queries
+ *     (jscc_proj_em_noalloc alone uses a static buffer of the library);
  *   - index tables that mirror the reference's 1-based tensors (RotMat,
  *     RotMatInv) are int64 with an element stride, exactly as torch stores a
  *     column of an [P, R] int64 tensor;
@@ -215,6 +216,26 @@ int jscc_proj_em(const void* mat, int dtype, int64_t ld, int64_t n_cols, const i
                  float* w, int64_t w_len, float* em_x, const float* em_s, int em_mode, float em_eps, float em_c1,
                  float* em_w2, float em_c2, int zero_w, void* workspace, int64_t workspace_bytes, int32_t* err,
                  void* stream);
+
+/* jscc_proj_em without the workspace arguments: the partial back-projections
+ * go to a static device buffer of JSCC_NOALLOC_SCRATCH_BYTES that the library
+ * holds on each GPU it runs on, so the call allocates nothing (no hipMalloc,
+ * no caller workspace).  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid allocation).  Subsets whose
+ * jscc_proj_workspace_bytes(n_rows, n_cols) exceed the buffer are rejected
+ * (above about 34,600 columns at 256 work-groups, so on a 256-CU GPU every
+ * width the projection accepts, up to 32768 columns, fits).  All calls on one GPU share its buffer: calls that overlap
+ * on one GPU (other streams, other threads) race.  The stream must belong to
+ * the current device. */
+#define JSCC_NOALLOC_SCRATCH_BYTES (36LL << 20)
+int jscc_proj_em_noalloc(const void* mat, int dtype, int64_t ld, int64_t n_cols, const int32_t* rows,
+                         int64_t n_rows, int64_t row_base, const float* row_scale, const float* x,
+                         const int64_t* x_perm, int64_t x_perm_stride, int64_t x_len, const float* y,
+                         int64_t y_stride, float eps, int use_eps, const int64_t* inv_perm, int64_t inv_perm_stride,
+                         float out_scale, int accumulate, int flags, float* w, int64_t w_len, float* em_x,
+                         const float* em_s, int em_mode, float em_eps, float em_c1, float* em_w2, float em_c2,
+                         int zero_w, int32_t* err, void* stream);
 
 /* One JSCC-SD subset step of osem_joint_mode (recon_osem_plane.py:100-125) in
  * ONE projection launch: the single-photon rows (mat, rows [row_base,

@@ -50,6 +50,9 @@ SIGNATURES = {
     "jscc_proj_em": (c_int, [vp, c_int, c_i64, c_i64, vp, c_i64, c_i64, vp, vp, vp, c_i64, c_i64, vp, c_i64, c_f32,
                              c_int, vp, c_i64, c_f32, c_int, c_int, vp, c_i64, vp, vp, c_int, c_f32, c_f32, vp, c_f32,
                              c_int, vp, c_i64, vp, vp]),
+    "jscc_proj_em_noalloc": (c_int, [vp, c_int, c_i64, c_i64, vp, c_i64, c_i64, vp, vp, vp, c_i64, c_i64, vp, c_i64,
+                                     c_f32, c_int, vp, c_i64, c_f32, c_int, c_int, vp, c_i64, vp, vp, c_int, c_f32,
+                                     c_f32, vp, c_f32, c_int, vp, vp]),
     "jscc_proj_joint_em": (c_int, [vp, c_int, c_i64, c_i64, vp, c_i64, c_i64, vp, vp, c_i64, c_int, c_f32,
                                    vp, c_i64, vp, c_i64, c_i64, vp, vp, c_i64, c_int, c_f32,
                                    vp, c_f32, c_int, c_int, vp, c_i64, vp, vp, c_int, c_f32, c_int, vp, c_i64, vp, vp]),

@@ -15,6 +15,9 @@
 // rotation gather and the alpha weight.  Rows are FP32, BF16 or FP8-e4m3fn
 // (with a per-row scale), always accumulated in FP32.
 #include "jscc_common.h"
+#include "jscc_hip.h"
+
+#include <mutex>
 
 namespace {
 
@@ -996,6 +999,69 @@ extern "C" int jscc_proj_em(const void* mat, int dtype, int64_t ld, int64_t n_co
   return proj_impl(0, mat, dtype, ld, n_cols, rows, n_rows, row_base, row_scale, x, x_perm, x_perm_stride, x_len, y,
                    y_stride, nullptr, eps, use_eps, nullptr, inv_perm, inv_perm_stride, out_scale, accumulate, flags,
                    w, w_len, workspace, workspace_bytes, err, stream, em);
+}
+
+// ------------------------------------------------------------------ allocation-free subset step
+// The partial back-projections of jscc_proj_em_noalloc live in a file-scope
+// device array (each GPU holds its own copy, at its own address, for the life
+// of the process).  In host code `scratch` is only a stand-in for the symbol:
+// its device address is looked up once per device with hipGetSymbolAddress
+// and only that address is handed to kernels.  Every work-group writes its
+// whole partial row and the reduction writes every row it reads, so nothing
+// in the buffer needs clearing between calls.  Calls that overlap on one GPU
+// (on different streams, or from different threads) share the buffer and race.
+static __device__ __align__(256) unsigned char scratch[JSCC_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_addr[kMaxDevices] = {};
+
+// The scratch address on the device of `stream` (which must be the current device).
+int noalloc_scratch(hipStream_t stream, void** out) {
+  int dev = 0;
+  JSCC_CHECK_HIP(hipGetDevice(&dev));
+  JSCC_REQUIRE(dev >= 0 && dev < kMaxDevices, "jscc_proj_em_noalloc: device %d beyond the table of %d", dev,
+               kMaxDevices);
+  if (stream != nullptr) {
+    int sdev = -1;
+    JSCC_CHECK_HIP(hipStreamGetDevice(stream, &sdev));
+    JSCC_REQUIRE(sdev == dev, "jscc_proj_em_noalloc: the stream belongs to device %d, the current device is %d",
+                 sdev, dev);
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_addr[dev] == nullptr) {
+    void* p = nullptr;
+    JSCC_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    JSCC_REQUIRE(p != nullptr, "jscc_proj_em_noalloc: no device address for the scratch buffer");
+    g_scratch_addr[dev] = p;
+  }
+  *out = g_scratch_addr[dev];
+  return 0;
+}
+}  // namespace
+
+extern "C" int jscc_proj_em_noalloc(const void* mat, int dtype, int64_t ld, int64_t n_cols, const int32_t* rows,
+                                    int64_t n_rows, int64_t row_base, const float* row_scale, const float* x,
+                                    const int64_t* x_perm, int64_t x_perm_stride, int64_t x_len, const float* y,
+                                    int64_t y_stride, float eps, int use_eps, const int64_t* inv_perm,
+                                    int64_t inv_perm_stride, float out_scale, int accumulate, int flags, float* w,
+                                    int64_t w_len, float* em_x, const float* em_s, int em_mode, float em_eps,
+                                    float em_c1, float* em_w2, float em_c2, int zero_w, int32_t* err,
+                                    void* stream) {
+  JSCC_REQUIRE(n_rows > 0 && n_cols > 0, "jscc_proj_em_noalloc: empty subset or no columns");
+  const int64_t need = jscc_proj_workspace_bytes(n_rows, n_cols);
+  JSCC_REQUIRE(need <= (int64_t)JSCC_NOALLOC_SCRATCH_BYTES,
+               "jscc_proj_em_noalloc: %lld rows x %lld columns need %lld bytes of scratch, above the %lld bytes "
+               "of the library's static buffer (use jscc_proj_em with a workspace)",
+               (long long)n_rows, (long long)n_cols, (long long)need, (long long)JSCC_NOALLOC_SCRATCH_BYTES);
+  void* ws = nullptr;
+  const int rc = noalloc_scratch((hipStream_t)stream, &ws);
+  if (rc != 0) return rc;
+  return jscc_proj_em(mat, dtype, ld, n_cols, rows, n_rows, row_base, row_scale, x, x_perm, x_perm_stride, x_len, y,
+                      y_stride, eps, use_eps, inv_perm, inv_perm_stride, out_scale, accumulate, flags, w, w_len,
+                      em_x, em_s, em_mode, em_eps, em_c1, em_w2, em_c2, zero_w, ws,
+                      (int64_t)JSCC_NOALLOC_SCRATCH_BYTES, err, stream);
 }
 
 // One JSCC-SD subset step in one projection launch: the rows of the first
