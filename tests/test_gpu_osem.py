@@ -277,8 +277,11 @@ def _oracle_joint(A, y, T, s_j, alpha, n_iter, k):
     return x
 
 
-@pytest.mark.parametrize("dtype,tol", [("fp32", 2e-5), ("bf16", 3e-3), ("fp16", 1e-3), ("fp8", 3e-2),
-                                       ("a-fp32", 3e-3)])
+# Storage types that meet the 1e-3 bar of the low-precision path are held to it; BF16 (measured 1.72e-3) and
+# FP8 (2.28e-2) storage of BOTH A and T miss it on this JSCCSD instance and are held to their measured bounds.
+# A in FP32 with T in BF16 ("a-fp32", 1.9e-5) meets it: the miss comes from rounding A, not T.
+@pytest.mark.parametrize("dtype,tol", [("fp32", 2e-5), ("bf16", 2e-3), ("fp16", 1e-3), ("fp8", 3e-2),
+                                       ("a-fp32", 1e-3)])
 def test_joint_osem_stores_vs_fp64_oracle(dtype, tol):
     """JSCCSD with A and T stored in FP32 / BF16 / FP8 (per-row scales), FP32
     accumulation, against the FP64 oracle of osem_joint_mode (3 iterations x 4 subsets).
@@ -297,24 +300,31 @@ def test_joint_osem_stores_vs_fp64_oracle(dtype, tol):
     s = torch.from_numpy(s_j.astype(np.float32)).to(DEV)
     x = torch.ones(A64.shape[1], device=DEV)
     joint_osem(a, torch.from_numpy(sc.y).to(DEV), t, x, s, alpha, 3, 4)
-    assert rel(x.cpu().numpy(), ref) < tol
+    err = rel(x.cpu().numpy(), ref)
+    print(f"joint osem {dtype}: image rel err vs FP64 oracle {err:.3e}")
+    assert err < tol
 
 
-@pytest.mark.parametrize("dtype,tol", [("fp32", 1e-5), ("bf16", 3e-3), ("fp16", 1e-3), ("fp8", 3e-2)])
-def test_system_store_osem_and_column_sums_vs_oracle(dtype, tol):
+# BF16 (measured 2.0e-4) and FP16 (1.9e-5) meet the 1e-3 bar; FP8 (2.6e-3) misses it and is held to its
+# measured bound.  s_tol bounds the column sums of the stored rows.
+@pytest.mark.parametrize("dtype,tol,s_tol", [("fp32", 1e-5, 1e-6), ("bf16", 1e-3, 1e-4), ("fp16", 1e-3, 1e-4),
+                                             ("fp8", 4e-3, 3e-3)])
+def test_system_store_osem_and_column_sums_vs_oracle(dtype, tol, s_tol):
     from jscc_recon_spect_cylinder_amd.engine import SystemStore
     from jscc_recon_spect_cylinder_amd import synth
     sc = synth.sc_osem(n_rings=32, n_angles=128, n_rot=2, bins_per_rot=1024, total_counts=2e5, seed=5)
     A64 = sc.A.astype(np.float64)
     a = SystemStore.from_dense(torch.from_numpy(sc.A).to(DEV), dtype)
     s = a.column_sums()
-    assert rel(s.cpu().numpy(), A64.sum(0)) < tol / 10
+    assert rel(s.cpu().numpy(), A64.sum(0)) < s_tol
     x = a.osem(torch.ones(A64.shape[1], device=DEV), torch.from_numpy(sc.y).to(DEV), s, 3, 2)
     xr = np.ones(A64.shape[1])
     for _ in range(3):
         for lo, hi in ((0, 1024), (1024, 2048)):
             xr = xr * osem.weight_single(A64[lo:hi], sc.y[lo:hi].astype(np.float64), xr) / A64.sum(0)
-    assert rel(x.cpu().numpy(), xr) < tol
+    err = rel(x.cpu().numpy(), xr)
+    print(f"system store osem {dtype}: image rel err vs FP64 oracle {err:.3e}")
+    assert err < tol
 
 
 def test_fp8_matrix_rows_scaled_by_their_own_maximum():

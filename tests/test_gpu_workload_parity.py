@@ -66,9 +66,10 @@ def jsccd_list():
     return cl, {"t": T, "kept": kept}, x, s
 
 
-# FP32 rows meet the 1e-4 bar of the FP32 path, BF16 and FP16 rows the 1e-3 bar of the low-precision path;
-# FP8 rows (per-row scale) are held to their measured bound and do not meet 1e-3.
-@pytest.mark.parametrize("dtype,tol", [("fp32", 1e-4), ("bf16", 1e-3), ("fp16", 1e-3), ("fp8", 3e-2)])
+# FP32 rows meet the 1e-4 bar of the FP32 path (measured 5.3e-6), BF16 (1.2e-4) and FP16 (1.4e-5) rows the
+# 1e-3 bar of the low-precision path; FP8 rows (per-row scale, measured 2.2e-3) miss it and are held to their
+# measured bound.
+@pytest.mark.parametrize("dtype,tol", [("fp32", 1e-4), ("bf16", 1e-3), ("fp16", 1e-3), ("fp8", 3e-3)])
 def test_jsccd_grid_10x8_vs_fp64_oracle(jsccd_list, dtype, tol):
     from jscc_recon_spect_cylinder_amd import synth
     from jscc_recon_spect_cylinder_amd.engine import ListModeStore

@@ -290,3 +290,30 @@ def test_sensitivity_from_t_invariants():
     np.testing.assert_allclose(one, s * n_c / n_s, rtol=1e-12)
     # the count ratio the other way round (n_s / n_c) fails the total
     assert not math.isclose(d.sum(), len(blocks) * s.sum() * n_s / n_c, rel_tol=1e-6)
+
+
+# ------------------------------------------------------------------ the CPU baseline's torch restatement
+def test_torch_cpu_baseline_agrees_with_the_fp64_oracle():
+    """The FP32 torch restatement that bench.py times as the reference's CPU path keeps the oracle's events
+    and reproduces its rows and its 10 x 8 list-mode image (FP32 against FP64)."""
+    from jscc_recon_spect_cylinder_amd import synth
+    from oracle import cone, torch_cpu
+    cl = synth.compton_list(n_events=700, n_rings=16, n_angles=64, seed=9)
+    tmax, tmin, tsum = synth.energy_thresholds(cl.e0)
+    sig = float(np.deg2rad(3.0))
+    g = torch.Generator().manual_seed(4)
+    n1, n2 = torch.randn(700, generator=g), torch.randn(700, generator=g)
+    ref = cone.backproj_list_single(None, cl.detector, cl.coor, cl.list_origin, 0.0, 0.0, cl.e0, 0.02, tmax, tmin,
+                                    tsum, n1.numpy(), n2.numpy(), fixed_sigma=sig)
+    T, kept = torch_cpu.cone_rows_fixed_sigma(torch.from_numpy(cl.detector), torch.from_numpy(cl.coor),
+                                              torch.from_numpy(cl.list_origin.copy()), cl.e0, 0.02, tmax, tmin, tsum,
+                                              n1, n2, sig)
+    assert np.array_equal(kept.numpy(), ref["kept"])
+    assert np.abs(T.double().numpy() - ref["t"]).max() / ref["t"].max() < 1e-4
+    x = torch_cpu.list_osem(T, 10, 8).double().numpy()
+    xr = np.ones(T.shape[1])
+    s = ref["t"].sum(0)
+    for _ in range(10):
+        for lo, hi in osem.split_bounds_even(ref["t"].shape[0], 8):
+            xr = osem.em_plain(xr, osem.weight_compton(ref["t"][lo:hi], xr), s)
+    assert np.linalg.norm(x - xr) / np.linalg.norm(xr) < 1e-4
