@@ -447,11 +447,15 @@ def run_sparse(args):
         packed[lo:hi] = packed[lo:hi][order]
     s = torch.ones(P, device=dev)
     w = torch.zeros(P, device=dev)
+    # the weight's form per subset (grouped when runs of one crystal are long), decided once outside the loop
+    run_len = [ops.crystal_run_length(packed[lo:hi]) for lo, hi in bounds]
+    forms = [bool(ax.za_idx is not None and r >= ops.GROUPED_MIN_RUN) for r in run_len]
 
     def run(x):
         for _ in range(n_iter):
-            for lo, hi in bounds:
-                ops.coarse_weight(packed[lo:hi], ax, A, x, w, normalize=True, eps=1e-12, accumulate=True)
+            for (lo, hi), form in zip(bounds, forms):
+                ops.coarse_weight(packed[lo:hi], ax, A, x, w, normalize=True, eps=1e-12, accumulate=True,
+                                  grouped=form)
                 ops.em_update(x, w, s, mode=ops.EM_SAFE, zero_after=w)
         return x
 
@@ -476,7 +480,8 @@ def run_sparse(args):
     xg = torch.ones(P, device=dev)
     for _ in range(3):
         for lo, hi in bs:
-            ops.coarse_weight(sample[lo:hi], ax, A, xg, w, normalize=True, eps=1e-12, accumulate=True)
+            ops.coarse_weight(sample[lo:hi], ax, A, xg, w, normalize=True, eps=1e-12, accumulate=True,
+                              grouped=forms[0])
             ops.em_update(xg, w, s, mode=ops.EM_SAFE, zero_after=w)
     # the oracle gets only the A rows the sample touches (crystal numbers remapped to them)
     cps, inv_idx = torch.unique(sample[:, 0], return_inverse=True)
@@ -494,7 +499,10 @@ def run_sparse(args):
                                    f"z 2 coarse grid ({Pc} pixels), A {M}x{P} FP32, {n_iter} iterations x {K} "
                                    "subsets"},
             "iter_s": round(n_iter / dt, 3),
-            "roofline": {"kernel": "k_coarse_weight<25, identity layer, uniform z, normalised>: one subset",
+            "roofline": {"kernel": ("k_coarse_grouped<25, normalised>" if all(forms) else
+                                    "k_coarse_weight<25, identity layer, uniform z, normalised>") + ": one subset",
+                         "form": "grouped" if all(forms) else ("per-row" if not any(forms) else "mixed"),
+                         "rows_per_crystal_run": round(float(np.mean(run_len)), 2),
                          "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(achieved / HBM_PEAK_GBS, 4),
                          "traffic": _traffic("f32", os.path.join(HERE, "profiles",

@@ -155,17 +155,26 @@ int jscc_materialize_rows(const float* packed, int64_t ld_packed, int64_t n_rows
  *   normalize == 0: get_weight_compton_sparse of recon_osem_plane_sparse.py:22-57
  *     (unnormalised rows, clamped ratio).
  * w[j] (+)= out_scale * nan_to_num(sum_i t_i[src(j)] / max(t_i . x_rot, eps)),
- * x_rot / src as in jscc_proj.  Fine pixels n_zf*n_lf <= 32768 (a multiple
+ * x_rot / src as in jscc_proj.  rows (optional): int32 order in which the
+ * n_rows packed rows are taken (row i of the call is packed row rows[i]).
+ * za_idx/za_w [n_zc][n_adj] (optional): for each coarse z slice the fine z
+ * slices with a tap on it and their weights (the transpose of z_expand,
+ * padded with weight 0).  With an identity layer map, n_adj = 4 and at most
+ * half as many coarse as fine pixels (up to 25600), the rows of one first-hit
+ * crystal are weighted in grouped form (the adjoints E^T(x*A[c]) and E^T A[c]
+ * once per run of equal crystal numbers in the row order, as the reference's
+ * grouped variant does); rows whose products could be non-finite are
+ * weighted product by product.  Fine pixels n_zf*n_lf <= 32768 (a multiple
  * of 4), coarse pixels <= 16384.  Error word bits: 2 x_perm, 4 inv_perm,
  * 64 tap table index out of range. */
 int64_t jscc_coarse_weight_workspace_bytes(int64_t n_rows, int64_t n_fine);
-int jscc_coarse_weight(const float* packed, int64_t ld_packed, int64_t n_rows, const int32_t* z_idx,
-                       const float* z_w, int n_zf, int n_zc, const int32_t* l_idx, const float* l_w, int n_lf,
-                       int n_lc, const float* sysmat, int64_t lda, int64_t n_arows, const float* x,
-                       const int64_t* x_perm, int64_t x_perm_stride, int64_t x_len, float eps, int use_eps,
-                       int normalize, const int64_t* inv_perm, int64_t inv_perm_stride, float out_scale,
-                       int accumulate, float* w, int64_t w_len, void* workspace, int64_t workspace_bytes,
-                       int32_t* err, void* stream);
+int jscc_coarse_weight(const float* packed, int64_t ld_packed, const int32_t* rows, int64_t n_rows,
+                       const int32_t* z_idx, const float* z_w, int n_zf, int n_zc, const int32_t* za_idx,
+                       const float* za_w, int n_adj, const int32_t* l_idx, const float* l_w, int n_lf, int n_lc,
+                       const float* sysmat, int64_t lda, int64_t n_arows, const float* x, const int64_t* x_perm,
+                       int64_t x_perm_stride, int64_t x_len, float eps, int use_eps, int normalize,
+                       const int64_t* inv_perm, int64_t inv_perm_stride, float out_scale, int accumulate, float* w,
+                       int64_t w_len, void* workspace, int64_t workspace_bytes, int32_t* err, void* stream);
 
 /* ---- OSEM projections --------------------------------------------------- */
 

@@ -75,9 +75,9 @@ SIGNATURES = {
                                        vp, c_i64, c_f32, c_int, c_int, vp, c_i64, c_int, c_f32, vp, c_i64, c_int, vp,
                                        c_i64, vp, vp]),
     "jscc_coarse_weight_workspace_bytes": (c_i64, [c_i64, c_i64]),
-    "jscc_coarse_weight": (c_int, [vp, c_i64, c_i64, vp, vp, c_int, c_int, vp, vp, c_int, c_int, vp, c_i64, c_i64, vp,
-                                   vp, c_i64, c_i64, c_f32, c_int, c_int, vp, c_i64, c_f32, c_int, vp, c_i64, vp,
-                                   c_i64, vp, vp]),
+    "jscc_coarse_weight": (c_int, [vp, c_i64, vp, c_i64, vp, vp, c_int, c_int, vp, vp, c_int, vp, vp, c_int, c_int,
+                                    vp, c_i64, c_i64, vp, vp, c_i64, c_i64, c_f32, c_int, c_int, vp, c_i64, c_f32, c_int,
+                                    vp, c_i64, vp, c_i64, vp, vp]),
     "jscc_prof_enable": (c_int, [c_int]),
     "jscc_prof_read": (c_int, [c_int, vp, vp]),
 }

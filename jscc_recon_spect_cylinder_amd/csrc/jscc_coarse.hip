@@ -56,7 +56,8 @@ __global__ void __launch_bounds__(kCoarseThreads)
                     const int32_t* __restrict__ l_idx, const float* __restrict__ l_w, int Lf, int Lc,
                     const float* __restrict__ A, int64_t lda, int64_t M, const float* __restrict__ x,
                     const int64_t* __restrict__ x_perm, int64_t xps, int64_t x_len, float eps, int use_eps,
-                    float* __restrict__ partial, int32_t* __restrict__ err) {
+                    float* __restrict__ partial, int32_t* __restrict__ err, const int32_t* __restrict__ row_list,
+                    const int32_t* __restrict__ row_count, int add_partial) {
   extern __shared__ float lds[];
   const int tid = threadIdx.x;
   const int P = Zf * Lf;
@@ -83,7 +84,8 @@ __global__ void __launch_bounds__(kCoarseThreads)
   }
   if (bad_tab) atomicOr(err, 64);
 
-  // ---- this work-group's rows
+  // ---- this work-group's rows (of the row list and its device count, when given)
+  if (row_count) n_rows = *row_count;
   const int G = gridDim.x, g = blockIdx.x;
   const int64_t per = n_rows / G, rem = n_rows % G;
   const int64_t rb = g * per + (g < rem ? g : rem);
@@ -140,8 +142,9 @@ __global__ void __launch_bounds__(kCoarseThreads)
   // (runtime loops: unrolled, their per-slot offsets would be hoisted into
   // registers held across the row loop)
   const int slots = (Pc + kCoarseThreads - 1) / kCoarseThreads;
+  auto row_id = [&](int64_t r) -> int64_t { return row_list ? (int64_t)row_list[r] : r; };
   auto fetch = [&](int64_t r, int b) {
-    const float* src = packed + r * ldp + 1;
+    const float* src = packed + row_id(r) * ldp + 1;
     float* dst = rowbuf + b * pc_pad;
 #pragma unroll 1
     for (int m = 0; m < slots; ++m) {
@@ -174,7 +177,7 @@ __global__ void __launch_bounds__(kCoarseThreads)
     // buffer loads over the row's P values (offsets past P read zero).  The
     // offset base passes through an empty asm each row so that the K offsets
     // are formed at use instead of being held in registers across rows.
-    const float cpf = rintf(packed[r * ldp]) - 1.0f;
+    const float cpf = rintf(packed[row_id(r) * ldp]) - 1.0f;
     int64_t cp = 0;
     if (cpf >= (float)(M - 1)) cp = M - 1;
     else if (cpf > 0.0f) cp = (int64_t)cpf;
@@ -332,11 +335,425 @@ __global__ void __launch_bounds__(kCoarseThreads)
   // with the prologue's and spilled across the row loop
   int jb = tid;
   asm volatile("" : "+v"(jb));
+  if (add_partial) {
+    // rows handed over by the grouped kernel: added to its partial of the same
+    // work-group index (a group without rows leaves that partial as it is)
+    if (rb < re) {
 #pragma unroll
-  for (int k = 0; k < K; ++k) {
-    const int j = jb + k * kCoarseThreads;
-    if (j < P) out[j] = wr[k];
+      for (int k = 0; k < K; ++k) {
+        const int j = jb + k * kCoarseThreads;
+        if (j < P) out[j] += wr[k];
+      }
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int j = jb + k * kCoarseThreads;
+      if (j < P) out[j] = wr[k];
+    }
   }
+}
+
+// ============================================================================
+// Grouped form (identity layer map).  The rows of one first-hit crystal c
+// share A[c]; with the expand E = z_expand (x) I,
+//   t_i . x = t_c,i . u_c,   u_c = E^T (A[c] * x)      (coarse, once per group)
+//   S_i     = t_c,i . v_c,   v_c = E^T A[c]
+//   w      += A[c] * E (sum over the group of coef_i t_c,i)
+// -- the algebra of get_weight_compton_sparse (recon_osem_plane_sparse.py:
+// 39-51) for both variants.  A[c] is read once per group instead of once per
+// row, and the row loop streams coarse rows with two dot products and one
+// update per value.  Runs of equal crystal numbers in the launch's row order
+// form the groups (rows ordered by crystal give the longest runs).
+//
+// The materialised variant's element-wise nan_to_num of t = E(t_c) * A[c]
+// matters only where a product is not finite: a row is taken here when its
+// coarse values are finite (a non-finite value poisons the reference's whole
+// fine row, which is then dropped), A[c] is finite and no product or partial
+// sum can overflow (max|t_c| max|A[c]| max(1, max|x|) 4 P < FLT_MAX); every
+// other row is appended to a device list and weighted exactly, one product at
+// a time, by k_coarse_weight, which adds into the same partial sums.
+//
+// Layout: one 1024-thread work-group per CU over a contiguous share of rows;
+// thread t owns fine pixels f = t + 1024 k (k < KF) and coarse pixels
+// q = t + 1024 k (k < KC): coarse row values, u, v and the group's coarse
+// weight in registers, its fine weight in registers.  One LDS buffer of P
+// floats carries A*x and then A for the two adjoints of a group, and the
+// group's coarse weight for its expansion.
+// ============================================================================
+constexpr int kGrpThreads = 1024;
+constexpr int kGrpWaves = kGrpThreads / JSCC_WAVE;
+constexpr float kFltMax = 3.402823466e38f;
+
+struct GrpZTap {  // fine z slice: word offsets of its two coarse taps from the fine index, and weights
+  int d0, d1;
+  float w0, w1;
+};
+struct GrpZAdj {  // coarse z slice: word offset of one contributing fine slice from the coarse index, weight
+  int d;
+  float w;
+};
+
+// a work-group-uniform value in a scalar register
+__device__ __forceinline__ float uniform_f(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
+__device__ __forceinline__ float abs_or_inf(float v) {
+  const float a = fabsf(v);
+  return a <= kFltMax ? a : __builtin_inff();  // NaN -> inf
+}
+
+// wave maximum of non-negative values, uniform across the wave
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_max_nonneg(float v) {
+  const int moved = __builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xf, false);
+  return fmaxf(v, __int_as_float(moved));
+}
+__device__ __forceinline__ float wave_max_nonneg(float v) {
+  v = dpp_max_nonneg<0xB1, 0xf>(v);
+  v = dpp_max_nonneg<0x4E, 0xf>(v);
+  v = dpp_max_nonneg<0x141, 0xf>(v);
+  v = dpp_max_nonneg<0x140, 0xf>(v);
+  v = dpp_max_nonneg<0x142, 0xa>(v);
+  v = dpp_max_nonneg<0x143, 0xc>(v);
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
+__global__ void __launch_bounds__(256) k_grp_prepare(const float* __restrict__ x, const int64_t* __restrict__ x_perm,
+                                                     int64_t xps, int64_t x_len, int P, float* __restrict__ x_rot,
+                                                     int32_t* __restrict__ count, int32_t* __restrict__ err) {
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j == 0) *count = 0;
+  if (x_perm == nullptr || j >= P) return;
+  const int64_t pv = x_perm[(int64_t)j * xps];
+  int64_t src = pv - 1;
+  if (pv < 1 || pv > x_len) {
+    atomicOr(err, 2);
+    src = 0;
+  }
+  x_rot[j] = x[src];
+}
+
+// word offset of the reduction scratch behind the buffer and the tap tables (a multiple of 4 words)
+__host__ __device__ __forceinline__ int grouped_red_offset(int P, int Zf, int Zc, int na) {
+  const int tab_words = ((P + 3) & ~3) + Zf * (int)(sizeof(GrpZTap) / 4) + Zc * na * (int)(sizeof(GrpZAdj) / 4);
+  return (tab_words + 3) & ~3;
+}
+
+// kGrpAdj contributions per coarse slice: linear z expansion at z stride 2 (the reference's default) gives
+// at most 4 fine slices per coarse slice; other strides take the per-product kernel
+constexpr int kGrpAdj = 4;
+
+template <int KF, bool NORM>
+__global__ void __launch_bounds__(kGrpThreads)
+    k_coarse_grouped(const float* __restrict__ packed, int64_t ldp, const int32_t* __restrict__ rows,
+                     int64_t n_rows, int L, int Zf, int Zc, const int32_t* __restrict__ z_idx,
+                     const float* __restrict__ z_w, const int32_t* __restrict__ za_idx,
+                     const float* __restrict__ za_w, const float* __restrict__ A, int64_t lda, int64_t M,
+                     const float* __restrict__ x, float eps, int use_eps, float* __restrict__ partial,
+                     int32_t* __restrict__ cx_list, int32_t* __restrict__ cx_count, int32_t* __restrict__ err) {
+  constexpr int KC = (KF + 1) / 2;
+  constexpr int na = kGrpAdj;
+  extern __shared__ float lds[];
+  const int tid = threadIdx.x, lane = jscc_lane(), wv = jscc_wave_id();
+  const int P = Zf * L, Pc = Zc * L;
+  float* F = lds;                                                       // [P]
+  GrpZTap* zt = reinterpret_cast<GrpZTap*>(lds + ((P + 3) & ~3));       // [Zf]
+  GrpZAdj* za = reinterpret_cast<GrpZAdj*>(zt + Zf);                    // [Zc][na]
+  float* red = lds + grouped_red_offset(P, Zf, Zc, na);                 // [2][3][kGrpWaves], 16-byte aligned
+
+  // ---- tap tables, as word offsets from the fine (or coarse) index: the fine
+  // pixel (zf, l) reads coarse (zc_a, l) = f + (zc_a - zf) L, the coarse pixel
+  // (zc, l) reads fine (zf_j, l) = q + (zf_j - zc) L
+  int bad_tab = 0;
+  for (int f = tid; f < Zf; f += kGrpThreads) {
+    int a0 = z_idx[2 * f], a1 = z_idx[2 * f + 1];
+    if (a0 < 0 || a0 >= Zc || a1 < 0 || a1 >= Zc) { bad_tab = 1; a0 = a1 = f < Zc ? f : 0; }
+    zt[f] = GrpZTap{(a0 - f) * L, (a1 - f) * L, z_w[2 * f], z_w[2 * f + 1]};
+  }
+  for (int e = tid; e < Zc * na; e += kGrpThreads) {
+    const int zc = e / na;
+    int fj = za_idx[e];
+    float wj = za_w[e];
+    if (fj < 0 || fj >= Zf) { bad_tab = 1; fj = zc; wj = 0.0f; }
+    za[e] = GrpZAdj{(fj - zc) * L, wj};
+  }
+  if (bad_tab) atomicOr(err, 64);
+
+  // ---- rows of this work-group
+  const int G = gridDim.x, g = blockIdx.x;
+  const int64_t per = n_rows / G, rem = n_rows % G;
+  const int64_t rb = g * per + (g < rem ? g : rem);
+  const int64_t re = rb + per + (g < rem ? 1 : 0);
+
+  // pixel slots: slot k of this thread is (z_k, l_k) with z_0 = tid / L, stepped by 1024 = zstep L + lstep
+  const int z0 = tid / L, l0 = tid - z0 * L;
+  const int zstep = kGrpThreads / L, lstep = kGrpThreads - zstep * L;
+
+  // Buffer loads over one row each (the range check covers the vector offset, which holds the whole
+  // offset: slots past the row's end read 0); the slot offsets and coordinates are formed at each use from
+  // a base passed through an empty asm: held as loop invariants they would take 3 KF registers across
+  // the row loop.
+  const __amdgpu_buffer_rsrc_t xrs = __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(x), (short)0, P * 4,
+                                                                       kBufWord3);
+  auto ld = [&](__amdgpu_buffer_rsrc_t rs, int vbase, int k) -> float {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, vbase + k * kGrpThreads * 4, 0, 0));
+  };
+  auto pinned_base = [&]() -> int {
+    int v = tid * 4;
+    asm volatile("" : "+v"(v));
+    return v;
+  };
+
+  // ---- the guard on products: max|x| over the image (every pixel is some thread's slot)
+  float xm = 0.0f;
+  {
+    const int vb = pinned_base();
+#pragma unroll
+    for (int k = 0; k < KF; ++k) xm = fmaxf(xm, abs_or_inf(ld(xrs, vb, k)));
+  }
+  xm = wave_max_nonneg(xm);
+  if (lane == 0) red[wv] = xm;
+  __syncthreads();  // also publishes the tap tables
+  float xmax = 0.0f;
+#pragma unroll
+  for (int q = 0; q < kGrpWaves; ++q) xmax = fmaxf(xmax, red[q]);
+  // rows pass the guard when max|t_c| * max|A[c]| < gthr
+  const float gthr = uniform_f(kFltMax / (4.0f * (float)P * fmaxf(1.0f, uniform_f(xmax))));
+  int rbuf = 1;  // reduction scratch in use (the prologue used the first half of buffer 0)
+
+  float w[KF], wc[KC], u[KC], v[KC];
+#pragma unroll
+  for (int k = 0; k < KF; ++k) w[k] = 0.0f;
+#pragma unroll
+  for (int k = 0; k < KC; ++k) { wc[k] = 0.0f; u[k] = 0.0f; v[k] = 0.0f; }
+
+  // coarse row values by buffer loads over the row's Pc values (slots past the end read 0)
+  auto row_of = [&](int64_t r) -> int64_t { return rows ? (int64_t)rows[r] : r; };
+  auto load_row = [&](int64_t r, float (&tc)[KC]) {
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<float*>(packed + row_of(r) * ldp + 1), (short)0, Pc * 4, kBufWord3);
+    const int vb = pinned_base();
+#pragma unroll
+    for (int k = 0; k < KC; ++k) tc[k] = ld(rs, vb, k);
+  };
+  auto crystal_of = [&](int64_t r) -> int64_t {
+    const float cpf = rintf(packed[row_of(r) * ldp]) - 1.0f;  // clamp(round(cpnum1) - 1, 0, M - 1)
+    if (cpf >= (float)(M - 1)) return M - 1;
+    if (cpf > 0.0f) return (int64_t)cpf;
+    return 0;
+  };
+  // out[k] = sum_j za[z_k][j].w * F[q_k + za[z_k][j].d] over the coarse slots
+  auto adjoint = [&](float (&out)[KC]) {
+    int z = z0, l = l0, qb = tid;
+    asm volatile("" : "+v"(z), "+v"(l), "+v"(qb));
+#pragma unroll
+    for (int k = 0; k < KC; ++k) {
+      const int q = qb + k * kGrpThreads;
+      float acc = 0.0f;
+      if (q < Pc) {
+        const GrpZAdj* t = za + z * na;
+#pragma unroll
+        for (int j = 0; j < na; ++j) acc = fmaf(t[j].w, F[q + t[j].d], acc);
+      }
+      out[k] = acc;
+      z += zstep;
+      l += lstep;
+      if (l >= L) { l -= L; z += 1; }
+    }
+  };
+
+  int64_t cur = -1;   // crystal of the open group
+  float amax = 0.0f;  // max|A[cur]|, inf when A[cur] holds a non-finite value
+
+  // u, v of crystal c through the LDS buffer (A * x, then A; A is read twice, the second time from L2)
+  auto form = [&](int64_t c) {
+    const __amdgpu_buffer_rsrc_t ars =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(A + c * lda), (short)0, P * 4, kBufWord3);
+    __syncthreads();  // no thread reads the buffer any more (the previous group's expansion)
+    float am = 0.0f;
+    int fb = tid;
+    asm volatile("" : "+v"(fb));
+    int vb = pinned_base();
+#pragma unroll
+    for (int k = 0; k < KF; ++k) {
+      const float a = ld(ars, vb, k);
+      am = fmaxf(am, abs_or_inf(a));
+      const int f = fb + k * kGrpThreads;
+      if (f < P) F[f] = a * ld(xrs, vb, k);
+    }
+    am = wave_max_nonneg(am);
+    float* rd = red + rbuf * 3 * kGrpWaves;
+    if (lane == 0) rd[wv] = am;
+    __syncthreads();
+    adjoint(u);
+    amax = 0.0f;
+#pragma unroll
+    for (int q = 0; q < kGrpWaves; ++q) amax = fmaxf(amax, rd[q]);
+    amax = uniform_f(amax);
+    rbuf ^= 1;
+    if (NORM) {  // the row sums (the grouped variant has no normalisation)
+      __syncthreads();
+      fb = tid;
+      asm volatile("" : "+v"(fb));
+      vb = pinned_base();
+#pragma unroll
+      for (int k = 0; k < KF; ++k) {
+        const float a = ld(ars, vb, k);
+        const int f = fb + k * kGrpThreads;
+        if (f < P) F[f] = a;
+      }
+      __syncthreads();
+      adjoint(v);
+    }
+  };
+
+  // w += A[c] * E(wc) and wc = 0.  A group whose A row is not finite handed all its rows over (its wc is
+  // zero) and adds nothing (A * 0 would be NaN where A is infinite).
+  auto flush = [&](int64_t c) {
+    if (NORM && amax == __builtin_inff()) return;
+    // (every thread finished the group's reads of the buffer before the barrier of its first row)
+    int qb = tid;
+    asm volatile("" : "+v"(qb));
+#pragma unroll
+    for (int k = 0; k < KC; ++k) {
+      const int q = qb + k * kGrpThreads;
+      if (q < Pc) F[q] = wc[k];
+      wc[k] = 0.0f;
+    }
+    __syncthreads();
+    const __amdgpu_buffer_rsrc_t ars =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(A + c * lda), (short)0, P * 4, kBufWord3);
+    int z = z0, l = l0, fb = tid;
+    asm volatile("" : "+v"(z), "+v"(l), "+v"(fb));
+    const int vb = pinned_base();
+#pragma unroll
+    for (int k = 0; k < KF; ++k) {
+      const int f = fb + k * kGrpThreads;
+      const float a = ld(ars, vb, k);
+      if (f < P) {
+        const GrpZTap t = zt[z];
+        float e = t.w0 * F[f + t.d0];
+        e = fmaf(t.w1, F[f + t.d1], e);
+        w[k] = fmaf(a, e, w[k]);
+      }
+      z += zstep;
+      l += lstep;
+      if (l >= L) { l -= L; z += 1; }
+    }
+  };
+
+  // one row: dot products, work-group sums (fixed order), coefficient, coarse weight update
+  auto consume = [&](int64_t r, const float (&tc)[KC]) {
+    const int64_t c = crystal_of(r);
+    if (c != cur) {  // block-uniform: every thread reads the same row id
+      if (cur >= 0) flush(cur);
+      form(c);
+      cur = c;
+    }
+    float D = 0.0f, S = 0.0f, m = 0.0f;
+#pragma unroll
+    for (int k = 0; k < KC; ++k) {
+      D = fmaf(tc[k], u[k], D);
+      if (NORM) S = fmaf(tc[k], v[k], S);
+      m = fmaxf(m, abs_or_inf(tc[k]));
+    }
+    D = jscc_wave_sum_dpp(D);
+    if (NORM) S = jscc_wave_sum_dpp(S);
+    m = wave_max_nonneg(m);
+    float* rd = red + rbuf * 3 * kGrpWaves;  // double-buffered: the next row's sums may land before slow
+    if (lane == 0) {                           // waves have read these
+      rd[wv] = D;
+      rd[kGrpWaves + wv] = S;
+      rd[2 * kGrpWaves + wv] = m;
+    }
+    __syncthreads();
+    // the wave sums in a fixed order, one 16-byte group of each at a time (read all at once, they
+    // would add 48 registers on top of the row buffers)
+    float Dt = 0.0f, St = 0.0f, mt = 0.0f;
+    const jscc_f4v* rd4 = reinterpret_cast<const jscc_f4v*>(rd);
+#pragma unroll
+    for (int q = 0; q < kGrpWaves / 4; ++q) {
+      const jscc_f4v a = rd4[q], b = rd4[kGrpWaves / 4 + q], e = rd4[kGrpWaves / 2 + q];
+      Dt += a.x; Dt += a.y; Dt += a.z; Dt += a.w;
+      St += b.x; St += b.y; St += b.z; St += b.w;
+      mt = fmaxf(fmaxf(mt, fmaxf(e.x, e.y)), fmaxf(e.z, e.w));
+      asm volatile("" : "+v"(Dt), "+v"(St), "+v"(mt));
+    }
+    Dt = uniform_f(Dt);
+    St = uniform_f(St);
+    mt = uniform_f(mt);
+    rbuf ^= 1;
+    float coef = 0.0f;
+    bool apply = true;  // (a dropped or handed-over row adds nothing: 0 * inf would be NaN)
+    if (NORM) {
+      if (mt == __builtin_inff()) {
+        apply = false;  // a non-finite coarse value: the reference's row is all NaN, then zero, then dropped
+      } else if (!(mt * amax < gthr)) {
+        apply = false;  // weighted exactly by the per-product kernel
+        if (tid == 0) cx_list[atomicAdd(cx_count, 1)] = (int32_t)row_of(r);
+      } else if (!(St > 0.0f)) {
+        apply = false;  // zero-sum row: dropped
+      } else {
+        float den = Dt / St;
+        if (use_eps && den < eps) den = eps;
+        coef = (1.0f / den) / St;
+      }
+    } else if (mt == __builtin_inff()) {
+      coef = __builtin_nanf("");  // a non-finite coarse value turns the block weight into NaN
+    } else {
+      float den = Dt;
+      if (use_eps && den < eps) den = eps;
+      coef = 1.0f / den;
+    }
+    if (apply) {
+#pragma unroll
+      for (int k = 0; k < KC; ++k) wc[k] = fmaf(coef, tc[k], wc[k]);
+    }
+  };
+
+  // ---- the next row is in flight while one is consumed; one call site of the row step (two would
+  // duplicate the group code), the prefetched row moved into place by register copies after the step
+  float ta[KC], tb[KC];
+  if (rb < re) load_row(rb, ta);
+#pragma unroll 1
+  for (int64_t r = rb; r < re; ++r) {
+    if (r + 1 < re) load_row(r + 1, tb);
+    consume(r, ta);
+#pragma unroll
+    for (int k = 0; k < KC; ++k) ta[k] = tb[k];
+  }
+  if (cur >= 0) flush(cur);
+
+  float* out = partial + (int64_t)g * P;
+  int jb = tid;
+  asm volatile("" : "+v"(jb));
+#pragma unroll
+  for (int k = 0; k < KF; ++k) {
+    const int f = jb + k * kGrpThreads;
+    if (f < P) out[f] = w[k];
+  }
+}
+
+size_t grouped_lds(int P, int Zf, int Zc, int na) {
+  return (size_t)grouped_red_offset(P, Zf, Zc, na) * 4 + 2 * 3 * kGrpWaves * sizeof(float);
+}
+
+template <int KF, bool NORM>
+int launch_grouped(size_t lds, int G, hipStream_t s, const float* packed, int64_t ldp, const int32_t* rows,
+                   int64_t n_rows, int L, int Zf, int Zc, const int32_t* z_idx, const float* z_w,
+                   const int32_t* za_idx, const float* za_w, const float* A, int64_t lda, int64_t M, const float* x,
+                   float eps, int use_eps, float* partial, int32_t* cx_list, int32_t* cx_count, int32_t* err) {
+  static bool attr_set = false;
+  if (!attr_set) {
+    JSCC_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_coarse_grouped<KF, NORM>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    attr_set = true;
+  }
+  hipLaunchKernelGGL((k_coarse_grouped<KF, NORM>), dim3(G), dim3(kGrpThreads), lds, s, packed, ldp, rows, n_rows, L,
+                     Zf, Zc, z_idx, z_w, za_idx, za_w, A, lda, M, x, eps, use_eps, partial, cx_list, cx_count, err);
+  return 0;
 }
 
 int coarse_grid(int64_t n_rows) {
@@ -355,7 +772,8 @@ template <int K, bool LID, bool ZU, bool NORM>
 int launch_coarse(size_t lds, int G, hipStream_t s, const float* packed, int64_t ldp, int64_t n_rows, int Pc,
                   const int32_t* z_idx, const float* z_w, int Zf, int Zc, const int32_t* l_idx, const float* l_w,
                   int Lf, int Lc, const float* A, int64_t lda, int64_t M, const float* x, const int64_t* x_perm,
-                  int64_t xps, int64_t x_len, float eps, int use_eps, float* partial, int32_t* err) {
+                  int64_t xps, int64_t x_len, float eps, int use_eps, float* partial, int32_t* err,
+                  const int32_t* row_list, const int32_t* row_count, int add_partial) {
   static bool attr_set = false;
   if (!attr_set) {
     JSCC_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_coarse_weight<K, LID, ZU, NORM>),
@@ -364,7 +782,7 @@ int launch_coarse(size_t lds, int G, hipStream_t s, const float* packed, int64_t
   }
   hipLaunchKernelGGL((k_coarse_weight<K, LID, ZU, NORM>), dim3(G), dim3(kCoarseThreads), lds, s, packed, ldp, n_rows,
                      Pc, z_idx, z_w, Zf, Zc, l_idx, l_w, Lf, Lc, A, lda, M, x, x_perm, xps, x_len, eps, use_eps,
-                     partial, err);
+                     partial, err, row_list, row_count, add_partial);
   return 0;
 }
 
@@ -374,10 +792,11 @@ int dispatch_coarse(bool lid, bool zu, bool norm, size_t lds, int G, hipStream_t
                     int64_t n_rows, int Pc, const int32_t* z_idx, const float* z_w, int Zf, int Zc,
                     const int32_t* l_idx, const float* l_w, int Lf, int Lc, const float* A, int64_t lda, int64_t M,
                     const float* x, const int64_t* x_perm, int64_t xps, int64_t x_len, float eps, int use_eps,
-                    float* partial, int32_t* err) {
+                    float* partial, int32_t* err, const int32_t* row_list, const int32_t* row_count,
+                    int add_partial) {
 #define JSCC_COARSE_ARGS                                                                                          \
   lds, G, s, packed, ldp, n_rows, Pc, z_idx, z_w, Zf, Zc, l_idx, l_w, Lf, Lc, A, lda, M, x, x_perm, xps, x_len, eps, \
-      use_eps, partial, err
+      use_eps, partial, err, row_list, row_count, add_partial
   const int code = (lid ? 4 : 0) | (zu ? 2 : 0) | (norm ? 1 : 0);
   switch (code) {
     case 0: return launch_coarse<K, false, false, false>(JSCC_COARSE_ARGS);
@@ -394,13 +813,18 @@ int dispatch_coarse(bool lid, bool zu, bool norm, size_t lds, int G, hipStream_t
 
 }  // namespace
 
+// workspace: partial rows | rotated image [P] | count of handed-over rows | their row ids [n_rows]
+static int64_t align256(int64_t b) { return (b + 255) / 256 * 256; }
+
 extern "C" int64_t jscc_coarse_weight_workspace_bytes(int64_t n_rows, int64_t n_fine) {
-  return jscc_partials_bytes(coarse_grid(n_rows), n_fine);
+  return align256(jscc_partials_bytes(coarse_grid(n_rows), n_fine)) + align256(n_fine * 4) + 256 +
+         align256((n_rows > 0 ? n_rows : 1) * 4);
 }
 
-extern "C" int jscc_coarse_weight(const float* packed, int64_t ld_packed, int64_t n_rows, const int32_t* z_idx,
-                                  const float* z_w, int n_zf, int n_zc, const int32_t* l_idx, const float* l_w,
-                                  int n_lf, int n_lc, const float* A, int64_t lda, int64_t n_arows, const float* x,
+extern "C" int jscc_coarse_weight(const float* packed, int64_t ld_packed, const int32_t* rows, int64_t n_rows,
+                                  const int32_t* z_idx, const float* z_w, int n_zf, int n_zc, const int32_t* za_idx,
+                                  const float* za_w, int n_adj, const int32_t* l_idx, const float* l_w, int n_lf,
+                                  int n_lc, const float* A, int64_t lda, int64_t n_arows, const float* x,
                                   const int64_t* x_perm, int64_t x_perm_stride, int64_t x_len, float eps, int use_eps,
                                   int normalize, const int64_t* inv_perm, int64_t inv_perm_stride, float out_scale,
                                   int accumulate, float* w, int64_t w_len, void* workspace, int64_t workspace_bytes,
@@ -415,6 +839,8 @@ extern "C" int jscc_coarse_weight(const float* packed, int64_t ld_packed, int64_
   JSCC_REQUIRE(l_idx != nullptr || n_lf == n_lc, "jscc_coarse_weight: an identity layer map needs n_lf == n_lc");
   JSCC_REQUIRE(z_idx && z_w && (l_idx == nullptr || l_w != nullptr) && A && x && w && err,
                "jscc_coarse_weight: null pointer");
+  JSCC_REQUIRE((za_idx == nullptr) == (za_w == nullptr) && (za_idx == nullptr || n_adj >= 1),
+               "jscc_coarse_weight: incomplete z adjoint table");
   JSCC_REQUIRE(x_perm != nullptr || x_len == P, "jscc_coarse_weight: image length mismatch");
   JSCC_REQUIRE(inv_perm != nullptr || w_len == P, "jscc_coarse_weight: weight length mismatch");
   hipStream_t s = (hipStream_t)stream;
@@ -424,30 +850,71 @@ extern "C" int jscc_coarse_weight(const float* packed, int64_t ld_packed, int64_
   }
   JSCC_REQUIRE(packed != nullptr, "jscc_coarse_weight: null rows");
   const int G = coarse_grid(n_rows);
-  JSCC_REQUIRE(workspace != nullptr && workspace_bytes >= jscc_partials_bytes(G, P),
+  JSCC_REQUIRE(workspace != nullptr && workspace_bytes >= jscc_coarse_weight_workspace_bytes(n_rows, P),
                "jscc_coarse_weight: workspace too small");
+  float* partial = (float*)workspace;
+  char* tail = (char*)workspace + align256(jscc_partials_bytes(G, P));
+  float* x_rot = (float*)tail;
+  int32_t* cx_count = (int32_t*)(tail + align256(P * 4));
+  int32_t* cx_list = (int32_t*)(tail + align256(P * 4) + 256);
   const bool lid = l_idx == nullptr;
   const size_t lds = coarse_lds((int)Pc, n_zf, n_lf, lid);
   JSCC_REQUIRE(lds <= 160 * 1024, "jscc_coarse_weight: the coarse row buffers and tap tables need %zu bytes of LDS",
                lds);
   const int kk = (int)((P + kCoarseThreads - 1) / kCoarseThreads);
+  const bool zu = n_lf % JSCC_WAVE == 0;
+  // the grouped form: identity layer map, an adjoint table, at most half as many coarse as fine pixels
+  // (the adjoint table has kGrpAdj entries per coarse slice, padded with zero weights; up to 25 fine slots)
+  const size_t glds = za_idx ? grouped_lds((int)P, n_zf, n_zc, kGrpAdj) : 0;
+  const bool grouped = lid && za_idx != nullptr && n_adj == kGrpAdj && 2 * Pc <= P && kk <= 25 &&
+                       glds <= 160 * 1024;
   int rc;
   jscc_prof_begin(JSCC_TAG_COARSE, s);
-  const bool zu = n_lf % JSCC_WAVE == 0;
-#define JSCC_COARSE(KV)                                                                                          \
+#define JSCC_COARSE(KV, XP, LIST, CNT, ADD)                                                                         \
   rc = dispatch_coarse<KV>(lid, zu, normalize != 0, lds, G, s, packed, ld_packed, n_rows, (int)Pc, z_idx, z_w, n_zf, \
-                           n_zc, l_idx, l_w, n_lf, n_lc, A, lda, n_arows, x, x_perm, x_perm_stride, x_len, eps,      \
-                           use_eps, (float*)workspace, err)
-  if (kk <= 4) JSCC_COARSE(4);
-  else if (kk <= 8) JSCC_COARSE(8);
-  else if (kk <= 16) JSCC_COARSE(16);
-  else if (kk <= 25) JSCC_COARSE(25);
-  else JSCC_COARSE(32);
+                           n_zc, l_idx, l_w, n_lf, n_lc, A, lda, n_arows, XP ? x : x_rot, XP ? x_perm : nullptr,     \
+                           x_perm_stride, XP ? x_len : P, eps, use_eps, partial, err, LIST, CNT, ADD)
+#define JSCC_COARSE_K(XP, LIST, CNT, ADD)        \
+  if (kk <= 4) JSCC_COARSE(4, XP, LIST, CNT, ADD);       \
+  else if (kk <= 8) JSCC_COARSE(8, XP, LIST, CNT, ADD);  \
+  else if (kk <= 16) JSCC_COARSE(16, XP, LIST, CNT, ADD); \
+  else if (kk <= 25) JSCC_COARSE(25, XP, LIST, CNT, ADD); \
+  else JSCC_COARSE(32, XP, LIST, CNT, ADD);
+  if (!grouped) {
+    JSCC_COARSE_K(true, rows, nullptr, 0)
+  } else {
+    // rotated image (when x_perm is given) and a cleared count of handed-over rows
+    hipLaunchKernelGGL(k_grp_prepare, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, s, x, x_perm, x_perm_stride,
+                       x_len, (int)P, x_rot, cx_count, err);
+    JSCC_CHECK_LAUNCH();
+    const float* xg = x_perm ? x_rot : x;
+#define JSCC_GRP(KV, NV)                                                                                           \
+  rc = launch_grouped<KV, NV>(glds, G, s, packed, ld_packed, rows, n_rows, n_lf, n_zf, n_zc, z_idx, z_w,             \
+                              za_idx, za_w, A, lda, n_arows, xg, eps, use_eps, partial, cx_list, cx_count, err)
+#define JSCC_GRP_N(KV) \
+  if (normalize) JSCC_GRP(KV, true); else JSCC_GRP(KV, false);
+    if (kk <= 4) { JSCC_GRP_N(4) }
+    else if (kk <= 8) { JSCC_GRP_N(8) }
+    else if (kk <= 16) { JSCC_GRP_N(16) }
+    else { JSCC_GRP_N(25) }
+#undef JSCC_GRP_N
+#undef JSCC_GRP
+    if (rc == 0 && normalize) {
+      JSCC_CHECK_LAUNCH();
+      // the handed-over rows, one product at a time, added into the same partials
+      if (x_perm) {
+        JSCC_COARSE_K(false, cx_list, cx_count, 1)
+      } else {
+        JSCC_COARSE_K(true, cx_list, cx_count, 1)
+      }
+    }
+  }
+#undef JSCC_COARSE_K
 #undef JSCC_COARSE
   jscc_prof_end(JSCC_TAG_COARSE, s);
   if (rc != 0) return rc;
   JSCC_CHECK_LAUNCH();
   const JsccEm none = {nullptr, nullptr, 0, 0.0f, 0, 1.0f, nullptr, 0.0f};
-  return jscc_reduce_partials((float*)workspace, G, (int)P, inv_perm, inv_perm_stride, out_scale, accumulate, 1, w,
-                              w_len, err, s, none);
+  return jscc_reduce_partials(partial, G, (int)P, inv_perm, inv_perm_stride, out_scale, accumulate, 1, w, w_len, err,
+                              s, none);
 }

@@ -396,6 +396,8 @@ class AxisTaps:
     l_w: torch.Tensor | None     # float32 [n_lf, 2]
     n_lf: int
     n_lc: int
+    za_idx: torch.Tensor | None = None  # int32 [n_zc, ZADJ]: fine z slices with a tap on each coarse slice
+    za_w: torch.Tensor | None = None    # float32 [n_zc, ZADJ]: their weights (padding: weight 0)
 
     @property
     def n_zf(self) -> int:
@@ -411,7 +413,8 @@ class AxisTaps:
 
     def to(self, device):
         mv = lambda t: None if t is None else t.to(device)  # noqa: E731
-        return AxisTaps(mv(self.z_idx), mv(self.z_w), self.n_zc, mv(self.l_idx), mv(self.l_w), self.n_lf, self.n_lc)
+        return AxisTaps(mv(self.z_idx), mv(self.z_w), self.n_zc, mv(self.l_idx), mv(self.l_w), self.n_lf, self.n_lc,
+                        mv(self.za_idx), mv(self.za_w))
 
 
 def _two_taps(m: np.ndarray, what: str):
@@ -431,17 +434,76 @@ def _two_taps(m: np.ndarray, what: str):
     return idx, w
 
 
+ZADJ = 4  # fine z slices per coarse slice in the grouped weight's adjoint table (linear expand, z stride 2)
+
+
+def _adjoint_taps(m: np.ndarray):
+    """(idx [n_coarse, ZADJ] int32, w [n_coarse, ZADJ] float32): the nonzeros of each column of a dense
+    interpolation matrix m [fine, coarse] (the fine indices with a tap on that coarse index), padded with
+    weight 0; None when a column has more than ZADJ nonzeros."""
+    n_c = m.shape[1]
+    idx = np.zeros((n_c, ZADJ), np.int32)
+    w = np.zeros((n_c, ZADJ), np.float32)
+    for c in range(n_c):
+        nz = np.flatnonzero(m[:, c])
+        if nz.size > ZADJ:
+            return None
+        idx[c, :nz.size] = nz
+        w[c, :nz.size] = m[nz, c]
+    return idx, w
+
+
 def axis_taps(z_expand, layer_expand) -> AxisTaps:
-    """Tap tables of the separable expand kron(z_expand, layer_expand) (dense [fine, coarse])."""
+    """Tap tables of the separable expand kron(z_expand, layer_expand) (dense [fine, coarse]), and for an
+    identity layer map the z adjoint table of the grouped weight."""
     ze = np.asarray(torch.as_tensor(z_expand).detach().cpu().float().numpy())
     le = np.asarray(torch.as_tensor(layer_expand).detach().cpu().float().numpy())
     zi, zw = _two_taps(ze, "z_expand")
     identity = le.shape[0] == le.shape[1] and np.array_equal(le, np.eye(le.shape[0], dtype=le.dtype))
     li = lw = None
+    za = None
     if not identity:
         a, b = _two_taps(le, "layer_expand")
         li, lw = torch.from_numpy(a), torch.from_numpy(b)
-    return AxisTaps(torch.from_numpy(zi), torch.from_numpy(zw), ze.shape[1], li, lw, le.shape[0], le.shape[1])
+    else:
+        za = _adjoint_taps(ze)
+    return AxisTaps(torch.from_numpy(zi), torch.from_numpy(zw), ze.shape[1], li, lw, le.shape[0], le.shape[1],
+                    None if za is None else torch.from_numpy(za[0]), None if za is None else torch.from_numpy(za[1]))
+
+
+_order_cache = SourceCache(64)
+
+
+def crystal_order(packed: torch.Tensor) -> torch.Tensor:
+    """int32 order of packed rows [cpnum1, t_c] by first-hit crystal (stable), cached per rows tensor and
+    version: the row order that gives jscc_coarse_weight's grouped form its longest runs."""
+    hit = _order_cache.get(packed)
+    if hit is not None:
+        return hit
+    order = torch.argsort(torch.round(packed[:, 0]), stable=True).to(torch.int32).contiguous()
+    return _order_cache.put(packed, (), order)
+
+
+# Rows per run of one first-hit crystal above which jscc_coarse_weight's grouped form beats its per-row form
+# (25000 rows of the 25600-voxel grid on MI355X: a run costs ~33 us of adjoints and expansion, a row ~2.3 us
+# less than in the per-row form; 2.6 rows per run: 1661 vs 618 us per launch, 25: 452 vs 546, 250: 327 vs 537)
+GROUPED_MIN_RUN = 16.0
+
+_runs_cache = SourceCache(64)
+
+
+def crystal_run_length(packed: torch.Tensor, rows: torch.Tensor | None = None) -> float:
+    """Mean number of consecutive rows (in the launch order: `rows`, or storage order) that share a first-hit
+    crystal, cached per rows tensor and version (one synchronisation when first computed)."""
+    key = (id(rows) if rows is not None else None,)
+    hit = _runs_cache.get(packed, key)
+    if hit is not None and (rows is None or hit[1] is rows):
+        return hit[0]
+    if packed.shape[0] == 0:
+        return 0.0
+    cp = torch.round(packed[:, 0] if rows is None else packed[rows.long(), 0])
+    runs = 1 + int((cp[1:] != cp[:-1]).sum().item())
+    return _runs_cache.put(packed, key, (packed.shape[0] / runs, rows))[0]
 
 
 _COARSE_MAX_FINE = 32 * 1024
@@ -460,9 +522,14 @@ def coarse_weight_fits(taps: AxisTaps) -> bool:
 
 def coarse_weight(packed: torch.Tensor, taps: AxisTaps, sysmat: torch.Tensor, x: torch.Tensor, w: torch.Tensor, *,
                   normalize: bool, eps: float | None = 1e-12, x_perm: torch.Tensor | None = None,
-                  inv_perm: torch.Tensor | None = None, out_scale: float = 1.0, accumulate: bool = True) -> None:
+                  inv_perm: torch.Tensor | None = None, out_scale: float = 1.0, accumulate: bool = True,
+                  rows: torch.Tensor | None = None, grouped: bool | str = "auto") -> None:
     """w (+)= out_scale * nan_to_num(T^T (1 / max(T x_rot, eps))) for packed coarse rows
-    [cpnum1, t_c] expanded on the fly (jscc_coarse_weight, include/jscc_hip.h)."""
+    [cpnum1, t_c] expanded on the fly (jscc_coarse_weight, include/jscc_hip.h).  rows: optional int32
+    order of the packed rows (rows ordered by first-hit crystal let the grouped form share each A row
+    over a run of rows; the weight is a sum over rows, so the order changes only rounding).
+    grouped: True for the grouped form where the grid allows it, False for the per-row form, "auto" for the
+    grouped form when runs of one crystal average GROUPED_MIN_RUN rows or more."""
     dev = packed.device
     if packed.ndim != 2 or packed.shape[1] != taps.n_coarse + 1:
         raise ValueError("packed rows must have shape [n, 1 + coarse_pixel_num]")
@@ -477,16 +544,23 @@ def coarse_weight(packed: torch.Tensor, taps: AxisTaps, sysmat: torch.Tensor, x:
         raise ValueError(f"RotMat column has {x_perm.shape[0]} entries for {taps.n_fine} pixels")
     if inv_perm is not None and inv_perm.shape[0] < w.shape[0]:
         raise ValueError("RotMatInv column shorter than the weight vector")
-    for t in (A, x, w, x_perm, inv_perm, taps.z_idx):
+    for t in (A, x, w, x_perm, inv_perm, taps.z_idx, taps.za_idx, rows):
         if t is not None and t.device != dev:
             raise ValueError("all operands must be on the rows' device")
+    if rows is not None and (rows.dtype != torch.int32 or rows.numel() != packed.shape[0] or
+                             not rows.is_contiguous()):
+        raise ValueError("rows must be a contiguous int32 order of the packed rows")
     if x.dtype != torch.float32 or not x.is_contiguous() or w.dtype != torch.float32 or not w.is_contiguous():
         raise ValueError("image and weight must be contiguous float32")
     n = packed.shape[0]
     ws = _workspace(dev, _lib.lib().jscc_coarse_weight_workspace_bytes(max(n, 1), taps.n_fine))
+    if grouped == "auto":
+        grouped = taps.za_idx is not None and n > 0 and crystal_run_length(packed, rows) >= GROUPED_MIN_RUN
+    za_i, za_w = (taps.za_idx, taps.za_w) if grouped else (None, None)
     rc = _lib.lib().jscc_coarse_weight(
-        packed.data_ptr() if n else None, packed.stride(0), n, taps.z_idx.data_ptr(), taps.z_w.data_ptr(),
-        taps.n_zf, taps.n_zc, _p(taps.l_idx), _p(taps.l_w), taps.n_lf, taps.n_lc, A.data_ptr(), A.stride(0),
+        packed.data_ptr() if n else None, packed.stride(0), _p(rows), n, taps.z_idx.data_ptr(), taps.z_w.data_ptr(),
+        taps.n_zf, taps.n_zc, _p(za_i), _p(za_w), ZADJ if za_i is not None else 0, _p(taps.l_idx), _p(taps.l_w),
+        taps.n_lf, taps.n_lc, A.data_ptr(), A.stride(0),
         A.shape[0], x.data_ptr(), _p(x_perm), x_perm.stride(0) if x_perm is not None else 0, x.shape[0],
         float(eps) if eps is not None else 0.0, 1 if eps is not None else 0, 1 if normalize else 0, _p(inv_perm),
         inv_perm.stride(0) if inv_perm is not None else 0, float(out_scale), 1 if accumulate else 0, w.data_ptr(),

@@ -39,9 +39,11 @@ def _materialized_weight(w, event_block, sysmat_full, x, projector, *, x_perm=No
     dev = x.device
     t = _tables(projector, dev)
     if t.axis is not None and ops.coarse_weight_fits(t.axis):
-        # materialise + normalise + ratio back-projection in one pass over the packed rows
-        ops.coarse_weight(event_block.to(dev), t.axis, sysmat_full, x, w, normalize=True, eps=1e-12, x_perm=x_perm,
-                          inv_perm=inv_perm, out_scale=scale, accumulate=True)
+        # materialise + normalise + ratio back-projection in one pass over the packed rows, taken in first-hit
+        # crystal order so that the rows of one crystal share its A row
+        blk = event_block.to(dev)
+        ops.coarse_weight(blk, t.axis, sysmat_full, x, w, normalize=True, eps=1e-12, x_perm=x_perm,
+                          inv_perm=inv_perm, out_scale=scale, accumulate=True, rows=ops.crystal_order(blk))
         return
     rows, flags = ops.materialize(event_block.to(dev), t.expand, sysmat_full, normalize=True)
     row_scale = flags.to(torch.float32)

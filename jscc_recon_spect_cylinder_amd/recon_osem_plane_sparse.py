@@ -33,9 +33,10 @@ def _sparse_block_weight(w, event_block, sysmat_full, x, sparse_projector, *, x_
     proj = sparse_projector if sparse_projector.coor_coarse.device == dev else sparse_projector.to(dev)
     axis = _tables(proj, dev).axis
     if axis is not None and ops.coarse_weight_fits(axis):
-        # one pass over the packed rows: expand, A-row factor and the ratio back-projection fused
+        # one pass over the packed rows: expand, A-row factor and the ratio back-projection fused; taken in
+        # first-hit crystal order, the rows of one crystal share its A row (the grouping of :39-51)
         ops.coarse_weight(rows_blk, axis, A, x, w, normalize=False, eps=1e-12, x_perm=x_perm, inv_perm=inv_perm,
-                          out_scale=scale, accumulate=True)
+                          out_scale=scale, accumulate=True, rows=ops.crystal_order(rows_blk))
         return
     t_fine, _ = materialize_sparse_event_rows_to_fine(rows_blk, A.float(), proj, normalize=False)
     core.add_weight(w, ops.proj_ready(t_fine, ops.DT_F32), x, eps=1e-12, x_perm=x_perm, inv_perm=inv_perm,

@@ -81,8 +81,12 @@ def make_rows(p, n, n_arows, seed, power=3):
     return np.hstack([cp[:, None], tc]), rng
 
 
+FORMS = pytest.mark.parametrize("form", [True, False], ids=["grouped_form", "per_row_form"])
+
+
+@FORMS
 @pytest.mark.parametrize("normalize", [True, False], ids=["materialised", "grouped"])
-def test_fused_weight_vs_oracle(grid, normalize):
+def test_fused_weight_vs_oracle(grid, normalize, form):
     from jscc_recon_spect_cylinder_amd import ops
     rot, inv, p, pj = grid
     ax = p.tables.axis
@@ -102,15 +106,17 @@ def test_fused_weight_vs_oracle(grid, normalize):
     ops.coarse_weight(torch.from_numpy(ev).to(DEV), ax.to(DEV), torch.from_numpy(A).to(DEV),
                       torch.from_numpy(x).to(DEV), w, normalize=normalize, eps=1e-12,
                       x_perm=torch.from_numpy(rot[:, r].copy()).to(DEV),
-                      inv_perm=torch.from_numpy(inv[:, r].copy()).to(DEV), out_scale=0.7, accumulate=True)
+                      inv_perm=torch.from_numpy(inv[:, r].copy()).to(DEV), out_scale=0.7, accumulate=True,
+                      grouped=form)
     ops.raise_on_index_error(torch.device(DEV))
     ref = oracle_weight(ev, A, osem.rot_gather(x, rot[:, r]), pj, normalize)
     expect = w0 + 0.7 * osem.rot_gather(ref, inv[:, r])
     assert rel(w.cpu().numpy() - w0, expect - w0) < 2e-6
 
 
+@FORMS
 @pytest.mark.parametrize("normalize", [True, False], ids=["materialised", "grouped"])
-def test_fused_weight_non_finite_rows(grid, normalize):
+def test_fused_weight_non_finite_rows(grid, normalize, form):
     """A non-finite coarse value spreads over its whole fine row in the reference's dense
     expansion: the row is dropped (materialised variant) or the block weight becomes
     NaN and is zeroed by the final nan_to_num (grouped variant)."""
@@ -125,7 +131,7 @@ def test_fused_weight_non_finite_rows(grid, normalize):
     x = (rng.random(p.fine_pixel_num) + 0.2).astype(np.float32)
     w = torch.zeros(p.fine_pixel_num, device=DEV)
     ops.coarse_weight(torch.from_numpy(ev).to(DEV), ax, torch.from_numpy(A).to(DEV), torch.from_numpy(x).to(DEV), w,
-                      normalize=normalize, eps=1e-12)
+                      normalize=normalize, eps=1e-12, grouped=form)
     ref = oracle_weight(ev, A, x, pj, normalize)
     got = w.cpu().numpy()
     if normalize:
@@ -137,8 +143,9 @@ def test_fused_weight_non_finite_rows(grid, normalize):
         assert not np.any(ref) and not np.any(got)
 
 
+@FORMS
 @pytest.mark.parametrize("normalize", [True, False], ids=["materialised", "grouped"])
-def test_fused_weight_non_finite_system_matrix(grid, normalize):
+def test_fused_weight_non_finite_system_matrix(grid, normalize, form):
     """Infinite and NaN entries in the first-hit A rows with finite coarse values: the
     materialised variant's element-wise nan_to_num zeroes those products and keeps the
     rows (the kernel filters only when a row's sum comes out non-finite), the grouped
@@ -157,7 +164,7 @@ def test_fused_weight_non_finite_system_matrix(grid, normalize):
     x = (rng.random(p.fine_pixel_num) + 0.2).astype(np.float32)
     w = torch.zeros(p.fine_pixel_num, device=DEV)
     ops.coarse_weight(torch.from_numpy(ev).to(DEV), ax, torch.from_numpy(A).to(DEV), torch.from_numpy(x).to(DEV), w,
-                      normalize=normalize, eps=1e-12)
+                      normalize=normalize, eps=1e-12, grouped=form)
     ref = oracle_weight(ev, A, x, pj, normalize)
     got = w.cpu().numpy()
     if normalize:
@@ -190,7 +197,8 @@ def test_fused_weight_equals_materialise_then_project(grid):
         assert rel(w1.cpu(), w2.cpu()) < 1e-6
 
 
-def test_fused_weight_row_counts_and_empty(grid):
+@FORMS
+def test_fused_weight_row_counts_and_empty(grid, form):
     """Fewer rows than work-groups, one row, and no rows (weight untouched / zeroed)."""
     from jscc_recon_spect_cylinder_amd import ops
     rot, inv, p, pj = grid
@@ -202,7 +210,7 @@ def test_fused_weight_row_counts_and_empty(grid):
     Ad, xd = torch.from_numpy(A).to(DEV), torch.from_numpy(x).to(DEV)
     for k in (1, 3):
         w = torch.zeros(p.fine_pixel_num, device=DEV)
-        ops.coarse_weight(torch.from_numpy(ev[:k]).to(DEV), ax, Ad, xd, w, normalize=True)
+        ops.coarse_weight(torch.from_numpy(ev[:k]).to(DEV), ax, Ad, xd, w, normalize=True, grouped=form)
         assert rel(w.cpu(), oracle_weight(ev[:k], A, x, pj, True)) < 2e-6
     w = torch.full((p.fine_pixel_num,), 2.0, device=DEV)
     ops.coarse_weight(torch.zeros((0, p.coarse_pixel_num + 1), device=DEV), ax, Ad, xd, w, normalize=True)
@@ -230,3 +238,35 @@ def test_sparse_drop_ins_on_a_grid_beyond_the_fused_kernel():
     w = loc.get_weight_compton_sparse(torch.from_numpy(ev).to(DEV), torch.from_numpy(A).to(DEV),
                                       torch.from_numpy(x[:, None]).to(DEV), p.to(DEV))
     assert rel(w.cpu().numpy().ravel(), oracle_weight(ev, A, x, pj, True)) < 2e-6
+
+
+@pytest.mark.parametrize("normalize", [True, False], ids=["materialised", "grouped"])
+@pytest.mark.parametrize("n_cryst", [3, 40])
+def test_grouped_form_equals_per_row_form(grid, normalize, n_cryst):
+    """Rows in first-hit crystal order (as a row list, and physically sorted): the grouped form (adjoints
+    once per run of one crystal, runs spanning work-group boundaries) gives the per-row form's weight and the
+    oracle's; with a rotation gather, both scatters and accumulation."""
+    from jscc_recon_spect_cylinder_amd import ops
+    rot, inv, p, pj = grid
+    ax = p.tables.axis.to(DEV)
+    n = 2000
+    ev, rng = make_rows(p, n, n_cryst, 16)
+    A = (rng.random((n_cryst, p.fine_pixel_num)) + 0.05).astype(np.float32)
+    x = (rng.random(p.fine_pixel_num) + 0.2).astype(np.float32)
+    evd, Ad, xd = torch.from_numpy(ev).to(DEV), torch.from_numpy(A).to(DEV), torch.from_numpy(x).to(DEV)
+    order = torch.from_numpy(np.argsort(ev[:, 0], kind="stable").astype(np.int32)).to(DEV)
+    r = 5
+    kw = dict(normalize=normalize, eps=1e-12, x_perm=torch.from_numpy(rot[:, r].copy()).to(DEV),
+              inv_perm=torch.from_numpy(inv[:, r].copy()).to(DEV), out_scale=1.3, accumulate=True)
+    w0 = rng.random(p.fine_pixel_num).astype(np.float32)
+    outs = []
+    for rows, grouped, src in ((order, True, evd), (None, True, evd[order.long()].contiguous()),
+                               (order, False, evd)):
+        w = torch.from_numpy(w0.copy()).to(DEV)
+        ops.coarse_weight(src, ax, Ad, xd, w, rows=rows, grouped=grouped, **kw)
+        outs.append(w.cpu().numpy().astype(np.float64) - w0)
+    ops.raise_on_index_error(torch.device(DEV))
+    ref = 1.3 * osem.rot_gather(oracle_weight(ev, A, osem.rot_gather(x, rot[:, r]), pj, normalize), inv[:, r])
+    for o in outs:
+        assert rel(o, ref) < 2e-6
+    assert rel(outs[0], outs[2]) < 1e-6 and rel(outs[1], outs[2]) < 1e-6

@@ -116,3 +116,22 @@ def test_axis_taps_reject_three_tap_rows():
     from jscc_recon_spect_cylinder_amd import ops
     with pytest.raises(ValueError):
         ops.axis_taps(np.eye(3), np.array([[0.5, 0.25, 0.25], [0, 1, 0], [0, 0, 1]]))
+
+
+def test_axis_taps_adjoint_table_is_the_transpose_of_z_expand(coor_full):
+    """The grouped weight's z adjoint table lists, per coarse slice, exactly the nonzero column entries of
+    z_expand (padded with weight 0), so E^T computed from it is the transpose of the expand; it is absent
+    for non-identity layer maps and for strides whose columns have more than four entries."""
+    from jscc_recon_spect_cylinder_amd import ops
+    from jscc_recon_spect_cylinder_amd.compton_sparse_ops import build_compton_sparse_projector
+    p = build_compton_sparse_projector(torch.from_numpy(coor_full), 1, 2, 20)
+    ax = p.tables.axis
+    assert ax.l_idx is None and ax.za_idx is not None
+    ze = p.z_expand.numpy().astype(np.float64)
+    rebuilt = np.zeros_like(ze)
+    for zc in range(ax.n_zc):
+        for j in range(ops.ZADJ):
+            rebuilt[ax.za_idx[zc, j], zc] += ax.za_w[zc, j]
+    np.testing.assert_array_equal(rebuilt, ze)
+    assert build_compton_sparse_projector(torch.from_numpy(coor_full), 2, 2, 20).tables.axis.za_idx is None
+    assert build_compton_sparse_projector(torch.from_numpy(coor_full), 1, 4, 20).tables.axis.za_idx is None
