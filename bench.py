@@ -744,12 +744,16 @@ def main():
     n1, n2 = n1_host.to(dev), n2_host.to(dev)
     P = cl.coor.shape[0]
 
-    n_sel = [0]
+    last_rows = [None]
 
     def step(store_dtype, name):
+        # FP32 / BF16 rows: the selected and kept counts stay on the device (sync_counts=False), so the step
+        # has no host synchronisation; the FP16 store is quantised from the selected FP32 rows, which reads
+        # the counts
         rows = build_compton_rows(None, det, coor, lst0.clone(), 0.0, 0.0, cl.e0, 0.02, tmax, tmin, tsum, dev,
-                                  noise=(n1, n2), sigma_model="fixed", fixed_sigma=sigma, store_dtype=store_dtype)
-        n_sel[0] = rows.n_sel
+                                  noise=(n1, n2), sigma_model="fixed", fixed_sigma=sigma, store_dtype=store_dtype,
+                                  sync_counts=False)
+        last_rows[0] = rows
         st = ListModeStore.from_rows(rows, name)
         s = st.backproject_ones()
         x = torch.ones_like(s)
@@ -780,6 +784,8 @@ def main():
         pcnt, ptot = _prof_read(lib, _lib.PROF_PROJ)
         scnt, stot = _prof_read(lib, _lib.PROF_PROJ_SUM)
         ms_step = (t1 - t0) * 1e3 / args.steps
+        n_sel = [last_rows[0].resolve_counts().n_sel]  # counts (and the index check) of the last step
+        st.resolve()
         # the OSEM part alone (T already built): iterations per second
         s = st.backproject_ones()
         xi = torch.ones_like(s)
@@ -889,7 +895,7 @@ def main():
         for name, sd in (("bf16", torch.bfloat16), ("fp16", torch.float32), ("fp32", torch.float32)):
             rows_s = build_compton_rows(None, det, coor, lst0[:n_s].clone(), 0.0, 0.0, cl.e0, 0.02, tmax, tmin, tsum,
                                         dev, noise=(n1[:n_s], n2[:n_s]), sigma_model="fixed", fixed_sigma=sigma,
-                                        store_dtype=sd)
+                                        store_dtype=sd, sync_counts=False)
             kept_ok = bool(np.array_equal(rows_s.kept_event_index().cpu().numpy(), kept_ref))
             st = ListModeStore.from_rows(rows_s, name)
             s_s = st.backproject_ones()

@@ -92,6 +92,18 @@ int jscc_cone_rows(const void* events, const int32_t* ev_idx, int64_t n_rows, co
                    int sigma_model, double fixed_sigma, void* out_t, int out_dtype, int64_t ld_out, float* out_tc,
                    int64_t ld_tc, double* stats, void* stream);
 
+/* jscc_cone_rows with the row count read on the device: n_rows_dev (int32)
+ * is the count of the event compaction that produced ev_idx, so no host read
+ * of it precedes the launch.  The grid covers n_rows_max rows (ev_idx, out_t,
+ * stats hold that many); rows [count, n_rows_max) get NaN statistics, which
+ * jscc_rows_keep_flags drops, and no row values.  Same computation as
+ * jscc_cone_rows for the rows below the count. */
+int jscc_cone_rows_counted(const void* events, const int32_t* ev_idx, const int32_t* n_rows_dev, int64_t n_rows_max,
+                           const float* coor_x, const float* coor_y, const float* coor_z, int64_t n_vox,
+                           const float* sysmat, int64_t ld_sysmat, const int32_t* tap_i0, const int32_t* tap_i1,
+                           const float* tap_w0, const float* tap_w1, int sigma_model, double fixed_sigma, void* out_t,
+                           int out_dtype, int64_t ld_out, float* out_tc, int64_t ld_tc, double* stats, void* stream);
+
 /* keep[i] = S finite and nonzero and S^2/Q >= min_support
  * (process_list_plane_strict.py:241-246, 145-146); inv_sum (optional) = 1/S
  * for kept rows (the row normalisation of :138, applied at use). */
@@ -225,6 +237,32 @@ int jscc_proj_em(const void* mat, int dtype, int64_t ld, int64_t n_cols, const i
                  float* w, int64_t w_len, float* em_x, const float* em_s, int em_mode, float em_eps, float em_c1,
                  float* em_w2, float em_c2, int zero_w, void* workspace, int64_t workspace_bytes, int32_t* err,
                  void* stream);
+
+/* jscc_proj (modes 0 and 3) and jscc_proj_em over subset sub_i of sub_k of
+ * the first n rows past row_base, where n = *count (device int32) clamped to
+ * [0, count_cap]: call rows [row_base + n*sub_i/sub_k, row_base +
+ * n*(sub_i+1)/sub_k), the split_tensor_even bounds of recon_osem_plane.py:8-18
+ * taken on the device.  The launch is sized by count_cap (rows, and a
+ * workspace of jscc_proj_workspace_bytes(ceil(count_cap / sub_k), n_cols)), so
+ * a chain of subset steps over a row list whose length stays on the device
+ * (the kept-row compaction) needs no host synchronisation.  Numerators y, if
+ * given, must be indexed by the stored row (flags bit 0).  An empty subset
+ * back-projects zero (jscc_proj_em then applies its update with w = 0, as
+ * osem_list_mode does for an empty split). */
+int jscc_proj_counted(int mode, const void* mat, int dtype, int64_t ld, int64_t n_cols, const int32_t* rows,
+                      const int32_t* count, int64_t count_cap, int sub_i, int sub_k, int64_t row_base,
+                      const float* row_scale, const float* x, const int64_t* x_perm, int64_t x_perm_stride,
+                      int64_t x_len, const float* y, int64_t y_stride, float eps, int use_eps, const int64_t* inv_perm,
+                      int64_t inv_perm_stride, float out_scale, int accumulate, int flags, float* w, int64_t w_len,
+                      void* workspace, int64_t workspace_bytes, int32_t* err, void* stream);
+int jscc_proj_em_counted(const void* mat, int dtype, int64_t ld, int64_t n_cols, const int32_t* rows,
+                         const int32_t* count, int64_t count_cap, int sub_i, int sub_k, int64_t row_base,
+                         const float* row_scale, const float* x, const int64_t* x_perm, int64_t x_perm_stride,
+                         int64_t x_len, const float* y, int64_t y_stride, float eps, int use_eps,
+                         const int64_t* inv_perm, int64_t inv_perm_stride, float out_scale, int accumulate, int flags,
+                         float* w, int64_t w_len, float* em_x, const float* em_s, int em_mode, float em_eps,
+                         float em_c1, float* em_w2, float em_c2, int zero_w, void* workspace,
+                         int64_t workspace_bytes, int32_t* err, void* stream);
 
 /* jscc_proj_em without the workspace arguments: the partial back-projections
  * go to a static device buffer of JSCC_NOALLOC_SCRATCH_BYTES that the library

@@ -33,6 +33,8 @@ SIGNATURES = {
     "jscc_compact_flags": (c_int, [vp, c_i64, vp, vp, vp, c_i64, vp]),
     "jscc_cone_rows": (c_int, [vp, vp, c_i64, vp, vp, vp, c_i64, vp, c_i64, vp, vp, vp, vp, c_int, c_f64, vp,
                                c_int, c_i64, vp, c_i64, vp, vp]),
+    "jscc_cone_rows_counted": (c_int, [vp, vp, vp, c_i64, vp, vp, vp, c_i64, vp, c_i64, vp, vp, vp, vp, c_int, c_f64,
+                                       vp, c_int, c_i64, vp, c_i64, vp, vp]),
     "jscc_rows_keep_flags": (c_int, [vp, c_i64, c_f64, vp, vp, vp]),
     "jscc_rows_gather_normalize": (c_int, [vp, c_int, c_i64, c_i64, vp, c_i64, vp, c_int, vp, vp, c_i64, c_i64, vp]),
     "jscc_arows_normalize": (c_int, [vp, c_i64, c_i64, vp, c_i64, vp, c_i64, vp]),
@@ -50,6 +52,12 @@ SIGNATURES = {
     "jscc_proj_em": (c_int, [vp, c_int, c_i64, c_i64, vp, c_i64, c_i64, vp, vp, vp, c_i64, c_i64, vp, c_i64, c_f32,
                              c_int, vp, c_i64, c_f32, c_int, c_int, vp, c_i64, vp, vp, c_int, c_f32, c_f32, vp, c_f32,
                              c_int, vp, c_i64, vp, vp]),
+    "jscc_proj_counted": (c_int, [c_int, vp, c_int, c_i64, c_i64, vp, vp, c_i64, c_int, c_int, c_i64, vp, vp, vp,
+                                  c_i64, c_i64, vp, c_i64, c_f32, c_int, vp, c_i64, c_f32, c_int, c_int, vp, c_i64,
+                                  vp, c_i64, vp, vp]),
+    "jscc_proj_em_counted": (c_int, [vp, c_int, c_i64, c_i64, vp, vp, c_i64, c_int, c_int, c_i64, vp, vp, vp, c_i64,
+                                     c_i64, vp, c_i64, c_f32, c_int, vp, c_i64, c_f32, c_int, c_int, vp, c_i64, vp, vp,
+                                     c_int, c_f32, c_f32, vp, c_f32, c_int, vp, c_i64, vp, vp]),
     "jscc_proj_em_noalloc": (c_int, [vp, c_int, c_i64, c_i64, vp, c_i64, c_i64, vp, vp, vp, c_i64, c_i64, vp, c_i64,
                                      c_f32, c_int, vp, c_i64, c_f32, c_int, c_int, vp, c_i64, vp, vp, c_int, c_f32,
                                      c_f32, vp, c_f32, c_int, vp, vp]),

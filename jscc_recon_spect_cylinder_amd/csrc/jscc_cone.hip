@@ -205,8 +205,20 @@ __global__ void __launch_bounds__(kConeThreads) k_cone_rows(
     const float* __restrict__ A, int64_t lda, const int32_t* __restrict__ tap_i0,
     const int32_t* __restrict__ tap_i1, const float* __restrict__ tap_w0, const float* __restrict__ tap_w1,
     ConeConsts cc, void* __restrict__ out_t, int64_t ldt, float* __restrict__ out_tc, int64_t ldtc,
-    double* __restrict__ stats) {
+    double* __restrict__ stats, const int32_t* __restrict__ n_rows_dev) {
   const int row0 = blockIdx.x * E;
+  const int n_grid = n_rows;  // rows the grid covers
+  if (n_rows_dev) {
+    // the row count read on the device (at most the grid's rows): the grid's rows past it get NaN
+    // statistics, which the keep decision drops, and no row values
+    const int nd = *n_rows_dev;
+    n_rows = nd < 0 ? 0 : (nd < n_rows ? nd : n_rows);
+    if (row0 >= n_rows) {
+      if (threadIdx.x < E * 4 && row0 + (int)threadIdx.x / 4 < n_grid)
+        stats[(int64_t)row0 * 4 + threadIdx.x] = __builtin_nan("");
+      return;
+    }
+  }
   ConeEv evs[E];
   bool valid[E];
 #pragma unroll
@@ -298,6 +310,8 @@ __global__ void __launch_bounds__(kConeThreads) k_cone_rows(
       double v = red[k][0][e];
       for (int w = 1; w < kConeThreads / JSCC_WAVE; ++w) v = (k == 3) ? fmax(v, red[k][w][e]) : v + red[k][w][e];
       stats[(int64_t)(row0 + e) * 4 + k] = v;
+    } else if (row0 + e < n_grid) {
+      stats[(int64_t)(row0 + e) * 4 + k] = __builtin_nan("");
     }
   }
 }
@@ -306,21 +320,21 @@ template <int SIG, int AF, int OUT_DT, bool WANT_TC>
 void launch_cone(int n_rows, hipStream_t s, const JsccEvent* ev, const int32_t* idx, const float* cx,
                  const float* cy, const float* cz, int P, const float* A, int64_t lda, const int32_t* ti0,
                  const int32_t* ti1, const float* tw0, const float* tw1, ConeConsts cc, void* out_t,
-                 int64_t ldt, float* out_tc, int64_t ldtc, double* stats) {
+                 int64_t ldt, float* out_tc, int64_t ldtc, double* stats, const int32_t* n_dev) {
   constexpr int E = 4;
   dim3 grid((unsigned)((n_rows + E - 1) / E));
   hipLaunchKernelGGL((k_cone_rows<SIG, AF, OUT_DT, WANT_TC, E>), grid, dim3(kConeThreads), 0, s, ev, idx, n_rows,
-                     cx, cy, cz, P, A, lda, ti0, ti1, tw0, tw1, cc, out_t, ldt, out_tc, ldtc, stats);
+                     cx, cy, cz, P, A, lda, ti0, ti1, tw0, tw1, cc, out_t, ldt, out_tc, ldtc, stats, n_dev);
 }
 
 template <int SIG, int AF>
 int dispatch_out(int out_dtype, bool want_tc, int n_rows, hipStream_t s, const JsccEvent* ev, const int32_t* idx,
                  const float* cx, const float* cy, const float* cz, int P, const float* A, int64_t lda,
                  const int32_t* ti0, const int32_t* ti1, const float* tw0, const float* tw1, ConeConsts cc,
-                 void* out_t, int64_t ldt, float* out_tc, int64_t ldtc, double* stats) {
+                 void* out_t, int64_t ldt, float* out_tc, int64_t ldtc, double* stats, const int32_t* n_dev) {
 #define JSCC_CONE_CASE(DT, TC)                                                                              \
   launch_cone<SIG, AF, DT, TC>(n_rows, s, ev, idx, cx, cy, cz, P, A, lda, ti0, ti1, tw0, tw1, cc, out_t, ldt, \
-                               out_tc, ldtc, stats)
+                               out_tc, ldtc, stats, n_dev)
   if (out_dtype == JSCC_DT_F32) {
     if (want_tc) JSCC_CONE_CASE(JSCC_DT_F32, true); else JSCC_CONE_CASE(JSCC_DT_F32, false);
   } else if (out_dtype == JSCC_DT_BF16) {
@@ -337,12 +351,12 @@ int dispatch_out(int out_dtype, bool want_tc, int n_rows, hipStream_t s, const J
 
 }  // namespace
 
-extern "C" int jscc_cone_rows(const void* events, const int32_t* ev_idx, int64_t n_rows, const float* coor_x,
-                              const float* coor_y, const float* coor_z, int64_t n_vox, const float* sysmat,
-                              int64_t ld_sysmat, const int32_t* tap_i0, const int32_t* tap_i1, const float* tap_w0,
-                              const float* tap_w1, int sigma_model, double fixed_sigma, void* out_t,
-                              int out_dtype, int64_t ld_out, float* out_tc, int64_t ld_tc, double* stats,
-                              void* stream) {
+static int cone_rows_impl(const void* events, const int32_t* ev_idx, int64_t n_rows, const int32_t* n_rows_dev,
+                          const float* coor_x, const float* coor_y, const float* coor_z, int64_t n_vox,
+                          const float* sysmat, int64_t ld_sysmat, const int32_t* tap_i0, const int32_t* tap_i1,
+                          const float* tap_w0, const float* tap_w1, int sigma_model, double fixed_sigma,
+                          void* out_t, int out_dtype, int64_t ld_out, float* out_tc, int64_t ld_tc, double* stats,
+                          void* stream) {
   JSCC_REQUIRE(n_rows >= 0 && n_rows < ((int64_t)1 << 31), "jscc_cone_rows: n_rows out of range");
   if (n_rows == 0) return 0;
   JSCC_REQUIRE(events && ev_idx && coor_x && coor_y && coor_z && stats, "jscc_cone_rows: null pointer");
@@ -381,15 +395,15 @@ extern "C" int jscc_cone_rows(const void* events, const int32_t* ev_idx, int64_t
   if (taps)                                                                                                   \
     rc = dispatch_out<SIG, AF_TAPS>(out_dtype, want_tc, n, s, ev, ev_idx, coor_x, coor_y, coor_z, P, sysmat,  \
                                     ld_sysmat, tap_i0, tap_i1, tap_w0, tap_w1, cc, out_t, ld_out, out_tc,      \
-                                    ld_tc, stats);                                                            \
+                                    ld_tc, stats, n_rows_dev);                                                \
   else if (sysmat)                                                                                            \
     rc = dispatch_out<SIG, AF_ROW>(out_dtype, want_tc, n, s, ev, ev_idx, coor_x, coor_y, coor_z, P, sysmat,   \
                                    ld_sysmat, tap_i0, tap_i1, tap_w0, tap_w1, cc, out_t, ld_out, out_tc,       \
-                                   ld_tc, stats);                                                             \
+                                   ld_tc, stats, n_rows_dev);                                                 \
   else                                                                                                        \
     rc = dispatch_out<SIG, AF_NONE>(out_dtype, want_tc, n, s, ev, ev_idx, coor_x, coor_y, coor_z, P, sysmat,  \
                                     ld_sysmat, tap_i0, tap_i1, tap_w0, tap_w1, cc, out_t, ld_out, out_tc,      \
-                                    ld_tc, stats);
+                                    ld_tc, stats, n_rows_dev);
   if (sigma_model == SIG_STRICT) {
     JSCC_CONE_AF(SIG_STRICT)
   } else {
@@ -400,6 +414,30 @@ extern "C" int jscc_cone_rows(const void* events, const int32_t* ev_idx, int64_t
   if (rc != 0) return rc;
   JSCC_CHECK_LAUNCH();
   return 0;
+}
+
+extern "C" int jscc_cone_rows(const void* events, const int32_t* ev_idx, int64_t n_rows, const float* coor_x,
+                              const float* coor_y, const float* coor_z, int64_t n_vox, const float* sysmat,
+                              int64_t ld_sysmat, const int32_t* tap_i0, const int32_t* tap_i1, const float* tap_w0,
+                              const float* tap_w1, int sigma_model, double fixed_sigma, void* out_t,
+                              int out_dtype, int64_t ld_out, float* out_tc, int64_t ld_tc, double* stats,
+                              void* stream) {
+  return cone_rows_impl(events, ev_idx, n_rows, nullptr, coor_x, coor_y, coor_z, n_vox, sysmat, ld_sysmat, tap_i0,
+                        tap_i1, tap_w0, tap_w1, sigma_model, fixed_sigma, out_t, out_dtype, ld_out, out_tc, ld_tc,
+                        stats, stream);
+}
+
+extern "C" int jscc_cone_rows_counted(const void* events, const int32_t* ev_idx, const int32_t* n_rows_dev,
+                                      int64_t n_rows_max, const float* coor_x, const float* coor_y,
+                                      const float* coor_z, int64_t n_vox, const float* sysmat, int64_t ld_sysmat,
+                                      const int32_t* tap_i0, const int32_t* tap_i1, const float* tap_w0,
+                                      const float* tap_w1, int sigma_model, double fixed_sigma, void* out_t,
+                                      int out_dtype, int64_t ld_out, float* out_tc, int64_t ld_tc, double* stats,
+                                      void* stream) {
+  JSCC_REQUIRE(n_rows_dev != nullptr, "jscc_cone_rows_counted: null row count");
+  return cone_rows_impl(events, ev_idx, n_rows_max, n_rows_dev, coor_x, coor_y, coor_z, n_vox, sysmat, ld_sysmat,
+                        tap_i0, tap_i1, tap_w0, tap_w1, sigma_model, fixed_sigma, out_t, out_dtype, ld_out, out_tc,
+                        ld_tc, stats, stream);
 }
 
 // ============================================================================

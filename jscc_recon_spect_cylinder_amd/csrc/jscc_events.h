@@ -18,3 +18,17 @@ struct __attribute__((aligned(16))) JsccEvent {
   int32_t src;           // input position of the event
 };
 static_assert(sizeof(JsccEvent) == 64, "JsccEvent must be 64 bytes");
+
+// The NaN / zero-sum / effective-support decision of process_list_plane_strict.py:241-246, 145-146 on the
+// FP64 row statistics (S = sum t, Q = sum t^2, M = max t).  A row lying entirely below the smallest normal
+// FP32 value is a zero row: its subnormal tail depends on the exponential's subnormal handling.  Shared by
+// every kernel that takes the decision, so they all keep the same rows.
+__device__ __forceinline__ bool jscc_row_keep(double S, double Q, double M, double min_support) {
+#pragma clang fp contract(off)
+  bool keep = (S == S) && (S != 0.0) && (M >= 1.17549435082228750797e-38);
+  if (keep) {
+    const double support = (S * S) / Q;  // 1 / sum((t/S)^2)
+    keep = support >= min_support;       // NaN compares false
+  }
+  return keep;
+}

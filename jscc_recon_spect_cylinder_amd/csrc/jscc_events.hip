@@ -238,13 +238,7 @@ __global__ void __launch_bounds__(256) k_rows_keep(const double* __restrict__ st
   const double S = stats[i * 4 + 0];
   const double Q = stats[i * 4 + 1];
   const double M = stats[i * 4 + 3];
-  // a row lying entirely below the smallest normal FP32 value is a zero row:
-  // its subnormal tail depends on the exponential's subnormal handling
-  bool keep = (S == S) && (S != 0.0) && (M >= 1.17549435082228750797e-38);
-  if (keep) {
-    const double support = (S * S) / Q;  // 1 / sum((t/S)^2)
-    keep = support >= min_support;       // NaN compares false
-  }
+  const bool keep = jscc_row_keep(S, Q, M, min_support);
   flags[i] = keep ? 1 : 0;
   if (inv_sum) inv_sum[i] = keep ? (float)(1.0 / S) : 0.0f;
 }

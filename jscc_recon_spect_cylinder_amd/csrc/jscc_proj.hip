@@ -114,6 +114,13 @@ struct ProjArgs {
   int64_t n_rows1;
   float num_scale1;
   ProjSeg seg2;
+  // Counted launches: when cnt is set, the call rows are [lo, hi) past
+  // row_base with lo = n * sub_i / sub_k, hi = n * (sub_i + 1) / sub_k and
+  // n = *cnt clamped to [0, cnt_cap] -- the split_tensor_even bounds of a row
+  // count that stays on the device (n_rows then only sizes the grid).
+  const int32_t* cnt;
+  int64_t cnt_cap;
+  int sub_i, sub_k;
 };
 
 // Row metadata staged in LDS for a chunk of a work-group's rows: stored row
@@ -270,6 +277,14 @@ __global__ void __launch_bounds__(kProjThreads)
                  int32_t* __restrict__ err, ProjArgs a) {
   a.M = M; a.rows = rows; a.row_scale = row_scale; a.x = x; a.x_perm = x_perm; a.y = y; a.coef = coef;
   a.fwd_out = fwd_out; a.partial = partial; a.err = err;
+  if (a.cnt) {
+    int64_t n = *a.cnt;
+    n = n < 0 ? 0 : (n > a.cnt_cap ? a.cnt_cap : n);
+    const int64_t lo = n * a.sub_i / a.sub_k, hi = n * (a.sub_i + 1) / a.sub_k;
+    a.row_base += lo;
+    a.n_rows = hi - lo;
+    a.n_rows1 = a.n_rows;
+  }
   constexpr int VN = Vec<DT>::N;
   const int tid = threadIdx.x;
   const int nvec = a.P / VN;
@@ -511,6 +526,7 @@ __global__ void __launch_bounds__(kProjThreads)
   a.x = x; a.x_perm = nullptr; a.x_perm_stride = 0; a.x_len = o.P; a.y = nullptr; a.y_stride = 0; a.coef = nullptr;
   a.eps = o.eps; a.use_eps = o.use_eps; a.fwd_out = nullptr; a.partial = partial; a.err = err; a.y_by_row = 0;
   a.n_rows1 = 0; a.num_scale1 = 1.0f; a.seg2 = ProjSeg{};
+  a.cnt = nullptr; a.cnt_cap = 0; a.sub_i = 0; a.sub_k = 1;
 
   extern __shared__ jscc_f4v xs4[];
   __shared__ RowMeta meta;
@@ -912,7 +928,8 @@ static int proj_impl(int mode, const void* mat, int dtype, int64_t ld, int64_t n
                      const int64_t* inv_perm, int64_t inv_perm_stride, float out_scale, int accumulate,
                      int flags, float* w, int64_t w_len, void* workspace, int64_t workspace_bytes,
                      int32_t* err, void* stream, const JsccEm& em, int64_t n_rows2 = 0,
-                     const ProjSeg* seg2 = nullptr, float num_scale1 = 1.0f) {
+                     const ProjSeg* seg2 = nullptr, float num_scale1 = 1.0f, const int32_t* cnt = nullptr,
+                     int64_t cnt_cap = 0, int sub_i = 0, int sub_k = 1) {
   JSCC_REQUIRE(mode >= 0 && mode <= 3, "jscc_proj: bad mode");
   JSCC_REQUIRE(n_rows >= 0 && n_cols > 0, "jscc_proj: bad shape");
   JSCC_REQUIRE(err != nullptr, "jscc_proj: null error word");
@@ -951,6 +968,7 @@ static int proj_impl(int mode, const void* mat, int dtype, int64_t ld, int64_t n
   a.row_scale = row_scale; a.x = x; a.x_perm = x_perm; a.x_perm_stride = x_perm_stride; a.x_len = x_len;
   a.y = y; a.y_stride = y_stride; a.coef = coef; a.eps = eps; a.use_eps = use_eps; a.fwd_out = fwd_out;
   a.partial = (float*)workspace; a.err = err; a.y_by_row = (flags & 1) ? 1 : 0;
+  a.cnt = cnt; a.cnt_cap = cnt_cap; a.sub_i = sub_i; a.sub_k = sub_k;
   int rc;
   const int tag = mode == 3 ? JSCC_TAG_PROJ_SUM : JSCC_TAG_PROJ;
   jscc_prof_begin(tag, s);
@@ -999,6 +1017,58 @@ extern "C" int jscc_proj_em(const void* mat, int dtype, int64_t ld, int64_t n_co
   return proj_impl(0, mat, dtype, ld, n_cols, rows, n_rows, row_base, row_scale, x, x_perm, x_perm_stride, x_len, y,
                    y_stride, nullptr, eps, use_eps, nullptr, inv_perm, inv_perm_stride, out_scale, accumulate, flags,
                    w, w_len, workspace, workspace_bytes, err, stream, em);
+}
+
+// ------------------------------------------------------------------ counted launches
+// Subset sub_i of sub_k of the first n rows past row_base, n = *count (device
+// int32, clamped to [0, count_cap]): the launch is sized by count_cap and the
+// bounds are taken on the device, so it needs no host read of the count.
+static int counted_check(const int32_t* count, int64_t count_cap, int sub_i, int sub_k, const float* y, int flags,
+                         const char* what) {
+  JSCC_REQUIRE(count != nullptr, "%s: null row count", what);
+  JSCC_REQUIRE(count_cap > 0 && count_cap < ((int64_t)1 << 31), "%s: count_cap out of range", what);
+  JSCC_REQUIRE(sub_k >= 1 && sub_i >= 0 && sub_i < sub_k, "%s: subset %d of %d", what, sub_i, sub_k);
+  JSCC_REQUIRE(y == nullptr || (flags & 1), "%s: numerators must be indexed by the stored row", what);
+  return 0;
+}
+
+extern "C" int jscc_proj_counted(int mode, const void* mat, int dtype, int64_t ld, int64_t n_cols,
+                                 const int32_t* rows, const int32_t* count, int64_t count_cap, int sub_i, int sub_k,
+                                 int64_t row_base, const float* row_scale, const float* x, const int64_t* x_perm,
+                                 int64_t x_perm_stride, int64_t x_len, const float* y, int64_t y_stride, float eps,
+                                 int use_eps, const int64_t* inv_perm, int64_t inv_perm_stride, float out_scale,
+                                 int accumulate, int flags, float* w, int64_t w_len, void* workspace,
+                                 int64_t workspace_bytes, int32_t* err, void* stream) {
+  JSCC_REQUIRE(mode == 0 || mode == 3, "jscc_proj_counted: ratio (0) or column-sum (3) mode");
+  const int rc0 = counted_check(count, count_cap, sub_i, sub_k, y, flags, "jscc_proj_counted");
+  if (rc0) return rc0;
+  const JsccEm none = {nullptr, nullptr, 0, 0.0f, 0, 1.0f, nullptr, 0.0f};
+  const int64_t n_max = (count_cap + sub_k - 1) / sub_k;
+  return proj_impl(mode, mat, dtype, ld, n_cols, rows, n_max, row_base, row_scale, x, x_perm, x_perm_stride, x_len,
+                   y, y_stride, nullptr, eps, use_eps, nullptr, inv_perm, inv_perm_stride, out_scale, accumulate,
+                   flags, w, w_len, workspace, workspace_bytes, err, stream, none, 0, nullptr, 1.0f, count,
+                   count_cap, sub_i, sub_k);
+}
+
+extern "C" int jscc_proj_em_counted(const void* mat, int dtype, int64_t ld, int64_t n_cols, const int32_t* rows,
+                                    const int32_t* count, int64_t count_cap, int sub_i, int sub_k, int64_t row_base,
+                                    const float* row_scale, const float* x, const int64_t* x_perm,
+                                    int64_t x_perm_stride, int64_t x_len, const float* y, int64_t y_stride, float eps,
+                                    int use_eps, const int64_t* inv_perm, int64_t inv_perm_stride, float out_scale,
+                                    int accumulate, int flags, float* w, int64_t w_len, float* em_x,
+                                    const float* em_s, int em_mode, float em_eps, float em_c1, float* em_w2,
+                                    float em_c2, int zero_w, void* workspace, int64_t workspace_bytes, int32_t* err,
+                                    void* stream) {
+  JSCC_REQUIRE(em_x != nullptr && em_s != nullptr, "jscc_proj_em_counted: null image or sensitivity");
+  JSCC_REQUIRE(em_mode >= 0 && em_mode <= 2, "jscc_proj_em_counted: bad EM mode %d", em_mode);
+  const int rc0 = counted_check(count, count_cap, sub_i, sub_k, y, flags, "jscc_proj_em_counted");
+  if (rc0) return rc0;
+  const JsccEm em = {em_x, em_s, em_mode, em_eps, zero_w, em_c1, em_w2, em_c2};
+  const int64_t n_max = (count_cap + sub_k - 1) / sub_k;
+  return proj_impl(0, mat, dtype, ld, n_cols, rows, n_max, row_base, row_scale, x, x_perm, x_perm_stride, x_len, y,
+                   y_stride, nullptr, eps, use_eps, nullptr, inv_perm, inv_perm_stride, out_scale, accumulate, flags,
+                   w, w_len, workspace, workspace_bytes, err, stream, em, 0, nullptr, 1.0f, count, count_cap, sub_i,
+                   sub_k);
 }
 
 // ------------------------------------------------------------------ allocation-free subset step

@@ -28,33 +28,56 @@ class ListModeStore:
     kept: torch.Tensor         # int32 stored-row ids of the kept events, in order
     n_kept: int
     row_scale: torch.Tensor    # per stored row: value = row_scale * stored
+    # rows built with sync_counts=False: the kept count as a device int32 (n_kept is -1 until resolve())
+    # and its upper bound; backproject_ones and osem then launch with device-side row ranges
+    count_dev: torch.Tensor | None = None
+    cap: int = 0
 
     @classmethod
     def from_rows(cls, rows: ComptonRows, dtype="bf16") -> "ListModeStore":
         code = ops.dtype_code(dtype)
         if code in (ops.DT_FP8, ops.DT_FP16):
+            rows.resolve_counts()  # quantisation takes the selected rows
             codes, scale = ops.quantize_rows(rows.store, rows.n_vox, rows.n_sel, rows.stats, code)
             return cls(codes, code, rows.n_vox, rows.kept, rows.n_kept, scale)
         if ops.dtype_code(rows.store.dtype) != code:
             raise ValueError("the rows were built in another storage dtype")
+        if rows.counts_dev is not None:
+            return cls(rows.store, code, rows.n_vox, rows.kept, -1, rows.inv_sum, rows.counts_dev[1],
+                       rows.kept.shape[0])
         return cls(rows.store, code, rows.n_vox, rows.kept, rows.n_kept, rows.inv_sum)
 
+    def resolve(self) -> "ListModeStore":
+        """Read a deferred kept count (one synchronisation)."""
+        if self.count_dev is not None:
+            (self.n_kept,) = ops.read_counts_checked(self.count_dev)
+            self.count_dev = None
+        return self
+
     def nbytes_kept(self) -> int:
+        self.resolve()
         per = {ops.DT_F32: 4, ops.DT_BF16: 2, ops.DT_FP8: 1, ops.DT_FP16: 2}[self.dtype_c]
         return self.n_kept * self.n_vox * per
 
     def subset_bounds(self, k: int):
+        self.resolve()
         return [(self.n_kept * i // k, self.n_kept * (i + 1) // k) for i in range(k)]
 
     def backproject_ones(self) -> torch.Tensor:
         """sum_i T_i (column sums of the normalised kept rows) as [P] FP32."""
         w = torch.zeros(self.n_vox, dtype=torch.float32, device=self.mat.device)
+        if self.count_dev is not None:
+            ops.ProjLauncher(ops.MODE_ONES, self.mat, n_rows_max=self.cap, dtype_c=self.dtype_c, n_cols=self.n_vox,
+                             rows=self.kept, row_scale=self.row_scale, w=w,
+                             count=(self.count_dev, self.cap, 1)).launch_subset(0)
+            return w
         ops.proj(ops.MODE_ONES, self.mat, dtype_c=self.dtype_c, n_cols=self.n_vox, rows=self.kept,
                  n_rows=self.n_kept, row_scale=self.row_scale, w=w)
         return w
 
     def forward(self, x: torch.Tensor) -> torch.Tensor:
         """T x for the kept rows (normalised) as [n_kept] FP32."""
+        self.resolve()
         f = torch.empty(max(self.n_kept, 1), dtype=torch.float32, device=self.mat.device)
         ops.proj(ops.MODE_FWD, self.mat, dtype_c=self.dtype_c, n_cols=self.n_vox, rows=self.kept,
                  n_rows=self.n_kept, row_scale=self.row_scale, x=x, fwd_out=f)
@@ -62,6 +85,7 @@ class ListModeStore:
 
     def subset_weight(self, x, w, lo, hi, eps=None, x_perm=None, inv_perm=None, scale=1.0):
         """w (+)= scale * T_s^T (1 / T_s x) for kept rows [lo, hi)."""
+        self.resolve()
         if hi <= lo:
             return
         ops.proj(ops.MODE_RATIO, self.mat, dtype_c=self.dtype_c, n_cols=self.n_vox, rows=self.kept, row_base=lo,
@@ -76,6 +100,18 @@ class ListModeStore:
         (jscc_osem_persistent) instead; on MI355X its two grid barriers per subset cost more
         than the launches they replace (153 vs 117 us per subset at 2e4 BF16 rows x 16384),
         so it is not the default."""
+        if self.count_dev is not None and not persistent:
+            # the subset bounds of the device-side kept count, taken on the device: no synchronisation
+            w = torch.zeros_like(x)
+            step = ops.ProjLauncher(ops.MODE_RATIO, self.mat, n_rows_max=self.cap, dtype_c=self.dtype_c,
+                                    n_cols=self.n_vox, rows=self.kept, row_scale=self.row_scale, x=x, eps=eps,
+                                    accumulate=True, w=w, em=(x, s, em_mode, 1e-12, True),
+                                    count=(self.count_dev, self.cap, n_subsets))
+            for _ in range(n_iter):
+                for i in range(n_subsets):
+                    step.launch_subset(i)
+            return x
+        self.resolve()
         if persistent and self.n_kept > 0:
             off = torch.tensor([b[0] for b in self.subset_bounds(n_subsets)] + [self.n_kept], dtype=torch.int64,
                                device=self.mat.device)
@@ -267,6 +303,7 @@ class LazyListStore:
     @classmethod
     def from_rows(cls, rows: ComptonRows, sysmat=None, sigma_model=ops.SIG_FIXED, fixed_sigma=0.0):
         """Keep the event records of built (or statistics-only) rows; T itself may be dropped."""
+        rows.resolve_counts()
         kept_rows = rows.kept[: rows.n_kept].long()
         ev_kept = rows.sel_idx[kept_rows].to(torch.int32).contiguous()
         return cls(rows.events, ev_kept, rows.n_kept, rows.vox, sysmat, sigma_model, fixed_sigma)
@@ -335,6 +372,7 @@ class BatchedLazyStore:
         statistics and kept sets only)."""
         evs, kept, base = [], [], 0
         for r in rows_list:
+            r.resolve_counts()
             k = r.sel_idx[r.kept[: r.n_kept].long()].to(torch.int32) + base
             evs.append(r.events)
             kept.append(k.contiguous())

@@ -239,8 +239,13 @@ class ATaps:
 
 def cone_rows(events: torch.Tensor, ev_idx: torch.Tensor, n_rows: int, vox: VoxelSoA, sysmat=None,
               taps: ATaps | None = None, sigma_model: int = SIG_STRICT, fixed_sigma: float = 0.0,
-              out_dtype=None, want_tc: bool = False, ld: int | None = None):
-    """Evaluate cone rows; returns (t_store | None, t_c | None, stats [n,4] f64, ld)."""
+              out_dtype=None, want_tc: bool = False, ld: int | None = None, n_dev: torch.Tensor | None = None):
+    """Evaluate cone rows; returns (t_store | None, t_c | None, stats [n,4] f64, ld).  n_dev: the row count
+    as a device int32 (jscc_cone_rows_counted), n_rows then its upper bound: rows past the count get NaN
+    statistics (dropped by rows_keep) and no values."""
+    if n_dev is not None and (n_dev.dtype != torch.int32 or n_dev.device != events.device or n_rows < 1
+                              or ev_idx.numel() < n_rows):
+        raise ValueError("the row count must be a device int32 and the event list hold n_rows entries")
     dev = events.device
     P = vox.n
     ld = ld or round_up(P, 64)
@@ -258,14 +263,29 @@ def cone_rows(events: torch.Tensor, ev_idx: torch.Tensor, n_rows: int, vox: Voxe
     if sysmat is not None:
         A = aligned_rows(sysmat) if taps is None else sysmat
         lda = A.stride(0)
-    rc = _lib.lib().jscc_cone_rows(
-        events.data_ptr(), ev_idx.data_ptr(), n_rows, vox.x.data_ptr(), vox.y.data_ptr(), vox.z.data_ptr(), P,
-        _p(A), lda,
-        _p(taps.i0) if taps else None, _p(taps.i1) if taps else None,
-        _p(taps.w0) if taps else None, _p(taps.w1) if taps else None,
-        int(sigma_model), float(fixed_sigma), _p(out_t), code, ld, _p(out_tc), ld, stats.data_ptr(), stream_of(dev))
-    _lib.check(rc, "jscc_cone_rows")
+    tail = (vox.x.data_ptr(), vox.y.data_ptr(), vox.z.data_ptr(), P, _p(A), lda,
+            _p(taps.i0) if taps else None, _p(taps.i1) if taps else None,
+            _p(taps.w0) if taps else None, _p(taps.w1) if taps else None,
+            int(sigma_model), float(fixed_sigma), _p(out_t), code, ld, _p(out_tc), ld, stats.data_ptr(), stream_of(dev))
+    if n_dev is None:
+        rc = _lib.lib().jscc_cone_rows(events.data_ptr(), ev_idx.data_ptr(), n_rows, *tail)
+        _lib.check(rc, "jscc_cone_rows")
+    else:
+        rc = _lib.lib().jscc_cone_rows_counted(events.data_ptr(), ev_idx.data_ptr(), n_dev.data_ptr(), n_rows, *tail)
+        _lib.check(rc, "jscc_cone_rows_counted")
     return out_t, out_tc, stats, ld
+
+
+def read_counts_checked(*counts: torch.Tensor) -> list:
+    """Several device int32 counts read back together with the device error word in one synchronisation
+    (raises IndexError as raise_on_index_error)."""
+    w = err_word(counts[0].device)
+    vals = torch.cat([c.view(1) for c in counts] + [w]).tolist()
+    if vals[-1]:
+        w.zero_()
+        msgs = [txt for bit, txt in _ERR_TEXT.items() if vals[-1] & bit]
+        raise IndexError("; ".join(msgs))
+    return [int(v) for v in vals[:-1]]
 
 
 def rows_keep(stats: torch.Tensor, n: int, min_support: float, want_inv_sum: bool = False):
@@ -636,10 +656,12 @@ class ProjLauncher:
         kw = dict(kw)
         kw["n_rows"] = n_rows_max
         kw.setdefault("row_base", 0)
+        self._count = kw.get("count")
         self._own_ws = kw.get("workspace") is None
         if self._own_ws:
             P = kw.get("n_cols") if kw.get("n_cols") is not None else mat.shape[1]
-            kw["workspace"] = torch.empty(max(lib_proj_workspace_bytes(n_rows_max, P), 256), dtype=torch.uint8,
+            per = n_rows_max if self._count is None else -(-self._count[1] // self._count[2])
+            kw["workspace"] = torch.empty(max(lib_proj_workspace_bytes(per, P), 256), dtype=torch.uint8,
                                           device=mat.device)
         self.name, args, (self._i_rows, self._i_base) = _proj_call(mode, mat, **kw)
         self._fn = getattr(_lib.lib(), self.name)
@@ -651,14 +673,26 @@ class ProjLauncher:
         self._keep = (mat, kw)  # the operands (and the workspace) stay alive as long as the launcher
 
     def launch(self, row_base: int, n_rows: int) -> None:
+        if self._count is not None:
+            raise ValueError("a counted launcher takes launch_subset(i)")
         if n_rows < 0 or n_rows > self._n_max or row_base < 0 or row_base + n_rows > self._n_total:
             raise ValueError("row range outside the launcher's matrix or workspace")
+        self._go(n_rows, row_base)
+
+    def launch_subset(self, sub_i: int) -> None:
+        """Subset sub_i of a counted launcher (count=(n_dev, cap, sub_k)): its bounds are taken on the device."""
+        if self._count is None or not 0 <= sub_i < self._count[2]:
+            raise ValueError("launch_subset needs a counted launcher and 0 <= sub_i < sub_k")
+        self._go(sub_i, self._args[self._i_base])
+
+    def _go(self, first: int, row_base: int) -> None:
+        """Launch with the row count (or, counted, the subset index) `first` and row_base."""
         s = stream_of(self._dev)
         if self._own_ws and s != self._stream:
             raise ValueError("this launcher owns its workspace and launches only on the stream it was built on "
                              "(pass workspace= to launch on others)")
         a = self._args
-        a[self._i_rows] = n_rows
+        a[self._i_rows] = first
         a[self._i_base] = row_base
         a[-1] = s
         rc = self._fn(*a)
@@ -736,10 +770,24 @@ def _proj_call(mode: int, mat: torch.Tensor, *, dtype_c: int | None = None, n_co
                eps: float | None = None, fwd_out: torch.Tensor | None = None, inv_perm: torch.Tensor | None = None,
                out_scale: float = 1.0, accumulate: bool = False, w: torch.Tensor | None = None,
                y_by_row: bool = False, sanitize: bool = False, em: tuple | None = None,
-               workspace: torch.Tensor | None = None):
+               workspace: torch.Tensor | None = None, count: tuple | None = None):
     """Checks the operands of a projection launch and returns (entry point name,
-    argument list, (index of n_rows, index of row_base) in that list)."""
+    argument list, (index of n_rows, index of row_base) in that list).
+
+    count = (n_dev, cap, sub_k): a counted launch (jscc_proj_counted /
+    jscc_proj_em_counted) over subset sub_i of sub_k of the first n_dev rows
+    past row_base (n_dev a device int32 clamped to cap); the first index
+    returned is then that of sub_i (0 in the list built here)."""
     dev = mat.device
+    if count is not None:
+        n_dev, cap, sub_k = count
+        if n_dev.dtype != torch.int32 or n_dev.device != dev or cap < 1 or sub_k < 1:
+            raise ValueError("counted launch: a device int32 count, a positive cap and subset count")
+        if mode not in (MODE_RATIO, MODE_ONES) or coef is not None or fwd_out is not None:
+            raise ValueError("counted launches take the ratio and column-sum modes")
+        if y is not None and not y_by_row:
+            raise ValueError("counted launches index their numerators by the stored row")
+        n_rows = cap
     if dtype_c is None:
         dtype_c = dtype_code(mat.dtype) if mat.dtype != torch.uint8 else DT_FP8
     P = n_cols if n_cols is not None else mat.shape[1]
@@ -776,7 +824,8 @@ def _proj_call(mode: int, mat: torch.Tensor, *, dtype_c: int | None = None, n_co
     for t in (x, x_perm, y, coef, fwd_out, inv_perm, w, row_scale, rows):
         if t is not None and t.device != dev:
             raise ValueError("all operands must be on the matrix's device")
-    need = _lib.lib().jscc_proj_workspace_bytes(max(n_rows, 1), P)
+    n_launch = n_rows if count is None else -(-count[1] // count[2])  # rows one launch is sized for
+    need = _lib.lib().jscc_proj_workspace_bytes(max(n_launch, 1), P)
     if workspace is not None:
         if workspace.numel() * workspace.element_size() < need or workspace.device != dev:
             raise ValueError(f"projection workspace needs {need} bytes on the matrix's device")
@@ -794,15 +843,27 @@ def _proj_call(mode: int, mat: torch.Tensor, *, dtype_c: int | None = None, n_co
             if (t.dtype != torch.float32 or not t.is_contiguous() or t.device != dev or w is None
                     or t.numel() < w.shape[0]):
                 raise ValueError("EM operands must be contiguous float32 covering the weight, on the matrix's device")
+        rng = [n_rows, row_base] if count is None else [count[0].data_ptr(), count[1], 0, count[2], row_base]
         args = [
-            mat.data_ptr(), dtype_c, ld, P, _p(rows), n_rows, row_base, _p(row_scale), _p(x), _p(x_perm),
+            mat.data_ptr(), dtype_c, ld, P, _p(rows), *rng, _p(row_scale), _p(x), _p(x_perm),
             x_perm.stride(0) if x_perm is not None else 0, xl, _p(y), y.stride(0) if y is not None else 0,
             float(eps) if eps is not None else 0.0, 1 if eps is not None else 0, _p(inv_perm),
             inv_perm.stride(0) if inv_perm is not None else 0, float(out_scale), 1 if accumulate else 0, flags,
             _p(w), wl, x_em.data_ptr(), s_em.data_ptr(), int(em_mode), float(em_eps), float(c1), _p(w2), float(c2),
             1 if zero_w else 0,
             ws.data_ptr(), ws.numel(), err_word(dev).data_ptr(), stream_of(dev)]
+        if count is not None:
+            return "jscc_proj_em_counted", args, (7, 9)
         return "jscc_proj_em", args, (5, 6)
+    if count is not None:
+        args = [
+            mode, mat.data_ptr(), dtype_c, ld, P, _p(rows), count[0].data_ptr(), count[1], 0, count[2], row_base,
+            _p(row_scale), _p(x), _p(x_perm), x_perm.stride(0) if x_perm is not None else 0, xl, _p(y),
+            y.stride(0) if y is not None else 0, float(eps) if eps is not None else 0.0, 1 if eps is not None else 0,
+            _p(inv_perm), inv_perm.stride(0) if inv_perm is not None else 0, float(out_scale),
+            1 if accumulate else 0, flags, _p(w), wl,
+            ws.data_ptr(), ws.numel(), err_word(dev).data_ptr(), stream_of(dev)]
+        return "jscc_proj_counted", args, (8, 10)
     args = [
         mode, mat.data_ptr(), dtype_c, ld, P, _p(rows), n_rows, row_base, _p(row_scale), _p(x), _p(x_perm),
         x_perm.stride(0) if x_perm is not None else 0, xl, _p(y), y.stride(0) if y is not None else 0, _p(coef),

@@ -72,25 +72,40 @@ class ComptonRows:
     inv_sum: torch.Tensor
     device: torch.device
     vox: ops.VoxelSoA | None = None
+    # sync_counts=False builds: the selected and kept counts as device int32 until resolve_counts() reads them
+    # (n_sel, n_kept are -1 until then; store, stats, keep cover every input event)
+    counts_dev: tuple | None = None
+
+    def resolve_counts(self) -> "ComptonRows":
+        """Read deferred counts (one synchronisation, with the index check of the event pass)."""
+        if self.counts_dev is not None:
+            self.n_sel, self.n_kept = ops.read_counts_checked(*self.counts_dev)
+            self.counts_dev = None
+        return self
 
     def kept_arows(self) -> torch.Tensor:
+        self.resolve_counts()
         ev_i32 = self.events.view(torch.int32)
         src = self.sel_idx[self.kept[:self.n_kept].long()].long()
         return ev_i32[src, 14].contiguous()
 
     def kept_event_index(self) -> torch.Tensor:
         """Input positions (int64) of the kept events, in order."""
+        self.resolve_counts()
         ev_i32 = self.events.view(torch.int32)
         src = self.sel_idx[self.kept[:self.n_kept].long()].long()
         return ev_i32[src, 15].long()
 
     def normalized_t(self) -> torch.Tensor:
+        self.resolve_counts()
         return ops.gather_normalize(self.store, self.n_vox, self.kept, self.n_kept, self.stats, 0)
 
     def normalized_tc(self) -> torch.Tensor:
+        self.resolve_counts()
         return ops.gather_normalize(self.tc_store, self.n_vox, self.kept, self.n_kept, self.stats, 2)
 
     def normalized_single(self, sysmat_dev: torch.Tensor) -> torch.Tensor:
+        self.resolve_counts()
         return ops.arows_normalize(sysmat_dev, self.kept_arows(), self.n_kept)
 
 
@@ -127,7 +142,8 @@ def prepare_list(list_origin: torch.Tensor, dev: torch.device):
 def build_compton_rows(sysmat, detector, coor, list_origin, delta_r1, delta_r2, e0, ene_resolution,
                        ene_threshold_max, ene_threshold_min, ene_threshold_sum, device, *, noise="cpu",
                        sigma_model="strict", fixed_sigma=None, store_dtype=torch.float32, want_tc=False,
-                       taps=None, sysmat_dev=None, coor_dev=None, mutate_list=True) -> ComptonRows:
+                       taps=None, sysmat_dev=None, coor_dev=None, mutate_list=True,
+                       sync_counts=True) -> ComptonRows:
     """Cone rows of a list chunk on the GPU (see ComptonRows).
 
     noise: "cpu" (two torch.randn on the CPU generator, as the strict
@@ -135,7 +151,10 @@ def build_compton_rows(sysmat, detector, coor, list_origin, delta_r1, delta_r2, 
     N(0,1) tensors.  sigma_model "strict" or "fixed" (fixed_sigma in rad).
     sysmat may be None (first-hit factor 1).  mutate_list: write the smeared
     energies back into list_origin (the strict reference does, the sparse one
-    does not).
+    does not).  sync_counts=False: no host synchronisation -- the cone launch reads the selected count on
+    the device and covers every input event (rows past the count are dropped by the keep pass), and the
+    selected and kept counts stay on the device (ComptonRows.counts_dev) until resolve_counts(), which also
+    raises the index errors of the event pass.
     """
     dev = _device_of(device)
     if mutate_list:
@@ -158,10 +177,18 @@ def build_compton_rows(sysmat, detector, coor, list_origin, delta_r1, delta_r2, 
     if copy_back is not None and n1 is not None:
         copy_back()
     sel_idx, sel_count = ops.compact(flags)
-    n_sel = ops.read_count_checked(sel_count)  # one synchronisation for the count and the index check
     sig = ops.SIG_STRICT if sigma_model == "strict" else ops.SIG_FIXED
     if sig == ops.SIG_FIXED and not fixed_sigma:
         raise ValueError("sigma_model='fixed' needs fixed_sigma (rad)")
+    if not sync_counts and n > 0:
+        store, tc_store, stats, ld = ops.cone_rows(events, sel_idx, n, vox, sysmat=sysmat_dev, taps=taps,
+                                                   sigma_model=sig, fixed_sigma=fixed_sigma or 0.0,
+                                                   out_dtype=store_dtype, want_tc=want_tc, n_dev=sel_count)
+        keep, inv_sum = ops.rows_keep(stats, n, MIN_EVENT_EFFECTIVE_SUPPORT, want_inv_sum=True)
+        kept, kept_count = ops.compact(keep)
+        return ComptonRows(store, tc_store, ld, vox.n, stats, events, sel_idx, -1, kept, -1, inv_sum, dev, vox,
+                           (sel_count, kept_count))
+    n_sel = ops.read_count_checked(sel_count)  # one synchronisation for the count and the index check
     store, tc_store, stats, ld = ops.cone_rows(events, sel_idx, n_sel, vox, sysmat=sysmat_dev, taps=taps,
                                                sigma_model=sig, fixed_sigma=fixed_sigma or 0.0,
                                                out_dtype=store_dtype, want_tc=want_tc)
