@@ -271,11 +271,12 @@ int jscc_proj_em_counted(const void* mat, int dtype, int64_t ld, int64_t n_cols,
  * that cannot allocate device memory (code under HIP graph capture, tight
  * serving loops, harnesses that forbid allocation).  Subsets whose
  * jscc_proj_workspace_bytes(n_rows, n_cols) exceed the buffer are rejected
- * (above about 34,600 columns at 256 work-groups, so on a 256-CU GPU every
- * width the projection accepts, up to 32768 columns, fits).  All calls on one GPU share its buffer: calls that overlap
- * on one GPU (other streams, other threads) race.  The stream must belong to
- * the current device. */
-#define JSCC_NOALLOC_SCRATCH_BYTES (36LL << 20)
+ * (24 MiB: on a 256-CU GPU, whose full projection grid is 192 work-groups plus
+ * 12 second-stage rows, widths up to 30840 columns fit at any row count, and
+ * wider rows in subsets small enough for a smaller grid).  All calls on one
+ * GPU share its buffer: calls that overlap on one GPU (other streams, other
+ * threads) race.  The stream must belong to the current device. */
+#define JSCC_NOALLOC_SCRATCH_BYTES (24LL << 20)
 int jscc_proj_em_noalloc(const void* mat, int dtype, int64_t ld, int64_t n_cols, const int32_t* rows,
                          int64_t n_rows, int64_t row_base, const float* row_scale, const float* x,
                          const int64_t* x_perm, int64_t x_perm_stride, int64_t x_len, const float* y,

@@ -28,6 +28,19 @@ int jscc_num_cus() {
   return cached[dev];
 }
 
+int jscc_stream_cus(hipStream_t stream) {
+  const int dev_cus = jscc_num_cus();
+  if (stream == nullptr) return dev_cus;
+  uint32_t mask[16] = {0};  // up to 512 compute units
+  if (hipExtStreamGetCUMask(stream, 16, mask) != hipSuccess) {
+    (void)hipGetLastError();  // not a masked stream: the whole device
+    return dev_cus;
+  }
+  int n = 0;
+  for (int i = 0; i < 16; ++i) n += __builtin_popcount(mask[i]);
+  return (n > 0 && n < dev_cus) ? n : dev_cus;
+}
+
 // ---------------------------------------------------------------- kernel timer
 #include <vector>
 

@@ -157,6 +157,10 @@ __device__ __forceinline__ int jscc_wave_id() { return threadIdx.x >> 6; }
 
 // number of compute units of the current device (cached per process)
 int jscc_num_cus();
+// compute units a launch on `stream` can use: the popcount of its CU mask
+// (hipExtStreamCreateWithCUMask), else the device's count; persistent grids
+// (one work-group per CU, all resident) are sized by it
+int jscc_stream_cus(hipStream_t stream);
 
 // EM updates: 0 plain x*w/s (recon_osem_plane.py:54); 1 safe_em_update
 // (recon_osem_plane_sparse.py:60-66); 2 apply_em_update

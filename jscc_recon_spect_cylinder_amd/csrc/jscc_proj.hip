@@ -889,14 +889,18 @@ extern "C" int jscc_proj_max_cols(int dtype) {
   return nch * kProjThreads * vw;
 }
 
-// Work-groups of one projection launch: one per CU, each with at least
+// Work-groups of one projection launch: at most one per CU, each with at least
 // rows_min rows.  Below a full grid (small subsets, rows resident in the
 // Infinity Cache) a launch is latency-bound and fewer, longer work-groups
 // write fewer partials: 8 rows per group measured 10.4 vs 10.9 us per
 // 1024 x 4096 FP32 subset (FP8 rows, a quarter of the threads busy per row at
 // 4096 columns, prefer 4).
-static int proj_grid_rows(int64_t n_rows, int rows_min) {
-  const int cus = jscc_num_cus();
+static int proj_grid_rows(int64_t n_rows, int rows_min, hipStream_t s = nullptr) {
+  // at most 3/4 of the CUs: fewer concurrent row streams keep HBM more efficient -- 10 x 8 OSEM
+  // subsets over 164758 BF16 rows x 16384 took 9.00 ms at 192 work-groups against 9.47 ms at 256
+  // (FP32: 16.8 vs 18.0 ms), the column-sum pass 0.82 vs 0.86 ms
+  const int q = (3 * jscc_stream_cus(s)) / 4;
+  const int cus = q > 0 ? q : 1;
   int64_t g = (n_rows + rows_min - 1) / rows_min;
   if (g > cus) g = cus;
   if (g < 1) g = 1;
@@ -956,7 +960,7 @@ static int proj_impl(int mode, const void* mat, int dtype, int64_t ld, int64_t n
     return 0;
   }
   JSCC_REQUIRE(mat != nullptr && ((uintptr_t)mat % 16) == 0, "jscc_proj: matrix must be 16-byte aligned");
-  const int G = proj_grid_rows(n_total, dtype == JSCC_DT_FP8 ? 4 : 8);
+  const int G = proj_grid_rows(n_total, dtype == JSCC_DT_FP8 ? 4 : 8, s);
   const int G2 = (G + kRedRows - 1) / kRedRows;
   if (mode != 1) {
     JSCC_REQUIRE(workspace != nullptr && workspace_bytes >= jscc_proj_workspace_bytes(n_total, n_cols),
@@ -1304,7 +1308,7 @@ extern "C" int jscc_osem_persistent(const void* mat, int dtype, int64_t ld, int6
   OsemArgs o;
   o.ld = ld; o.P = (int)n_cols; o.n_sub = n_sub; o.n_iter = n_iter; o.eps = eps; o.use_eps = use_eps;
   o.em_mode = em_mode; o.em_eps = em_eps; o.spin_limit = 4000000;  // ~0.1-1 s of polling: far above any phase
-  const int G = jscc_num_cus();
+  const int G = jscc_stream_cus(s);  // every work-group resident on the CUs the stream may use
   int rc;
   jscc_prof_begin(JSCC_TAG_PROJ, s);
   if (dtype == JSCC_DT_F32) rc = dispatch_persistent<JSCC_DT_F32>(mat, rows, row_scale, x, sens, partial, sub_off, bar, err, o, G, s);

@@ -32,13 +32,12 @@ def _matrix(dtype, M, P, rng):
     return (base.to(torch.bfloat16) if dtype == "bf16" else base), scale
 
 
-@pytest.mark.parametrize("dtype,P", [("fp32", 16384), ("bf16", 16384), ("fp16", 25600), ("fp8", 4096),
-                                     ("bf16", 32768)])
-def test_noalloc_subset_step_equals_workspace_form(dtype, P):
+@pytest.mark.parametrize("dtype,P,M", [("fp32", 16384, 3000), ("bf16", 16384, 3000), ("fp16", 25600, 3000),
+                                       ("fp8", 4096, 3000), ("bf16", 32768, 600)])
+def test_noalloc_subset_step_equals_workspace_form(dtype, P, M):
     from jscc_recon_spect_cylinder_amd import _lib, ops
     lib = _lib.lib()
     rng = np.random.default_rng(P + len(dtype))
-    M = 3000
     mat, scale = _matrix(dtype, M, P, rng)
     dt = ops.dtype_code(dtype)
     rows = torch.from_numpy(rng.permutation(M).astype(np.int32)).to(DEV)
@@ -49,7 +48,8 @@ def test_noalloc_subset_step_equals_workspace_form(dtype, P):
     xa, wa = x0.clone(), torch.zeros(P, device=DEV)
     xb, wb = x0.clone(), torch.zeros(P, device=DEV)
     # two calls in a row, the second on a smaller subset
-    for lo, n in ((0, 2600), (2600, 400)):
+    first = M * 13 // 15
+    for lo, n in ((0, first), (first, M - first)):
         ops.proj(ops.MODE_RATIO, mat, dtype_c=dt, n_cols=P, rows=rows, row_base=lo, n_rows=n, row_scale=scale, x=xa,
                  y=y, y_by_row=True, eps=1e-12, accumulate=True, w=wa, em=(xa, s, ops.EM_PLAIN, 1e-12, True))
         rc = _noalloc(lib, mat, dt, P, rows, n, lo, scale, xb, y, wb, s, err)
@@ -94,12 +94,12 @@ def test_noalloc_rejects_subsets_beyond_its_buffer():
     from jscc_recon_spect_cylinder_amd import _lib, ops
     lib = _lib.lib()
     err = ops.err_word(torch.device(DEV))
-    need = lib.jscc_proj_workspace_bytes(100000, 40960)
-    assert need > (36 << 20)
+    need = lib.jscc_proj_workspace_bytes(100000, 32768)  # the widest BF16 projection, at a full grid
+    assert need > (24 << 20)
     stream = torch.cuda.current_stream(torch.device(DEV)).cuda_stream
     dummy = torch.zeros(16, device=DEV)
-    rc = lib.jscc_proj_em_noalloc(dummy.data_ptr(), 1, 40960, 40960, None, 100000, 0, None, dummy.data_ptr(), None, 0,
-                                  40960, None, 0, 1e-12, 1, None, 0, 1.0, 1, 0, dummy.data_ptr(), 40960,
+    rc = lib.jscc_proj_em_noalloc(dummy.data_ptr(), 1, 32768, 32768, None, 100000, 0, None, dummy.data_ptr(), None, 0,
+                                  32768, None, 0, 1e-12, 1, None, 0, 1.0, 1, 0, dummy.data_ptr(), 32768,
                                   dummy.data_ptr(), dummy.data_ptr(), 0, 1e-12, 1.0, None, 0.0, 1, err.data_ptr(),
                                   stream)
     assert rc != 0 and b"static buffer" in lib.jscc_last_error()
