@@ -826,7 +826,7 @@ def main():
         rel_err = float(np.linalg.norm(bf["img"] - f) / np.linalg.norm(f))
         results["fp16"]["rel_err_vs_fp32_image"] = float(np.linalg.norm(results["fp16"]["img"] - f) / np.linalg.norm(f))
     traffic = _traffic("bf16")
-    roof = {"kernel": "k_proj_fused<bf16,2,4,ratio>: one OSEM subset pass T_s^T(1/T_s x)", "bound": "hbm",
+    roof = {"kernel": "k_proj_fused<bf16,2,2,ratio>: one OSEM subset pass T_s^T(1/T_s x)", "bound": "hbm",
             "achieved": round(bf["osem_GBs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(bf["osem_GBs"] / HBM_PEAK_GBS, 4),
             "traffic": None if traffic is None else int(traffic),

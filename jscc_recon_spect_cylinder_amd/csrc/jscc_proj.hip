@@ -815,9 +815,15 @@ int launch_fused(int mode, const ProjArgs& a, int G, hipStream_t s) {
 
 // Rows per register buffer: two buffers of R rows x NCH 16-byte vectors take
 // at most 64 VGPRs next to the NCH x VN accumulators (128 per lane in all);
-// FP8 decodes 16 values per vector and gets half as many rows.
+// FP8 decodes 16 values per vector and gets half as many rows.  From two
+// vectors per thread on, a buffer holds 4 / NCH rows (64 KB in flight per
+// 1024-thread work-group): with more the memory system is oversubscribed --
+// 10 x 8 OSEM subsets over 164758 x 16384 took 8.76 ms with 2 BF16 rows per
+// buffer against 9.00 ms with 4, and 16.40 against 16.72 ms with 1 FP32 row
+// against 2 (192 work-groups); a streaming read peaks at 32-64 KB in flight
+// per CU.
 constexpr int rows_per_buf(int vn, int nch) {
-  return nch * vn >= 64 ? 1 : ((vn == 16 ? 4 : 8) / nch > 1 ? (vn == 16 ? 4 : 8) / nch : 1);
+  return nch * vn >= 64 ? 1 : (nch == 1 ? (vn == 16 ? 4 : 8) : (4 / nch > 1 ? 4 / nch : 1));
 }
 
 template <int DT>
