@@ -475,11 +475,19 @@ struct ConeProjArgs {
   float* partial;
 };
 
+// Rows of more than 16384 voxels (8 quads per thread) keep their values in the
+// thread's own LDS slots (NCH x BLOCK float4, dynamic) instead of registers:
+// held next to the accumulators they would spill.
+template <int BLOCK, int NCH>
+constexpr bool cp_row_in_lds() { return NCH >= 8; }
+
 template <int SIG, int AF, int BLOCK, int NCH, int MODE, bool DSIG>
 __global__ void __launch_bounds__(BLOCK) k_cone_proj(ConeProjArgs a) {
   constexpr int NW = BLOCK / JSCC_WAVE;
+  constexpr bool TRL = cp_row_in_lds<BLOCK, NCH>();
   const int tid = threadIdx.x;
-  float tr[NCH][4];
+  extern __shared__ float4 cp_rows[];
+  float tr[TRL ? 1 : NCH][4];
   float wr[NCH][4];
 #pragma unroll
   for (int c = 0; c < NCH; ++c)
@@ -495,7 +503,10 @@ __global__ void __launch_bounds__(BLOCK) k_cone_proj(ConeProjArgs a) {
     float S = 0.0f, D = 0.0f, dS = 0.0f, dD = 0.0f;
 #pragma unroll
     for (int c = 0; c < NCH; ++c) {
-      const int base = (c * BLOCK + tid) * 4;
+      int base = (c * BLOCK + tid) * 4;
+      // formed at use: the chunks' loads are not hoisted ahead of their arithmetic (which would hold
+      // every chunk's coordinates in registers at once)
+      asm volatile("" : "+v"(base));
       if (base < a.P) {
         const float4 vx4 = *reinterpret_cast<const float4*>(a.cx + base);
         const float4 vy4 = *reinterpret_cast<const float4*>(a.cy + base);
@@ -509,10 +520,11 @@ __global__ void __launch_bounds__(BLOCK) k_cone_proj(ConeProjArgs a) {
         afactor4<AF>(e, a.A, base, nullptr, nullptr, nullptr, nullptr, av);
         float tcv[4], dl[4];
         cone_tc4<SIG, DSIG>(e, vcx, vcy, vcz, a.cc, tcv, dl);
+        float tq[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
           const float t = tcv[k] * av[k];
-          tr[c][k] = t;
+          tq[k] = t;
           S += t;
           D = fmaf(t, xv[k], D);
           if (DSIG) {
@@ -521,9 +533,19 @@ __global__ void __launch_bounds__(BLOCK) k_cone_proj(ConeProjArgs a) {
             dD = fmaf(dt, xv[k], dD);
           }
         }
-      } else {
+        if constexpr (TRL) {
+          cp_rows[c * BLOCK + tid] = make_float4(tq[0], tq[1], tq[2], tq[3]);
+        } else {
 #pragma unroll
-        for (int k = 0; k < 4; ++k) tr[c][k] = 0.0f;
+          for (int k = 0; k < 4; ++k) tr[TRL ? 0 : c][k] = tq[k];
+        }
+      } else {
+        if constexpr (TRL) {
+          cp_rows[c * BLOCK + tid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        } else {
+#pragma unroll
+          for (int k = 0; k < 4; ++k) tr[TRL ? 0 : c][k] = 0.0f;
+        }
       }
     }
     // wave sums in FP32 by DPP (64 partials of <= 32 values each), across waves in FP64
@@ -565,9 +587,18 @@ __global__ void __launch_bounds__(BLOCK) k_cone_proj(ConeProjArgs a) {
         coef = (float)((double)a.coef[i] * inv);
       }
 #pragma unroll
-      for (int c = 0; c < NCH; ++c)
+      for (int c = 0; c < NCH; ++c) {
+        float tq[4];
+        if constexpr (TRL) {
+          const float4 v = cp_rows[c * BLOCK + tid];
+          tq[0] = v.x; tq[1] = v.y; tq[2] = v.z; tq[3] = v.w;
+        } else {
 #pragma unroll
-        for (int k = 0; k < 4; ++k) wr[c][k] = fmaf(coef, tr[c][k], wr[c][k]);
+          for (int k = 0; k < 4; ++k) tq[k] = tr[TRL ? 0 : c][k];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) wr[c][k] = fmaf(coef, tq[k], wr[c][k]);
+      }
     }
   }
   if (MODE != CP_FWD) {
@@ -581,28 +612,44 @@ __global__ void __launch_bounds__(BLOCK) k_cone_proj(ConeProjArgs a) {
   }
 }
 
+// dynamic LDS of a launch (the row values of cp_row_in_lds shapes), with the
+// attribute that allows more than 64 KB set once per kernel
+template <typename K>
+int lds_launch_attr(K* fn, size_t lds, bool* done) {
+  if (lds > 65536 && !*done) {
+    JSCC_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds));
+    *done = true;
+  }
+  return 0;
+}
+
+template <int SIG, int AF, int BLOCK, int NCH, int MODE, bool DSIG>
+int launch_cp_one(const ConeProjArgs& a, int G, hipStream_t s) {
+  constexpr size_t lds = cp_row_in_lds<BLOCK, NCH>() ? (size_t)NCH * BLOCK * sizeof(float4) : 0;
+  static bool attr = false;
+  const int rc = lds_launch_attr(&k_cone_proj<SIG, AF, BLOCK, NCH, MODE, DSIG>, lds, &attr);
+  if (rc) return rc;
+  hipLaunchKernelGGL((k_cone_proj<SIG, AF, BLOCK, NCH, MODE, DSIG>), dim3(G), dim3(BLOCK), lds, s, a);
+  return 0;
+}
+
 template <int SIG, int AF, int BLOCK, int NCH, bool DSIG>
-void launch_cp_mode(int mode, const ConeProjArgs& a, int G, hipStream_t s) {
-  if (mode == CP_RATIO)
-    hipLaunchKernelGGL((k_cone_proj<SIG, AF, BLOCK, NCH, CP_RATIO, DSIG>), dim3(G), dim3(BLOCK), 0, s, a);
-  else if (mode == CP_FWD)
-    hipLaunchKernelGGL((k_cone_proj<SIG, AF, BLOCK, NCH, CP_FWD, DSIG>), dim3(G), dim3(BLOCK), 0, s, a);
-  else
-    hipLaunchKernelGGL((k_cone_proj<SIG, AF, BLOCK, NCH, CP_COEF, DSIG>), dim3(G), dim3(BLOCK), 0, s, a);
+int launch_cp_mode(int mode, const ConeProjArgs& a, int G, hipStream_t s) {
+  if (mode == CP_RATIO) return launch_cp_one<SIG, AF, BLOCK, NCH, CP_RATIO, DSIG>(a, G, s);
+  if (mode == CP_FWD) return launch_cp_one<SIG, AF, BLOCK, NCH, CP_FWD, DSIG>(a, G, s);
+  return launch_cp_one<SIG, AF, BLOCK, NCH, CP_COEF, DSIG>(a, G, s);
 }
 
 template <int SIG, int AF, bool DSIG>
 int launch_cp_size(int mode, const ConeProjArgs& a, int G, hipStream_t s) {
-  // the row lives in registers: 4 * NCH voxels per thread
-  if (a.P <= 4 * 256 * 4) launch_cp_mode<SIG, AF, 256, 4, DSIG>(mode, a, G, s);
-  else if (a.P <= 4 * 512 * 4) launch_cp_mode<SIG, AF, 512, 4, DSIG>(mode, a, G, s);
-  else if (a.P <= 4 * 1024 * 4) launch_cp_mode<SIG, AF, 1024, 4, DSIG>(mode, a, G, s);
-  else if (a.P <= 4 * 1024 * 8) launch_cp_mode<SIG, AF, 1024, 8, DSIG>(mode, a, G, s);
-  else {
-    jscc_set_error("jscc_cone_proj: %d voxels exceed the register-resident row limit (32768)", a.P);
-    return -3;
-  }
-  return 0;
+  // the row lives in registers (4 * NCH voxels per thread), or in LDS above 16384 voxels
+  if (a.P <= 4 * 256 * 4) return launch_cp_mode<SIG, AF, 256, 4, DSIG>(mode, a, G, s);
+  if (a.P <= 4 * 512 * 4) return launch_cp_mode<SIG, AF, 512, 4, DSIG>(mode, a, G, s);
+  if (a.P <= 4 * 1024 * 4) return launch_cp_mode<SIG, AF, 1024, 4, DSIG>(mode, a, G, s);
+  if (a.P <= 4 * 1024 * 8) return launch_cp_mode<SIG, AF, 1024, 8, DSIG>(mode, a, G, s);
+  jscc_set_error("jscc_cone_proj: %d voxels exceed the row limit (32768)", a.P);
+  return -3;
 }
 
 }  // namespace
@@ -725,6 +772,12 @@ __device__ __forceinline__ int wg_of_row(int64_t n, int G, int64_t r) {
   return (int)(rem + (r - big) / per);
 }
 
+// Steps of 8 or more quads per thread (R rows x NCH quads at 1024 threads)
+// keep the row values in the thread's own LDS slots: in registers next to the
+// accumulators they would spill.
+template <int BLOCK, int NCH, int R>
+constexpr bool bp_rows_in_lds() { return BLOCK >= 1024 && NCH * R >= 8; }
+
 // R rows per work-group step: the rows are evaluated into registers, their
 // sums reduced by DPP within each wave, and one barrier serves all R rows;
 // lane q of every wave then finishes row q (cross-wave FP64 sum and
@@ -740,6 +793,8 @@ __global__ void __launch_bounds__(BLOCK)
   a.ev = ev; a.ev_idx = ev_idx; a.row_off = row_off; a.cx = cx; a.cy = cy; a.cz = cz; a.A = A; a.x = x;
   a.partial = partial;
   constexpr int NW = BLOCK / JSCC_WAVE;
+  constexpr bool TRL = bp_rows_in_lds<BLOCK, NCH, R>();
+  extern __shared__ float4 bp_rows[];  // TRL: slot (q * NCH + c) * BLOCK + tid
   const int tid = threadIdx.x;
   const int lane = jscc_lane();
   const int g = blockIdx.x;
@@ -766,7 +821,7 @@ __global__ void __launch_bounds__(BLOCK)
 #pragma unroll
       for (int k = 0; k < 4; ++k) wr[c][k] = 0.0f;
     for (; i < seg_end; i += R) {
-      float tr[R][NCH][4];
+      float tr[TRL ? 1 : R][TRL ? 1 : NCH][4];
       float S[R], D[R];
       ConeEv e[R];
 #pragma unroll
@@ -800,19 +855,31 @@ __global__ void __launch_bounds__(BLOCK)
             afactor4<AF>(e[q], a.A, base, nullptr, nullptr, nullptr, nullptr, av);
             float tcv[4];
             cone_tc4<SIG, false>(e[q], vcx, vcy, vcz, a.cc, tcv, nullptr);
+            float tq[4];
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
               const float t = tcv[k] * av[k];
-              tr[q][c][k] = t;
+              tq[k] = t;
               S[q] += t;
               if (MODE == BP_OSEM) D[q] = fmaf(t, xv[k], D[q]);
+            }
+            if constexpr (TRL) {
+              bp_rows[(q * NCH + c) * BLOCK + tid] = make_float4(tq[0], tq[1], tq[2], tq[3]);
+            } else {
+#pragma unroll
+              for (int k = 0; k < 4; ++k) tr[q][c][k] = tq[k];
             }
           }
         } else {
 #pragma unroll
-          for (int q = 0; q < R; ++q)
+          for (int q = 0; q < R; ++q) {
+            if constexpr (TRL) {
+              bp_rows[(q * NCH + c) * BLOCK + tid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            } else {
 #pragma unroll
-            for (int k = 0; k < 4; ++k) tr[q][c][k] = 0.0f;
+              for (int k = 0; k < 4; ++k) tr[q][c][k] = 0.0f;
+            }
+          }
         }
       }
 #pragma unroll
@@ -851,9 +918,18 @@ __global__ void __launch_bounds__(BLOCK)
       for (int q = 0; q < R; ++q) {
         const float coef = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cl), q));
 #pragma unroll
-        for (int c = 0; c < NCH; ++c)
+        for (int c = 0; c < NCH; ++c) {
+          float tq[4];
+          if constexpr (TRL) {
+            const float4 v = bp_rows[(q * NCH + c) * BLOCK + tid];
+            tq[0] = v.x; tq[1] = v.y; tq[2] = v.z; tq[3] = v.w;
+          } else {
 #pragma unroll
-          for (int k = 0; k < 4; ++k) wr[c][k] = fmaf(coef, tr[q][c][k], wr[c][k]);
+            for (int k = 0; k < 4; ++k) tq[k] = tr[q][c][k];
+          }
+#pragma unroll
+          for (int k = 0; k < 4; ++k) wr[c][k] = fmaf(coef, tq[k], wr[c][k]);
+        }
       }
     }
     float* dst = a.partial + (int64_t)(g + b) * a.P;
@@ -908,28 +984,32 @@ __global__ void __launch_bounds__(256) k_reduce_batched(const float* __restrict_
   }
 }
 
+template <int SIG, int AF, int BLOCK, int NCH, int R, int MODE>
+int launch_bp_one(const ConeBatchArgs& a, int G, hipStream_t s) {
+  constexpr size_t lds = bp_rows_in_lds<BLOCK, NCH, R>() ? (size_t)R * NCH * BLOCK * sizeof(float4) : 0;
+  static bool attr = false;
+  const int rc = lds_launch_attr(&k_cone_proj_batched<SIG, AF, BLOCK, NCH, R, MODE>, lds, &attr);
+  if (rc) return rc;
+  hipLaunchKernelGGL((k_cone_proj_batched<SIG, AF, BLOCK, NCH, R, MODE>), dim3(G), dim3(BLOCK), lds, s, a.ev,
+                     a.ev_idx, a.row_off, a.cx, a.cy, a.cz, a.A, a.x, a.partial, a);
+  return 0;
+}
+
 template <int SIG, int AF, int BLOCK, int NCH>
-void launch_bp_mode(int mode, const ConeBatchArgs& a, int G, hipStream_t s) {
-  constexpr int R = NCH <= 4 ? 2 : 1;  // rows per step: tr[R][NCH][4] next to the accumulators
-  if (mode == BP_OSEM)
-    hipLaunchKernelGGL((k_cone_proj_batched<SIG, AF, BLOCK, NCH, R, BP_OSEM>), dim3(G), dim3(BLOCK), 0, s, a.ev,
-                       a.ev_idx, a.row_off, a.cx, a.cy, a.cz, a.A, a.x, a.partial, a);
-  else
-    hipLaunchKernelGGL((k_cone_proj_batched<SIG, AF, BLOCK, NCH, R, BP_SUM>), dim3(G), dim3(BLOCK), 0, s, a.ev,
-                       a.ev_idx, a.row_off, a.cx, a.cy, a.cz, a.A, a.x, a.partial, a);
+int launch_bp_mode(int mode, const ConeBatchArgs& a, int G, hipStream_t s) {
+  constexpr int R = NCH <= 4 ? 2 : 1;  // rows per step
+  if (mode == BP_OSEM) return launch_bp_one<SIG, AF, BLOCK, NCH, R, BP_OSEM>(a, G, s);
+  return launch_bp_one<SIG, AF, BLOCK, NCH, R, BP_SUM>(a, G, s);
 }
 
 template <int SIG, int AF>
 int launch_bp_size(int mode, const ConeBatchArgs& a, int G, hipStream_t s) {
-  if (a.P <= 4 * 256 * 4) launch_bp_mode<SIG, AF, 256, 4>(mode, a, G, s);
-  else if (a.P <= 4 * 512 * 4) launch_bp_mode<SIG, AF, 512, 4>(mode, a, G, s);
-  else if (a.P <= 4 * 1024 * 4) launch_bp_mode<SIG, AF, 1024, 4>(mode, a, G, s);
-  else if (a.P <= 4 * 1024 * 8) launch_bp_mode<SIG, AF, 1024, 8>(mode, a, G, s);
-  else {
-    jscc_set_error("jscc_cone_proj_batched: %d voxels exceed the register-resident row limit (32768)", a.P);
-    return -3;
-  }
-  return 0;
+  if (a.P <= 4 * 256 * 4) return launch_bp_mode<SIG, AF, 256, 4>(mode, a, G, s);
+  if (a.P <= 4 * 512 * 4) return launch_bp_mode<SIG, AF, 512, 4>(mode, a, G, s);
+  if (a.P <= 4 * 1024 * 4) return launch_bp_mode<SIG, AF, 1024, 4>(mode, a, G, s);
+  if (a.P <= 4 * 1024 * 8) return launch_bp_mode<SIG, AF, 1024, 8>(mode, a, G, s);
+  jscc_set_error("jscc_cone_proj_batched: %d voxels exceed the row limit (32768)", a.P);
+  return -3;
 }
 
 }  // namespace

@@ -815,7 +815,8 @@ int launch_fused(int mode, const ProjArgs& a, int G, hipStream_t s) {
 
 // Rows per register buffer: two buffers of R rows x NCH 16-byte vectors take
 // at most 64 VGPRs next to the NCH x VN accumulators (128 per lane in all);
-// FP8 decodes 16 values per vector and gets half as many rows.  From two
+// FP8 decodes 16 values per vector and gets half as many rows (one from two
+// vectors per thread on: with two its decoded rows would spill).  From two
 // vectors per thread on, a buffer holds 4 / NCH rows (64 KB in flight per
 // 1024-thread work-group): with more the memory system is oversubscribed --
 // 10 x 8 OSEM subsets over 164758 x 16384 took 8.76 ms with 2 BF16 rows per
@@ -823,17 +824,24 @@ int launch_fused(int mode, const ProjArgs& a, int G, hipStream_t s) {
 // against 2 (192 work-groups); a streaming read peaks at 32-64 KB in flight
 // per CU.
 constexpr int rows_per_buf(int vn, int nch) {
-  return nch * vn >= 64 ? 1 : (nch == 1 ? (vn == 16 ? 4 : 8) : (4 / nch > 1 ? 4 / nch : 1));
+  return nch * vn >= 32 ? 1 : (nch == 1 ? (vn == 16 ? 4 : 8) : (4 / nch > 1 ? 4 / nch : 1));
 }
 
 template <int DT>
 int dispatch_nch(int mode, const ProjArgs& a, int G, hipStream_t s) {
   constexpr int VN = Vec<DT>::N;
+  // chunk counts up to the widest row jscc_proj_max_cols admits (32768 columns: 8 FP32, 4 BF16 / FP16 and
+  // 2 FP8 vectors per thread); no wider shape is instantiated
+  constexpr int kMaxNch = 32768 / VN / kProjThreads;
   const int nch = (a.P / VN + kProjThreads - 1) / kProjThreads;
   if (nch <= 1) return launch_fused<DT, 1, rows_per_buf(VN, 1)>(mode, a, G, s);
   if (nch <= 2) return launch_fused<DT, 2, rows_per_buf(VN, 2)>(mode, a, G, s);
-  if (nch <= 4) return launch_fused<DT, 4, rows_per_buf(VN, 4)>(mode, a, G, s);
-  if (DT != JSCC_DT_FP8 && nch <= 8) return launch_fused<DT, (DT == JSCC_DT_FP8 ? 4 : 8), 1>(mode, a, G, s);
+  if constexpr (kMaxNch >= 4) {
+    if (nch <= 4) return launch_fused<DT, 4, rows_per_buf(VN, 4)>(mode, a, G, s);
+  }
+  if constexpr (kMaxNch >= 8) {
+    if (nch <= 8) return launch_fused<DT, 8, 1>(mode, a, G, s);
+  }
   jscc_set_error("jscc_proj: %d columns exceed the register-resident limit for this dtype", a.P);
   return -3;
 }
@@ -871,7 +879,9 @@ int dispatch_persistent(const void* M, const int32_t* rows, const float* row_sca
       M, rows, row_scale, x, sens, partial, sub_off, bar, err, o, G, s)
   if (nch <= 1) JSCC_PERSIST(1);
   if (nch <= 2) JSCC_PERSIST(2);
-  if (nch <= 4) JSCC_PERSIST(4);
+  if constexpr (32768 / VN / kProjThreads >= 4) {
+    if (nch <= 4) JSCC_PERSIST(4);
+  }
 #undef JSCC_PERSIST
   jscc_set_error("jscc_osem_persistent: %d columns exceed the persistent kernel's limit for this dtype", o.P);
   return -3;
