@@ -505,8 +505,10 @@ def run_sparse(args):
                          "rows_per_crystal_run": round(float(np.mean(run_len)), 2),
                          "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(achieved / HBM_PEAK_GBS, 4),
-                         "traffic": _traffic("f32", os.path.join(HERE, "profiles",
-                                                                 "k_coarse_weight_sparse_2e5x25600_traffic.json")),
+                         # the committed counter summary covers the per-row form on 2e5 rows only
+                         "traffic": (_traffic("f32", os.path.join(HERE, "profiles",
+                                                                  "k_coarse_weight_sparse_2e5x25600_traffic.json"))
+                                     if n == 200_000 and not any(forms) else None),
                          "launch_us": round(launch_ms * 1e3, 2), "bytes_per_launch": int(nbytes),
                          "distinct_arows_per_subset": int(np.mean(distinct))},
             "rel_err_vs_fp64_oracle_sample": _rel(xg.cpu().numpy(), xref),

@@ -271,9 +271,10 @@ int jscc_proj_em_counted(const void* mat, int dtype, int64_t ld, int64_t n_cols,
  * that cannot allocate device memory (code under HIP graph capture, tight
  * serving loops, harnesses that forbid allocation).  Subsets whose
  * jscc_proj_workspace_bytes(n_rows, n_cols) exceed the buffer are rejected
- * (24 MiB: on a 256-CU GPU, whose full projection grid is 192 work-groups plus
- * 12 second-stage rows, widths up to 30840 columns fit at any row count, and
- * wider rows in subsets small enough for a smaller grid).  All calls on one
+ * (24 MiB: on a 256-CU GPU, whose full BF16 / FP16 / FP32 projection grid is
+ * 192 work-groups plus 12 second-stage rows, widths up to 30840 columns fit at
+ * any row count -- FP8 rows, whose grid is 256 + 16, up to 23130 -- and wider
+ * rows in subsets small enough for a smaller grid).  All calls on one
  * GPU share its buffer: calls that overlap on one GPU (other streams, other
  * threads) race.  The stream must belong to the current device. */
 #define JSCC_NOALLOC_SCRATCH_BYTES (24LL << 20)

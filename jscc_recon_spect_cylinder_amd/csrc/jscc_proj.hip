@@ -911,11 +911,9 @@ extern "C" int jscc_proj_max_cols(int dtype) {
 // write fewer partials: 8 rows per group measured 10.4 vs 10.9 us per
 // 1024 x 4096 FP32 subset (FP8 rows, a quarter of the threads busy per row at
 // 4096 columns, prefer 4).
-static int proj_grid_rows(int64_t n_rows, int rows_min, hipStream_t s = nullptr) {
-  // at most 3/4 of the CUs: fewer concurrent row streams keep HBM more efficient -- 10 x 8 OSEM
-  // subsets over 164758 BF16 rows x 16384 took 9.00 ms at 192 work-groups against 9.47 ms at 256
-  // (FP32: 16.8 vs 18.0 ms), the column-sum pass 0.82 vs 0.86 ms
-  const int q = (3 * jscc_stream_cus(s)) / 4;
+static int proj_grid_rows(int64_t n_rows, int rows_min, bool all_cus, hipStream_t s) {
+  const int all = jscc_stream_cus(s);
+  const int q = all_cus ? all : (3 * all) / 4;
   const int cus = q > 0 ? q : 1;
   int64_t g = (n_rows + rows_min - 1) / rows_min;
   if (g > cus) g = cus;
@@ -923,8 +921,24 @@ static int proj_grid_rows(int64_t n_rows, int rows_min, hipStream_t s = nullptr)
   return (int)g;
 }
 
+// The grid of a launch over rows of `dtype`.  BF16 / FP16 / FP32 rows take at
+// most 3/4 of the CUs: fewer concurrent row streams keep HBM more efficient --
+// 10 x 8 OSEM subsets over 164758 BF16 rows x 16384 took 9.00 ms at 192
+// work-groups against 9.47 ms at 256 (FP32: 16.8 vs 18.0 ms), the column-sum
+// pass 0.82 vs 0.86 ms.  FP8 rows (half the bytes in flight per row) keep every
+// CU: the JSCCSD FP8 iteration took 3.07 ms at 192 against 2.52 ms at 256.
+static int proj_grid_for(int dtype, int64_t n_rows, hipStream_t s) {
+  const bool fp8 = dtype == JSCC_DT_FP8;
+  return proj_grid_rows(n_rows, fp8 ? 4 : 8, fp8, s);
+}
+
+// workspace bytes a launch of `dtype` over n_rows rows on stream s needs
+static int64_t proj_need(int dtype, int64_t n_rows, int64_t P, hipStream_t s) {
+  return jscc_partials_bytes(proj_grid_for(dtype, n_rows, s), P);
+}
+
 extern "C" int jscc_proj_grid(int64_t n_rows) {
-  return proj_grid_rows(n_rows, 4);  // the most work-groups any dtype's launch uses
+  return proj_grid_rows(n_rows, 4, true, nullptr);  // the most work-groups any dtype's launch uses
 }
 
 extern "C" int64_t jscc_proj_workspace_bytes(int64_t n_rows, int64_t P) {
@@ -976,10 +990,10 @@ static int proj_impl(int mode, const void* mat, int dtype, int64_t ld, int64_t n
     return 0;
   }
   JSCC_REQUIRE(mat != nullptr && ((uintptr_t)mat % 16) == 0, "jscc_proj: matrix must be 16-byte aligned");
-  const int G = proj_grid_rows(n_total, dtype == JSCC_DT_FP8 ? 4 : 8, s);
+  const int G = proj_grid_for(dtype, n_total, s);
   const int G2 = (G + kRedRows - 1) / kRedRows;
   if (mode != 1) {
-    JSCC_REQUIRE(workspace != nullptr && workspace_bytes >= jscc_proj_workspace_bytes(n_total, n_cols),
+    JSCC_REQUIRE(workspace != nullptr && workspace_bytes >= proj_need(dtype, n_total, n_cols, s),
                  "jscc_proj: workspace too small");
   }
   ProjArgs a;
@@ -1140,7 +1154,10 @@ extern "C" int jscc_proj_em_noalloc(const void* mat, int dtype, int64_t ld, int6
                                     float em_c1, float* em_w2, float em_c2, int zero_w, int32_t* err,
                                     void* stream) {
   JSCC_REQUIRE(n_rows > 0 && n_cols > 0, "jscc_proj_em_noalloc: empty subset or no columns");
-  const int64_t need = jscc_proj_workspace_bytes(n_rows, n_cols);
+  // the bytes this launch uses (jscc_proj_workspace_bytes bounds every dtype and stream from above)
+  const int vw = jscc_proj_vec_width(dtype);
+  JSCC_REQUIRE(vw > 0, "jscc_proj_em_noalloc: bad dtype %d", dtype);
+  const int64_t need = proj_need(dtype, n_rows, n_cols, (hipStream_t)stream);
   JSCC_REQUIRE(need <= (int64_t)JSCC_NOALLOC_SCRATCH_BYTES,
                "jscc_proj_em_noalloc: %lld rows x %lld columns need %lld bytes of scratch, above the %lld bytes "
                "of the library's static buffer (use jscc_proj_em with a workspace)",
