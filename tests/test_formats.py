@@ -1,4 +1,6 @@
-"""On-disk formats (CPU): the reference's file layouts round-trip, and the T-shard subset plan."""
+"""On-disk formats (CPU): the readers on files the reference itself holds (Factors/511keV_RotateNum20:
+Detector.csv, coor_polar.csv, RotMat.csv, RotMatInv.csv, Sensi_d, copied as fixtures), the reference's file
+layouts round-trip, and the T-shard subset plan."""
 import json
 import os
 
@@ -6,6 +8,39 @@ import numpy as np
 import torch
 
 from jscc_recon_spect_cylinder_amd import formats
+from tests.conftest import GOLDEN
+
+REF_FILES = os.path.join(GOLDEN, "reference_files", "511keV_RotateNum20")
+FACTORS = os.path.join(GOLDEN, "factors_511keV_RotateNum20.npz")
+
+
+def test_readers_on_the_reference_factor_files():
+    """Detector.csv (main_plane.py:174-196 reads columns 1..3), coor_polar.csv, RotMat.csv / RotMatInv.csv
+    (1-based, one column per rotation) and Sensi_d (raw FP32, one value per pixel) as the reference stores them;
+    the expected arrays come from the factors fixture (written from the same files with pandas / numpy)."""
+    g = np.load(FACTORS)
+    det = formats.read_detector_csv(os.path.join(REF_FILES, "Detector.csv"))
+    assert det.dtype == torch.float32 and det.shape == (10496, 3)
+    np.testing.assert_array_equal(det.numpy(), g["detector"])
+    coor = formats.read_coor_polar_csv(os.path.join(REF_FILES, "coor_polar.csv"))
+    assert coor.shape == (1280, 2)
+    np.testing.assert_array_equal(coor.numpy(), g["coor_layer0"][:, :2].astype(np.float32))
+    rot = formats.read_rotmat_csv(os.path.join(REF_FILES, "RotMat.csv"))
+    inv = formats.read_rotmat_csv(os.path.join(REF_FILES, "RotMatInv.csv"))
+    assert rot.dtype == torch.int64 and rot.shape == (1280, 20)
+    np.testing.assert_array_equal(rot.numpy(), g["rot_layer0"].astype(np.int64))
+    # RotMatInv undoes RotMat rotation by rotation: rot[inv[p, r] - 1, r] == p + 1
+    r = rot.numpy()
+    v = inv.numpy()
+    for k in range(20):
+        np.testing.assert_array_equal(r[v[:, k] - 1, k], np.arange(1, 1281))
+    s = formats.read_sensi(os.path.join(REF_FILES, "Sensi_d"), 25600)
+    assert s.shape == (25600, 1) and s.dtype == torch.float32
+    raw = np.fromfile(os.path.join(REF_FILES, "Sensi_d"), dtype="<f4")
+    np.testing.assert_array_equal(s.numpy()[:, 0], raw)
+    assert np.isfinite(raw).all() and (raw > 0).all()
+    s2 = formats.read_sensi(os.path.join(REF_FILES, "Sensi_d"), 25600, intensity=2.0)
+    np.testing.assert_array_equal(s2.numpy(), 2 * s.numpy())
 
 
 def test_sysmat_polar_is_pixel_major_and_transposed_on_load(tmp_path):
