@@ -796,6 +796,17 @@ def main():
         st.osem(xi, s, args.iters, args.subsets)
         torch.cuda.synchronize()
         osem_s = (time.perf_counter() - t2) / args.iters
+        # the same iterations replayed from one captured iteration (ListModeStore.capture)
+        xg = torch.ones_like(s)
+        gi = st.capture(xg, s, args.subsets)
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        for _ in range(args.iters):
+            gi.replay()
+        torch.cuda.synchronize()
+        osem_graph_s = (time.perf_counter() - t3) / args.iters
+        graph_equal = bool(torch.equal(xg, xi))
+        del gi, xg
         b = {"bf16": 2, "fp32": 4, "fp16": 2}[name]
         # algorithmic bytes of the OSEM launches of one step: every kept row read once per iteration
         # (+ the image read and the weight written per subset), divided over the subset launches
@@ -818,6 +829,8 @@ def main():
             "colsum_ms": stot / max(scnt, 1),
             "colsum_GBs": st.n_kept * P * b / (stot / max(scnt, 1) * 1e-3) / 1e9,
             "osem_iter_s": 1.0 / osem_s,
+            "osem_graph_iter_s": 1.0 / osem_graph_s,
+            "osem_graph_equals_eager": graph_equal,
         }
         del st, x
 

@@ -131,3 +131,33 @@ def test_quantised_store_from_deferred_rows_reads_the_counts():
     assert a.n_kept == b.n_kept and b.count_dev is None
     sa, sb = a.backproject_ones(), b.backproject_ones()
     assert torch.equal(sa, sb)
+
+
+@pytest.mark.parametrize("sync", [True, False], ids=["host_counts", "device_counts"])
+def test_list_mode_graph_replays_equal_eager_iterations(sync):
+    """One captured list-mode iteration replayed 3 times, with eager work and synchronisation in between,
+    reproduces each eager iteration exactly -- with the kept count read on the host, and left on the device
+    (the captured launches take their subset bounds from it)."""
+    from jscc_recon_spect_cylinder_amd import synth
+    from jscc_recon_spect_cylinder_amd.engine import ListModeStore
+    cl = synth.compton_list(n_events=6000, n_rings=32, n_angles=128, seed=23)
+    st = ListModeStore.from_rows(_build(cl, torch.bfloat16, sync), "bf16")
+    assert (st.count_dev is None) == sync
+    s = st.backproject_ones()
+    xe = torch.ones(st.n_vox, dtype=torch.float32, device=DEV)
+    eager = []
+    for _ in range(3):
+        st.osem(xe, s, 1, 8)
+        eager.append(xe.clone())
+    xg = torch.ones(st.n_vox, dtype=torch.float32, device=DEV)
+    gi = st.capture(xg, s, 8)
+    for it in range(3):
+        gi.replay()
+        torch.cuda.synchronize()
+        junk = torch.randn(1 << 16, device=DEV).sum()  # eager work between replays
+        torch.cuda.synchronize()
+        assert torch.equal(xg, eager[it]), f"replay {it}"
+        del junk
+    x3 = torch.ones(st.n_vox, dtype=torch.float32, device=DEV)
+    st.osem(x3, s, 3, 8, graph=True)
+    assert torch.equal(x3, eager[2])
