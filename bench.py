@@ -500,14 +500,14 @@ def run_sparse(args):
                                    "subsets"},
             "iter_s": round(n_iter / dt, 3),
             "roofline": {"kernel": ("k_coarse_grouped<25, normalised>" if all(forms) else
-                                    "k_coarse_weight<25, identity layer, uniform z, normalised>") + ": one subset",
+                                    "k_coarse_pf<13, normalised> (prefetching per-row form)") + ": one subset",
                          "form": "grouped" if all(forms) else ("per-row" if not any(forms) else "mixed"),
                          "rows_per_crystal_run": round(float(np.mean(run_len)), 2),
                          "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(achieved / HBM_PEAK_GBS, 4),
                          # the committed counter summary covers the per-row form on 2e5 rows only
                          "traffic": (_traffic("f32", os.path.join(HERE, "profiles",
-                                                                  "k_coarse_weight_sparse_2e5x25600_traffic.json"))
+                                                                  "k_coarse_pf_sparse_2e5x25600_traffic.json"))
                                      if n == 200_000 and not any(forms) else None),
                          "launch_us": round(launch_ms * 1e3, 2), "bytes_per_launch": int(nbytes),
                          "distinct_arows_per_subset": int(np.mean(distinct))},

@@ -41,6 +41,7 @@ constexpr int kMaxCoarse = kRowSlots * kCoarseThreads;
 constexpr int kBufWord3 = 0x00020000;
 typedef __attribute__((address_space(3))) void* lds_vp;
 typedef float jscc_f4v __attribute__((ext_vector_type(4)));
+typedef int jscc_i4v __attribute__((ext_vector_type(4)));
 
 struct AxisTap {
   int i0, i1;    // coarse indices (z: already multiplied by the coarse layer count)
@@ -350,6 +351,311 @@ __global__ void __launch_bounds__(kCoarseThreads)
     for (int k = 0; k < K; ++k) {
       const int j = jb + k * kCoarseThreads;
       if (j < P) out[j] = wr[k];
+    }
+  }
+}
+
+// ============================================================================
+// Prefetching per-row form (identity layer map, 256 | Lf, P <= 2048 * K4): the
+// same weight as k_coarse_weight, arranged so that no row waits for memory it
+// issues itself and every memory access moves 16 bytes per lane.  While row i
+// is consumed, the copy of row i+1 into LDS and -- when row i+1 has another
+// first-hit crystal -- the A row of that crystal into registers are in
+// flight; both are waited for at the row's barrier.  They are issued a pixel
+// step at a time inside the first pass over row i, so no wave stalls on a
+// full queue of loads before it computes.  Rows of one crystal in a row share
+// its A values (no reload); the crystal numbers of the work-group's rows sit
+// in an LDS table (read without a memory wait).
+//
+// Thread t owns the pixel quads q = t + 512 k (pixels 4q .. 4q+3): image
+// values, weight accumulators, the row's products and the A values of the
+// next row in registers (16 K4 of its 256).  A wave's 256 pixels of one step
+// share a fine z slice (256 | Lf), so their two z taps are wave-uniform: one
+// broadcast LDS read per step from a per-wave tap table; the coarse values of
+// a quad are two 16-byte LDS reads.  Memory -> LDS copies of 16 bytes per lane read the packed rows at
+// any 4-byte alignment (the buffer range zeroes the words past the row), A
+// rows are 16-byte buffer loads whose range is the row.
+// ============================================================================
+constexpr int kPfThreads = 512;
+constexpr int kPfWaves = kPfThreads / JSCC_WAVE;
+constexpr int kPfChunk = 4 * JSCC_WAVE;  // coarse words of one wave's 16-byte copy
+constexpr int kPfMaxK4 = 13;             // pixel quads per thread: P <= 26624
+
+// LDS words of the prefetching form: two row buffers of pc_pad words (a multiple of kPfChunk), the wave
+// sums [3 buffers][3 sums][waves], the tap table [waves][K4] of int4, then the crystal table of the
+// work-group's rows
+__host__ __device__ constexpr int pf_pc_pad(int Pc) { return (Pc + kPfChunk - 1) / kPfChunk * kPfChunk; }
+__host__ __device__ constexpr int pf_red_words() { return 3 * 3 * kPfWaves; }
+// 16-byte copies each wave issues per row: the coarse row holds at most pf_copies(K4) * 2048 values
+__host__ __device__ constexpr int pf_copies(int K4) { return (K4 + 1) / 2; }
+
+template <int K4, bool NORM>
+__global__ void __launch_bounds__(kPfThreads)
+    k_coarse_pf(const float* __restrict__ packed, int64_t ldp, int64_t n_rows, int Pc,
+                const int32_t* __restrict__ z_idx, const float* __restrict__ z_w, int Zf, int Zc, int Lf,
+                const float* __restrict__ A, int64_t lda, int64_t M, const float* __restrict__ x,
+                const int64_t* __restrict__ x_perm, int64_t xps, int64_t x_len, float eps, int use_eps,
+                float* __restrict__ partial, int32_t* __restrict__ err, const int32_t* __restrict__ row_list) {
+  extern __shared__ float lds[];
+  const int tid = threadIdx.x, lane = jscc_lane(), wv = jscc_wave_id();
+  const int P = Zf * Lf;
+  const int pc_pad = pf_pc_pad(Pc);
+  float* rowbuf = lds;                                                     // [2][pc_pad]
+  float* red = lds + 2 * pc_pad;                                           // [3][3][kPfWaves]
+  int4* taps = reinterpret_cast<int4*>(red + pf_red_words());             // [kPfWaves][K4]
+  int32_t* cps = reinterpret_cast<int32_t*>(taps + kPfWaves * K4);       // [rows of this group]
+
+  // ---- this work-group's rows and their first-hit crystals clamp(round(cpnum1) - 1, 0, M - 1)
+  // (compton_sparse_ops.py:390-391)
+  const int G = gridDim.x, g = blockIdx.x;
+  const int64_t per = n_rows / G, rem = n_rows % G;
+  const int64_t rb = g * per + (g < rem ? g : rem);
+  const int nr = (int)(per + (g < rem ? 1 : 0));
+  auto row_id = [&](int i) -> int64_t { return row_list ? (int64_t)row_list[rb + i] : rb + i; };
+  for (int i = tid; i < nr; i += kPfThreads) {
+    const float cpf = rintf(packed[row_id(i) * ldp]) - 1.0f;
+    int cp = 0;
+    if (cpf >= (float)(M - 1)) cp = (int)(M - 1);
+    else if (cpf > 0.0f) cp = (int)cpf;
+    cps[i] = cp;
+  }
+
+  // ---- z tap table (indices checked once; a bad table sets bit 64 and reads a clamped index)
+  {
+    int bad_tab = 0;
+    for (int f = tid; f < Zf; f += kPfThreads) {
+      const int a0 = z_idx[2 * f], a1 = z_idx[2 * f + 1];
+      if (a0 < 0 || a0 >= Zc || a1 < 0 || a1 >= Zc) bad_tab = 1;
+    }
+    if (bad_tab) atomicOr(err, 64);
+  }
+  // the taps of pixel step k of wave w, taps[w * K4 + k] = (coarse word offsets of both taps, their weights)
+  for (int e = tid; e < kPfWaves * K4; e += kPfThreads) {
+    const int w = e / K4, k = e - w * K4;
+    const int j0 = 4 * (w * JSCC_WAVE + k * kPfThreads);
+    int zf = j0 / Lf;
+    const int lfw = j0 - zf * Lf;  // before the clamp: a step past P still reads inside the row
+    if (zf >= Zf) zf = Zf - 1;
+    int a0 = z_idx[2 * zf], a1 = z_idx[2 * zf + 1];
+    a0 = a0 < 0 ? 0 : (a0 >= Zc ? Zc - 1 : a0);
+    a1 = a1 < 0 ? 0 : (a1 >= Zc ? Zc - 1 : a1);
+    taps[e] = make_int4((a0 * Lf + lfw) >> 2, (a1 * Lf + lfw) >> 2, __float_as_int(z_w[2 * zf]),
+                        __float_as_int(z_w[2 * zf + 1]));
+  }
+  const int wvu = __builtin_amdgcn_readfirstlane(wv);
+
+  // ---- image (rotation gather) and accumulators
+  float xr[4 * K4], wr[4 * K4], t[4 * K4], an[4 * K4];
+  {
+    int bad = 0;
+#pragma unroll
+    for (int k = 0; k < 4 * K4; ++k) {
+      const int j = 4 * (tid + (k >> 2) * kPfThreads) + (k & 3);
+      float v = 0.0f;
+      if (j < P) {
+        int64_t src = j;
+        if (x_perm) {
+          const int64_t pv = x_perm[(int64_t)j * xps];
+          if (pv < 1 || pv > x_len) { bad = 1; src = 0; } else { src = pv - 1; }
+        }
+        v = x[src];
+      }
+      xr[k] = v;
+      wr[k] = 0.0f;
+    }
+    if (bad) atomicOr(err, 2);
+  }
+
+  // A values of crystal cp for pixel step k into an: one 16-byte buffer load whose range is the row's P
+  // values (quads past P read zero)
+  auto a_rsrc = [&](int cp) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(A + (int64_t)cp * lda), (short)0, P * 4, kBufWord3);
+  };
+  // (the step's offset is the scalar offset, so all steps share one lane-offset register)
+  const int lane16 = tid * 16;
+  auto load_a = [&](__amdgpu_buffer_rsrc_t ars, int k) {
+    const jscc_f4v v =
+        __builtin_bit_cast(jscc_f4v, __builtin_amdgcn_raw_buffer_load_b128(ars, lane16, k * kPfThreads * 16, 0));
+    an[4 * k] = v.x;
+    an[4 * k + 1] = v.y;
+    an[4 * k + 2] = v.z;
+    an[4 * k + 3] = v.w;
+  };
+  // chunk ch (256 words) of coarse row i into LDS buffer b: a 16-byte buffer-to-LDS copy whose range is the
+  // row's Pc values (the padding up to pc_pad reads zero).  Issued by inline asm: the compiler would wait
+  // for every such copy before the next LDS read (it cannot tell the buffers apart); the row's barrier
+  // waits for them instead.  The asm has no register outputs.
+  const int n_chunks = pc_pad / kPfChunk;
+  auto r_rsrc = [&](int i) {
+    const uint64_t a = reinterpret_cast<uint64_t>(packed + row_id(i) * ldp + 1);
+    jscc_i4v r;
+    r.x = (int)(uint32_t)a;
+    r.y = (int)((uint32_t)(a >> 32) & 0xffffu);  // stride 0
+    r.z = Pc * 4;                                // range in bytes
+    r.w = kBufWord3;
+    return r;
+  };
+  auto copy_chunk = [&](const jscc_i4v& rrs, int b, int ch) {
+    const uint32_t la = (uint32_t)(uintptr_t)(lds_vp)(rowbuf + b * pc_pad + ch * kPfChunk);
+    asm volatile("buffer_load_dwordx4 %0, %1, %2 offen lds" ::"v"(lane * 16), "s"(rrs), "s"(ch * kPfChunk * 4),
+                 "{m0}"(la)
+                 : "memory");
+  };
+
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  __syncthreads();  // the crystal table
+  if (nr > 0) {
+    const __amdgpu_buffer_rsrc_t ars = a_rsrc(cps[0]);
+    const jscc_i4v rrs = r_rsrc(0);
+#pragma unroll
+    for (int k = 0; k < K4; ++k) load_a(ars, k);
+    for (int ch = wvu; ch < n_chunks; ch += kPfWaves) copy_chunk(rrs, 0, ch);
+  }
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  __syncthreads();
+
+  for (int i = 0; i < nr; ++i) {
+    const int b = i & 1;
+    // this row's A values become its products in place; the next row's A values go into an (a repeated
+    // crystal keeps them there) and its coarse values into the other LDS buffer, a chunk per step of pass 1,
+    // so that no wave stalls on a queue of copies before it computes
+    const bool next = i + 1 < nr;  // the last row copies itself into the other buffer (no branch per step)
+    const int cp = __builtin_amdgcn_readfirstlane(cps[i]);
+    const int cpn = next ? __builtin_amdgcn_readfirstlane(cps[i + 1]) : cp;
+    const bool need_a = cpn != cp;
+    const jscc_i4v rrs = r_rsrc(next ? i + 1 : i);
+#pragma unroll
+    for (int k = 0; k < 4 * K4; ++k) t[k] = an[k];
+    if (need_a) {
+      const __amdgpu_buffer_rsrc_t ars = a_rsrc(cpn);
+#pragma unroll
+      for (int k = 0; k < K4; ++k) load_a(ars, k);
+    }
+    const float* c = rowbuf + b * pc_pad;
+    float* rd = red + b * 3 * kPfWaves;  // wave sums, double-buffered
+
+    // a non-finite coarse value anywhere in the row (0 * v is NaN for v = +-inf or NaN; the padding is zero)
+    float zsum = 0.0f;
+#pragma unroll 1
+    for (int q = tid; q < pc_pad / 4; q += kPfThreads) {
+      const jscc_f4v v = reinterpret_cast<const jscc_f4v*>(c)[q];
+      zsum = fmaf(v.x, 0.0f, zsum);
+      zsum = fmaf(v.y, 0.0f, zsum);
+      zsum = fmaf(v.z, 0.0f, zsum);
+      zsum = fmaf(v.w, 0.0f, zsum);
+    }
+    const int bad = zsum != 0.0f;
+
+    // pass 1: products (expanded coarse values times the A row, rounded as stored), their row sum and the
+    // forward projection
+    float S = 0.0f, D = 0.0f;
+    const jscc_f4v* c4 = reinterpret_cast<const jscc_f4v*>(c) + lane;  // this lane's quads (offsets are multiples of 4)
+#pragma unroll
+      for (int k = 0; k < K4; ++k) {
+#pragma clang fp contract(off)
+        const int4 tp = taps[wvu * K4 + k];  // one broadcast read
+        const jscc_f4v c0 = c4[tp.x];
+        const jscc_f4v c1 = c4[tp.y];
+        const float w0 = __int_as_float(tp.z), w1 = __int_as_float(tp.w);
+        const float cv0[4] = {c0.x, c0.y, c0.z, c0.w}, cv1[4] = {c1.x, c1.y, c1.z, c1.w};
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          float u = w0 * cv0[e];
+          u = fmaf(w1, cv1[e], u);
+          const float v = u * t[4 * k + e];
+          t[4 * k + e] = v;
+          S += v;
+          D = fmaf(v, xr[4 * k + e], D);
+        }
+        // one copy per step of the first pf_copies(K4) steps, without a branch: the chunk index is clamped to
+        // the row's last chunk (a repeated copy writes the same values again)
+        if (k < pf_copies(K4)) copy_chunk(rrs, b ^ 1, min(wvu + k * kPfWaves, n_chunks - 1));
+      }
+    S = jscc_wave_sum_dpp(S);
+    D = jscc_wave_sum_dpp(D);
+    const int wbad = __builtin_amdgcn_ballot_w64(bad != 0) != 0;
+    if (lane == 0) {
+      rd[wv] = S;
+      rd[kPfWaves + wv] = D;
+      rd[2 * kPfWaves + wv] = wbad ? 1.0f : 0.0f;
+    }
+    // one barrier per row: after it the wave sums are complete, the next row's copy and A values have landed
+    // (every wave waited for its own loads) and no wave reads this row's buffer any more
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    float St = 0.0f, Dt = 0.0f, Bt = 0.0f;
+    auto block_sums = [&](const float* rs) {
+      St = 0.0f; Dt = 0.0f; Bt = 0.0f;
+#pragma unroll
+      for (int q = 0; q < kPfWaves; ++q) {
+        St += rs[q];
+        Dt += rs[kPfWaves + q];
+        Bt += rs[2 * kPfWaves + q];
+      }
+    };
+    block_sums(rd);
+    if (NORM && Bt == 0.0f && !finite_f(St)) {
+      // some product is not finite (or the sum overflowed): nan_to_num the products and sum again (the
+      // same decision in every thread); the third sums buffer is not the next row's
+      float* rf = red + 2 * 3 * kPfWaves;
+      S = 0.0f;
+      D = 0.0f;
+#pragma unroll
+      for (int k = 0; k < 4 * K4; ++k) {
+        const float v = finite_f(t[k]) ? t[k] : 0.0f;
+        t[k] = v;
+        S += v;
+        D = fmaf(v, xr[k], D);
+      }
+      S = jscc_wave_sum_dpp(S);
+      D = jscc_wave_sum_dpp(D);
+      if (lane == 0) {
+        rf[wv] = S;
+        rf[kPfWaves + wv] = D;
+        rf[2 * kPfWaves + wv] = 0.0f;
+      }
+      __syncthreads();
+      block_sums(rf);
+    }
+
+    float cf;
+    bool use = true;
+    if (NORM) {
+      if (Bt != 0.0f || !finite_f(St) || !(St > 0.0f)) {
+        use = false;
+        cf = 0.0f;
+      } else {
+        float den = Dt / St;
+        if (use_eps && den < eps) den = eps;
+        cf = (1.0f / den) / St;
+      }
+    } else if (Bt != 0.0f) {
+      cf = __builtin_nanf("");
+    } else {
+      float den = Dt;
+      if (use_eps && den < eps) den = eps;
+      cf = 1.0f / den;
+    }
+    // pass 2: w += t * cf (an invalid normalised row contributes nothing)
+    if (use) {
+#pragma unroll
+      for (int k = 0; k < 4 * K4; ++k) wr[k] = fmaf(t[k], cf, wr[k]);
+    }
+  }
+
+  float* out = partial + (int64_t)g * P;
+  int jb = 4 * tid;
+  asm volatile("" : "+v"(jb));
+#pragma unroll
+  for (int k = 0; k < K4; ++k) {
+    const int j = jb + k * 4 * kPfThreads;
+    if (j < P) {
+      jscc_f4v v;
+      v.x = wr[4 * k];
+      v.y = wr[4 * k + 1];
+      v.z = wr[4 * k + 2];
+      v.w = wr[4 * k + 3];
+      *reinterpret_cast<jscc_f4v*>(out + j) = v;
     }
   }
 }
@@ -756,6 +1062,27 @@ int launch_grouped(size_t lds, int G, hipStream_t s, const float* packed, int64_
   return 0;
 }
 
+// LDS bytes of the prefetching form with a crystal table of `rows` rows
+size_t pf_lds(int Pc, int64_t rows) {
+  return ((size_t)2 * pf_pc_pad(Pc) + pf_red_words() + (size_t)4 * kPfWaves * kPfMaxK4 + (size_t)rows) * 4;
+}
+
+template <int K4, bool NORM>
+int launch_pf(size_t lds, int G, hipStream_t s, const float* packed, int64_t ldp, int64_t n_rows, int Pc,
+              const int32_t* z_idx, const float* z_w, int Zf, int Zc, int Lf, const float* A, int64_t lda, int64_t M,
+              const float* x, const int64_t* x_perm, int64_t xps, int64_t x_len, float eps, int use_eps,
+              float* partial, int32_t* err, const int32_t* row_list) {
+  static bool attr_set = false;
+  if (!attr_set) {
+    JSCC_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_coarse_pf<K4, NORM>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    attr_set = true;
+  }
+  hipLaunchKernelGGL((k_coarse_pf<K4, NORM>), dim3(G), dim3(kPfThreads), lds, s, packed, ldp, n_rows, Pc, z_idx, z_w,
+                     Zf, Zc, Lf, A, lda, M, x, x_perm, xps, x_len, eps, use_eps, partial, err, row_list);
+  return 0;
+}
+
 int coarse_grid(int64_t n_rows) {
   const int cus = jscc_num_cus();
   int64_t g = n_rows < cus ? n_rows : cus;
@@ -868,6 +1195,14 @@ extern "C" int jscc_coarse_weight(const float* packed, int64_t ld_packed, const 
   const size_t glds = za_idx ? grouped_lds((int)P, n_zf, n_zc, kGrpAdj) : 0;
   const bool grouped = lid && za_idx != nullptr && n_adj == kGrpAdj && 2 * Pc <= P && kk <= 25 &&
                        glds <= 160 * 1024;
+  // the prefetching per-row form: identity layer map with 256 | Lf, up to 26624 fine pixels, both row
+  // buffers and the crystal table of a work-group's rows within the LDS
+  const int64_t group_rows = (n_rows + G - 1) / G;
+  const int k4 = (int)((P + 4 * kPfThreads - 1) / (4 * kPfThreads));
+  const int kv = k4 <= 2 ? 2 : (k4 <= 4 ? 4 : (k4 <= 8 ? 8 : kPfMaxK4));  // the instantiated quad count
+  const size_t plds = pf_lds((int)Pc, group_rows);
+  const bool pf = lid && n_lf % 256 == 0 && k4 <= kPfMaxK4 && Pc <= pf_copies(kv) * kPfWaves * kPfChunk &&
+                  plds <= 160 * 1024 && n_arows <= INT32_MAX;
   int rc;
   jscc_prof_begin(JSCC_TAG_COARSE, s);
 #define JSCC_COARSE(KV, XP, LIST, CNT, ADD)                                                                         \
@@ -880,7 +1215,19 @@ extern "C" int jscc_coarse_weight(const float* packed, int64_t ld_packed, const 
   else if (kk <= 16) JSCC_COARSE(16, XP, LIST, CNT, ADD); \
   else if (kk <= 25) JSCC_COARSE(25, XP, LIST, CNT, ADD); \
   else JSCC_COARSE(32, XP, LIST, CNT, ADD);
-  if (!grouped) {
+  if (!grouped && pf) {
+#define JSCC_PF(KV, NV)                                                                                           \
+  rc = launch_pf<KV, NV>(plds, G, s, packed, ld_packed, n_rows, (int)Pc, z_idx, z_w, n_zf, n_zc, n_lf, A, lda,      \
+                         n_arows, x, x_perm, x_perm_stride, x_len, eps, use_eps, partial, err, rows)
+#define JSCC_PF_N(KV) \
+  if (normalize) JSCC_PF(KV, true); else JSCC_PF(KV, false);
+    if (kv == 2) { JSCC_PF_N(2) }
+    else if (kv == 4) { JSCC_PF_N(4) }
+    else if (kv == 8) { JSCC_PF_N(8) }
+    else { JSCC_PF_N(kPfMaxK4) }
+#undef JSCC_PF_N
+#undef JSCC_PF
+  } else if (!grouped) {
     JSCC_COARSE_K(true, rows, nullptr, 0)
   } else {
     // rotated image (when x_perm is given) and a cleared count of handed-over rows
