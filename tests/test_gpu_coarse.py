@@ -38,9 +38,20 @@ def _deep_grid():
     return coor, ident, ident
 
 
+def _layer_grid(n_z):
+    """The 1280-pixel layer of the 511keV factors over n_z slices (identity rotation maps): 12, 6 and 2
+    slices take the 8-, 4- and 2-quad instantiations of the prefetching per-row kernel."""
+    from jscc_recon_spect_cylinder_amd import synth
+    coor = synth.polar_grid_3d(np.arange(6, 151, 6), [20, 40, 60, 80], np.arange(n_z) * 3.0).astype(np.float32)
+    ident = np.arange(1, coor.shape[0] + 1, dtype=np.int64)[:, None].repeat(8, axis=1)
+    return coor, ident, ident
+
+
 @pytest.fixture(scope="module",
-                params=[("ref", 1, 2), ("ref", 2, 2), ("ref", 2, 4), ("odd", 1, 2), ("odd", 2, 2), ("deep", 1, 2)],
-                ids=["theta1_z2", "theta2_z2", "theta2_z4", "odd_theta1_z2", "odd_theta2_z2", "deep_theta1_z2"])
+                params=[("ref", 1, 2), ("ref", 2, 2), ("ref", 2, 4), ("odd", 1, 2), ("odd", 2, 2), ("deep", 1, 2),
+                        ("layer12", 1, 2), ("layer6", 1, 2), ("layer2", 1, 2)],
+                ids=["theta1_z2", "theta2_z2", "theta2_z4", "odd_theta1_z2", "odd_theta2_z2", "deep_theta1_z2",
+                     "slices12_theta1_z2", "slices6_theta1_z2", "slices2_theta1_z2"])
 def grid(request):
     which, ts, zs = request.param
     if which == "ref":
@@ -48,6 +59,9 @@ def grid(request):
         rnum = 20
     elif which == "odd":
         coor, rot, inv = _odd_grid(6)
+        rnum = None
+    elif which.startswith("layer"):
+        coor, rot, inv = _layer_grid(int(which[5:]))
         rnum = None
     else:
         coor, rot, inv = _deep_grid()
