@@ -27,6 +27,7 @@ import time
 import numpy as np
 
 HBM_PEAK_GBS = 8000.0
+MFMA_BF16_PEAK_TFLOPS = 2500.0  # dense BF16 matrix-core peak of MI355X (no sparsity)
 # MI355X VALU issue rate without packed FP32 (kept out of every kernel): one 64-lane instruction per 4 cycles
 # per SIMD, 1024 SIMDs x 16 lanes x 2.4 GHz = 3.93e13 lane instructions/s (the 157.3 TFLOP/s FP32 figure
 # counts packed FMAs: 2 lanes x 2 flops per lane instruction)
@@ -45,12 +46,15 @@ def parse():
     ap.add_argument("--subsets", type=int, default=8)
     ap.add_argument("--cpu-sample", type=int, default=12000, help="events in the CPU-baseline sample (0: skip)")
     ap.add_argument("--skip-fp32", action="store_true", help="time the BF16 path only")
-    ap.add_argument("--workload", default="jsccd", choices=["jsccd", "sc", "jsccsd", "grad", "batched", "sparse"],
+    ap.add_argument("--workload", default="jsccd",
+                    choices=["jsccd", "sc", "jsccsd", "grad", "batched", "sparse", "frames"],
                     help="jsccd (default, the reported metric): JSCCD list mode, 2e5 events; sc: single-photon "
                          "OSEM, A 5120x4096; jsccsd: joint mode, A 81920x16384 + 1e6 events, FP32/BF16/FP8; "
                          "grad: Poisson log-likelihood gradient through the no-store cone operator; batched: 48 "
                          "reconstructions of 1e4-2e6 events in one launch per subset step; sparse: coarse-T "
-                         "list-mode OSEM with on-the-fly expansion, 2e5 packed rows on the 25600-voxel grid")
+                         "list-mode OSEM with on-the-fly expansion, 2e5 packed rows on the 25600-voxel grid; "
+                         "frames: 16 single-photon data sets sharing the 81920x16384 matrix, projected as GEMMs on "
+                         "the matrix cores")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (HBM counter calibration) and exit")
     return ap.parse_args()
@@ -515,6 +519,108 @@ def run_sparse(args):
             "cpu_baseline": cpu}
 
 
+def run_frames(args):
+    """K = 16 single-photon data sets sharing one system matrix (the time frames of a dynamic acquisition):
+    A 81920 x 16384 with the single-photon value law (the configuration-2 matrix) stored in BF16, Poisson data
+    of the rod phantom on the 64 x 256 grid at count levels 1e5-1e7, OSEM with 10 subsets per iteration.  The
+    frames' subset products run as GEMMs on the matrix cores (engine.FrameStore: forward over the row-major
+    matrix, back-projection over its transpose), beside the same frames one at a time through the fused
+    single-frame projection with BF16 rows (SystemStore.osem).  The error is measured per frame against the
+    FP64 oracle with the stored matrix on a reduced instance (A 4096 x 16384, 2 frames, 2 iterations x 4
+    subsets), which is also the CPU baseline's sample."""
+    import torch
+    from jscc_recon_spect_cylinder_amd import _lib, synth
+    from jscc_recon_spect_cylinder_amd.engine import FrameStore, SystemStore
+    dev = torch.device("cuda:0")
+    lib = _lib.load()
+    K, M, n_sub = 16, 81920, 10
+    coor = synth.polar_grid(64, 256, 50.0)
+    P = coor.shape[0]
+    x_true = torch.from_numpy(synth.rod_phantom(coor, 50.0).astype(np.float32)).to(dev)
+
+    def data(M_, seed):
+        A = _big_sysmat(M_, P, seed, dev)
+        f = SystemStore.from_dense(A, "fp32").forward(x_true)
+        g = torch.Generator(device=dev)
+        g.manual_seed(seed)
+        levels = torch.logspace(5, 7, K, device=dev)
+        Y = torch.poisson(f[:, None] * (levels / f.sum())[None, :], generator=g).contiguous()
+        return A, Y
+
+    A, Y = data(M, 20260336)
+    fs = FrameStore.from_dense(A)
+    ss = SystemStore.from_dense(A, "bf16")
+    del A
+    torch.cuda.empty_cache()
+    s = fs.column_sums()
+    X = torch.ones((P, K), device=dev)
+    _prof_read(lib, _lib.PROF_FRAMES)
+    lib.jscc_prof_enable(1)
+    dt, _ = _timed(lambda: fs.osem(X.fill_(1.0), Y, s, 1, n_sub), args.steps, args.warmup)
+    lib.jscc_prof_enable(0)
+    cnt, tot = _prof_read(lib, _lib.PROF_FRAMES)
+    assert cnt == 2 * n_sub * (args.steps + args.warmup), cnt
+    gemm_ms = tot / cnt
+    x1 = torch.ones(P, device=dev)
+    y0 = Y[:, 0].contiguous()
+    dt1, _ = _timed(lambda: ss.osem(x1.fill_(1.0), y0, s, 1, n_sub), args.steps, args.warmup)
+    # one GEMM launch (forward or back-projection of one subset, their mean): its matrix rows in BF16 once,
+    # the frame operand (two BF16 terms, 4 B per element and frame: K x P forward, K x n_s back) and the
+    # FP32 result (n_s x K forward, P x K back); 2 MFMA products (hi, lo) of 2 flops per element and frame
+    n_s = M // n_sub
+    nbytes = n_s * P * 2 + K * (n_s + P) * 4
+    flops = 2 * 2 * n_s * P * K
+    gbs = nbytes / (gemm_ms * 1e-3) / 1e9
+    tfl = flops / (gemm_ms * 1e-3) / 1e12
+    del fs, ss, Y, X
+    torch.cuda.empty_cache()
+    # reduced instance: error per frame against the FP64 oracle with the stored (BF16) matrix; CPU baseline
+    sys.path.insert(0, HERE)
+    from threadpoolctl import threadpool_limits
+    from oracle import osem
+    Mr, n_it, ks = 4096, 2, 4
+    Ar, Yr = data(Mr, 20260337)
+    fr = FrameStore.from_dense(Ar)
+    sr = fr.column_sums()
+    Xr = torch.ones((P, K), device=dev)
+    fr.osem(Xr, Yr, sr, n_it, ks)
+    Ab = Ar.to(torch.bfloat16).double().cpu().numpy()
+    Yn = Yr.double().cpu().numpy()
+    s64 = Ab.sum(0)
+    errs = []
+    with threadpool_limits(limits=1):
+        t0 = time.perf_counter()
+        for k in (0, K - 1):
+            x = np.ones(P)
+            for _ in range(n_it):
+                for lo, hi in fr.subset_bounds(ks):
+                    x = osem.em_plain(x, osem.weight_single(Ab[lo:hi], Yn[lo:hi, k], x), s64)
+            errs.append(_rel(Xr[:, k].cpu().numpy(), x))
+        t_cpu = (time.perf_counter() - t0) / (2 * n_it)  # seconds per frame-iteration of the reduced matrix
+    cpu_value = 1.0 / (t_cpu * M / Mr)
+    return {"metric": "multi-frame SC OSEM frame-iterations/s (16 frames sharing A 81920x16384, BF16, MFMA)",
+            "value": round(K / dt, 2), "unit": "frame-iterations/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "bf16 matrix and frame terms, fp32 accumulate", "data": "synthetic (seeded)",
+            "config": {"workload": f"{K} frames, A {M}x{P} BF16 (single-photon value law), 64x256 rod phantom, "
+                                   f"counts 1e5-1e7 per frame, 1 iteration = {n_sub} subsets"},
+            "single_frame_fused": {"frame_iterations_s": round(1.0 / dt1, 2), "ms_per_frame_iteration":
+                                   round(dt1 * 1e3, 3)},
+            "speedup_vs_single_frame": round(dt1 * K / dt, 2),
+            "roofline": {"kernel": "k_frames_gemm<1> (v_mfma_f32_16x16x32_bf16): one subset's forward or "
+                                   "back-projection GEMM, 16 frames", "bound": "hbm", "achieved": round(gbs, 1),
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+                         "traffic": None, "launch_us": round(gemm_ms * 1e3, 2), "bytes_per_launch": int(nbytes),
+                         "mfma_tflops": round(tfl, 1), "mfma_peak_tflops": MFMA_BF16_PEAK_TFLOPS,
+                         "mfma_frac": round(tfl / MFMA_BF16_PEAK_TFLOPS, 4)},
+            "rel_err_vs_fp64_oracle_reduced": {"worst_frame": max(errs), "bar": 1e-4,
+                                               "sample": f"A {Mr}x{P} (stored BF16 matrix), frames 0 and {K - 1}, "
+                                                         f"{n_it} iterations x {ks} subsets"},
+            "cpu_baseline": {"value": round(cpu_value, 4), "unit": "frame-iterations/s", "cores": 1, "kind": "port",
+                             "sample": f"numpy FP64 oracle (1 thread) on A {Mr}x{P}: {t_cpu:.3f} s per frame-"
+                                       f"iteration, scaled by rows to {M}", "host_cpu": _cpu_model()}}
+
+
 def run_grad(args):
     """Gradient of the list-mode Poisson log-likelihood w.r.t. the image and the angular blur sigma through the
     no-store cone operator (T recomputed in the backward), on the 2e5-event list at sigma = 3 deg."""
@@ -725,7 +831,7 @@ def main():
         return
     if args.workload != "jsccd":
         print(json.dumps({"sc": run_sc, "jsccsd": run_jsccsd, "grad": run_grad, "batched": run_batched,
-                          "sparse": run_sparse}[args.workload](args)))
+                          "sparse": run_sparse, "frames": run_frames}[args.workload](args)))
         return
     import torch
     from jscc_recon_spect_cylinder_amd import _lib, synth

@@ -401,12 +401,68 @@ int jscc_sum_f64(const float* v, int64_t n, int64_t stride, double* out, void* s
  * every e4m3fn code, out512[256..511] the library's software decode. */
 int jscc_fp8_selftest(float* out512, void* stream);
 
+/* ---- K frames sharing one BF16 system matrix (MFMA) ---------------------- */
+
+/* No counterpart in the reference, which reconstructs one data set per call:
+ * K single-photon data sets y_k with one system matrix (the time frames of a
+ * dynamic acquisition, bootstrap replicates) each run the per-subset weight
+ * get_weight_single (recon_osem_plane.py:28-30) and the EM update of
+ * osem_bin_mode (:45-54).  Batched, the subset products are GEMMs on the
+ * matrix cores (v_mfma_f32_16x16x32_bf16):
+ *   forward  F[i][f] = sum_j R[i][j] X[j][f]   a = R, row-major [rows x lda]
+ *   back     W[j][f] = sum_i R[i][j] C[i][f]   a = R^T, [P x lda] (the matrix
+ *                                              kept a second time, transposed)
+ * The frame operand is FP32, passed as two BF16 terms hi = bf16(v),
+ * lo = bf16(v - hi), frame-major [k_frames x ldb] with the reduced index
+ * contiguous; frames-minor FP32 arrays ([n x ld], frame f of item i at
+ * i * ld + f) hold images X [P x K], data Y [M x K] and forward projections.
+ * k_frames is 16, 32 or 64 (pad with frames whose data are zero). */
+
+/* dst[c][r] = src[r][c] for a BF16 matrix [rows x cols] (row stride ld_src);
+ * dst rows are ld_dst >= rows long and their columns [rows, ld_dst) zeroed. */
+int jscc_frames_transpose_bf16(const uint16_t* src, int64_t rows, int64_t cols, int64_t ld_src, uint16_t* dst,
+                               int64_t ld_dst, void* stream);
+
+/* hi/lo[f][i] = terms of x[i * ldx + f] for i < n, zero for n <= i < n_pad. */
+int jscc_frames_split(const float* x, int64_t n, int64_t n_pad, int k_frames, int64_t ldx, uint16_t* hi,
+                      uint16_t* lo, int64_t ldt, void* stream);
+
+/* Chunks of the reduced index for a GEMM of m output rows on `cus` compute
+ * units (about 16 waves per unit), and the partial-sum bytes of a GEMM. */
+int jscc_frames_split_count(int64_t m, int64_t kdim, int cus);
+int64_t jscc_frames_partial_bytes(int64_t m, int k_frames, int n_split);
+
+/* partial[sp][i][f] = sum over chunk sp of the reduced index k < kdim of
+ * a[i * lda + k] * (b_hi[f * ldb + k] + b_lo[f * ldb + k]), i < m; kdim a
+ * multiple of 128 (zero-padded operands), 16-byte aligned operands with
+ * strides a multiple of 8.  n_split must be the value jscc_frames_split_count
+ * returns for (m, kdim). */
+int jscc_frames_gemm(const uint16_t* a, int64_t lda, int64_t m, int64_t kdim, const uint16_t* b_hi,
+                     const uint16_t* b_lo, int64_t ldb, int k_frames, int n_split, float* partial,
+                     int64_t partial_bytes, void* stream);
+
+/* The ratio step of a subset of n rows: F[i][f] = sum_sp partial[sp][i][f],
+ * c = y[i * ldy + f] / F (/ max(F, eps) when use_eps), written as the hi/lo
+ * terms of C^T at columns c_off + i of [k_frames x ldc]; the other columns of
+ * [0, n_pad) are zeroed.  f_out [n x k_frames] (F) may be NULL. */
+int jscc_frames_ratio(const float* partial, int n_split, int64_t n, int64_t n_pad, int64_t c_off, int k_frames,
+                      const float* y, int64_t ldy, float eps, int use_eps, uint16_t* c_hi, uint16_t* c_lo,
+                      int64_t ldc, float* f_out, void* stream);
+
+/* The EM update of every frame: W[j][f] = sum_sp partial[sp][j][f] (zero when
+ * partial is NULL, an empty subset), x[j * ldx + f] = update(x, W, s[j]) with
+ * the modes of jscc_em_update, and the hi/lo terms of the updated image at
+ * t[f][j] when t_hi is given. */
+int jscc_frames_update(const float* partial, int n_split, int64_t n_pix, int k_frames, const float* s, float* x,
+                       int64_t ldx, int em_mode, float em_eps, uint16_t* t_hi, uint16_t* t_lo, int64_t ldt,
+                       void* stream);
+
 /* Built-in kernel timer (no counterpart in the reference).  While enabled
  * (per calling thread), the main kernel of every launch is bracketed by a HIP
  * event pair on its own stream, by tag: 0 cone rows (jscc_cone_rows),
  * 1 projection modes 0-2 (jscc_proj / jscc_proj_em), 2 projection mode 3
  * (column sums), 3 no-store cone projection (jscc_cone_proj), 4 fused coarse
- * row weight (jscc_coarse_weight).
+ * row weight (jscc_coarse_weight), 5 multi-frame GEMM (jscc_frames_gemm).
  * jscc_prof_read synchronises the pairs of one tag and returns their count and
  * summed milliseconds. */
 int jscc_prof_enable(int on);

@@ -86,11 +86,19 @@ SIGNATURES = {
     "jscc_coarse_weight": (c_int, [vp, c_i64, vp, c_i64, vp, vp, c_int, c_int, vp, vp, c_int, vp, vp, c_int, c_int,
                                     vp, c_i64, c_i64, vp, vp, c_i64, c_i64, c_f32, c_int, c_int, vp, c_i64, c_f32, c_int,
                                     vp, c_i64, vp, c_i64, vp, vp]),
+    "jscc_frames_transpose_bf16": (c_int, [vp, c_i64, c_i64, c_i64, vp, c_i64, vp]),
+    "jscc_frames_split": (c_int, [vp, c_i64, c_i64, c_int, c_i64, vp, vp, c_i64, vp]),
+    "jscc_frames_split_count": (c_int, [c_i64, c_i64, c_int]),
+    "jscc_frames_partial_bytes": (c_i64, [c_i64, c_int, c_int]),
+    "jscc_frames_gemm": (c_int, [vp, c_i64, c_i64, c_i64, vp, vp, c_i64, c_int, c_int, vp, c_i64, vp]),
+    "jscc_frames_ratio": (c_int, [vp, c_int, c_i64, c_i64, c_i64, c_int, vp, c_i64, c_f32, c_int, vp, vp, c_i64, vp,
+                                  vp]),
+    "jscc_frames_update": (c_int, [vp, c_int, c_i64, c_int, vp, vp, c_i64, c_int, c_f32, vp, vp, c_i64, vp]),
     "jscc_prof_enable": (c_int, [c_int]),
     "jscc_prof_read": (c_int, [c_int, vp, vp]),
 }
 
-PROF_CONE, PROF_PROJ, PROF_PROJ_SUM, PROF_CONE_PROJ, PROF_COARSE = 0, 1, 2, 3, 4
+PROF_CONE, PROF_PROJ, PROF_PROJ_SUM, PROF_CONE_PROJ, PROF_COARSE, PROF_FRAMES = 0, 1, 2, 3, 4, 5
 
 
 class JsccError(RuntimeError):

@@ -27,6 +27,7 @@ SOURCES = [
     ("jscc_proj.hip", []),
     ("jscc_rows.hip", []),
     ("jscc_coarse.hip", []),
+    ("jscc_frames.hip", []),
 ]
 
 COMMON_FLAGS = [

@@ -215,7 +215,8 @@ enum {
   JSCC_TAG_PROJ_SUM = 2,
   JSCC_TAG_CONE_PROJ = 3,
   JSCC_TAG_COARSE = 4,
-  JSCC_NUM_TAGS = 5
+  JSCC_TAG_FRAMES = 5,
+  JSCC_NUM_TAGS = 6
 };
 bool jscc_prof_on();
 void jscc_prof_begin(int tag, hipStream_t s);

@@ -281,6 +281,86 @@ class SystemStore:
         return x
 
 
+
+@dataclass
+class FrameStore:
+    """K single-photon data sets that share one system matrix A [M, P] (the time frames of a dynamic
+    acquisition, bootstrap replicates): their OSEM subset steps as GEMMs on the matrix cores.  The
+    reference reconstructs one data set per call (osem_bin_mode, recon_osem_plane.py:45-54, with
+    get_weight_single, :28-30); per frame this computes the same update, for all frames in one pass over
+    the subset's rows in each direction:
+        F = A_s X  (forward, [n_s, K]),  C = Y_s / F,  W = A_s^T C  (back, [P, K]),  X <- X * W / s.
+    A is held in BF16 twice, row-major for the forward GEMM and transposed for the back-projection (each
+    GEMM reduces over a contiguous index); images and coefficients enter the GEMMs as two BF16 terms
+    (hi = bf16(v), lo = bf16(v - hi)), accumulation FP32.  Frames are padded to 16, 32 or 64 with
+    zero-data frames.  Subsets are contiguous bin ranges with split_tensor_even bounds, as SystemStore.osem."""
+    mat: torch.Tensor    # [M, ldp] BF16, ldp = P rounded up to ops.FRAMES_ALIGN, zero-padded
+    matt: torch.Tensor   # [P, ldm] BF16, the transpose, ldm = M rounded up to ops.FRAMES_ALIGN
+    n_vox: int
+    n_rows: int
+
+    @classmethod
+    def from_dense(cls, A: torch.Tensor) -> "FrameStore":
+        ops.require_gpu(A, "sysmat")
+        M, P = A.shape
+        ldp, ldm = ops.round_up(P, ops.FRAMES_ALIGN), ops.round_up(M, ops.FRAMES_ALIGN)
+        mat = torch.zeros((M, ldp), dtype=torch.bfloat16, device=A.device)
+        mat[:, :P] = A
+        return cls(mat, ops.frames_transpose(mat, M, P, ldm), P, M)
+
+    def subset_bounds(self, k: int):
+        return [(self.n_rows * i // k, self.n_rows * (i + 1) // k) for i in range(k)]
+
+    def column_sums(self) -> torch.Tensor:
+        """s = A^T 1 of the stored (BF16) matrix, FP32 (the fused projection's column-sum mode)."""
+        w = torch.zeros(self.n_vox, dtype=torch.float32, device=self.mat.device)
+        ops.proj(ops.MODE_ONES, self.mat, dtype_c=ops.DT_BF16, n_cols=self.n_vox, n_rows=self.n_rows, w=w)
+        return w
+
+    def osem(self, X: torch.Tensor, Y: torch.Tensor, s: torch.Tensor, n_iter: int, n_subsets: int, eps=None,
+             em_mode: int = ops.EM_PLAIN) -> torch.Tensor:
+        """OSEM of every frame: X [P, K] (frames contiguous) updated in place from the data Y [M, K], with the
+        sensitivity s [P] shared by the frames."""
+        P, M = self.n_vox, self.n_rows
+        if X.ndim != 2 or X.shape[0] != P or Y.ndim != 2 or Y.shape != (M, X.shape[1]):
+            raise ValueError(f"frames: X must be [{P}, K] and Y [{M}, K]")
+        for t, nm in ((X, "images"), (Y, "data"), (s, "sensitivity")):
+            ops.require_gpu(t, nm)
+        K = X.shape[1]
+        if K > ops.FRAMES_K[-1]:
+            raise ValueError(f"frames: at most {ops.FRAMES_K[-1]} frames per store call")
+        Kp = next(k for k in ops.FRAMES_K if k >= K)
+        dev = X.device
+        ldp, ldm = self.mat.stride(0), self.matt.stride(0)
+        xp = torch.ones((P, Kp), dtype=torch.float32, device=dev)
+        xp[:, :K] = X
+        yp = torch.zeros((M, Kp), dtype=torch.float32, device=dev)
+        yp[:, :K] = Y
+        s32 = s.float().contiguous()
+        bounds = self.subset_bounds(n_subsets)
+        al = ops.FRAMES_ALIGN
+        pads = [((lo // al) * al, ops.round_up(hi, al)) for lo, hi in bounds]
+        n_max = max((hi - lo for lo, hi in bounds), default=0)
+        c_pad = max((b - a for a, b in pads), default=al)
+        splits_f = [ops.frames_split_count(hi - lo, ldp, dev) if hi > lo else 0 for lo, hi in bounds]
+        splits_b = [ops.frames_split_count(P, b - a, dev) if hi > lo else 0 for (lo, hi), (a, b) in zip(bounds, pads)]
+        part = torch.empty(max([n * Kp * (hi - lo) for n, (lo, hi) in zip(splits_f, bounds)] +
+                               [n * Kp * P for n in splits_b] + [1]), dtype=torch.float32, device=dev)
+        xt = torch.empty((2, Kp, ldp), dtype=torch.bfloat16, device=dev)
+        ct = torch.empty((2, Kp, c_pad), dtype=torch.bfloat16, device=dev)
+        ops.frames_split(xp, P, ldp, xt[0], xt[1])
+        for _ in range(n_iter):
+            for (lo, hi), (a0, a1), nf, nb in zip(bounds, pads, splits_f, splits_b):
+                if hi <= lo:
+                    ops.frames_update(None, 0, s32, xp, em_mode, xt[0], xt[1])
+                    continue
+                ops.frames_gemm(self.mat[lo:], hi - lo, ldp, xt[0], xt[1], nf, part)
+                ops.frames_ratio(part, nf, hi - lo, a1 - a0, lo - a0, yp[lo:], eps, ct[0], ct[1])
+                ops.frames_gemm(self.matt[:, a0:], P, a1 - a0, ct[0], ct[1], nb, part)
+                ops.frames_update(part, nb, s32, xp, em_mode, xt[0], xt[1])
+        X.copy_(xp[:, :K])
+        return X
+
 def joint_osem(a: SystemStore, y: torch.Tensor, t: "ListModeStore", x: torch.Tensor, s_joint: torch.Tensor,
                alpha: float, n_iter: int, n_subsets: int, eps=None, em_mode: int = ops.EM_PLAIN) -> torch.Tensor:
     """JSCC-SD OSEM (osem_joint_mode, recon_osem_plane.py:100-125): per subset

@@ -1058,3 +1058,102 @@ def fp8_selftest(device) -> torch.Tensor:
     rc = _lib.lib().jscc_fp8_selftest(out.data_ptr(), stream_of(out.device))
     _lib.check(rc, "jscc_fp8_selftest")
     return out
+
+
+# ----------------------------------------------------------------------------- K frames on the matrix cores
+FRAMES_K = (16, 32, 64)   # frame counts of one multi-frame GEMM (pad with zero-data frames)
+FRAMES_ALIGN = 128        # reduced-index granularity of a multi-frame GEMM
+
+
+def _check_bf16_operand(t: torch.Tensor, name: str) -> None:
+    if t.dtype != torch.bfloat16 or t.ndim != 2 or t.stride(1) != 1 or t.stride(0) % 8 or t.data_ptr() % 16:
+        raise ValueError(f"{name} must be a 2-D BF16 tensor, rows contiguous, row stride a multiple of 8, "
+                         "16-byte aligned")
+
+
+def frames_transpose(mat: torch.Tensor, n_rows: int, n_cols: int, ld_out: int) -> torch.Tensor:
+    """The transpose [n_cols, ld_out] of the BF16 matrix mat[:n_rows, :n_cols] (columns past n_rows zero)."""
+    require_gpu(mat, "matrix")
+    _check_bf16_operand(mat, "matrix")
+    out = torch.empty((n_cols, ld_out), dtype=torch.bfloat16, device=mat.device)
+    _lib.check(_lib.lib().jscc_frames_transpose_bf16(mat.data_ptr(), n_rows, n_cols, mat.stride(0), out.data_ptr(),
+                                                     ld_out, stream_of(mat.device)), "jscc_frames_transpose_bf16")
+    return out
+
+
+def frames_split(x: torch.Tensor, n: int, n_pad: int, hi: torch.Tensor, lo: torch.Tensor) -> None:
+    """hi/lo[f, i] = the BF16 terms of x[i, f] (i < n; zero up to n_pad) for a frames-minor FP32 array x."""
+    _check_bf16_operand(hi, "hi")
+    _check_bf16_operand(lo, "lo")
+    if x.dtype != torch.float32 or x.stride(1) != 1 or hi.shape[0] != x.shape[1] or hi.shape != lo.shape:
+        raise ValueError("frames_split: x must be FP32 [n, K] (frames contiguous) and hi/lo [K, >= n_pad]")
+    _lib.check(_lib.lib().jscc_frames_split(x.data_ptr(), n, n_pad, x.shape[1], x.stride(0), hi.data_ptr(),
+                                            lo.data_ptr(), hi.stride(0), stream_of(x.device)), "jscc_frames_split")
+
+
+_cu_counts: dict[int, int] = {}
+
+
+def device_cu_count(device: torch.device) -> int:
+    """Compute units of a GPU (jscc_device_cu_count on it, cached)."""
+    idx = device.index if device.index is not None else torch.cuda.current_device()
+    if idx not in _cu_counts:
+        import ctypes
+        v = ctypes.c_int(0)
+        with torch.cuda.device(idx):
+            _lib.check(_lib.lib().jscc_device_cu_count(ctypes.byref(v)), "jscc_device_cu_count")
+        _cu_counts[idx] = v.value
+    return _cu_counts[idx]
+
+
+def frames_split_count(m: int, kdim: int, device: torch.device) -> int:
+    """Chunks of the reduced index of a multi-frame GEMM with m output rows (about 16 waves per CU)."""
+    return int(_lib.lib().jscc_frames_split_count(m, kdim, device_cu_count(device)))
+
+
+def frames_gemm(a: torch.Tensor, m: int, kdim: int, b_hi: torch.Tensor, b_lo: torch.Tensor, n_split: int,
+                partial: torch.Tensor) -> None:
+    """partial[sp, i, f] = sum over chunk sp of the reduced index of a[i, k] * (b_hi[f, k] + b_lo[f, k])
+    (jscc_frames_gemm): a [>= m, >= kdim] and b [K, >= kdim] BF16, kdim a multiple of FRAMES_ALIGN."""
+    for t, nm in ((a, "a"), (b_hi, "b_hi"), (b_lo, "b_lo")):
+        require_gpu(t, nm)
+        _check_bf16_operand(t, nm)
+    K = b_hi.shape[0]
+    if K not in FRAMES_K or b_lo.shape != b_hi.shape or b_hi.stride(0) != b_lo.stride(0):
+        raise ValueError(f"frames_gemm: the frame operand must hold {FRAMES_K} frames in matching hi/lo tensors")
+    if a.shape[0] < m or a.shape[1] < kdim or b_hi.shape[1] < kdim or kdim % FRAMES_ALIGN:
+        raise ValueError("frames_gemm: operand shapes do not cover (m, kdim)")
+    if partial.dtype != torch.float32 or partial.numel() * 4 < n_split * m * K * 4:
+        raise ValueError("frames_gemm: partial too small")
+    _lib.check(_lib.lib().jscc_frames_gemm(a.data_ptr(), a.stride(0), m, kdim, b_hi.data_ptr(), b_lo.data_ptr(),
+                                           b_hi.stride(0), K, n_split, partial.data_ptr(), partial.numel() * 4,
+                                           stream_of(a.device)), "jscc_frames_gemm")
+
+
+def frames_ratio(partial: torch.Tensor, n_split: int, n: int, n_pad: int, c_off: int, y: torch.Tensor, eps,
+                 c_hi: torch.Tensor, c_lo: torch.Tensor, f_out: torch.Tensor | None = None) -> None:
+    """C^T terms of c = y / F (/ max(F, eps)) for a subset of n rows, F summed from the forward GEMM's partials
+    (jscc_frames_ratio); y [n, K] FP32 frames-minor, c_hi/c_lo [K, >= n_pad]."""
+    K = c_hi.shape[0]
+    if y.dtype != torch.float32 or y.stride(1) != 1 or y.shape[0] < n or y.shape[1] != K:
+        raise ValueError("frames_ratio: y must be FP32 [>= n, K] with frames contiguous")
+    if c_hi.shape != c_lo.shape or c_hi.shape[1] < n_pad or not 0 <= c_off <= n_pad - n:
+        raise ValueError("frames_ratio: coefficient buffers do not cover the subset")
+    _lib.check(_lib.lib().jscc_frames_ratio(partial.data_ptr(), n_split, n, n_pad, c_off, K, y.data_ptr(), y.stride(0),
+                                            float(eps) if eps is not None else 0.0, 1 if eps is not None else 0,
+                                            c_hi.data_ptr(), c_lo.data_ptr(), c_hi.stride(0), _p(f_out),
+                                            stream_of(y.device)), "jscc_frames_ratio")
+
+
+def frames_update(partial: torch.Tensor | None, n_split: int, s: torch.Tensor, x: torch.Tensor, em_mode: int = EM_PLAIN,
+                  t_hi: torch.Tensor | None = None, t_lo: torch.Tensor | None = None, em_eps: float = 1e-12) -> None:
+    """x[j, f] <- update(x[j, f], W[j, f], s[j]) for every frame, W summed from the back-projection GEMM's partials
+    (zero when partial is None), and the BF16 terms of the new image into t_hi/t_lo [K, >= P]
+    (jscc_frames_update)."""
+    P, K = x.shape
+    if x.dtype != torch.float32 or x.stride(1) != 1 or s.shape[0] < P:
+        raise ValueError("frames_update: x must be FP32 [P, K] with frames contiguous, s [P]")
+    _lib.check(_lib.lib().jscc_frames_update(_p(partial), n_split, P, K, s.data_ptr(), x.data_ptr(), x.stride(0),
+                                             em_mode, em_eps, _p(t_hi), _p(t_lo),
+                                             t_hi.stride(0) if t_hi is not None else 0, stream_of(x.device)),
+               "jscc_frames_update")
