@@ -428,7 +428,8 @@ int jscc_frames_split(const float* x, int64_t n, int64_t n_pad, int k_frames, in
                       uint16_t* lo, int64_t ldt, void* stream);
 
 /* Chunks of the reduced index for a GEMM of m output rows on `cus` compute
- * units (about 16 waves per unit), and the partial-sum bytes of a GEMM. */
+ * units (about one 64-row work-group per unit), and the partial-sum bytes of
+ * a GEMM. */
 int jscc_frames_split_count(int64_t m, int64_t kdim, int cus);
 int64_t jscc_frames_partial_bytes(int64_t m, int k_frames, int n_split);
 

@@ -15,11 +15,12 @@
 // R is exact in BF16 (it is stored so), v is kept to 16 significant bits, the
 // accumulation is FP32 (v_mfma_f32_16x16x32_bf16).
 //
-// GEMM layout: a wave owns 32 output rows (two 16-row tiles) and all K frames
-// (NT tiles of 16) over one of n_split contiguous chunks of the reduced index;
-// its partial sums go to [n_split][rows][K] and the element-wise kernel that
-// follows adds them in a fixed order.  Operands are 16-byte buffer loads whose
-// range ends at the operand's last row (rows past it read zero).
+// GEMM layout: a work-group owns 64 output rows and all K frames over one of
+// n_split contiguous chunks of the reduced index; its partial sums go to
+// [n_split][rows][K] and the element-wise kernel that follows adds them in a
+// fixed order.  Operands reach LDS by 16-byte memory -> LDS copies of
+// contiguous row segments whose range ends at the operand's last row (rows
+// past it read zero), three or four stages in flight.
 #include "jscc_common.h"
 
 namespace {
@@ -27,11 +28,16 @@ namespace {
 typedef __bf16 jscc_bf8 __attribute__((ext_vector_type(8)));
 typedef float jscc_mf4 __attribute__((ext_vector_type(4)));
 
-constexpr int kFrWaves = 4;        // independent waves per work-group
-constexpr int kFrRows = 32;        // output rows per wave (two 16-row MFMA tiles)
+constexpr int kFrWaves = 4;        // waves per work-group
+constexpr int kFrRowsWg = 64;      // output rows per work-group (a 16-row MFMA tile per wave)
 constexpr int kFrK = 32;           // reduced indices per MFMA step
-constexpr int kFrUnroll = 4;       // MFMA steps whose loads are issued together
+constexpr int kFrAlign = 256;      // the reduced length is a multiple of every stage length
 constexpr int kBufWord3 = 0x00020000;
+typedef __attribute__((address_space(3))) void* lds_vp;
+typedef int jscc_i4v __attribute__((ext_vector_type(4)));
+
+// s_waitcnt immediate of gfx9 waiting for vmcnt <= n only (expcnt and lgkmcnt at their maxima)
+constexpr int jscc_vmcnt_enc(int n) { return (n & 0xf) | ((n >> 4) << 14) | (0x7 << 4) | (0xf << 8); }
 
 __device__ __forceinline__ uint16_t bf16_rne(float v) {
   // round to nearest even (the operands are finite or NaN; NaN stays NaN)
@@ -41,82 +47,128 @@ __device__ __forceinline__ uint16_t bf16_rne(float v) {
 }
 __device__ __forceinline__ float bf16_val(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-__device__ __forceinline__ jscc_bf8 load_frag(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
-  return __builtin_bit_cast(jscc_bf8, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+
+// Stage geometry of the GEMM: KS reduced indices per stage (512 B of a BF16 row for 16 frames, 256 B
+// otherwise), NSTAGE stages in flight, LDS = NSTAGE * (64 rows + 2 * 16 NT frame rows) * KS * 2 bytes.
+template <int NT> struct FrGeom {
+  static constexpr int KS = NT == 1 ? 256 : 128;
+  static constexpr int NSTAGE = NT == 2 ? 4 : 3;
+  static constexpr int RB = KS * 2;                     // bytes of one row in a stage
+  static constexpr int CH = RB / 16;                    // 16-byte chunks of a row in a stage
+  static constexpr int A_BYTES = kFrRowsWg * RB;        // the matrix tile of a stage
+  static constexpr int B_BYTES = 16 * NT * RB;          // one frame term of a stage
+  static constexpr int STAGE = A_BYTES + 2 * B_BYTES;
+  static constexpr int LDS = NSTAGE * STAGE;
+  static constexpr int A_COPIES = A_BYTES / 1024 / kFrWaves;  // 1 KB copies per wave and stage
+  static constexpr int B_COPIES = B_BYTES / 1024 / kFrWaves;  // per frame term
+};
+
+// 16-byte memory -> LDS copy of one lane (buffer range checked); inline asm, so that the compiler does not
+// wait for it before every later LDS read (the stage barriers wait for it instead); no register outputs
+__device__ __forceinline__ void dma16(const jscc_i4v& rsrc, int voff, int soff, uint32_t lds_byte) {
+  asm volatile("buffer_load_dwordx4 %0, %1, %2 offen lds" ::"v"(voff), "s"(rsrc), "s"(soff), "{m0}"(lds_byte)
+               : "memory");
 }
 
-// partial[sp][i][f] = sum over the reduced indices of chunk sp of a[i][k] * (bh[f][k] + bl[f][k])
+__device__ __forceinline__ jscc_i4v rsrc_of(const void* base, int64_t bytes) {
+  const uint64_t a = reinterpret_cast<uint64_t>(base);
+  jscc_i4v r;
+  r.x = (int)(uint32_t)a;
+  r.y = (int)((uint32_t)(a >> 32) & 0xffffu);
+  r.z = (int)bytes;
+  r.w = kBufWord3;
+  return r;
+}
+
+// partial[sp][i][f] = sum over the reduced indices of chunk sp of a[i][k] * (bh[f][k] + bl[f][k]).
+// A work-group of 4 waves owns 64 rows (wave w the 16-row MFMA tile w) and all 16 NT frames.  Each stage
+// brings the rows' next KS reduced indices and both frame terms into LDS by 16-byte copies of contiguous
+// row segments (a row of a stage is RB contiguous bytes in memory); chunk c of row r sits at chunk slot
+// c ^ (r & 15), so that the 16 rows of an MFMA fragment read 16 different bank groups.
 template <int NT>
 __global__ void __launch_bounds__(kFrWaves * JSCC_WAVE)
     k_frames_gemm(const uint16_t* __restrict__ a, int64_t lda, int64_t m, int64_t kdim,
                   const uint16_t* __restrict__ bh, const uint16_t* __restrict__ bl, int64_t ldb, int64_t kchunk,
                   int n_split, float* __restrict__ partial) {
-  const int lane = jscc_lane();
-  const int64_t wave = (int64_t)blockIdx.x * kFrWaves + jscc_wave_id();
-  const int64_t m_tiles = (m + kFrRows - 1) / kFrRows;
-  const int64_t mt = wave % m_tiles, sp = wave / m_tiles;
-  if (sp >= n_split) return;
-  const int64_t m0 = mt * kFrRows;
+  using Gm = FrGeom<NT>;
+  extern __shared__ __attribute__((aligned(16))) unsigned char frlds[];
+  const int lane = jscc_lane(), wv = jscc_wave_id();
+  const int64_t m_tiles = (m + kFrRowsWg - 1) / kFrRowsWg;
+  const int64_t mt = blockIdx.x % m_tiles, sp = blockIdx.x / m_tiles;
+  const int64_t m0 = mt * kFrRowsWg;
   const int64_t k0 = sp * kchunk;
   const int64_t k1 = k0 + kchunk < kdim ? k0 + kchunk : kdim;
-  const int64_t rows = m - m0 < kFrRows ? m - m0 : kFrRows;
+  const int n_st = (int)((k1 - k0) / Gm::KS);
+  const int64_t rows = m - m0 < kFrRowsWg ? m - m0 : kFrRowsWg;
   constexpr int KF = 16 * NT;
 
-  const __amdgpu_buffer_rsrc_t ra = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint16_t*>(a + m0 * lda), (short)0,
-                                                                      (int)(rows * lda * 2), kBufWord3);
-  const __amdgpu_buffer_rsrc_t rh =
-      __builtin_amdgcn_make_buffer_rsrc(const_cast<uint16_t*>(bh), (short)0, (int)(KF * ldb * 2), kBufWord3);
-  const __amdgpu_buffer_rsrc_t rl =
-      __builtin_amdgcn_make_buffer_rsrc(const_cast<uint16_t*>(bl), (short)0, (int)(KF * ldb * 2), kBufWord3);
-  // lane l supplies row (l & 15) of a 16-row tile and the 8 reduced indices 8 (l >> 4) .. +7 of a step
-  const int r16 = lane & 15, g8 = 8 * (lane >> 4);
-  const int va0 = (int)((r16 * lda + g8) * 2), va1 = (int)(((16 + r16) * lda + g8) * 2);
-  int vb[NT];
+  const jscc_i4v ra = rsrc_of(a + m0 * lda, rows * lda * 2);
+  const jscc_i4v rh = rsrc_of(bh, (int64_t)KF * ldb * 2), rl = rsrc_of(bl, (int64_t)KF * ldb * 2);
+  const uint32_t lds0 = (uint32_t)(uintptr_t)(lds_vp)frlds;
+  // copy q of a wave fills LDS bytes [1 KB * (q * waves + w), +1 KB) of a tile: lane j -> row
+  // (that KB * 1024 + 16 j) / RB, slot (16 j / 16) % CH, which holds the row's chunk slot ^ (row & 15)
+  auto row_of = [&](int kb) { return (kb * 1024 + 16 * lane) / Gm::RB; };
+  auto chunk_of = [&](int kb) { return ((lane % Gm::CH) ^ (row_of(kb) & 15)); };
+  auto issue = [&](int st) {
+    const int buf = st % Gm::NSTAGE;
+    const uint32_t sb = lds0 + buf * Gm::STAGE;
+    const int so = (int)((k0 + (int64_t)st * Gm::KS) * 2);
 #pragma unroll
-  for (int u = 0; u < NT; ++u) vb[u] = (int)(((16 * u + r16) * ldb + g8) * 2);
-
-  jscc_mf4 acc[2][NT];
-#pragma unroll
-  for (int t = 0; t < 2; ++t)
-#pragma unroll
-    for (int u = 0; u < NT; ++u) acc[t][u] = jscc_mf4{0.0f, 0.0f, 0.0f, 0.0f};
-
-  for (int64_t k = k0; k < k1; k += kFrK * kFrUnroll) {
-    jscc_bf8 fa[kFrUnroll][2], fh[kFrUnroll][NT], fl[kFrUnroll][NT];
-#pragma unroll
-    for (int s = 0; s < kFrUnroll; ++s) {
-      // kchunk and kdim are multiples of kFrK * kFrUnroll: every step of the group lies in the chunk
-      const int so = (int)((k + s * kFrK) * 2);
-      fa[s][0] = load_frag(ra, va0, so);
-      fa[s][1] = load_frag(ra, va1, so);
-#pragma unroll
-      for (int u = 0; u < NT; ++u) {
-        fh[s][u] = load_frag(rh, vb[u], so);
-        fl[s][u] = load_frag(rl, vb[u], so);
-      }
+    for (int q = 0; q < Gm::A_COPIES; ++q) {
+      const int kb = q * kFrWaves + wv;
+      dma16(ra, (int)(row_of(kb) * lda * 2) + chunk_of(kb) * 16, so, sb + kb * 1024);
     }
 #pragma unroll
-    for (int s = 0; s < kFrUnroll; ++s)
+    for (int q = 0; q < Gm::B_COPIES; ++q) {
+      const int kb = q * kFrWaves + wv;
+      const int vo = (int)(row_of(kb) * ldb * 2) + chunk_of(kb) * 16;
+      dma16(rh, vo, so, sb + Gm::A_BYTES + kb * 1024);
+      dma16(rl, vo, so, sb + Gm::A_BYTES + Gm::B_BYTES + kb * 1024);
+    }
+  };
+  constexpr int PER_STAGE = Gm::A_COPIES + 2 * Gm::B_COPIES;  // copies per wave and stage
+
+  jscc_mf4 acc[NT];
 #pragma unroll
-      for (int t = 0; t < 2; ++t)
+  for (int u = 0; u < NT; ++u) acc[u] = jscc_mf4{0.0f, 0.0f, 0.0f, 0.0f};
+  for (int st = 0; st < Gm::NSTAGE - 1 && st < n_st; ++st) issue(st);
+
+  const int r16 = lane & 15, g = lane >> 4;
+  for (int st = 0; st < n_st; ++st) {
+    // stage st has landed once at most the copies of the later stages already issued are outstanding
+    const int later = n_st - 1 - st < Gm::NSTAGE - 2 ? n_st - 1 - st : Gm::NSTAGE - 2;
+    if (later >= 2) __builtin_amdgcn_s_waitcnt(jscc_vmcnt_enc(2 * PER_STAGE));
+    else if (later == 1) __builtin_amdgcn_s_waitcnt(jscc_vmcnt_enc(PER_STAGE));
+    else __builtin_amdgcn_s_waitcnt(jscc_vmcnt_enc(0));
+    __syncthreads();  // every wave's copies of stage st; every wave is past stage st - 1 (its buffer is free)
+    if (st + Gm::NSTAGE - 1 < n_st) issue(st + Gm::NSTAGE - 1);
+    const unsigned char* sa = frlds + (st % Gm::NSTAGE) * Gm::STAGE;
+    const unsigned char* sbh = sa + Gm::A_BYTES;
+    const unsigned char* sbl = sbh + Gm::B_BYTES;
 #pragma unroll
-        for (int u = 0; u < NT; ++u) {
-          acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[s][t], fh[s][u], acc[t][u], 0, 0, 0);
-          acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[s][t], fl[s][u], acc[t][u], 0, 0, 0);
-        }
+    for (int s = 0; s < Gm::KS / kFrK; ++s) {
+      const int slot = (4 * s + g) ^ r16;
+      const jscc_bf8 fa = *reinterpret_cast<const jscc_bf8*>(sa + (16 * wv + r16) * Gm::RB + slot * 16);
+#pragma unroll
+      for (int u = 0; u < NT; ++u) {
+        const int off = (16 * u + r16) * Gm::RB + slot * 16;
+        const jscc_bf8 fh = *reinterpret_cast<const jscc_bf8*>(sbh + off);
+        const jscc_bf8 fl = *reinterpret_cast<const jscc_bf8*>(sbl + off);
+        acc[u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa, fh, acc[u], 0, 0, 0);
+        acc[u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa, fl, acc[u], 0, 0, 0);
+      }
+    }
   }
   // lane l holds rows 4 (l >> 4) + j (j < 4) of column l & 15 of each 16 x 16 tile
   float* out = partial + (sp * m + m0) * KF;
 #pragma unroll
-  for (int t = 0; t < 2; ++t)
+  for (int j = 0; j < 4; ++j) {
+    const int r = 16 * wv + 4 * g + j;
+    if (r < rows) {
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int r = 16 * t + 4 * (lane >> 4) + j;
-      if (r < rows) {
-#pragma unroll
-        for (int u = 0; u < NT; ++u) out[(int64_t)r * KF + 16 * u + r16] = acc[t][u][j];
-      }
+      for (int u = 0; u < NT; ++u) out[(int64_t)r * KF + 16 * u + r16] = acc[u][j];
     }
+  }
 }
 
 // hi/lo BF16 terms of v (frame-major, reduced index contiguous)
@@ -195,7 +247,20 @@ __global__ void __launch_bounds__(256) k_frames_transpose(const uint16_t* __rest
   }
 }
 
-int frames_grid(int64_t waves) { return (int)((waves + kFrWaves - 1) / kFrWaves); }
+template <int NT>
+int launch_frames_gemm(hipStream_t s, int64_t G, const uint16_t* a, int64_t lda, int64_t m, int64_t kdim,
+                       const uint16_t* bh, const uint16_t* bl, int64_t ldb, int64_t kchunk, int n_split,
+                       float* partial) {
+  static bool attr_set = false;
+  if (!attr_set) {
+    JSCC_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_frames_gemm<NT>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, FrGeom<NT>::LDS));
+    attr_set = true;
+  }
+  hipLaunchKernelGGL(k_frames_gemm<NT>, dim3((unsigned)G), dim3(kFrWaves * JSCC_WAVE), FrGeom<NT>::LDS, s, a, lda,
+                     m, kdim, bh, bl, ldb, kchunk, n_split, partial);
+  return 0;
+}
 
 }  // namespace
 
@@ -223,12 +288,18 @@ extern "C" int jscc_frames_split(const float* x, int64_t n, int64_t n_pad, int k
 }
 
 extern "C" int jscc_frames_split_count(int64_t m, int64_t kdim, int cus) {
-  // about 16 waves per compute unit, each over at least 8 MFMA steps of the reduced index
-  const int64_t m_tiles = (m + kFrRows - 1) / kFrRows;
-  const int64_t want = (16LL * (cus > 0 ? cus : 256) + m_tiles - 1) / m_tiles;
-  const int64_t most = kdim / (kFrK * kFrUnroll * 2) > 0 ? kdim / (kFrK * kFrUnroll * 2) : 1;
+  // about one work-group per compute unit (one is resident: 96-144 KB of LDS), each over at least four
+  // stages of the reduced index (measured on MI355X at 8192 x 16384 and 16384 x 8192, 16 frames: one
+  // work-group per unit 54.9 / 52.0 us, two 60.5 / 54.2 us, four 61.0-64.4 / 60.8 us)
+  const int64_t m_tiles = (m + kFrRowsWg - 1) / kFrRowsWg;
+  const int64_t want = ((cus > 0 ? cus : 256) + m_tiles - 1) / m_tiles;
+  const int64_t most = kdim / (4 * kFrAlign) > 0 ? kdim / (4 * kFrAlign) : 1;
   int64_t ns = want < most ? want : most;
-  return (int)(ns < 1 ? 1 : ns);
+  ns = ns < 1 ? 1 : ns;
+  // the count the GEMM accepts: chunks of a multiple of kFrAlign that split kdim into exactly ns
+  int64_t kchunk = (kdim + ns - 1) / ns;
+  kchunk = (kchunk + kFrAlign - 1) / kFrAlign * kFrAlign;
+  return (int)((kdim + kchunk - 1) / kchunk);
 }
 
 extern "C" int64_t jscc_frames_partial_bytes(int64_t m, int k_frames, int n_split) {
@@ -242,31 +313,29 @@ extern "C" int jscc_frames_gemm(const uint16_t* a, int64_t lda, int64_t m, int64
   JSCC_REQUIRE(k_frames == 16 || k_frames == 32 || k_frames == 64, "jscc_frames_gemm: %d frames (16, 32 or 64)",
                k_frames);
   JSCC_REQUIRE(m > 0 && kdim > 0 && n_split >= 1, "jscc_frames_gemm: bad shape");
-  JSCC_REQUIRE(kdim % (kFrK * kFrUnroll) == 0 && lda >= kdim && ldb >= kdim,
+  JSCC_REQUIRE(kdim % kFrAlign == 0 && lda >= kdim && ldb >= kdim,
                "jscc_frames_gemm: the reduced length %lld must be a multiple of %d within both strides",
-               (long long)kdim, kFrK * kFrUnroll);
+               (long long)kdim, kFrAlign);
   JSCC_REQUIRE((uintptr_t)a % 16 == 0 && (uintptr_t)b_hi % 16 == 0 && (uintptr_t)b_lo % 16 == 0 && lda % 8 == 0 &&
                    ldb % 8 == 0,
                "jscc_frames_gemm: operands must be 16-byte aligned with strides a multiple of 8");
-  JSCC_REQUIRE((double)kFrRows * lda * 2 < 2147483647.0 && (double)k_frames * ldb * 2 < 2147483647.0,
+  JSCC_REQUIRE((double)kFrRowsWg * lda * 2 < 2147483647.0 && (double)k_frames * ldb * 2 < 2147483647.0,
                "jscc_frames_gemm: operand tile beyond the 32-bit buffer range");
   JSCC_REQUIRE(partial_bytes >= jscc_frames_partial_bytes(m, k_frames, n_split), "jscc_frames_gemm: partial too small");
   int64_t kchunk = (kdim + n_split - 1) / n_split;
-  kchunk = (kchunk + kFrK * kFrUnroll - 1) / (kFrK * kFrUnroll) * (kFrK * kFrUnroll);
+  kchunk = (kchunk + kFrAlign - 1) / kFrAlign * kFrAlign;
   const int ns = (int)((kdim + kchunk - 1) / kchunk);
   JSCC_REQUIRE(ns == n_split, "jscc_frames_gemm: %d chunks of the reduced index do not split %lld evenly into "
-               "multiples of %d (use jscc_frames_split_count)", n_split, (long long)kdim, kFrK * kFrUnroll);
-  const int64_t waves = (m + kFrRows - 1) / kFrRows * n_split;
+               "multiples of %d (use jscc_frames_split_count)", n_split, (long long)kdim, kFrAlign);
+  const int64_t G = (m + kFrRowsWg - 1) / kFrRowsWg * n_split;
   hipStream_t s = (hipStream_t)stream;
+  int rc;
   jscc_prof_begin(JSCC_TAG_FRAMES, s);
-#define JSCC_FR(NTV)                                                                                         \
-  hipLaunchKernelGGL(k_frames_gemm<NTV>, dim3(frames_grid(waves)), dim3(kFrWaves * JSCC_WAVE), 0, s, a, lda, m, \
-                     kdim, b_hi, b_lo, ldb, kchunk, n_split, partial)
-  if (k_frames == 16) JSCC_FR(1);
-  else if (k_frames == 32) JSCC_FR(2);
-  else JSCC_FR(4);
-#undef JSCC_FR
+  if (k_frames == 16) rc = launch_frames_gemm<1>(s, G, a, lda, m, kdim, b_hi, b_lo, ldb, kchunk, n_split, partial);
+  else if (k_frames == 32) rc = launch_frames_gemm<2>(s, G, a, lda, m, kdim, b_hi, b_lo, ldb, kchunk, n_split, partial);
+  else rc = launch_frames_gemm<4>(s, G, a, lda, m, kdim, b_hi, b_lo, ldb, kchunk, n_split, partial);
   jscc_prof_end(JSCC_TAG_FRAMES, s);
+  if (rc != 0) return rc;
   JSCC_CHECK_LAUNCH();
   return 0;
 }

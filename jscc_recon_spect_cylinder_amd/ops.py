@@ -1062,7 +1062,7 @@ def fp8_selftest(device) -> torch.Tensor:
 
 # ----------------------------------------------------------------------------- K frames on the matrix cores
 FRAMES_K = (16, 32, 64)   # frame counts of one multi-frame GEMM (pad with zero-data frames)
-FRAMES_ALIGN = 128        # reduced-index granularity of a multi-frame GEMM
+FRAMES_ALIGN = 256        # reduced-index granularity of a multi-frame GEMM
 
 
 def _check_bf16_operand(t: torch.Tensor, name: str) -> None:
