@@ -453,7 +453,7 @@ def run_sparse(args):
     w = torch.zeros(P, device=dev)
     # the weight's form per subset (grouped when runs of one crystal are long), decided once outside the loop
     run_len = [ops.crystal_run_length(packed[lo:hi]) for lo, hi in bounds]
-    forms = [bool(ax.za_idx is not None and r >= ops.GROUPED_MIN_RUN) for r in run_len]
+    forms = [bool(ax.za_idx is not None and r >= ops.grouped_min_run(ax)) for r in run_len]
 
     def run(x):
         for _ in range(n_iter):

@@ -504,10 +504,31 @@ def crystal_order(packed: torch.Tensor) -> torch.Tensor:
     return _order_cache.put(packed, (), order)
 
 
-# Rows per run of one first-hit crystal above which jscc_coarse_weight's grouped form beats its per-row form
-# (25000 rows of the 25600-voxel grid on MI355X: a run costs ~33 us of adjoints and expansion, a row ~2.3 us
-# less than in the per-row form; 2.6 rows per run: 1661 vs 618 us per launch, 25: 452 vs 546, 250: 327 vs 537)
+# Rows per run of one first-hit crystal above which jscc_coarse_weight's grouped form beats its per-row form.
+# Grids the prefetching per-row kernel takes (identity layer map, 256 | layer size, <= 26624 fine pixels: the
+# reference grids) read each A row once per run in both forms; measured on the 25600-voxel grid (100000 rows):
+# 50 rows per run 1.453 (grouped) vs 1.177 ms, 100: 1.228 vs 1.140, 250: 1.126 vs 1.137, 500: 1.106 vs 1.142;
+# 250000 rows at 23.8 per run 3.99 vs 2.85 ms.  Other grids: the first per-row kernel reloads the A row of
+# every row (25000 rows: 2.6 rows per run 1661 vs 618 us per launch, 25: 452 vs 546, 250: 327 vs 537).
 GROUPED_MIN_RUN = 16.0
+GROUPED_MIN_RUN_PF = 250.0
+_PF_MAX_K4 = 13
+
+
+def coarse_pf_fits(taps: "AxisTaps") -> bool:
+    """Whether jscc_coarse_weight's per-row form is the prefetching kernel on this grid (the grid conditions;
+    the row count must also leave the LDS room for the work-group's crystal table, about 14000 rows per
+    work-group on the reference grid)."""
+    P, Pc = taps.n_fine, taps.n_coarse
+    k4 = -(-P // 2048)
+    kv = 2 if k4 <= 2 else 4 if k4 <= 4 else 8 if k4 <= 8 else _PF_MAX_K4
+    return (taps.l_idx is None and taps.n_lf % 256 == 0 and k4 <= _PF_MAX_K4 and Pc <= (kv + 1) // 2 * 2048 and
+            (2 * round_up(Pc, 256) + 72 + 4 * 8 * _PF_MAX_K4) * 4 <= 160 * 1024)
+
+
+def grouped_min_run(taps: "AxisTaps") -> float:
+    """The crystal run length from which coarse_weight(grouped="auto") takes the grouped form on this grid."""
+    return GROUPED_MIN_RUN_PF if coarse_pf_fits(taps) else GROUPED_MIN_RUN
 
 _runs_cache = SourceCache(64)
 
@@ -549,7 +570,7 @@ def coarse_weight(packed: torch.Tensor, taps: AxisTaps, sysmat: torch.Tensor, x:
     order of the packed rows (rows ordered by first-hit crystal let the grouped form share each A row
     over a run of rows; the weight is a sum over rows, so the order changes only rounding).
     grouped: True for the grouped form where the grid allows it, False for the per-row form, "auto" for the
-    grouped form when runs of one crystal average GROUPED_MIN_RUN rows or more."""
+    grouped form when runs of one crystal average grouped_min_run(taps) rows or more."""
     dev = packed.device
     if packed.ndim != 2 or packed.shape[1] != taps.n_coarse + 1:
         raise ValueError("packed rows must have shape [n, 1 + coarse_pixel_num]")
@@ -575,7 +596,7 @@ def coarse_weight(packed: torch.Tensor, taps: AxisTaps, sysmat: torch.Tensor, x:
     n = packed.shape[0]
     ws = _workspace(dev, _lib.lib().jscc_coarse_weight_workspace_bytes(max(n, 1), taps.n_fine))
     if grouped == "auto":
-        grouped = taps.za_idx is not None and n > 0 and crystal_run_length(packed, rows) >= GROUPED_MIN_RUN
+        grouped = taps.za_idx is not None and n > 0 and crystal_run_length(packed, rows) >= grouped_min_run(taps)
     za_i, za_w = (taps.za_idx, taps.za_w) if grouped else (None, None)
     rc = _lib.lib().jscc_coarse_weight(
         packed.data_ptr() if n else None, packed.stride(0), _p(rows), n, taps.z_idx.data_ptr(), taps.z_w.data_ptr(),
