@@ -706,7 +706,7 @@ def run_batched(args):
     P = coor.shape[0]
     s = bs.sensitivity()
     x = torch.ones((B, P), device=dev)
-    groups = bs.autotune(x, s, K)
+    tile, groups, _ = bs.autotune(x, s, K)
     _prof_read(lib, _lib.PROF_CONE_PROJ)
     gi = bs.capture(x, s, K)
     torch.cuda.synchronize()
@@ -725,7 +725,9 @@ def run_batched(args):
             "config": {"workload": f"{B} reconstructions, 64x256 polar grid, event counts log-uniform [1e4, 2e6] "
                                    f"(total {int(counts.sum())}, kept {total}), {n_iter} iterations x {K} subsets, "
                                    "one launch per subset step for all images, HIP-graph replay"},
-            "components": {"groups_per_cu": groups, "kept_min": int(min(n_kept)), "kept_max": int(max(n_kept)),
+            "components": {"groups_per_cu": groups, "tile": dict(zip(("threads", "voxel_quads", "rows_per_step"),
+                                                                     ops.cone_batch_tiles(P)[tile])),
+                           "kept_min": int(min(n_kept)), "kept_max": int(max(n_kept)),
                            "pairs_per_s": round(total * P * n_iter / dt, 1),
                            "list_generation_and_filters_s": round(t_prep, 2),
                            "image_min": float(x.min()), "image_max": float(x.max())}}

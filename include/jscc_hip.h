@@ -371,17 +371,22 @@ int jscc_cone_proj(int mode, const void* events, const int32_t* ev_idx, int64_t 
  *           independent reconstructions); w (optional) receives the weights;
  *   mode 1 (column sums): w[b, :] = sum of the normalised rows of b (the
  *           list-mode sensitivity of main_plane.py:348-355).
- * x, sens, w: FP32 [B, ld*] with 16-byte aligned rows.  groups_per_cu: work-
- * groups per CU (0: 4), the tile choice an autotuner may vary.  No
- * counterpart in the reference (it reconstructs one image per process). */
+ * x, sens, w: FP32 [B, ld*] with 16-byte aligned rows.  The launch shape is
+ * what an autotuner varies: groups_per_cu, work-groups per CU (0: 4), and
+ * tile, one of jscc_cone_batch_tiles(n_vox) tile shapes (threads per work-
+ * group, voxel quads per thread, rows per step; jscc_cone_batch_tile_shape
+ * describes shape `tile`; 0 is the default).  No counterpart in the reference
+ * (it reconstructs one image per process). */
+int jscc_cone_batch_tiles(int64_t n_vox);
+int jscc_cone_batch_tile_shape(int64_t n_vox, int tile, int* threads, int* quads, int* rows_per_step);
 int jscc_cone_batch_groups(int64_t n_rows, int groups_per_cu);
 int64_t jscc_cone_batch_workspace_bytes(int64_t n_rows, int n_recon, int64_t n_vox, int groups_per_cu);
 int jscc_cone_proj_batched(int mode, const void* events, const int32_t* ev_idx, int64_t n_rows, const int64_t* row_off,
                            int n_recon, const float* coor_x, const float* coor_y, const float* coor_z, int64_t n_vox,
                            const float* sysmat, int64_t ld_sysmat, int sigma_model, double fixed_sigma, float* x,
                            int64_t ldx, float eps, int use_eps, int normalize, const float* sens, int64_t ld_sens,
-                           int em_mode, float em_eps, float* w, int64_t ldw, int groups_per_cu, void* workspace,
-                           int64_t workspace_bytes, int32_t* err, void* stream);
+                           int em_mode, float em_eps, float* w, int64_t ldw, int groups_per_cu, int tile,
+                           void* workspace, int64_t workspace_bytes, int32_t* err, void* stream);
 
 /* Count thinning: out[j * out_stride] = number of positions[0..n_pos) that fall
  * in bin j, where bin j owns the copy positions [cum[j-1], cum[j]) (cum: the

@@ -80,8 +80,10 @@ SIGNATURES = {
     "jscc_cone_batch_groups": (c_int, [c_i64, c_int]),
     "jscc_cone_batch_workspace_bytes": (c_i64, [c_i64, c_int, c_i64, c_int]),
     "jscc_cone_proj_batched": (c_int, [c_int, vp, vp, c_i64, vp, c_int, vp, vp, vp, c_i64, vp, c_i64, c_int, c_f64,
-                                       vp, c_i64, c_f32, c_int, c_int, vp, c_i64, c_int, c_f32, vp, c_i64, c_int, vp,
-                                       c_i64, vp, vp]),
+                                       vp, c_i64, c_f32, c_int, c_int, vp, c_i64, c_int, c_f32, vp, c_i64, c_int,
+                                       c_int, vp, c_i64, vp, vp]),
+    "jscc_cone_batch_tiles": (c_int, [c_i64]),
+    "jscc_cone_batch_tile_shape": (c_int, [c_i64, c_int, vp, vp, vp]),
     "jscc_coarse_weight_workspace_bytes": (c_i64, [c_i64, c_i64]),
     "jscc_coarse_weight": (c_int, [vp, c_i64, vp, c_i64, vp, vp, c_int, c_int, vp, vp, c_int, vp, vp, c_int, c_int,
                                     vp, c_i64, c_i64, vp, vp, c_i64, c_i64, c_f32, c_int, c_int, vp, c_i64, c_f32, c_int,

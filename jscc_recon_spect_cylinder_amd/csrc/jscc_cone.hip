@@ -995,24 +995,56 @@ int launch_bp_one(const ConeBatchArgs& a, int G, hipStream_t s) {
   return 0;
 }
 
-template <int SIG, int AF, int BLOCK, int NCH>
-int launch_bp_mode(int mode, const ConeBatchArgs& a, int G, hipStream_t s) {
-  constexpr int R = NCH <= 4 ? 2 : 1;  // rows per step
+template <int SIG, int AF, int BLOCK, int NCH, int R>
+int launch_bp_shape(int mode, const ConeBatchArgs& a, int G, hipStream_t s) {
   if (mode == BP_OSEM) return launch_bp_one<SIG, AF, BLOCK, NCH, R, BP_OSEM>(a, G, s);
   return launch_bp_one<SIG, AF, BLOCK, NCH, R, BP_SUM>(a, G, s);
 }
 
+// Tile shapes (threads per work-group, voxel quads per thread, rows per step) by voxel count; shape 0 is
+// the default, the others are the autotuner's alternatives (every shape covers the row: BLOCK x NCH x 4 >= P).
+constexpr int kBpShapes[4][3][3] = {
+    {{256, 4, 2}, {256, 4, 1}, {512, 2, 2}},     // P <= 4096
+    {{512, 4, 2}, {512, 4, 1}, {1024, 2, 2}},    // P <= 8192
+    {{1024, 4, 2}, {1024, 4, 1}, {512, 8, 1}},   // P <= 16384
+    {{1024, 8, 1}, {0, 0, 0}, {0, 0, 0}}};       // P <= 32768
+
+int bp_class(int64_t P) { return P <= 4096 ? 0 : (P <= 8192 ? 1 : (P <= 16384 ? 2 : (P <= 32768 ? 3 : -1))); }
+int bp_shape_count(int64_t P) { const int c = bp_class(P); return c < 0 ? 0 : (c == 3 ? 1 : 3); }
+
 template <int SIG, int AF>
-int launch_bp_size(int mode, const ConeBatchArgs& a, int G, hipStream_t s) {
-  if (a.P <= 4 * 256 * 4) return launch_bp_mode<SIG, AF, 256, 4>(mode, a, G, s);
-  if (a.P <= 4 * 512 * 4) return launch_bp_mode<SIG, AF, 512, 4>(mode, a, G, s);
-  if (a.P <= 4 * 1024 * 4) return launch_bp_mode<SIG, AF, 1024, 4>(mode, a, G, s);
-  if (a.P <= 4 * 1024 * 8) return launch_bp_mode<SIG, AF, 1024, 8>(mode, a, G, s);
+int launch_bp_size(int mode, const ConeBatchArgs& a, int G, hipStream_t s, int tile) {
+  switch (bp_class(a.P) * 3 + tile) {
+    case 0: return launch_bp_shape<SIG, AF, 256, 4, 2>(mode, a, G, s);
+    case 1: return launch_bp_shape<SIG, AF, 256, 4, 1>(mode, a, G, s);
+    case 2: return launch_bp_shape<SIG, AF, 512, 2, 2>(mode, a, G, s);
+    case 3: return launch_bp_shape<SIG, AF, 512, 4, 2>(mode, a, G, s);
+    case 4: return launch_bp_shape<SIG, AF, 512, 4, 1>(mode, a, G, s);
+    case 5: return launch_bp_shape<SIG, AF, 1024, 2, 2>(mode, a, G, s);
+    case 6: return launch_bp_shape<SIG, AF, 1024, 4, 2>(mode, a, G, s);
+    case 7: return launch_bp_shape<SIG, AF, 1024, 4, 1>(mode, a, G, s);
+    case 8: return launch_bp_shape<SIG, AF, 512, 8, 1>(mode, a, G, s);
+    case 9: return launch_bp_shape<SIG, AF, 1024, 8, 1>(mode, a, G, s);
+    default: break;
+  }
   jscc_set_error("jscc_cone_proj_batched: %d voxels exceed the row limit (32768)", a.P);
   return -3;
 }
 
 }  // namespace
+
+extern "C" int jscc_cone_batch_tiles(int64_t n_vox) { return bp_shape_count(n_vox); }
+
+extern "C" int jscc_cone_batch_tile_shape(int64_t n_vox, int tile, int* threads, int* quads, int* rows_per_step) {
+  JSCC_REQUIRE(threads && quads && rows_per_step, "jscc_cone_batch_tile_shape: null pointer");
+  JSCC_REQUIRE(tile >= 0 && tile < bp_shape_count(n_vox), "jscc_cone_batch_tile_shape: tile %d of %d for %lld voxels",
+               tile, bp_shape_count(n_vox), (long long)n_vox);
+  const int* sh = kBpShapes[bp_class(n_vox)][tile];
+  *threads = sh[0];
+  *quads = sh[1];
+  *rows_per_step = sh[2];
+  return 0;
+}
 
 extern "C" int jscc_cone_batch_groups(int64_t n_rows, int groups_per_cu) {
   int64_t g = (int64_t)(groups_per_cu > 0 ? groups_per_cu : 4) * jscc_num_cus();
@@ -1030,8 +1062,8 @@ extern "C" int jscc_cone_proj_batched(int mode, const void* events, const int32_
                                       const float* coor_z, int64_t n_vox, const float* sysmat, int64_t ld_sysmat,
                                       int sigma_model, double fixed_sigma, float* x, int64_t ldx, float eps,
                                       int use_eps, int normalize, const float* sens, int64_t ld_sens, int em_mode,
-                                      float em_eps, float* w, int64_t ldw, int groups_per_cu, void* workspace,
-                                      int64_t workspace_bytes, int32_t* err, void* stream) {
+                                      float em_eps, float* w, int64_t ldw, int groups_per_cu, int tile,
+                                      void* workspace, int64_t workspace_bytes, int32_t* err, void* stream) {
   JSCC_REQUIRE(mode == BP_OSEM || mode == BP_SUM, "jscc_cone_proj_batched: bad mode");
   JSCC_REQUIRE(n_recon >= 1 && n_recon <= 65535, "jscc_cone_proj_batched: bad reconstruction count");
   JSCC_REQUIRE(n_rows >= 0 && n_rows < ((int64_t)1 << 31) && n_vox > 0 && n_vox % 4 == 0,
@@ -1049,6 +1081,8 @@ extern "C" int jscc_cone_proj_batched(int mode, const void* events, const int32_
   } else {
     JSCC_REQUIRE(w != nullptr && ldw >= n_vox, "jscc_cone_proj_batched: null weight output");
   }
+  JSCC_REQUIRE(tile >= 0 && tile < (bp_shape_count(n_vox) > 0 ? bp_shape_count(n_vox) : 1),
+               "jscc_cone_proj_batched: tile %d of %d for %lld voxels", tile, bp_shape_count(n_vox), (long long)n_vox);
   if (n_rows == 0) return 0;
   const int G = jscc_cone_batch_groups(n_rows, groups_per_cu);
   JSCC_REQUIRE(workspace && workspace_bytes >= jscc_cone_batch_workspace_bytes(n_rows, n_recon, n_vox, groups_per_cu),
@@ -1062,9 +1096,11 @@ extern "C" int jscc_cone_proj_batched(int mode, const void* events, const int32_
   int rc;
   jscc_prof_begin(JSCC_TAG_CONE_PROJ, s);
   if (sigma_model == SIG_STRICT) {
-    rc = sysmat ? launch_bp_size<SIG_STRICT, AF_ROW>(mode, a, G, s) : launch_bp_size<SIG_STRICT, AF_NONE>(mode, a, G, s);
+    rc = sysmat ? launch_bp_size<SIG_STRICT, AF_ROW>(mode, a, G, s, tile)
+                : launch_bp_size<SIG_STRICT, AF_NONE>(mode, a, G, s, tile);
   } else {
-    rc = sysmat ? launch_bp_size<SIG_FIXED, AF_ROW>(mode, a, G, s) : launch_bp_size<SIG_FIXED, AF_NONE>(mode, a, G, s);
+    rc = sysmat ? launch_bp_size<SIG_FIXED, AF_ROW>(mode, a, G, s, tile)
+                : launch_bp_size<SIG_FIXED, AF_NONE>(mode, a, G, s, tile);
   }
   jscc_prof_end(JSCC_TAG_CONE_PROJ, s);
   if (rc != 0) return rc;

@@ -482,6 +482,7 @@ class BatchedLazyStore:
     sigma_model: int
     fixed_sigma: float
     groups_per_cu: int = 0
+    tile: int = 0
 
     @classmethod
     def from_rows(cls, rows_list, sysmat=None, sigma_model=ops.SIG_FIXED, fixed_sigma=0.0):
@@ -521,7 +522,8 @@ class BatchedLazyStore:
     def _call(self, mode, ev_idx, row_off, **kw):
         ops.cone_proj_batched(mode, self.events, ev_idx, row_off, self.vox, sysmat=self.sysmat,
                               sigma_model=self.sigma_model, fixed_sigma=self.fixed_sigma,
-                              groups_per_cu=kw.pop("groups_per_cu", self.groups_per_cu), **kw)
+                              groups_per_cu=kw.pop("groups_per_cu", self.groups_per_cu),
+                              tile=kw.pop("tile", self.tile), **kw)
 
     def sensitivity(self) -> torch.Tensor:
         """[B, P] list-mode sensitivities: column sums of each reconstruction's normalised rows."""
@@ -531,27 +533,35 @@ class BatchedLazyStore:
         self._call(ops.BP_SUM, ev_idx, off, w=w)
         return w
 
-    def autotune(self, x: torch.Tensor, s: torch.Tensor, n_subsets: int, candidates=(1, 2, 4, 8)) -> int:
-        """Pick the work-groups per CU of the projection launch by timing one subset step of each
-        candidate on a copy of the images; the choice is kept in groups_per_cu and returned."""
+    def autotune(self, x: torch.Tensor, s: torch.Tensor, n_subsets: int, candidates=(1, 2, 4, 8),
+                 tiles=None, reps: int = 2):
+        """Pick the launch shape of the projection: every tile shape of ops.cone_batch_tiles (threads per
+        work-group, voxel quads per thread, rows per step) times every work-groups-per-CU candidate, each
+        timed on one subset step (the fastest of `reps` timed runs after a warm-up) on a copy of the
+        images.  The choice is kept (tile, groups_per_cu) and returned with its time in ms."""
         ev_idx, off = self.tables(n_subsets)[0]
-        best, best_t = candidates[0], float("inf")
-        for c in candidates:
-            xs = x.clone()
-            ws = ops.cone_batch_workspace(x.device, ev_idx.numel(), self.n_recon, self.vox.n, c)
-            self._call(ops.BP_OSEM, ev_idx, off, x=xs, sens=s, workspace=ws, groups_per_cu=c)  # warm
-            torch.cuda.synchronize()
-            t0 = torch.cuda.Event(enable_timing=True)
-            t1 = torch.cuda.Event(enable_timing=True)
-            t0.record()
-            self._call(ops.BP_OSEM, ev_idx, off, x=xs, sens=s, workspace=ws, groups_per_cu=c)
-            t1.record()
-            t1.synchronize()
-            t = t0.elapsed_time(t1)
-            if t < best_t:
-                best, best_t = c, t
-        self.groups_per_cu = best
-        return best
+        n_tiles = len(ops.cone_batch_tiles(self.vox.n))
+        tiles = range(n_tiles) if tiles is None else tiles
+        best, best_t = (self.tile, self.groups_per_cu), float("inf")
+        xs = x.clone()
+        for t in tiles:
+            for c in candidates:
+                ws = ops.cone_batch_workspace(x.device, ev_idx.numel(), self.n_recon, self.vox.n, c)
+                self._call(ops.BP_OSEM, ev_idx, off, x=xs, sens=s, workspace=ws, groups_per_cu=c, tile=t)  # warm
+                ts = []
+                for _ in range(reps):
+                    xs.copy_(x)
+                    t0 = torch.cuda.Event(enable_timing=True)
+                    t1 = torch.cuda.Event(enable_timing=True)
+                    t0.record()
+                    self._call(ops.BP_OSEM, ev_idx, off, x=xs, sens=s, workspace=ws, groups_per_cu=c, tile=t)
+                    t1.record()
+                    t1.synchronize()
+                    ts.append(t0.elapsed_time(t1))
+                if min(ts) < best_t:
+                    best, best_t = (t, c), min(ts)
+        self.tile, self.groups_per_cu = best
+        return self.tile, self.groups_per_cu, best_t
 
     def _iteration(self, x, s, n_subsets, eps, em_mode):
         """The launches of one OSEM iteration and the buffers they read (tables, workspace)."""

@@ -980,10 +980,11 @@ def cone_proj_batched(mode: int, events: torch.Tensor, ev_idx: torch.Tensor, row
                       *, x: torch.Tensor | None = None, sens: torch.Tensor | None = None, w: torch.Tensor | None = None,
                       sysmat=None, sigma_model: int = SIG_FIXED, fixed_sigma: float = 0.0, eps=None,
                       normalize: bool = True, em_mode: int = 0, em_eps: float = 1e-12, groups_per_cu: int = 0,
-                      workspace: torch.Tensor | None = None) -> None:
+                      tile: int = 0, workspace: torch.Tensor | None = None) -> None:
     """Batched no-store cone projection over B reconstructions (see include/jscc_hip.h):
     BP_OSEM updates the images x [B, ldx] in place (one OSEM subset step each),
-    BP_SUM writes the column sums of the normalised rows to w [B, ldw]."""
+    BP_SUM writes the column sums of the normalised rows to w [B, ldw].  groups_per_cu and tile (one of
+    cone_batch_tiles(P) shapes, 0 the default) choose the launch shape."""
     dev = events.device
     P = vox.n
     B = row_off.numel() - 1
@@ -1006,9 +1007,22 @@ def cone_proj_batched(mode: int, events: torch.Tensor, ev_idx: torch.Tensor, row
         vox.y.data_ptr(), vox.z.data_ptr(), P, _p(A), A.stride(0) if A is not None else 0, int(sigma_model),
         float(fixed_sigma), _p(x), x.stride(0) if x is not None else 0, float(eps) if eps is not None else 0.0,
         1 if eps is not None else 0, 1 if normalize else 0, _p(sens), sens.stride(0) if sens is not None else 0,
-        int(em_mode), float(em_eps), _p(w), w.stride(0) if w is not None else 0, int(groups_per_cu), ws.data_ptr(),
-        ws.numel(), err_word(dev).data_ptr(), stream_of(dev))
+        int(em_mode), float(em_eps), _p(w), w.stride(0) if w is not None else 0, int(groups_per_cu), int(tile),
+        ws.data_ptr(), ws.numel(), err_word(dev).data_ptr(), stream_of(dev))
     _lib.check(rc, "jscc_cone_proj_batched")
+
+
+def cone_batch_tiles(n_vox: int) -> list:
+    """The tile shapes (threads per work-group, voxel quads per thread, rows per step) of the batched
+    no-store projection at n_vox voxels; index 0 is the default."""
+    import ctypes
+    out = []
+    for t in range(int(_lib.lib().jscc_cone_batch_tiles(n_vox))):
+        v = [ctypes.c_int(0) for _ in range(3)]
+        _lib.check(_lib.lib().jscc_cone_batch_tile_shape(n_vox, t, *[ctypes.byref(c) for c in v]),
+                   "jscc_cone_batch_tile_shape")
+        out.append(tuple(c.value for c in v))
+    return out
 
 
 def cone_batch_workspace(dev, n_rows: int, n_recon: int, P: int, groups_per_cu: int = 0) -> torch.Tensor:
