@@ -904,12 +904,14 @@ def run_jsccd(args):
         # FP32 / BF16 rows: the selected and kept counts stay on the device (sync_counts=False), so the step
         # has no host synchronisation; the FP16 store is quantised from the selected FP32 rows, which reads
         # the counts
+        # the sensitivity (column sums of the kept, normalised rows) is formed by the cone pass itself
+        # (want_sens), from the rows as stored, instead of a second pass over T
         rows = build_compton_rows(None, det, coor, lst0.clone(), 0.0, 0.0, cl.e0, 0.02, tmax, tmin, tsum, dev,
                                   noise=(n1, n2), sigma_model="fixed", fixed_sigma=sigma, store_dtype=store_dtype,
-                                  sync_counts=False)
+                                  sync_counts=False, want_sens=True)
         last_rows[0] = rows
         st = ListModeStore.from_rows(rows, name)
-        s = st.backproject_ones()
+        s = rows.sens
         x = torch.ones_like(s)
         st.osem(x, s, args.iters, args.subsets)
         return st, x
@@ -978,8 +980,7 @@ def run_jsccd(args):
             "osem_launches_per_step": launches,
             "osem_bytes_per_launch": subset_bytes,
             "osem_GBs": subset_bytes / (proj_ms * 1e-3) / 1e9,
-            "colsum_ms": stot / max(scnt, 1),
-            "colsum_GBs": st.n_kept * P * b / (stot / max(scnt, 1) * 1e-3) / 1e9,
+            "sensitivity": "formed in the cone pass (want_sens)" if scnt == 0 else "column-sum pass",
             "osem_iter_s": 1.0 / osem_s,
             "osem_graph_iter_s": 1.0 / osem_graph_s,
             "osem_graph_equals_eager": graph_equal,
@@ -1062,10 +1063,10 @@ def run_jsccd(args):
         for name, sd in (("bf16", torch.bfloat16), ("fp16", torch.float32), ("fp32", torch.float32)):
             rows_s = build_compton_rows(None, det, coor, lst0[:n_s].clone(), 0.0, 0.0, cl.e0, 0.02, tmax, tmin, tsum,
                                         dev, noise=(n1[:n_s], n2[:n_s]), sigma_model="fixed", fixed_sigma=sigma,
-                                        store_dtype=sd, sync_counts=False)
+                                        store_dtype=sd, sync_counts=False, want_sens=True)
             kept_ok = bool(np.array_equal(rows_s.kept_event_index().cpu().numpy(), kept_ref))
             st = ListModeStore.from_rows(rows_s, name)
-            s_s = st.backproject_ones()
+            s_s = rows_s.sens
             x_s = st.osem(torch.ones_like(s_s), s_s, args.iters, args.subsets)
             e = _rel(x_s.cpu().numpy(), x_ref)
             errs[name] = {"rel_l2": e, "bar": 1e-4 if name == "fp32" else 1e-3,

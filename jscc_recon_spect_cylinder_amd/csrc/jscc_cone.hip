@@ -20,6 +20,8 @@
 //   * rows are written unnormalised (FP32 or BF16) and the row statistics
 //     S = sum t, Q = sum t^2, Sc = sum t_c and max t are reduced in FP64; the
 //     normalisation 1/S is applied by whoever consumes the row.
+#include <type_traits>
+
 #include "jscc_common.h"
 #include "jscc_events.h"
 
@@ -198,6 +200,76 @@ __device__ __forceinline__ void load_taps(int base, int P, const int32_t* __rest
   }
 }
 
+// One tile of 4 voxels for the E events of a work-group: the row values (stored when valid) and the row
+// statistics (S, Q in FP64, Sc, max) accumulated per thread.
+template <int SIG, int AF, int OUT_DT, bool WANT_TC, int E>
+__device__ __forceinline__ void cone_rows_tile_at(const ConeEv* evs, const bool* valid, int row0, int base, int P,
+                                                  float4 vx4, float4 vy4, float4 vz4, const float* __restrict__ A,
+                                                  const int32_t* __restrict__ tap_i0,
+                                                  const int32_t* __restrict__ tap_i1,
+                                                  const float* __restrict__ tap_w0, const float* __restrict__ tap_w1,
+                                                  const ConeConsts& cc, void* __restrict__ out_t, int64_t ldt,
+                                                  float* __restrict__ out_tc, int64_t ldtc, float* accS, double* accQ,
+                                                  float* accC, float* accM) {
+  const float vcx[4] = {vx4.x, vx4.y, vx4.z, vx4.w};
+  const float vcy[4] = {vy4.x, vy4.y, vy4.z, vy4.w};
+  const float vcz[4] = {vz4.x, vz4.y, vz4.z, vz4.w};
+  float tw0[4], tw1[4];
+  int ti0[4], ti1[4];
+  load_taps<AF>(base, P, tap_i0, tap_i1, tap_w0, tap_w1, ti0, ti1, tw0, tw1);
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    float av[4];
+    afactor4<AF>(evs[e], A, base, ti0, ti1, tw0, tw1, av);
+    float tv[4], tcv[4];
+    cone_tc4<SIG, false>(evs[e], vcx, vcy, vcz, cc, tcv, nullptr);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) tv[k] = tcv[k] * av[k];
+    if (base + 4 > P) {  // the padded tail of the last tile (P not a multiple of 4)
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (base + k >= P) { tcv[k] = 0.0f; tv[k] = 0.0f; }
+    }
+    if (valid[e]) {
+      const int64_t r = (int64_t)(row0 + e);
+      if (OUT_DT == JSCC_DT_F32) {
+        *reinterpret_cast<float4*>(reinterpret_cast<float*>(out_t) + r * ldt + base) =
+            make_float4(tv[0], tv[1], tv[2], tv[3]);
+      } else if (OUT_DT == JSCC_DT_BF16) {
+        uint2 pk;
+        pk.x = jscc_pack_bf16x2(tv[0], tv[1]);
+        pk.y = jscc_pack_bf16x2(tv[2], tv[3]);
+        *reinterpret_cast<uint2*>(reinterpret_cast<uint16_t*>(out_t) + r * ldt + base) = pk;
+      }
+      if (WANT_TC) {
+        *reinterpret_cast<float4*>(out_tc + r * ldtc + base) = make_float4(tcv[0], tcv[1], tcv[2], tcv[3]);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      accS[e] += tv[k];
+      accQ[e] = fma((double)tv[k], (double)tv[k], accQ[e]);
+      if (AF != AF_NONE) accC[e] += tcv[k];  // without an A factor t = t_c: Sc = S (set below)
+      accM[e] = fmaxf(accM[e], tv[k]);  // fmaxf drops NaN; NaN rows are caught by S
+    }
+  }
+}
+
+template <int SIG, int AF, int OUT_DT, bool WANT_TC, int E>
+__device__ __forceinline__ void cone_rows_tile(const ConeEv* evs, const bool* valid, int row0, int base, int P,
+                                               const float* __restrict__ cx, const float* __restrict__ cy,
+                                               const float* __restrict__ cz, const float* __restrict__ A,
+                                               const int32_t* __restrict__ tap_i0, const int32_t* __restrict__ tap_i1,
+                                               const float* __restrict__ tap_w0, const float* __restrict__ tap_w1,
+                                               const ConeConsts& cc, void* __restrict__ out_t, int64_t ldt,
+                                               float* __restrict__ out_tc, int64_t ldtc, float* accS, double* accQ,
+                                               float* accC, float* accM) {
+  cone_rows_tile_at<SIG, AF, OUT_DT, WANT_TC, E>(evs, valid, row0, base, P, *reinterpret_cast<const float4*>(cx + base),
+                                                 *reinterpret_cast<const float4*>(cy + base),
+                                                 *reinterpret_cast<const float4*>(cz + base), A, tap_i0, tap_i1,
+                                                 tap_w0, tap_w1, cc, out_t, ldt, out_tc, ldtc, accS, accQ, accC, accM);
+}
+
 template <int SIG, int AF, int OUT_DT, bool WANT_TC, int E>
 __global__ void __launch_bounds__(kConeThreads) k_cone_rows(
     const JsccEvent* __restrict__ ev, const int32_t* __restrict__ ev_idx, int n_rows,
@@ -236,53 +308,9 @@ __global__ void __launch_bounds__(kConeThreads) k_cone_rows(
 #pragma unroll
   for (int e = 0; e < E; ++e) { accS[e] = 0.0f; accQ[e] = 0.0; accC[e] = 0.0f; accM[e] = 0.0f; }
 
-  for (int base = threadIdx.x * kConeVec; base < P; base += kConeThreads * kConeVec) {
-    const float4 vx4 = *reinterpret_cast<const float4*>(cx + base);
-    const float4 vy4 = *reinterpret_cast<const float4*>(cy + base);
-    const float4 vz4 = *reinterpret_cast<const float4*>(cz + base);
-    const float vcx[4] = {vx4.x, vx4.y, vx4.z, vx4.w};
-    const float vcy[4] = {vy4.x, vy4.y, vy4.z, vy4.w};
-    const float vcz[4] = {vz4.x, vz4.y, vz4.z, vz4.w};
-    float tw0[4], tw1[4];
-    int ti0[4], ti1[4];
-    load_taps<AF>(base, P, tap_i0, tap_i1, tap_w0, tap_w1, ti0, ti1, tw0, tw1);
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-      float av[4];
-      afactor4<AF>(evs[e], A, base, ti0, ti1, tw0, tw1, av);
-      float tv[4], tcv[4];
-      cone_tc4<SIG, false>(evs[e], vcx, vcy, vcz, cc, tcv, nullptr);
-#pragma unroll
-      for (int k = 0; k < 4; ++k) tv[k] = tcv[k] * av[k];
-      if (base + 4 > P) {  // the padded tail of the last tile (P not a multiple of 4)
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-          if (base + k >= P) { tcv[k] = 0.0f; tv[k] = 0.0f; }
-      }
-      if (valid[e]) {
-        const int64_t r = (int64_t)(row0 + e);
-        if (OUT_DT == JSCC_DT_F32) {
-          *reinterpret_cast<float4*>(reinterpret_cast<float*>(out_t) + r * ldt + base) =
-              make_float4(tv[0], tv[1], tv[2], tv[3]);
-        } else if (OUT_DT == JSCC_DT_BF16) {
-          uint2 pk;
-          pk.x = jscc_pack_bf16x2(tv[0], tv[1]);
-          pk.y = jscc_pack_bf16x2(tv[2], tv[3]);
-          *reinterpret_cast<uint2*>(reinterpret_cast<uint16_t*>(out_t) + r * ldt + base) = pk;
-        }
-        if (WANT_TC) {
-          *reinterpret_cast<float4*>(out_tc + r * ldtc + base) = make_float4(tcv[0], tcv[1], tcv[2], tcv[3]);
-        }
-      }
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        accS[e] += tv[k];
-        accQ[e] = fma((double)tv[k], (double)tv[k], accQ[e]);
-        if (AF != AF_NONE) accC[e] += tcv[k];  // without an A factor t = t_c: Sc = S (set below)
-        accM[e] = fmaxf(accM[e], tv[k]);  // fmaxf drops NaN; NaN rows are caught by S
-      }
-    }
-  }
+  for (int base = threadIdx.x * kConeVec; base < P; base += kConeThreads * kConeVec)
+    cone_rows_tile<SIG, AF, OUT_DT, WANT_TC, E>(evs, valid, row0, base, P, cx, cy, cz, A, tap_i0, tap_i1, tap_w0,
+                                                tap_w1, cc, out_t, ldt, out_tc, ldtc, accS, accQ, accC, accM);
 
   if (AF == AF_NONE) {
 #pragma unroll
@@ -347,6 +375,262 @@ int dispatch_out(int out_dtype, bool want_tc, int n_rows, hipStream_t s, const J
   }
 #undef JSCC_CONE_CASE
   return 0;
+}
+
+// ----------------------------------------------------------------------------
+// Cone rows with the list-mode sensitivity in the same pass.  A persistent
+// grid (kSensBlocksPerCu work-groups of 512 threads per CU; 1 of 1024 above
+// 16384 voxels) takes groups of E rows from a ticket counter, so work-groups
+// on slower CUs simply take fewer groups (a static split measured 25 % between
+// the fastest and slowest work-group).  Every row is evaluated and stored as in
+// k_cone_rows, its statistics reduced, the keep decision of
+// jscc_rows_keep_flags taken (f = 1/S for a kept row, 0 otherwise).  The rows
+// of group g are read back (each thread its own stored values, the same bits
+// the column-sum pass would read) during the tile loop of the work-group's
+// next group, one voxel quad per tile, so the loads have a whole tile of cone
+// arithmetic to arrive; f x value is added into the work-group's LDS column
+// sums (each thread owns its quads: no barrier).  At the end every work-group
+// writes its column sums and k_sens_reduce adds them in work-group order.
+// The split of groups between work-groups follows the ticket order, so the
+// FP32 grouping of the sums (the last bits of sens) can differ between runs.
+// ----------------------------------------------------------------------------
+constexpr int kSensBlocksPerCu = 2;  // 512-thread work-groups per CU (64 KB of column sums each)
+
+template <int THREADS>
+constexpr int sens_blocks_per_cu() { return THREADS == 512 ? kSensBlocksPerCu : 1; }
+
+__global__ void k_sens_ticket_reset(int* ticket) { *ticket = 0; }
+
+template <int SIG, int AF, int OUT_DT, int THREADS>
+__global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) k_cone_rows_sens(
+    const JsccEvent* __restrict__ ev, const int32_t* __restrict__ ev_idx, int n_grid,
+    const int32_t* __restrict__ n_rows_dev, const float* __restrict__ cx, const float* __restrict__ cy,
+    const float* __restrict__ cz, int P, const float* __restrict__ A, int64_t lda, ConeConsts cc,
+    void* __restrict__ out_t, int64_t ldt, double* __restrict__ stats, double min_support,
+    float* __restrict__ partial, int64_t ldp, int* __restrict__ ticket) {
+  constexpr int E = 4;
+  constexpr int NW = THREADS / JSCC_WAVE;
+  using Raw = typename std::conditional<OUT_DT == JSCC_DT_F32, float4, uint2>::type;
+  extern __shared__ float4 sacc[];  // [ldp / 4] column sums; quad q is owned by thread q % THREADS
+  __shared__ double red[4][NW][E];
+  __shared__ float fsh[E];
+  __shared__ int next_g;
+  const int tid = threadIdx.x;
+  int n_rows = n_grid;
+  if (n_rows_dev) {
+    const int nd = *n_rows_dev;
+    n_rows = nd < 0 ? 0 : (nd < n_grid ? nd : n_grid);
+  }
+  // rows of the grid past the device count: NaN statistics (dropped by the keep decision)
+  for (int64_t r = (int64_t)n_rows + (int64_t)blockIdx.x * THREADS + tid; r < n_grid;
+       r += (int64_t)gridDim.x * THREADS) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) stats[r * 4 + k] = __builtin_nan("");
+  }
+  const int n_quads = (int)(ldp / 4);
+  for (int q = tid; q < n_quads; q += THREADS) sacc[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+  const int n_groups = (n_rows + E - 1) / E;
+  if (tid == 0) next_g = atomicAdd(ticket, 1);
+  __syncthreads();
+  int g = next_g;
+  __syncthreads();
+  // the previous group of this work-group: its first row and the factors f of its rows (uniform)
+  int prev_row0 = -1;
+  float pf[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) pf[e] = 0.0f;
+
+  while (g < n_groups) {
+    const int row0 = g * E;
+    ConeEv evs[E];
+    bool valid[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      valid[e] = (row0 + e) < n_rows;
+      const int r = valid[e] ? (row0 + e) : (n_rows - 1);
+      evs[e] = load_event(ev[ev_idx[r]], lda);
+    }
+    float accS[E], accC[E], accM[E];
+    double accQ[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) { accS[e] = 0.0f; accQ[e] = 0.0; accC[e] = 0.0f; accM[e] = 0.0f; }
+    // rows of the previous group, as stored (a dropped row reads the group's first row and adds nothing;
+    // without a previous group every pointer is row 0)
+    const Raw* prow[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int64_t r = prev_row0 < 0 ? 0 : (int64_t)(pf[e] != 0.0f ? prev_row0 + e : prev_row0);
+      prow[e] = reinterpret_cast<const Raw*>(reinterpret_cast<const char*>(out_t) +
+                                             r * ldt * (int64_t)(sizeof(Raw) / 4));
+    }
+    for (int q = tid; q * kConeVec < P; q += THREADS) {
+      const int base = q * kConeVec;
+      const float4 vx4 = *reinterpret_cast<const float4*>(cx + base);
+      const float4 vy4 = *reinterpret_cast<const float4*>(cy + base);
+      const float4 vz4 = *reinterpret_cast<const float4*>(cz + base);
+      // the read-back loads are issued after the coordinate loads (memory clobber: no reordering), so waiting
+      // for the coordinates does not wait for them (loads complete in order)
+      asm volatile("" ::: "memory");
+      // unconditional (a branch here would make the coordinate wait drain them too); before the first group
+      // they read row 0, which is allocated, and nothing is added
+      Raw rb[E];
+#pragma unroll
+      for (int e = 0; e < E; ++e) rb[e] = prow[e][q];
+      cone_rows_tile_at<SIG, AF, OUT_DT, false, E>(evs, valid, row0, base, P, vx4, vy4, vz4, A, nullptr, nullptr,
+                                                   nullptr, nullptr, cc, out_t, ldt, nullptr, 0, accS, accQ, accC,
+                                                   accM);
+      if (prev_row0 >= 0) {
+        float4 a = sacc[q];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          if (pf[e] != 0.0f) {  // uniform: dropped rows add nothing
+            float v[4];
+            if constexpr (OUT_DT == JSCC_DT_F32) {
+              v[0] = rb[e].x; v[1] = rb[e].y; v[2] = rb[e].z; v[3] = rb[e].w;
+            } else {
+              v[0] = __uint_as_float(rb[e].x << 16); v[1] = __uint_as_float(rb[e].x & 0xffff0000u);
+              v[2] = __uint_as_float(rb[e].y << 16); v[3] = __uint_as_float(rb[e].y & 0xffff0000u);
+            }
+            a.x = fmaf(pf[e], v[0], a.x); a.y = fmaf(pf[e], v[1], a.y);
+            a.z = fmaf(pf[e], v[2], a.z); a.w = fmaf(pf[e], v[3], a.w);
+          }
+        }
+        sacc[q] = a;
+      }
+    }
+    if (AF == AF_NONE) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) accC[e] = accS[e];
+    }
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const double sS = jscc_wave_sum((double)accS[e]);
+      const double sQ = jscc_wave_sum(accQ[e]);
+      const double sC = jscc_wave_sum((double)accC[e]);
+      const float sM = jscc_wave_max(accM[e]);
+      if (jscc_lane() == 0) {
+        red[0][jscc_wave_id()][e] = sS;
+        red[1][jscc_wave_id()][e] = sQ;
+        red[2][jscc_wave_id()][e] = sC;
+        red[3][jscc_wave_id()][e] = (double)sM;
+      }
+    }
+    if (tid == 0) next_g = atomicAdd(ticket, 1);
+    __syncthreads();
+    const int g_next = next_g;
+    if (tid < E) {
+      const int e = tid;
+      float f = 0.0f;
+      if (row0 + e < n_rows) {
+        double v[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          double t = red[k][0][e];
+          for (int w = 1; w < NW; ++w) t = (k == 3) ? fmax(t, red[k][w][e]) : t + red[k][w][e];
+          v[k] = t;
+          stats[(int64_t)(row0 + e) * 4 + k] = t;
+        }
+        if (jscc_row_keep(v[0], v[1], v[3], min_support)) f = (float)(1.0 / v[0]);
+      }
+      fsh[e] = f;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < E; ++e) pf[e] = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(fsh[e])));
+    prev_row0 = row0;
+    g = g_next;
+  }
+  // the rows of the work-group's last group
+  if (prev_row0 >= 0) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      if (pf[e] != 0.0f) {
+        const Raw* pr = reinterpret_cast<const Raw*>(reinterpret_cast<const char*>(out_t) +
+                                                     (int64_t)(prev_row0 + e) * ldt * (int64_t)(sizeof(Raw) / 4));
+        for (int q = tid; q * kConeVec < P; q += THREADS) {
+          const Raw w = pr[q];
+          float v[4];
+          if constexpr (OUT_DT == JSCC_DT_F32) {
+            v[0] = w.x; v[1] = w.y; v[2] = w.z; v[3] = w.w;
+          } else {
+            v[0] = __uint_as_float(w.x << 16); v[1] = __uint_as_float(w.x & 0xffff0000u);
+            v[2] = __uint_as_float(w.y << 16); v[3] = __uint_as_float(w.y & 0xffff0000u);
+          }
+          float4 a = sacc[q];
+          a.x = fmaf(pf[e], v[0], a.x); a.y = fmaf(pf[e], v[1], a.y);
+          a.z = fmaf(pf[e], v[2], a.z); a.w = fmaf(pf[e], v[3], a.w);
+          sacc[q] = a;
+        }
+      }
+    }
+  }
+  float4* dst = reinterpret_cast<float4*>(partial + (int64_t)blockIdx.x * ldp);
+  for (int q = tid; q < n_quads; q += THREADS) dst[q] = sacc[q];
+}
+
+// sens[p] = sum over work-groups b (in order) of partial[b, p]: 64 columns per block, 16 groups of 16 threads
+// sum every 16th partial row, then one thread per column quad adds the 16 group sums in order.
+__global__ void __launch_bounds__(256) k_sens_reduce(const float* __restrict__ partial, int G, int64_t ldp, int P,
+                                                     float* __restrict__ sens) {
+  __shared__ float4 acc[16][16];
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  const int col = blockIdx.x * 64 + tx * 4;
+  float4 sum = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (col < P) {
+    for (int b = ty; b < G; b += 16) {
+      const float4 v = *reinterpret_cast<const float4*>(partial + (int64_t)b * ldp + col);
+      sum.x += v.x; sum.y += v.y; sum.z += v.z; sum.w += v.w;
+    }
+  }
+  acc[ty][tx] = sum;
+  __syncthreads();
+  if (ty == 0 && col < P) {
+    float4 t = acc[0][tx];
+#pragma unroll
+    for (int k = 1; k < 16; ++k) {
+      t.x += acc[k][tx].x; t.y += acc[k][tx].y; t.z += acc[k][tx].z; t.w += acc[k][tx].w;
+    }
+    const float tv[4] = {t.x, t.y, t.z, t.w};
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (col + k < P) sens[col + k] = tv[k];
+  }
+}
+
+// threads per work-group of the fused pass at P voxels (0: too many voxels)
+int sens_threads(int64_t P) { return P <= 16384 ? 512 : (P <= 32768 ? 1024 : 0); }
+int sens_grid(int64_t P) {
+  return (sens_threads(P) == 512 ? kSensBlocksPerCu : 1) * jscc_num_cus();
+}
+size_t sens_lds(int64_t P) { return (size_t)((P + 3) / 4) * sizeof(float4); }
+
+template <int SIG, int AF, int OUT_DT, int THREADS>
+int launch_sens_t(int G, hipStream_t s, const JsccEvent* ev, const int32_t* idx, int n_grid, const int32_t* n_dev,
+                  const float* cx, const float* cy, const float* cz, int P, const float* A, int64_t lda,
+                  ConeConsts cc, void* out_t, int64_t ldt, double* stats, double min_support, float* partial,
+                  int64_t ldp, int* ticket) {
+  static bool attr = false;  // more than 64 KB of dynamic LDS needs the attribute (set once)
+  if (sens_lds(P) > 65536 && !attr) {
+    JSCC_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_cone_rows_sens<SIG, AF, OUT_DT, THREADS>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)sens_lds(32768)));
+    attr = true;
+  }
+  hipLaunchKernelGGL(k_sens_ticket_reset, dim3(1), dim3(1), 0, s, ticket);
+  hipLaunchKernelGGL((k_cone_rows_sens<SIG, AF, OUT_DT, THREADS>), dim3(G), dim3(THREADS), sens_lds(P), s, ev, idx,
+                     n_grid, n_dev, cx, cy, cz, P, A, lda, cc, out_t, ldt, stats, min_support, partial, ldp, ticket);
+  return 0;
+}
+
+template <int SIG, int AF, int OUT_DT>
+int launch_sens(int G, hipStream_t s, const JsccEvent* ev, const int32_t* idx, int n_grid, const int32_t* n_dev,
+                const float* cx, const float* cy, const float* cz, int P, const float* A, int64_t lda, ConeConsts cc,
+                void* out_t, int64_t ldt, double* stats, double min_support, float* partial, int64_t ldp,
+                int* ticket) {
+  if (sens_threads(P) == 512)
+    return launch_sens_t<SIG, AF, OUT_DT, 512>(G, s, ev, idx, n_grid, n_dev, cx, cy, cz, P, A, lda, cc, out_t, ldt,
+                                               stats, min_support, partial, ldp, ticket);
+  return launch_sens_t<SIG, AF, OUT_DT, 1024>(G, s, ev, idx, n_grid, n_dev, cx, cy, cz, P, A, lda, cc, out_t, ldt,
+                                              stats, min_support, partial, ldp, ticket);
 }
 
 }  // namespace
@@ -438,6 +722,81 @@ extern "C" int jscc_cone_rows_counted(const void* events, const int32_t* ev_idx,
   return cone_rows_impl(events, ev_idx, n_rows_max, n_rows_dev, coor_x, coor_y, coor_z, n_vox, sysmat, ld_sysmat,
                         tap_i0, tap_i1, tap_w0, tap_w1, sigma_model, fixed_sigma, out_t, out_dtype, ld_out, out_tc,
                         ld_tc, stats, stream);
+}
+
+extern "C" int jscc_cone_sens_grid(int64_t n_vox) { return sens_threads(n_vox) ? sens_grid(n_vox) : 0; }
+
+extern "C" int64_t jscc_cone_sens_workspace_bytes(int64_t n_vox) {
+  // the work-groups' column sums, then the ticket counter
+  return (int64_t)jscc_cone_sens_grid(n_vox) * ((n_vox + 3) / 4 * 4) * (int64_t)sizeof(float) + 256;
+}
+
+extern "C" int jscc_cone_rows_sens(const void* events, const int32_t* ev_idx, const int32_t* n_rows_dev,
+                                   int64_t n_rows_max, const float* coor_x, const float* coor_y, const float* coor_z,
+                                   int64_t n_vox, const float* sysmat, int64_t ld_sysmat, int sigma_model,
+                                   double fixed_sigma, void* out_t, int out_dtype, int64_t ld_out, double* stats,
+                                   double min_support, float* sens, void* workspace, int64_t workspace_bytes,
+                                   void* stream) {
+  JSCC_REQUIRE(n_rows_max >= 0 && n_rows_max < ((int64_t)1 << 31), "jscc_cone_rows_sens: n_rows out of range");
+  JSCC_REQUIRE(events && ev_idx && coor_x && coor_y && coor_z && stats && out_t && sens,
+               "jscc_cone_rows_sens: null pointer");
+  JSCC_REQUIRE(n_vox > 0 && sens_threads(n_vox) > 0, "jscc_cone_rows_sens: %lld voxels (1 to 32768)",
+               (long long)n_vox);
+  JSCC_REQUIRE(((uintptr_t)coor_x | (uintptr_t)coor_y | (uintptr_t)coor_z) % 16 == 0,
+               "jscc_cone_rows_sens: coordinate arrays must be 16-byte aligned and padded to a multiple of 4");
+  JSCC_REQUIRE(sigma_model == SIG_STRICT || sigma_model == SIG_FIXED, "jscc_cone_rows_sens: bad sigma model");
+  JSCC_REQUIRE(out_dtype == JSCC_DT_F32 || out_dtype == JSCC_DT_BF16,
+               "jscc_cone_rows_sens: rows are stored as FP32 or BF16");
+  JSCC_REQUIRE(ld_out >= n_vox && ld_out % 4 == 0 && (uintptr_t)out_t % 16 == 0,
+               "jscc_cone_rows_sens: out_t must be 16-byte aligned with ld >= n_vox, a multiple of 4");
+  if (sysmat) {
+    JSCC_REQUIRE(ld_sysmat >= n_vox && ld_sysmat % 4 == 0 && (uintptr_t)sysmat % 16 == 0,
+                 "jscc_cone_rows_sens: sysmat rows must be 16-byte aligned with ld a multiple of 4");
+  }
+  if (sigma_model == SIG_FIXED) JSCC_REQUIRE(fixed_sigma > 0.0, "jscc_cone_rows_sens: fixed sigma must be positive");
+  JSCC_REQUIRE(workspace && workspace_bytes >= jscc_cone_sens_workspace_bytes(n_vox) && (uintptr_t)workspace % 16 == 0,
+               "jscc_cone_rows_sens: workspace too small or misaligned");
+  hipStream_t s = (hipStream_t)stream;
+  const int P = (int)n_vox;
+  const int64_t ldp = (n_vox + 3) / 4 * 4;
+  if (n_rows_max == 0) {  // no rows: zero sums (the reduction over no partial rows)
+    hipLaunchKernelGGL(k_sens_reduce, dim3((unsigned)((P + 63) / 64)), dim3(256), 0, s, (const float*)workspace, 0,
+                       ldp, P, sens);
+    JSCC_CHECK_LAUNCH();
+    return 0;
+  }
+  const ConeConsts cc = make_consts(sigma_model, fixed_sigma);
+  const JsccEvent* ev = (const JsccEvent*)events;
+  const int G = sens_grid(n_vox);
+  const int n = (int)n_rows_max;
+  float* partial = (float*)workspace;
+  int* ticket = (int*)((char*)workspace + (int64_t)G * ldp * (int64_t)sizeof(float));
+  int rc = 0;
+  jscc_prof_begin(JSCC_TAG_CONE, s);
+#define JSCC_SENS(SIG, AF, DT)                                                                                        \
+  rc = launch_sens<SIG, AF, DT>(G, s, ev, ev_idx, n, n_rows_dev, coor_x, coor_y, coor_z, P, sysmat, ld_sysmat, cc, \
+                                out_t, ld_out, stats, min_support, partial, ldp, ticket)
+  if (sigma_model == SIG_STRICT) {
+    if (sysmat) {
+      if (out_dtype == JSCC_DT_F32) JSCC_SENS(SIG_STRICT, AF_ROW, JSCC_DT_F32); else JSCC_SENS(SIG_STRICT, AF_ROW, JSCC_DT_BF16);
+    } else {
+      if (out_dtype == JSCC_DT_F32) JSCC_SENS(SIG_STRICT, AF_NONE, JSCC_DT_F32); else JSCC_SENS(SIG_STRICT, AF_NONE, JSCC_DT_BF16);
+    }
+  } else {
+    if (sysmat) {
+      if (out_dtype == JSCC_DT_F32) JSCC_SENS(SIG_FIXED, AF_ROW, JSCC_DT_F32); else JSCC_SENS(SIG_FIXED, AF_ROW, JSCC_DT_BF16);
+    } else {
+      if (out_dtype == JSCC_DT_F32) JSCC_SENS(SIG_FIXED, AF_NONE, JSCC_DT_F32); else JSCC_SENS(SIG_FIXED, AF_NONE, JSCC_DT_BF16);
+    }
+  }
+#undef JSCC_SENS
+  if (rc != 0) return rc;
+  JSCC_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_sens_reduce, dim3((unsigned)((P + 63) / 64)), dim3(256), 0, s, (const float*)partial, G, ldp, P,
+                     sens);
+  jscc_prof_end(JSCC_TAG_CONE, s);
+  JSCC_CHECK_LAUNCH();
+  return 0;
 }
 
 // ============================================================================

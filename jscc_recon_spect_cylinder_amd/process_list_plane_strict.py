@@ -75,6 +75,8 @@ class ComptonRows:
     # sync_counts=False builds: the selected and kept counts as device int32 until resolve_counts() reads them
     # (n_sel, n_kept are -1 until then; store, stats, keep cover every input event)
     counts_dev: tuple | None = None
+    # want_sens builds: column sums of the kept rows divided by their sums (the list-mode sensitivity), [P] FP32
+    sens: torch.Tensor | None = None
 
     def resolve_counts(self) -> "ComptonRows":
         """Read deferred counts (one synchronisation, with the index check of the event pass)."""
@@ -143,7 +145,7 @@ def build_compton_rows(sysmat, detector, coor, list_origin, delta_r1, delta_r2, 
                        ene_threshold_max, ene_threshold_min, ene_threshold_sum, device, *, noise="cpu",
                        sigma_model="strict", fixed_sigma=None, store_dtype=torch.float32, want_tc=False,
                        taps=None, sysmat_dev=None, coor_dev=None, mutate_list=True,
-                       sync_counts=True) -> ComptonRows:
+                       sync_counts=True, want_sens=False) -> ComptonRows:
     """Cone rows of a list chunk on the GPU (see ComptonRows).
 
     noise: "cpu" (two torch.randn on the CPU generator, as the strict
@@ -154,7 +156,9 @@ def build_compton_rows(sysmat, detector, coor, list_origin, delta_r1, delta_r2, 
     does not).  sync_counts=False: no host synchronisation -- the cone launch reads the selected count on
     the device and covers every input event (rows past the count are dropped by the keep pass), and the
     selected and kept counts stay on the device (ComptonRows.counts_dev) until resolve_counts(), which also
-    raises the index errors of the event pass.
+    raises the index errors of the event pass.  want_sens: the cone pass also forms ComptonRows.sens, the column
+    sums of the kept rows divided by their sums (the list-mode sensitivity of main_plane.py:348-355 before its
+    scale), from each row as stored, so no second pass over T is needed (FP32 / BF16 store, no taps, no t_c).
     """
     dev = _device_of(device)
     if mutate_list:
@@ -180,22 +184,34 @@ def build_compton_rows(sysmat, detector, coor, list_origin, delta_r1, delta_r2, 
     sig = ops.SIG_STRICT if sigma_model == "strict" else ops.SIG_FIXED
     if sig == ops.SIG_FIXED and not fixed_sigma:
         raise ValueError("sigma_model='fixed' needs fixed_sigma (rad)")
-    if not sync_counts and n > 0:
-        store, tc_store, stats, ld = ops.cone_rows(events, sel_idx, n, vox, sysmat=sysmat_dev, taps=taps,
+    if want_sens and (taps is not None or want_tc or store_dtype not in (torch.float32, torch.bfloat16)):
+        raise ValueError("want_sens needs an FP32 or BF16 store, no tap tables and no t_c rows")
+
+    def rows_of(n_rows, n_dev):
+        if want_sens:
+            store, stats, ld, sens = ops.cone_rows_sens(events, sel_idx, n_rows, vox, MIN_EVENT_EFFECTIVE_SUPPORT,
+                                                        sysmat=sysmat_dev, sigma_model=sig,
+                                                        fixed_sigma=fixed_sigma or 0.0, out_dtype=store_dtype,
+                                                        n_dev=n_dev)
+            return store, None, stats, ld, sens
+        store, tc_store, stats, ld = ops.cone_rows(events, sel_idx, n_rows, vox, sysmat=sysmat_dev, taps=taps,
                                                    sigma_model=sig, fixed_sigma=fixed_sigma or 0.0,
-                                                   out_dtype=store_dtype, want_tc=want_tc, n_dev=sel_count)
+                                                   out_dtype=store_dtype, want_tc=want_tc, n_dev=n_dev)
+        return store, tc_store, stats, ld, None
+
+    if not sync_counts and n > 0:
+        store, tc_store, stats, ld, sens = rows_of(n, sel_count)
         keep, inv_sum = ops.rows_keep(stats, n, MIN_EVENT_EFFECTIVE_SUPPORT, want_inv_sum=True)
         kept, kept_count = ops.compact(keep)
         return ComptonRows(store, tc_store, ld, vox.n, stats, events, sel_idx, -1, kept, -1, inv_sum, dev, vox,
-                           (sel_count, kept_count))
+                           (sel_count, kept_count), sens)
     n_sel = ops.read_count_checked(sel_count)  # one synchronisation for the count and the index check
-    store, tc_store, stats, ld = ops.cone_rows(events, sel_idx, n_sel, vox, sysmat=sysmat_dev, taps=taps,
-                                               sigma_model=sig, fixed_sigma=fixed_sigma or 0.0,
-                                               out_dtype=store_dtype, want_tc=want_tc)
+    store, tc_store, stats, ld, sens = rows_of(n_sel, None)
     keep, inv_sum = ops.rows_keep(stats, n_sel, MIN_EVENT_EFFECTIVE_SUPPORT, want_inv_sum=True)
     kept, kept_count = ops.compact(keep)
     n_kept = int(kept_count.item())
-    return ComptonRows(store, tc_store, ld, vox.n, stats, events, sel_idx, n_sel, kept, n_kept, inv_sum, dev, vox)
+    return ComptonRows(store, tc_store, ld, vox.n, stats, events, sel_idx, n_sel, kept, n_kept, inv_sum, dev, vox,
+                       None, sens)
 
 
 def get_compton_backproj_list_single(sysmat, detector, coor, list_origin, delta_r1, delta_r2, e0, ene_resolution,

@@ -1,0 +1,136 @@
+"""The cone pass that also forms the list-mode sensitivity (jscc_cone_rows_sens, build_compton_rows(want_sens=True)):
+the rows are the bits jscc_cone_rows stores, the statistics equal up to the order of the per-row sums, the kept
+rows are the same, and the sensitivity -- torch.sum(t, dim=0) of the normalised kept T that main_plane.py:348-355
+scales into Sensi_d -- matches the FP64 column sums of the stored rows, the projection kernel's column-sum pass and
+the reference-written golden T."""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import cone
+from tests.conftest import GOLDEN
+from tests.golden.golden_inputs import golden_sysmat
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+FIX = os.path.join(GOLDEN, "factors_511keV_RotateNum20.npz")
+STRICT = os.path.join(GOLDEN, "strict_cone_small.npz")
+
+
+def _rel(a, b):
+    a = np.asarray(a, np.float64).ravel()
+    b = np.asarray(b, np.float64).ravel()
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _check_against_plain_pass(ra, rb, P, sens_bar=2e-6):
+    """rb (want_sens) against ra (jscc_cone_rows) built from the same list."""
+    from jscc_recon_spect_cylinder_amd.engine import ListModeStore
+    ra.resolve_counts()
+    rb.resolve_counts()
+    assert ra.n_sel == rb.n_sel and ra.n_kept == rb.n_kept
+    n = ra.n_sel
+    assert torch.equal(ra.store[:n, :P], rb.store[:n, :P])          # the same row bits
+    sa, sb = ra.stats[:n].cpu().numpy(), rb.stats[:n].cpu().numpy()
+    assert np.array_equal(sa[:, 3], sb[:, 3])                      # max: exact
+    # S and Sc: per-thread FP32 partial sums over another thread layout, combined in FP64; Q: FP64 throughout
+    for k, rtol in ((0, 1e-6), (1, 1e-12), (2, 1e-6)):
+        np.testing.assert_allclose(sb[:, k], sa[:, k], rtol=rtol, atol=0)
+    assert torch.equal(ra.kept[:ra.n_kept], rb.kept[:rb.n_kept])
+    kept = rb.kept[:rb.n_kept].long()
+    rows64 = rb.store[kept, :P].double()
+    inv = rb.inv_sum[kept].double()
+    ref = (rows64 * inv[:, None]).sum(0).cpu().numpy()            # FP64 column sums of the stored rows
+    got = rb.sens.cpu().numpy()
+    assert got.shape == (P,)
+    assert _rel(got, ref) < sens_bar
+    name = "bf16" if rb.store.dtype == torch.bfloat16 else "fp32"
+    s_pass = ListModeStore.from_rows(rb, name).backproject_ones().cpu().numpy()
+    assert _rel(got, s_pass) < sens_bar                            # the column-sum pass over T
+    return got
+
+
+@pytest.mark.parametrize("store", [torch.float32, torch.bfloat16])
+def test_strict_sensitivity_matches_reference_golden_t(store):
+    """Strict sigma model with the first-hit factor on the reference grid layout (1280 voxels): the fused
+    sensitivity against the column sums of the reference-written normalised T (98 kept rows)."""
+    from jscc_recon_spect_cylinder_amd.process_list_plane_strict import build_compton_rows
+    g = np.load(STRICT)
+    det = np.load(FIX)["detector"]
+    coor = g["coor"]
+    A = golden_sysmat(det, coor)
+    dr1, dr2, e0, res, tmax, tmin, tsum = g["params"]
+    built = []
+    for want in (False, True):
+        built.append(build_compton_rows(torch.from_numpy(A), torch.from_numpy(det), torch.from_numpy(coor),
+                                        torch.from_numpy(g["list_in"].copy()), dr1, dr2, e0, res, tmax, tmin, tsum,
+                                        DEV, noise=(torch.from_numpy(g["noise1"]).to(DEV),
+                                                    torch.from_numpy(g["noise2"]).to(DEV)),
+                                        store_dtype=store, want_sens=want))
+    got = _check_against_plain_pass(built[0], built[1], coor.shape[0])
+    golden = g["t"].astype(np.float64).sum(0)
+    assert _rel(got, golden) < (1e-5 if store == torch.float32 else 2e-3)
+
+
+@pytest.mark.parametrize("rings,angles,store,sync", [
+    (16, 64, torch.float32, True),
+    (16, 64, torch.bfloat16, False),
+    (64, 256, torch.bfloat16, False),    # the 16384-voxel workload grid, device-side counts
+    (64, 256, torch.float32, True),
+    (100, 256, torch.bfloat16, False),   # 25600 voxels: 16 voxel quads per thread
+    (100, 256, torch.float32, True),
+])
+def test_fixed_sigma_sensitivity_matches_plain_pass_and_oracle(rings, angles, store, sync):
+    from jscc_recon_spect_cylinder_amd import synth
+    from jscc_recon_spect_cylinder_amd.process_list_plane_strict import build_compton_rows
+    cl = synth.compton_list(n_events=1500, n_rings=rings, n_angles=angles, seed=rings + angles)
+    tmax, tmin, tsum = synth.energy_thresholds(cl.e0)
+    sig = float(np.deg2rad(3.0))
+    P = cl.coor.shape[0]
+    print(f"P {P} store {store} sync {sync}", flush=True)
+    built = []
+    for want in (False, True):
+        built.append(build_compton_rows(None, torch.from_numpy(cl.detector), torch.from_numpy(cl.coor),
+                                        torch.from_numpy(cl.list_origin.copy()), 0.0, 0.0, cl.e0, 0.02, tmax, tmin,
+                                        tsum, DEV, noise="none", sigma_model="fixed", fixed_sigma=sig,
+                                        store_dtype=store, sync_counts=sync, want_sens=want))
+    rb = built[1]
+    if not sync:  # rows of the grid past the device count: NaN statistics, as jscc_cone_rows_counted
+        rb.resolve_counts()
+        assert rb.n_sel < cl.list_origin.shape[0]
+        assert torch.isnan(rb.stats[rb.n_sel:]).all()
+    got = _check_against_plain_pass(built[0], rb, P)
+    if store == torch.float32 and P <= 16384:
+        ref = cone.backproj_list_single(None, cl.detector, cl.coor, cl.list_origin, 0.0, 0.0, cl.e0, 0.02, tmax,
+                                        tmin, tsum, None, None, fixed_sigma=sig)
+        kept = rb.kept_event_index().cpu().numpy()
+        border = set(np.nonzero(np.abs(ref["support"] - 50.0) / 50.0 < 1e-4)[0].tolist())
+        assert set(kept.tolist()) - border == set(ref["kept"].tolist()) - border
+        if np.array_equal(kept, ref["kept"]):
+            assert _rel(got, np.asarray(ref["t"], np.float64).sum(0)) < 1e-5
+
+
+def test_sensitivity_of_no_rows_is_zero_and_bad_requests_raise():
+    from jscc_recon_spect_cylinder_amd import _lib, ops, synth
+    from jscc_recon_spect_cylinder_amd.process_list_plane_strict import build_compton_rows
+    cl = synth.compton_list(n_events=50, n_rings=8, n_angles=32, seed=3)
+    tmax, tmin, tsum = synth.energy_thresholds(cl.e0)
+    sig = float(np.deg2rad(3.0))
+    rows = build_compton_rows(None, torch.from_numpy(cl.detector), torch.from_numpy(cl.coor),
+                              torch.from_numpy(cl.list_origin.copy()), 0.0, 0.0, cl.e0, 0.02, tmax, tmin, tsum, DEV,
+                              noise="none", sigma_model="fixed", fixed_sigma=sig, store_dtype=torch.float32,
+                              want_sens=True)
+    _, _, _, sens = ops.cone_rows_sens(rows.events, rows.sel_idx, 0, rows.vox, 50.0, sigma_model=ops.SIG_FIXED,
+                                       fixed_sigma=sig, out_dtype=torch.float32)
+    assert sens.shape == (256,) and not torch.any(sens != 0)
+    for kw in ({"want_tc": True}, {"store_dtype": None}):
+        with pytest.raises(ValueError):
+            build_compton_rows(None, torch.from_numpy(cl.detector), torch.from_numpy(cl.coor),
+                               torch.from_numpy(cl.list_origin.copy()), 0.0, 0.0, cl.e0, 0.02, tmax, tmin, tsum, DEV,
+                               noise="none", sigma_model="fixed", fixed_sigma=sig, want_sens=True, **kw)
+    big = ops.voxel_soa(torch.from_numpy(np.random.default_rng(0).normal(size=(32772, 3)).astype(np.float32) * 40.0),
+                        DEV)
+    with pytest.raises(_lib.JsccError, match="voxels"):
+        ops.cone_rows_sens(rows.events, rows.sel_idx, 4, big, 50.0, sigma_model=ops.SIG_FIXED, fixed_sigma=sig)
