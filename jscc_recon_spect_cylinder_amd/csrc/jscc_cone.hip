@@ -209,7 +209,7 @@ __device__ __forceinline__ void cone_rows_tile_at(const ConeEv* evs, const bool*
                                                   const int32_t* __restrict__ tap_i1,
                                                   const float* __restrict__ tap_w0, const float* __restrict__ tap_w1,
                                                   const ConeConsts& cc, void* __restrict__ out_t, int64_t ldt,
-                                                  float* __restrict__ out_tc, int64_t ldtc, float* accS, double* accQ,
+                                                  float* __restrict__ out_tc, int64_t ldtc, float* accS, float* accQ,
                                                   float* accC, float* accM) {
   const float vcx[4] = {vx4.x, vx4.y, vx4.z, vx4.w};
   const float vcy[4] = {vy4.x, vy4.y, vy4.z, vy4.w};
@@ -248,7 +248,7 @@ __device__ __forceinline__ void cone_rows_tile_at(const ConeEv* evs, const bool*
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       accS[e] += tv[k];
-      accQ[e] = fma((double)tv[k], (double)tv[k], accQ[e]);
+      accQ[e] = fmaf(tv[k], tv[k], accQ[e]);
       if (AF != AF_NONE) accC[e] += tcv[k];  // without an A factor t = t_c: Sc = S (set below)
       accM[e] = fmaxf(accM[e], tv[k]);  // fmaxf drops NaN; NaN rows are caught by S
     }
@@ -262,12 +262,53 @@ __device__ __forceinline__ void cone_rows_tile(const ConeEv* evs, const bool* va
                                                const int32_t* __restrict__ tap_i0, const int32_t* __restrict__ tap_i1,
                                                const float* __restrict__ tap_w0, const float* __restrict__ tap_w1,
                                                const ConeConsts& cc, void* __restrict__ out_t, int64_t ldt,
-                                               float* __restrict__ out_tc, int64_t ldtc, float* accS, double* accQ,
+                                               float* __restrict__ out_tc, int64_t ldtc, float* accS, float* accQ,
                                                float* accC, float* accM) {
   cone_rows_tile_at<SIG, AF, OUT_DT, WANT_TC, E>(evs, valid, row0, base, P, *reinterpret_cast<const float4*>(cx + base),
                                                  *reinterpret_cast<const float4*>(cy + base),
                                                  *reinterpret_cast<const float4*>(cz + base), A, tap_i0, tap_i1,
                                                  tap_w0, tap_w1, cc, out_t, ldt, out_tc, ldtc, accS, accQ, accC, accM);
+}
+
+// Rows whose largest value is below kQ64Below: FP32 squares of their tail would fall under the normal range,
+// so their sum of squares is recomputed in FP64 (per thread over its tiles, DPP within the wave, waves in
+// order) from the same row values.  Every thread of the work-group calls it (it holds two barriers); wq is
+// LDS for one value per wave.
+constexpr double kQ64Below = 0x1p-40;
+
+template <int SIG, int AF, int THREADS>
+__device__ double cone_row_q64(const ConeEv& ev, int P, const float* __restrict__ cx, const float* __restrict__ cy,
+                               const float* __restrict__ cz, const float* __restrict__ A,
+                               const int32_t* __restrict__ tap_i0, const int32_t* __restrict__ tap_i1,
+                               const float* __restrict__ tap_w0, const float* __restrict__ tap_w1,
+                               const ConeConsts& cc, double* wq) {
+  double q = 0.0;
+  for (int base = threadIdx.x * kConeVec; base < P; base += THREADS * kConeVec) {
+    const float4 vx4 = *reinterpret_cast<const float4*>(cx + base);
+    const float4 vy4 = *reinterpret_cast<const float4*>(cy + base);
+    const float4 vz4 = *reinterpret_cast<const float4*>(cz + base);
+    const float vcx[4] = {vx4.x, vx4.y, vx4.z, vx4.w};
+    const float vcy[4] = {vy4.x, vy4.y, vy4.z, vy4.w};
+    const float vcz[4] = {vz4.x, vz4.y, vz4.z, vz4.w};
+    float tw0[4], tw1[4];
+    int ti0[4], ti1[4];
+    load_taps<AF>(base, P, tap_i0, tap_i1, tap_w0, tap_w1, ti0, ti1, tw0, tw1);
+    float av[4], tcv[4];
+    afactor4<AF>(ev, A, base, ti0, ti1, tw0, tw1, av);
+    cone_tc4<SIG, false>(ev, vcx, vcy, vcz, cc, tcv, nullptr);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const float t = base + k < P ? tcv[k] * av[k] : 0.0f;
+      q = fma((double)t, (double)t, q);
+    }
+  }
+  q = jscc_wave_sum(q);
+  if (jscc_lane() == 0) wq[jscc_wave_id()] = q;
+  __syncthreads();
+  double t = wq[0];
+  for (int w = 1; w < THREADS / JSCC_WAVE; ++w) t += wq[w];
+  __syncthreads();
+  return t;
 }
 
 template <int SIG, int AF, int OUT_DT, bool WANT_TC, int E>
@@ -300,13 +341,13 @@ __global__ void __launch_bounds__(kConeThreads) k_cone_rows(
     evs[e] = load_event(ev[ev_idx[r]], lda);
   }
   // per-thread partial sums in FP32 (at most P/1024 tiles x 4 values each),
-  // combined across threads in FP64.  The sum of squares is kept in FP64: a row
-  // of tiny weights (1e-25 and below) squares under the FP32 range, and its
-  // effective support S^2/Q must match the reference's scale-free 1/sum((t/S)^2).
-  float accS[E], accC[E], accM[E];
-  double accQ[E];
+  // combined across threads in FP64.  A row whose largest value is below 2^-40
+  // gets its sum of squares again in FP64 (cone_row_q64): its effective support
+  // S^2/Q must match the reference's scale-free 1/sum((t/S)^2) even where the
+  // squares of its tail fall under the FP32 normal range.
+  float accS[E], accQ[E], accC[E], accM[E];
 #pragma unroll
-  for (int e = 0; e < E; ++e) { accS[e] = 0.0f; accQ[e] = 0.0; accC[e] = 0.0f; accM[e] = 0.0f; }
+  for (int e = 0; e < E; ++e) { accS[e] = 0.0f; accQ[e] = 0.0f; accC[e] = 0.0f; accM[e] = 0.0f; }
 
   for (int base = threadIdx.x * kConeVec; base < P; base += kConeThreads * kConeVec)
     cone_rows_tile<SIG, AF, OUT_DT, WANT_TC, E>(evs, valid, row0, base, P, cx, cy, cz, A, tap_i0, tap_i1, tap_w0,
@@ -321,7 +362,7 @@ __global__ void __launch_bounds__(kConeThreads) k_cone_rows(
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     double s = jscc_wave_sum((double)accS[e]);
-    double q = jscc_wave_sum(accQ[e]);
+    double q = jscc_wave_sum((double)accQ[e]);
     double c = jscc_wave_sum((double)accC[e]);
     float m = jscc_wave_max(accM[e]);
     if (jscc_lane() == 0) {
@@ -332,11 +373,33 @@ __global__ void __launch_bounds__(kConeThreads) k_cone_rows(
     }
   }
   __syncthreads();
+  __shared__ double q64[E], wq[kConeThreads / JSCC_WAVE];
+  bool fix[E];
+  bool any_fix = false;
+#pragma unroll
+  for (int e = 0; e < E; ++e) {  // the row maxima, read by every thread: a uniform decision
+    double m = red[3][0][e];
+    for (int w = 1; w < kConeThreads / JSCC_WAVE; ++w) m = fmax(m, red[3][w][e]);
+    fix[e] = valid[e] && !(m >= kQ64Below);
+    any_fix |= fix[e];
+  }
+  if (any_fix) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      if (fix[e]) {
+        const double q = cone_row_q64<SIG, AF, kConeThreads>(evs[e], P, cx, cy, cz, A, tap_i0, tap_i1, tap_w0, tap_w1,
+                                                             cc, wq);
+        if (threadIdx.x == 0) q64[e] = q;
+      }
+    }
+    __syncthreads();
+  }
   if (threadIdx.x < E * 4) {
     const int e = threadIdx.x / 4, k = threadIdx.x % 4;
     if (row0 + e < n_rows) {
       double v = red[k][0][e];
       for (int w = 1; w < kConeThreads / JSCC_WAVE; ++w) v = (k == 3) ? fmax(v, red[k][w][e]) : v + red[k][w][e];
+      if (k == 1 && fix[e]) v = q64[e];
       stats[(int64_t)(row0 + e) * 4 + k] = v;
     } else if (row0 + e < n_grid) {
       stats[(int64_t)(row0 + e) * 4 + k] = __builtin_nan("");
@@ -413,6 +476,7 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4)
   using Raw = typename std::conditional<OUT_DT == JSCC_DT_F32, float4, uint2>::type;
   extern __shared__ float4 sacc[];  // [ldp / 4] column sums; quad q is owned by thread q % THREADS
   __shared__ double red[4][NW][E];
+  __shared__ double q64[E], wq[NW];
   __shared__ float fsh[E];
   __shared__ int next_g;
   const int tid = threadIdx.x;
@@ -450,10 +514,9 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4)
       const int r = valid[e] ? (row0 + e) : (n_rows - 1);
       evs[e] = load_event(ev[ev_idx[r]], lda);
     }
-    float accS[E], accC[E], accM[E];
-    double accQ[E];
+    float accS[E], accQ[E], accC[E], accM[E];
 #pragma unroll
-    for (int e = 0; e < E; ++e) { accS[e] = 0.0f; accQ[e] = 0.0; accC[e] = 0.0f; accM[e] = 0.0f; }
+    for (int e = 0; e < E; ++e) { accS[e] = 0.0f; accQ[e] = 0.0f; accC[e] = 0.0f; accM[e] = 0.0f; }
     // rows of the previous group, as stored (a dropped row reads the group's first row and adds nothing;
     // without a previous group every pointer is row 0)
     const Raw* prow[E];
@@ -505,7 +568,7 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4)
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const double sS = jscc_wave_sum((double)accS[e]);
-      const double sQ = jscc_wave_sum(accQ[e]);
+      const double sQ = jscc_wave_sum((double)accQ[e]);
       const double sC = jscc_wave_sum((double)accC[e]);
       const float sM = jscc_wave_max(accM[e]);
       if (jscc_lane() == 0) {
@@ -518,6 +581,26 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4)
     if (tid == 0) next_g = atomicAdd(ticket, 1);
     __syncthreads();
     const int g_next = next_g;
+    bool fix[E];
+    bool any_fix = false;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {  // rows below 2^-40: the sum of squares again in FP64 (uniform decision)
+      double m = red[3][0][e];
+      for (int w = 1; w < NW; ++w) m = fmax(m, red[3][w][e]);
+      fix[e] = valid[e] && !(m >= kQ64Below);
+      any_fix |= fix[e];
+    }
+    if (any_fix) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        if (fix[e]) {
+          const double q = cone_row_q64<SIG, AF, THREADS>(evs[e], P, cx, cy, cz, A, nullptr, nullptr, nullptr,
+                                                          nullptr, cc, wq);
+          if (tid == 0) q64[e] = q;
+        }
+      }
+      __syncthreads();
+    }
     if (tid < E) {
       const int e = tid;
       float f = 0.0f;
@@ -527,6 +610,7 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4)
         for (int k = 0; k < 4; ++k) {
           double t = red[k][0][e];
           for (int w = 1; w < NW; ++w) t = (k == 3) ? fmax(t, red[k][w][e]) : t + red[k][w][e];
+          if (k == 1 && fix[e]) t = q64[e];
           v[k] = t;
           stats[(int64_t)(row0 + e) * 4 + k] = t;
         }

@@ -168,6 +168,39 @@ def test_cone_rows_near_the_cone_axis(u_dir):
         assert np.all(t[i][~m] < 1e-30)
 
 
+def test_rows_of_tiny_weights_keep_their_effective_support():
+    """A row whose weights all lie near 1e-21 and below (every voxel 0.49-0.60 rad off the cone, sigma 0.05
+    rad): the FP32 squares of its values fall under the FP32 normal range, so the kernels take the sum of
+    squares of a row whose largest value is below 2^-40 again in FP64.  Its effective support S^2/Q must
+    match the oracle's scale-free 1/sum((t/S)^2) (process_list_plane_strict.py:145-146), in the plain cone
+    pass and in the pass that also forms the sensitivity."""
+    from jscc_recon_spect_cylinder_amd.process_list_plane_strict import build_compton_rows
+    p1 = np.array([90.0, 10.0, 3.0])
+    u = np.array([-0.6, -0.75, 0.1])
+    u /= np.linalg.norm(u)
+    w1 = np.cross(u, [0.0, 0.0, 1.0])
+    w1 /= np.linalg.norm(w1)
+    w2 = np.cross(u, w1)
+    th = 0.522  # the scattering angle of the event below
+    pts = [p1 - 50.0 * (np.cos(b) * u + np.sin(b) * (np.cos(ph) * w1 + np.sin(ph) * w2))
+           for b in np.linspace(th + 0.49, th + 0.60, 64) for ph in np.linspace(0.0, 2 * np.pi, 16, endpoint=False)]
+    coor = np.asarray(pts, np.float32)
+    det = np.stack([p1, p1 + 30.0 * u]).astype(np.float32)
+    lst = np.array([[1.0, 0.06, 2.0, 0.451]], np.float32)
+    e0, tmax, tmin, tsum, sig = 0.511, 0.3397, 0.05, 0.46, 0.05
+    ref = cone.backproj_list_single(None, det, coor, lst, 0.0, 0.0, e0, 0.02, tmax, tmin, tsum, None, None,
+                                    fixed_sigma=sig)
+    assert abs(ref["theta"][0] - th) < 1e-3
+    for want_sens in (False, True):
+        rows = build_compton_rows(None, torch.from_numpy(det), torch.from_numpy(coor), torch.from_numpy(lst.copy()),
+                                  0.0, 0.0, e0, 0.02, tmax, tmin, tsum, DEV, noise="none", sigma_model="fixed",
+                                  fixed_sigma=sig, store_dtype=torch.float32, want_sens=want_sens)
+        S, Q, _, M = rows.stats[0].cpu().numpy()
+        assert 1e-30 < M < 2.0 ** -40  # the FP64 sum-of-squares path
+        assert abs(S / ref["S"][0] - 1.0) < 1e-4
+        assert abs((S * S / Q) / ref["support"][0] - 1.0) < 1e-4
+
+
 def test_non_finite_voxel_coordinates_raise():
     from jscc_recon_spect_cylinder_amd.process_list_plane_strict import build_compton_rows
     g = np.load(STRICT)

@@ -35,8 +35,8 @@ def _check_against_plain_pass(ra, rb, P, sens_bar=2e-6):
     assert torch.equal(ra.store[:n, :P], rb.store[:n, :P])          # the same row bits
     sa, sb = ra.stats[:n].cpu().numpy(), rb.stats[:n].cpu().numpy()
     assert np.array_equal(sa[:, 3], sb[:, 3])                      # max: exact
-    # S and Sc: per-thread FP32 partial sums over another thread layout, combined in FP64; Q: FP64 throughout
-    for k, rtol in ((0, 1e-6), (1, 1e-12), (2, 1e-6)):
+    # S, Q, Sc: per-thread FP32 partial sums over another thread layout, combined in FP64
+    for k, rtol in ((0, 1e-6), (1, 1e-6), (2, 1e-6)):
         np.testing.assert_allclose(sb[:, k], sa[:, k], rtol=rtol, atol=0)
     assert torch.equal(ra.kept[:ra.n_kept], rb.kept[:rb.n_kept])
     kept = rb.kept[:rb.n_kept].long()
