@@ -462,7 +462,12 @@ constexpr int kSensBlocksPerCu = 2;  // 512-thread work-groups per CU (64 KB of 
 template <int THREADS>
 constexpr int sens_blocks_per_cu() { return THREADS == 512 ? kSensBlocksPerCu : 1; }
 
-__global__ void k_sens_ticket_reset(int* ticket) { *ticket = 0; }
+// the ticket counter to 0 and the zero row (read back in place of dropped rows) to zeros
+__global__ void k_sens_prep(int* ticket, float* zrow, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) *ticket = 0;
+  for (int j = i; j < n; j += gridDim.x * blockDim.x) zrow[j] = 0.0f;
+}
 
 template <int SIG, int AF, int OUT_DT, int THREADS>
 __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) k_cone_rows_sens(
@@ -470,7 +475,7 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4)
     const int32_t* __restrict__ n_rows_dev, const float* __restrict__ cx, const float* __restrict__ cy,
     const float* __restrict__ cz, int P, const float* __restrict__ A, int64_t lda, ConeConsts cc,
     void* __restrict__ out_t, int64_t ldt, double* __restrict__ stats, double min_support,
-    float* __restrict__ partial, int64_t ldp, int* __restrict__ ticket) {
+    float* __restrict__ partial, int64_t ldp, int* __restrict__ ticket, const float* __restrict__ zrow) {
   constexpr int E = 4;
   constexpr int NW = THREADS / JSCC_WAVE;
   using Raw = typename std::conditional<OUT_DT == JSCC_DT_F32, float4, uint2>::type;
@@ -517,15 +522,15 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4)
     float accS[E], accQ[E], accC[E], accM[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) { accS[e] = 0.0f; accQ[e] = 0.0f; accC[e] = 0.0f; accM[e] = 0.0f; }
-    // rows of the previous group, as stored (a dropped row reads the group's first row and adds nothing;
-    // without a previous group every pointer is row 0)
+    // rows of the previous group, as stored; a dropped row (f = 0, possibly holding NaN) and every row
+    // before the first group read the zero row instead, so the additions below need no branch
     const Raw* prow[E];
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const int64_t r = prev_row0 < 0 ? 0 : (int64_t)(pf[e] != 0.0f ? prev_row0 + e : prev_row0);
-      prow[e] = reinterpret_cast<const Raw*>(reinterpret_cast<const char*>(out_t) +
-                                             r * ldt * (int64_t)(sizeof(Raw) / 4));
-    }
+    for (int e = 0; e < E; ++e)
+      prow[e] = pf[e] != 0.0f ? reinterpret_cast<const Raw*>(reinterpret_cast<const char*>(out_t) +
+                                                             (int64_t)(prev_row0 + e) * ldt *
+                                                                 (int64_t)(sizeof(Raw) / 4))
+                              : reinterpret_cast<const Raw*>(zrow);
     for (int q = tid; q * kConeVec < P; q += THREADS) {
       const int base = q * kConeVec;
       const float4 vx4 = *reinterpret_cast<const float4*>(cx + base);
@@ -534,32 +539,28 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4)
       // the read-back loads are issued after the coordinate loads (memory clobber: no reordering), so waiting
       // for the coordinates does not wait for them (loads complete in order)
       asm volatile("" ::: "memory");
-      // unconditional (a branch here would make the coordinate wait drain them too); before the first group
-      // they read row 0, which is allocated, and nothing is added
+      // unconditional (a branch here would make the coordinate wait drain them too)
       Raw rb[E];
 #pragma unroll
       for (int e = 0; e < E; ++e) rb[e] = prow[e][q];
       cone_rows_tile_at<SIG, AF, OUT_DT, false, E>(evs, valid, row0, base, P, vx4, vy4, vz4, A, nullptr, nullptr,
                                                    nullptr, nullptr, cc, out_t, ldt, nullptr, 0, accS, accQ, accC,
                                                    accM);
-      if (prev_row0 >= 0) {
-        float4 a = sacc[q];
+      float4 a = sacc[q];
+      float w[4] = {a.x, a.y, a.z, a.w};
 #pragma unroll
-        for (int e = 0; e < E; ++e) {
-          if (pf[e] != 0.0f) {  // uniform: dropped rows add nothing
-            float v[4];
-            if constexpr (OUT_DT == JSCC_DT_F32) {
-              v[0] = rb[e].x; v[1] = rb[e].y; v[2] = rb[e].z; v[3] = rb[e].w;
-            } else {
-              v[0] = __uint_as_float(rb[e].x << 16); v[1] = __uint_as_float(rb[e].x & 0xffff0000u);
-              v[2] = __uint_as_float(rb[e].y << 16); v[3] = __uint_as_float(rb[e].y & 0xffff0000u);
-            }
-            a.x = fmaf(pf[e], v[0], a.x); a.y = fmaf(pf[e], v[1], a.y);
-            a.z = fmaf(pf[e], v[2], a.z); a.w = fmaf(pf[e], v[3], a.w);
-          }
+      for (int e = 0; e < E; ++e) {
+        float v[4];
+        if constexpr (OUT_DT == JSCC_DT_F32) {
+          v[0] = rb[e].x; v[1] = rb[e].y; v[2] = rb[e].z; v[3] = rb[e].w;
+        } else {
+          v[0] = __uint_as_float(rb[e].x << 16); v[1] = __uint_as_float(rb[e].x & 0xffff0000u);
+          v[2] = __uint_as_float(rb[e].y << 16); v[3] = __uint_as_float(rb[e].y & 0xffff0000u);
         }
-        sacc[q] = a;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) w[k] = fmaf(pf[e], v[k], w[k]);
       }
+      sacc[q] = make_float4(w[0], w[1], w[2], w[3]);
     }
     if (AF == AF_NONE) {
 #pragma unroll
@@ -632,13 +633,13 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4)
         const Raw* pr = reinterpret_cast<const Raw*>(reinterpret_cast<const char*>(out_t) +
                                                      (int64_t)(prev_row0 + e) * ldt * (int64_t)(sizeof(Raw) / 4));
         for (int q = tid; q * kConeVec < P; q += THREADS) {
-          const Raw w = pr[q];
+          const Raw rw = pr[q];
           float v[4];
           if constexpr (OUT_DT == JSCC_DT_F32) {
-            v[0] = w.x; v[1] = w.y; v[2] = w.z; v[3] = w.w;
+            v[0] = rw.x; v[1] = rw.y; v[2] = rw.z; v[3] = rw.w;
           } else {
-            v[0] = __uint_as_float(w.x << 16); v[1] = __uint_as_float(w.x & 0xffff0000u);
-            v[2] = __uint_as_float(w.y << 16); v[3] = __uint_as_float(w.y & 0xffff0000u);
+            v[0] = __uint_as_float(rw.x << 16); v[1] = __uint_as_float(rw.x & 0xffff0000u);
+            v[2] = __uint_as_float(rw.y << 16); v[3] = __uint_as_float(rw.y & 0xffff0000u);
           }
           float4 a = sacc[q];
           a.x = fmaf(pf[e], v[0], a.x); a.y = fmaf(pf[e], v[1], a.y);
@@ -648,6 +649,7 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4)
       }
     }
   }
+  __syncthreads();
   float4* dst = reinterpret_cast<float4*>(partial + (int64_t)blockIdx.x * ldp);
   for (int q = tid; q < n_quads; q += THREADS) dst[q] = sacc[q];
 }
@@ -699,9 +701,11 @@ int launch_sens_t(int G, hipStream_t s, const JsccEvent* ev, const int32_t* idx,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)sens_lds(32768)));
     attr = true;
   }
-  hipLaunchKernelGGL(k_sens_ticket_reset, dim3(1), dim3(1), 0, s, ticket);
+  float* zrow = reinterpret_cast<float*>(reinterpret_cast<char*>(ticket) + 256);
+  hipLaunchKernelGGL(k_sens_prep, dim3(16), dim3(256), 0, s, ticket, zrow, (int)ldp);
   hipLaunchKernelGGL((k_cone_rows_sens<SIG, AF, OUT_DT, THREADS>), dim3(G), dim3(THREADS), sens_lds(P), s, ev, idx,
-                     n_grid, n_dev, cx, cy, cz, P, A, lda, cc, out_t, ldt, stats, min_support, partial, ldp, ticket);
+                     n_grid, n_dev, cx, cy, cz, P, A, lda, cc, out_t, ldt, stats, min_support, partial, ldp, ticket,
+                     zrow);
   return 0;
 }
 
@@ -811,8 +815,8 @@ extern "C" int jscc_cone_rows_counted(const void* events, const int32_t* ev_idx,
 extern "C" int jscc_cone_sens_grid(int64_t n_vox) { return sens_threads(n_vox) ? sens_grid(n_vox) : 0; }
 
 extern "C" int64_t jscc_cone_sens_workspace_bytes(int64_t n_vox) {
-  // the work-groups' column sums, then the ticket counter
-  return (int64_t)jscc_cone_sens_grid(n_vox) * ((n_vox + 3) / 4 * 4) * (int64_t)sizeof(float) + 256;
+  // the work-groups' column sums, the ticket counter (256 bytes), the zero row
+  return ((int64_t)jscc_cone_sens_grid(n_vox) + 1) * ((n_vox + 3) / 4 * 4) * (int64_t)sizeof(float) + 256;
 }
 
 extern "C" int jscc_cone_rows_sens(const void* events, const int32_t* ev_idx, const int32_t* n_rows_dev,
