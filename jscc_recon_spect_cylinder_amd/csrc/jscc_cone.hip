@@ -200,10 +200,12 @@ __device__ __forceinline__ void load_taps(int base, int P, const int32_t* __rest
   }
 }
 
-// One tile of 4 voxels for the E events of a work-group: the row values (stored when valid) and the row
-// statistics (S, Q in FP64, Sc, max) accumulated per thread.
+// One tile of 4 voxels for the E events of a work-group: the row values, stored in rows rrow[e], and the row
+// statistics (S, Q, Sc, max) accumulated per thread in FP32.  A slot past the row count carries the event of
+// the last row and stores into that row (rrow = the last row): the same values to the same addresses by the
+// same thread, so the stores need no condition.
 template <int SIG, int AF, int OUT_DT, bool WANT_TC, int E>
-__device__ __forceinline__ void cone_rows_tile_at(const ConeEv* evs, const bool* valid, int row0, int base, int P,
+__device__ __forceinline__ void cone_rows_tile_at(const ConeEv* evs, const int* rrow, int base, int P,
                                                   float4 vx4, float4 vy4, float4 vz4, const float* __restrict__ A,
                                                   const int32_t* __restrict__ tap_i0,
                                                   const int32_t* __restrict__ tap_i1,
@@ -230,16 +232,18 @@ __device__ __forceinline__ void cone_rows_tile_at(const ConeEv* evs, const bool*
       for (int k = 0; k < 4; ++k)
         if (base + k >= P) { tcv[k] = 0.0f; tv[k] = 0.0f; }
     }
-    if (valid[e]) {
-      const int64_t r = (int64_t)(row0 + e);
+    {
+      const int64_t r = (int64_t)rrow[e];
+      // row start (uniform) plus a 32-bit byte offset: scalar base + vector offset addressing
       if (OUT_DT == JSCC_DT_F32) {
-        *reinterpret_cast<float4*>(reinterpret_cast<float*>(out_t) + r * ldt + base) =
-            make_float4(tv[0], tv[1], tv[2], tv[3]);
+        char* rowp = reinterpret_cast<char*>(out_t) + r * ldt * (int64_t)sizeof(float);
+        *reinterpret_cast<float4*>(rowp + (uint32_t)base * 4u) = make_float4(tv[0], tv[1], tv[2], tv[3]);
       } else if (OUT_DT == JSCC_DT_BF16) {
         uint2 pk;
         pk.x = jscc_pack_bf16x2(tv[0], tv[1]);
         pk.y = jscc_pack_bf16x2(tv[2], tv[3]);
-        *reinterpret_cast<uint2*>(reinterpret_cast<uint16_t*>(out_t) + r * ldt + base) = pk;
+        char* rowp = reinterpret_cast<char*>(out_t) + r * ldt * (int64_t)sizeof(uint16_t);
+        *reinterpret_cast<uint2*>(rowp + (uint32_t)base * 2u) = pk;
       }
       if (WANT_TC) {
         *reinterpret_cast<float4*>(out_tc + r * ldtc + base) = make_float4(tcv[0], tcv[1], tcv[2], tcv[3]);
@@ -256,7 +260,7 @@ __device__ __forceinline__ void cone_rows_tile_at(const ConeEv* evs, const bool*
 }
 
 template <int SIG, int AF, int OUT_DT, bool WANT_TC, int E>
-__device__ __forceinline__ void cone_rows_tile(const ConeEv* evs, const bool* valid, int row0, int base, int P,
+__device__ __forceinline__ void cone_rows_tile(const ConeEv* evs, const int* rrow, int base, int P,
                                                const float* __restrict__ cx, const float* __restrict__ cy,
                                                const float* __restrict__ cz, const float* __restrict__ A,
                                                const int32_t* __restrict__ tap_i0, const int32_t* __restrict__ tap_i1,
@@ -264,7 +268,7 @@ __device__ __forceinline__ void cone_rows_tile(const ConeEv* evs, const bool* va
                                                const ConeConsts& cc, void* __restrict__ out_t, int64_t ldt,
                                                float* __restrict__ out_tc, int64_t ldtc, float* accS, float* accQ,
                                                float* accC, float* accM) {
-  cone_rows_tile_at<SIG, AF, OUT_DT, WANT_TC, E>(evs, valid, row0, base, P, *reinterpret_cast<const float4*>(cx + base),
+  cone_rows_tile_at<SIG, AF, OUT_DT, WANT_TC, E>(evs, rrow, base, P, *reinterpret_cast<const float4*>(cx + base),
                                                  *reinterpret_cast<const float4*>(cy + base),
                                                  *reinterpret_cast<const float4*>(cz + base), A, tap_i0, tap_i1,
                                                  tap_w0, tap_w1, cc, out_t, ldt, out_tc, ldtc, accS, accQ, accC, accM);
@@ -334,11 +338,12 @@ __global__ void __launch_bounds__(kConeThreads) k_cone_rows(
   }
   ConeEv evs[E];
   bool valid[E];
+  int rrow[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     valid[e] = (row0 + e) < n_rows;
-    const int r = valid[e] ? (row0 + e) : (n_rows - 1);
-    evs[e] = load_event(ev[ev_idx[r]], lda);
+    rrow[e] = valid[e] ? (row0 + e) : (n_rows - 1);
+    evs[e] = load_event(ev[ev_idx[rrow[e]]], lda);
   }
   // per-thread partial sums in FP32 (at most P/1024 tiles x 4 values each),
   // combined across threads in FP64.  A row whose largest value is below 2^-40
@@ -350,7 +355,7 @@ __global__ void __launch_bounds__(kConeThreads) k_cone_rows(
   for (int e = 0; e < E; ++e) { accS[e] = 0.0f; accQ[e] = 0.0f; accC[e] = 0.0f; accM[e] = 0.0f; }
 
   for (int base = threadIdx.x * kConeVec; base < P; base += kConeThreads * kConeVec)
-    cone_rows_tile<SIG, AF, OUT_DT, WANT_TC, E>(evs, valid, row0, base, P, cx, cy, cz, A, tap_i0, tap_i1, tap_w0,
+    cone_rows_tile<SIG, AF, OUT_DT, WANT_TC, E>(evs, rrow, base, P, cx, cy, cz, A, tap_i0, tap_i1, tap_w0,
                                                 tap_w1, cc, out_t, ldt, out_tc, ldtc, accS, accQ, accC, accM);
 
   if (AF == AF_NONE) {
@@ -482,7 +487,8 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4)
   extern __shared__ float4 sacc[];  // [ldp / 4] column sums; quad q is owned by thread q % THREADS
   __shared__ double red[4][NW][E];
   __shared__ double q64[E], wq[NW];
-  __shared__ float fsh[E];
+  __shared__ float4 fsh4;  // the factors f of the E rows of the work-group's last finished group
+  float* fsh = reinterpret_cast<float*>(&fsh4);
   __shared__ int next_g;
   const int tid = threadIdx.x;
   int n_rows = n_grid;
@@ -499,6 +505,7 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4)
   const int n_quads = (int)(ldp / 4);
   for (int q = tid; q < n_quads; q += THREADS) sacc[q] = make_float4(0.f, 0.f, 0.f, 0.f);
   const int n_groups = (n_rows + E - 1) / E;
+  if (tid == 0) fsh4 = make_float4(0.f, 0.f, 0.f, 0.f);  // no previous group: nothing is added
   if (tid == 0) next_g = atomicAdd(ticket, 1);
   __syncthreads();
   int g = next_g;
@@ -513,11 +520,12 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4)
     const int row0 = g * E;
     ConeEv evs[E];
     bool valid[E];
+    int rrow[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       valid[e] = (row0 + e) < n_rows;
-      const int r = valid[e] ? (row0 + e) : (n_rows - 1);
-      evs[e] = load_event(ev[ev_idx[r]], lda);
+      rrow[e] = valid[e] ? (row0 + e) : (n_rows - 1);
+      evs[e] = load_event(ev[ev_idx[rrow[e]]], lda);
     }
     float accS[E], accQ[E], accC[E], accM[E];
 #pragma unroll
@@ -533,21 +541,28 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4)
                               : reinterpret_cast<const Raw*>(zrow);
     for (int q = tid; q * kConeVec < P; q += THREADS) {
       const int base = q * kConeVec;
-      const float4 vx4 = *reinterpret_cast<const float4*>(cx + base);
-      const float4 vy4 = *reinterpret_cast<const float4*>(cy + base);
-      const float4 vz4 = *reinterpret_cast<const float4*>(cz + base);
+      const uint32_t coff = (uint32_t)base * 4u;
+      const float4 vx4 = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(cx) + coff);
+      const float4 vy4 = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(cy) + coff);
+      const float4 vz4 = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(cz) + coff);
       // the read-back loads are issued after the coordinate loads (memory clobber: no reordering), so waiting
       // for the coordinates does not wait for them (loads complete in order)
       asm volatile("" ::: "memory");
       // unconditional (a branch here would make the coordinate wait drain them too)
       Raw rb[E];
+      const uint32_t qoff = (uint32_t)q * (uint32_t)sizeof(Raw);  // one 32-bit offset from the rows' scalar bases
 #pragma unroll
-      for (int e = 0; e < E; ++e) rb[e] = prow[e][q];
-      cone_rows_tile_at<SIG, AF, OUT_DT, false, E>(evs, valid, row0, base, P, vx4, vy4, vz4, A, nullptr, nullptr,
+      for (int e = 0; e < E; ++e)
+        rb[e] = *reinterpret_cast<const Raw*>(reinterpret_cast<const char*>(prow[e]) + qoff);
+      cone_rows_tile_at<SIG, AF, OUT_DT, false, E>(evs, rrow, base, P, vx4, vy4, vz4, A, nullptr, nullptr,
                                                    nullptr, nullptr, cc, out_t, ldt, nullptr, 0, accS, accQ, accC,
                                                    accM);
       float4 a = sacc[q];
       float w[4] = {a.x, a.y, a.z, a.w};
+      // f of the previous group's rows read from LDS (a vector register operand; the scalar registers are
+      // taken by the event parameters and row bases).  fsh holds them until the next group's finalisation.
+      const float4 f4 = fsh4;
+      const float fv[E] = {f4.x, f4.y, f4.z, f4.w};
 #pragma unroll
       for (int e = 0; e < E; ++e) {
         float v[4];
@@ -558,7 +573,7 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4)
           v[2] = __uint_as_float(rb[e].y << 16); v[3] = __uint_as_float(rb[e].y & 0xffff0000u);
         }
 #pragma unroll
-        for (int k = 0; k < 4; ++k) w[k] = fmaf(pf[e], v[k], w[k]);
+        for (int k = 0; k < 4; ++k) w[k] = fmaf(fv[e], v[k], w[k]);
       }
       sacc[q] = make_float4(w[0], w[1], w[2], w[3]);
     }
