@@ -113,7 +113,7 @@ int jscc_cone_rows_counted(const void* events, const int32_t* ev_idx, const int3
  * is read back as stored (FP32 or BF16) by the thread that wrote it, during
  * the cone arithmetic of the work-group's next rows, so the sums cost no
  * second pass over T in HBM.  Persistent grid of jscc_cone_sens_grid(n_vox)
- * work-groups taking groups of 4 rows from a ticket counter: the grouping of
+ * work-groups taking groups of 8 rows from a ticket counter: the grouping of
  * the FP32 sums, and so the last bits of sens, can differ between runs (rows
  * and statistics do not).  out_dtype FP32 or BF16, first-hit factor
  * sysmat[arow] or none (no tap tables), no t_c output, n_vox <= 32768.

@@ -448,21 +448,23 @@ int dispatch_out(int out_dtype, bool want_tc, int n_rows, hipStream_t s, const J
 // ----------------------------------------------------------------------------
 // Cone rows with the list-mode sensitivity in the same pass.  A persistent
 // grid (kSensBlocksPerCu work-groups of 512 threads per CU; 1 of 1024 above
-// 16384 voxels) takes groups of E rows from a ticket counter, so work-groups
-// on slower CUs simply take fewer groups (a static split measured 25 % between
-// the fastest and slowest work-group).  Every row is evaluated and stored as in
-// k_cone_rows, its statistics reduced, the keep decision of
-// jscc_rows_keep_flags taken (f = 1/S for a kept row, 0 otherwise).  The rows
-// of group g are read back (each thread its own stored values, the same bits
-// the column-sum pass would read) during the tile loop of the work-group's
-// next group, one voxel quad per tile, so the loads have a whole tile of cone
-// arithmetic to arrive; f x value is added into the work-group's LDS column
-// sums (each thread owns its quads: no barrier).  At the end every work-group
-// writes its column sums and k_sens_reduce adds them in work-group order.
-// The split of groups between work-groups follows the ticket order, so the
-// FP32 grouping of the sums (the last bits of sens) can differ between runs.
+// 16384 voxels) takes groups of 8 rows (two halves of 4 events) from a ticket
+// counter, so work-groups on slower CUs simply take fewer groups (a static
+// split measured 25 % between the fastest and slowest work-group).  Every row
+// is evaluated and stored as in k_cone_rows, its statistics reduced (one pair
+// of barriers per group), the keep decision of jscc_rows_keep_flags taken
+// (f = 1/S for a kept row, 0 otherwise).  The rows of group g are read back
+// (each thread its own stored values, the same bits the column-sum pass would
+// read) during the tile loop of the work-group's next group, one voxel quad
+// per tile, so the loads have a whole tile of cone arithmetic to arrive;
+// f x value is added into the work-group's LDS column sums (each thread owns
+// its quads: no barrier).  At the end every work-group writes its column sums
+// and k_sens_reduce adds them in work-group order.  The split of groups
+// between work-groups follows the ticket order, so the FP32 grouping of the
+// sums (the last bits of sens) can differ between runs.
 // ----------------------------------------------------------------------------
 constexpr int kSensBlocksPerCu = 2;  // 512-thread work-groups per CU (64 KB of column sums each)
+constexpr int kSensHalves = 2;       // groups of 2 x 4 rows: one pair of barriers per 8 rows
 
 template <int THREADS>
 constexpr int sens_blocks_per_cu() { return THREADS == 512 ? kSensBlocksPerCu : 1; }
@@ -481,14 +483,16 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4)
     const float* __restrict__ cz, int P, const float* __restrict__ A, int64_t lda, ConeConsts cc,
     void* __restrict__ out_t, int64_t ldt, double* __restrict__ stats, double min_support,
     float* __restrict__ partial, int64_t ldp, int* __restrict__ ticket, const float* __restrict__ zrow) {
-  constexpr int E = 4;
+  constexpr int E = 4;              // events evaluated together (scalar registers)
+  constexpr int H = kSensHalves;    // E-event halves per group: one barrier round per H * E rows
+  constexpr int R = H * E;          // rows per group
   constexpr int NW = THREADS / JSCC_WAVE;
   using Raw = typename std::conditional<OUT_DT == JSCC_DT_F32, float4, uint2>::type;
   extern __shared__ float4 sacc[];  // [ldp / 4] column sums; quad q is owned by thread q % THREADS
-  __shared__ double red[4][NW][E];
-  __shared__ double q64[E], wq[NW];
-  __shared__ float4 fsh4;  // the factors f of the E rows of the work-group's last finished group
-  float* fsh = reinterpret_cast<float*>(&fsh4);
+  __shared__ double red[4][NW][R];
+  __shared__ double q64[R], wq[NW];
+  __shared__ float4 fsh4[H];  // the factors f of the rows of the work-group's last finished group
+  float* fsh = reinterpret_cast<float*>(fsh4);
   __shared__ int next_g;
   const int tid = threadIdx.x;
   int n_rows = n_grid;
@@ -504,85 +508,90 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4)
   }
   const int n_quads = (int)(ldp / 4);
   for (int q = tid; q < n_quads; q += THREADS) sacc[q] = make_float4(0.f, 0.f, 0.f, 0.f);
-  const int n_groups = (n_rows + E - 1) / E;
-  if (tid == 0) fsh4 = make_float4(0.f, 0.f, 0.f, 0.f);  // no previous group: nothing is added
+  const int n_groups = (n_rows + R - 1) / R;
+  if (tid < H) fsh4[tid] = make_float4(0.f, 0.f, 0.f, 0.f);  // no previous group: nothing is added
   if (tid == 0) next_g = atomicAdd(ticket, 1);
   __syncthreads();
   int g = next_g;
   __syncthreads();
   // the previous group of this work-group: its first row and the factors f of its rows (uniform)
   int prev_row0 = -1;
-  float pf[E];
+  float pf[R];
 #pragma unroll
-  for (int e = 0; e < E; ++e) pf[e] = 0.0f;
+  for (int e = 0; e < R; ++e) pf[e] = 0.0f;
 
   while (g < n_groups) {
-    const int row0 = g * E;
-    ConeEv evs[E];
-    bool valid[E];
-    int rrow[E];
+    const int row0 = g * R;
+    bool valid[R];
+    float accS[R], accQ[R], accC[R], accM[R];
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
+    for (int e = 0; e < R; ++e) {
       valid[e] = (row0 + e) < n_rows;
-      rrow[e] = valid[e] ? (row0 + e) : (n_rows - 1);
-      evs[e] = load_event(ev[ev_idx[rrow[e]]], lda);
+      accS[e] = 0.0f; accQ[e] = 0.0f; accC[e] = 0.0f; accM[e] = 0.0f;
     }
-    float accS[E], accQ[E], accC[E], accM[E];
 #pragma unroll
-    for (int e = 0; e < E; ++e) { accS[e] = 0.0f; accQ[e] = 0.0f; accC[e] = 0.0f; accM[e] = 0.0f; }
-    // rows of the previous group, as stored; a dropped row (f = 0, possibly holding NaN) and every row
-    // before the first group read the zero row instead, so the additions below need no branch
-    const Raw* prow[E];
-#pragma unroll
-    for (int e = 0; e < E; ++e)
-      prow[e] = pf[e] != 0.0f ? reinterpret_cast<const Raw*>(reinterpret_cast<const char*>(out_t) +
-                                                             (int64_t)(prev_row0 + e) * ldt *
-                                                                 (int64_t)(sizeof(Raw) / 4))
-                              : reinterpret_cast<const Raw*>(zrow);
-    for (int q = tid; q * kConeVec < P; q += THREADS) {
-      const int base = q * kConeVec;
-      const uint32_t coff = (uint32_t)base * 4u;
-      const float4 vx4 = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(cx) + coff);
-      const float4 vy4 = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(cy) + coff);
-      const float4 vz4 = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(cz) + coff);
-      // the read-back loads are issued after the coordinate loads (memory clobber: no reordering), so waiting
-      // for the coordinates does not wait for them (loads complete in order)
-      asm volatile("" ::: "memory");
-      // unconditional (a branch here would make the coordinate wait drain them too)
-      Raw rb[E];
-      const uint32_t qoff = (uint32_t)q * (uint32_t)sizeof(Raw);  // one 32-bit offset from the rows' scalar bases
-#pragma unroll
-      for (int e = 0; e < E; ++e)
-        rb[e] = *reinterpret_cast<const Raw*>(reinterpret_cast<const char*>(prow[e]) + qoff);
-      cone_rows_tile_at<SIG, AF, OUT_DT, false, E>(evs, rrow, base, P, vx4, vy4, vz4, A, nullptr, nullptr,
-                                                   nullptr, nullptr, cc, out_t, ldt, nullptr, 0, accS, accQ, accC,
-                                                   accM);
-      float4 a = sacc[q];
-      float w[4] = {a.x, a.y, a.z, a.w};
-      // f of the previous group's rows read from LDS (a vector register operand; the scalar registers are
-      // taken by the event parameters and row bases).  fsh holds them until the next group's finalisation.
-      const float4 f4 = fsh4;
-      const float fv[E] = {f4.x, f4.y, f4.z, f4.w};
+    for (int h = 0; h < H; ++h) {
+      ConeEv evs[E];
+      int rrow[E];
 #pragma unroll
       for (int e = 0; e < E; ++e) {
-        float v[4];
-        if constexpr (OUT_DT == JSCC_DT_F32) {
-          v[0] = rb[e].x; v[1] = rb[e].y; v[2] = rb[e].z; v[3] = rb[e].w;
-        } else {
-          v[0] = __uint_as_float(rb[e].x << 16); v[1] = __uint_as_float(rb[e].x & 0xffff0000u);
-          v[2] = __uint_as_float(rb[e].y << 16); v[3] = __uint_as_float(rb[e].y & 0xffff0000u);
-        }
-#pragma unroll
-        for (int k = 0; k < 4; ++k) w[k] = fmaf(fv[e], v[k], w[k]);
+        rrow[e] = valid[h * E + e] ? (row0 + h * E + e) : (n_rows - 1);
+        evs[e] = load_event(ev[ev_idx[rrow[e]]], lda);
       }
-      sacc[q] = make_float4(w[0], w[1], w[2], w[3]);
+      // rows of the previous group's half h, as stored; a dropped row (f = 0, possibly holding NaN) and every
+      // row before the first group read the zero row instead, so the additions below need no branch
+      const Raw* prow[E];
+#pragma unroll
+      for (int e = 0; e < E; ++e)
+        prow[e] = pf[h * E + e] != 0.0f ? reinterpret_cast<const Raw*>(reinterpret_cast<const char*>(out_t) +
+                                                                       (int64_t)(prev_row0 + h * E + e) * ldt *
+                                                                           (int64_t)(sizeof(Raw) / 4))
+                                        : reinterpret_cast<const Raw*>(zrow);
+      for (int q = tid; q * kConeVec < P; q += THREADS) {
+        const int base = q * kConeVec;
+        const uint32_t coff = (uint32_t)base * 4u;
+        const float4 vx4 = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(cx) + coff);
+        const float4 vy4 = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(cy) + coff);
+        const float4 vz4 = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(cz) + coff);
+        // the read-back loads are issued after the coordinate loads (memory clobber: no reordering), so
+        // waiting for the coordinates does not wait for them (loads complete in order); unconditional, since
+        // a branch here would make that wait drain them too
+        asm volatile("" ::: "memory");
+        Raw rb[E];
+        const uint32_t qoff = (uint32_t)q * (uint32_t)sizeof(Raw);  // one 32-bit offset from scalar row bases
+#pragma unroll
+        for (int e = 0; e < E; ++e)
+          rb[e] = *reinterpret_cast<const Raw*>(reinterpret_cast<const char*>(prow[e]) + qoff);
+        cone_rows_tile_at<SIG, AF, OUT_DT, false, E>(evs, rrow, base, P, vx4, vy4, vz4, A, nullptr, nullptr,
+                                                     nullptr, nullptr, cc, out_t, ldt, nullptr, 0, accS + h * E,
+                                                     accQ + h * E, accC + h * E, accM + h * E);
+        float4 a = sacc[q];
+        float w[4] = {a.x, a.y, a.z, a.w};
+        // f of the previous group's rows read from LDS (a vector register operand; the scalar registers are
+        // taken by the event parameters and row bases).  fsh holds them until the next group's finalisation.
+        const float4 f4 = fsh4[h];
+        const float fv[E] = {f4.x, f4.y, f4.z, f4.w};
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          float v[4];
+          if constexpr (OUT_DT == JSCC_DT_F32) {
+            v[0] = rb[e].x; v[1] = rb[e].y; v[2] = rb[e].z; v[3] = rb[e].w;
+          } else {
+            v[0] = __uint_as_float(rb[e].x << 16); v[1] = __uint_as_float(rb[e].x & 0xffff0000u);
+            v[2] = __uint_as_float(rb[e].y << 16); v[3] = __uint_as_float(rb[e].y & 0xffff0000u);
+          }
+#pragma unroll
+          for (int k = 0; k < 4; ++k) w[k] = fmaf(fv[e], v[k], w[k]);
+        }
+        sacc[q] = make_float4(w[0], w[1], w[2], w[3]);
+      }
     }
     if (AF == AF_NONE) {
 #pragma unroll
-      for (int e = 0; e < E; ++e) accC[e] = accS[e];
+      for (int e = 0; e < R; ++e) accC[e] = accS[e];
     }
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
+    for (int e = 0; e < R; ++e) {
       const double sS = jscc_wave_sum((double)accS[e]);
       const double sQ = jscc_wave_sum((double)accQ[e]);
       const double sC = jscc_wave_sum((double)accC[e]);
@@ -597,10 +606,10 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4)
     if (tid == 0) next_g = atomicAdd(ticket, 1);
     __syncthreads();
     const int g_next = next_g;
-    bool fix[E];
+    bool fix[R];
     bool any_fix = false;
 #pragma unroll
-    for (int e = 0; e < E; ++e) {  // rows below 2^-40: the sum of squares again in FP64 (uniform decision)
+    for (int e = 0; e < R; ++e) {  // rows below 2^-40: the sum of squares again in FP64 (uniform decision)
       double m = red[3][0][e];
       for (int w = 1; w < NW; ++w) m = fmax(m, red[3][w][e]);
       fix[e] = valid[e] && !(m >= kQ64Below);
@@ -608,16 +617,17 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4)
     }
     if (any_fix) {
 #pragma unroll
-      for (int e = 0; e < E; ++e) {
+      for (int e = 0; e < R; ++e) {
         if (fix[e]) {
-          const double q = cone_row_q64<SIG, AF, THREADS>(evs[e], P, cx, cy, cz, A, nullptr, nullptr, nullptr,
-                                                          nullptr, cc, wq);
+          const ConeEv evf = load_event(ev[ev_idx[row0 + e]], lda);
+          const double q = cone_row_q64<SIG, AF, THREADS>(evf, P, cx, cy, cz, A, nullptr, nullptr, nullptr, nullptr,
+                                                          cc, wq);
           if (tid == 0) q64[e] = q;
         }
       }
       __syncthreads();
     }
-    if (tid < E) {
+    if (tid < R) {
       const int e = tid;
       float f = 0.0f;
       if (row0 + e < n_rows) {
@@ -636,14 +646,14 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4)
     }
     __syncthreads();
 #pragma unroll
-    for (int e = 0; e < E; ++e) pf[e] = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(fsh[e])));
+    for (int e = 0; e < R; ++e) pf[e] = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(fsh[e])));
     prev_row0 = row0;
     g = g_next;
   }
   // the rows of the work-group's last group
   if (prev_row0 >= 0) {
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
+    for (int e = 0; e < R; ++e) {
       if (pf[e] != 0.0f) {
         const Raw* pr = reinterpret_cast<const Raw*>(reinterpret_cast<const char*>(out_t) +
                                                      (int64_t)(prev_row0 + e) * ldt * (int64_t)(sizeof(Raw) / 4));
