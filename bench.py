@@ -850,10 +850,14 @@ def _summary(res):
     keep["config"] = res.get("config", {}).get("workload")
     keep.update({k: v for k, v in res.items() if k.startswith("rel_err")})
     if "components" in res and isinstance(res["components"], dict):
-        keep["components"] = {n: {k: v for k, v in c.items() if k in ("iter_s", "ms_per_iter", "hbm_frac",
-                                                                      "rel_err_vs_fp64_oracle_reduced",
-                                                                      "meets_error_bar")}
-                              for n, c in res["components"].items() if isinstance(c, dict)}
+        per = ("iter_s", "ms_per_iter", "hbm_frac", "rel_err_vs_fp64_oracle_reduced", "meets_error_bar")
+        comp = {}
+        for n, c in res["components"].items():
+            if isinstance(c, dict):  # per storage type: the figures above; a launch shape: as it is
+                comp[n] = {k: v for k, v in c.items() if k in per} or c
+            elif n in ("groups_per_cu", "kept_min", "kept_max"):
+                comp[n] = c
+        keep["components"] = comp
     cb = res.get("cpu_baseline")
     if isinstance(cb, dict):
         keep["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind") if k in cb}
@@ -905,13 +909,16 @@ def run_jsccd(args):
         # has no host synchronisation; the FP16 store is quantised from the selected FP32 rows, which reads
         # the counts
         # the sensitivity (column sums of the kept, normalised rows) is formed by the cone pass itself
-        # (want_sens), from the rows as stored, instead of a second pass over T
+        # (want_sens), from the rows as stored, instead of a second pass over T; FP16 rows are quantised after
+        # the cone pass, so their sensitivity is the column-sum pass over the quantised rows the OSEM projects
+        # (consistent with them: 1.8e-5 of the FP64 oracle's image, 2.8e-4 from the FP32 rows' sums)
+        fused = name != "fp16"
         rows = build_compton_rows(None, det, coor, lst0.clone(), 0.0, 0.0, cl.e0, 0.02, tmax, tmin, tsum, dev,
                                   noise=(n1, n2), sigma_model="fixed", fixed_sigma=sigma, store_dtype=store_dtype,
-                                  sync_counts=False, want_sens=True)
+                                  sync_counts=False, want_sens=fused)
         last_rows[0] = rows
         st = ListModeStore.from_rows(rows, name)
-        s = rows.sens
+        s = rows.sens if fused else st.backproject_ones()
         x = torch.ones_like(s)
         st.osem(x, s, args.iters, args.subsets)
         return st, x
@@ -980,7 +987,7 @@ def run_jsccd(args):
             "osem_launches_per_step": launches,
             "osem_bytes_per_launch": subset_bytes,
             "osem_GBs": subset_bytes / (proj_ms * 1e-3) / 1e9,
-            "sensitivity": "formed in the cone pass (want_sens)" if scnt == 0 else "column-sum pass",
+            "sensitivity": "formed in the cone pass (want_sens)" if name != "fp16" else "column-sum pass",
             "osem_iter_s": 1.0 / osem_s,
             "osem_graph_iter_s": 1.0 / osem_graph_s,
             "osem_graph_equals_eager": graph_equal,
@@ -1000,17 +1007,18 @@ def run_jsccd(args):
             "traffic": None if traffic is None else int(traffic),
             "launch_ms": round(bf["osem_launch_ms"], 4), "bytes_per_launch": int(bf["osem_bytes_per_launch"])}
     value = args.events / (bf["ms_per_step"] * 1e-3) / 1e6
-    # the cone kernel is VALU-bound: its instruction count per (event, voxel) pair on this workload, from
-    # SQ_INSTS_VALU x 64 / pairs (46.6, 5 of them sqrt / rsq / rcp / exp; the gfx950 ISA of k_cone_rows<fixed
-    # sigma, no A factor, BF16 out> has 727 per loop iteration of 4 events x 4 voxels on the common path plus
-    # 40 per event and lane quad near the cone axis), times the measured pair rate, against the non-packed
-    # issue rate at the 2.4 GHz engine clock (one wave64 instruction per 4 cycles per SIMD).  Under load the
-    # kernel is power-limited to ~2.15-2.3 GHz (GRBM_GUI_ACTIVE per dispatch), which caps the fraction below 1.
-    cone_roof = {"kernel": "k_cone_rows<fixed sigma, BF16>", "bound": "valu", "valu_instr_per_pair": 46.6,
-                 "transcendental_per_pair": 5, "pairs_s": round(bf["cone_pairs_s"], -8),
-                 "valu_lane_instr_s": round(bf["cone_pairs_s"] * 46.6, -10),
+    # the cone pass (rows, statistics and the sensitivity's read-back and column sums) is VALU-bound: its
+    # instruction count per (event, voxel) pair on this workload, from SQ_INSTS_VALU x 64 / pairs of
+    # k_cone_rows_sens<fixed sigma, no A factor, BF16> (47.6, 5 of them sqrt / rsq / rcp / exp), times the
+    # measured pair rate, against the non-packed issue rate at the 2.4 GHz engine clock (one wave64 instruction
+    # per 4 cycles per SIMD).  Under load the pass is power-limited to ~2.15-2.3 GHz (GRBM_GUI_ACTIVE per
+    # dispatch), which caps the fraction below 1.
+    cone_roof = {"kernel": "k_cone_rows_sens<fixed sigma, BF16>: rows + statistics + sensitivity", "bound": "valu",
+                 "valu_instr_per_pair": 47.6, "transcendental_per_pair": 5,
+                 "pairs_s": round(bf["cone_pairs_s"], -8),
+                 "valu_lane_instr_s": round(bf["cone_pairs_s"] * 47.6, -10),
                  "peak_lane_instr_s": FP32_LANE_PEAK,
-                 "frac": round(bf["cone_pairs_s"] * 46.6 / FP32_LANE_PEAK, 4),
+                 "frac": round(bf["cone_pairs_s"] * 47.6 / FP32_LANE_PEAK, 4),
                  "clock_GHz_profiled": 2.15}
     comp = {}
     for k, r in results.items():
@@ -1061,12 +1069,13 @@ def run_jsccd(args):
                                              sigma)
         errs = {}
         for name, sd in (("bf16", torch.bfloat16), ("fp16", torch.float32), ("fp32", torch.float32)):
+            fused = name != "fp16"
             rows_s = build_compton_rows(None, det, coor, lst0[:n_s].clone(), 0.0, 0.0, cl.e0, 0.02, tmax, tmin, tsum,
                                         dev, noise=(n1[:n_s], n2[:n_s]), sigma_model="fixed", fixed_sigma=sigma,
-                                        store_dtype=sd, sync_counts=False, want_sens=True)
+                                        store_dtype=sd, sync_counts=False, want_sens=fused)
             kept_ok = bool(np.array_equal(rows_s.kept_event_index().cpu().numpy(), kept_ref))
             st = ListModeStore.from_rows(rows_s, name)
-            s_s = rows_s.sens
+            s_s = rows_s.sens if fused else st.backproject_ones()
             x_s = st.osem(torch.ones_like(s_s), s_s, args.iters, args.subsets)
             e = _rel(x_s.cpu().numpy(), x_ref)
             errs[name] = {"rel_l2": e, "bar": 1e-4 if name == "fp32" else 1e-3,
