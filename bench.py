@@ -909,22 +909,21 @@ def run_jsccd(args):
         # has no host synchronisation; the FP16 store is quantised from the selected FP32 rows, which reads
         # the counts
         # the sensitivity (column sums of the kept, normalised rows) is formed by the cone pass itself
-        # (want_sens), from the rows as stored, instead of a second pass over T; FP16 rows are quantised after
-        # the cone pass, so their sensitivity is the column-sum pass over the quantised rows the OSEM projects
-        # (consistent with them: 1.8e-5 of the FP64 oracle's image, 2.8e-4 from the FP32 rows' sums)
-        fused = name != "fp16"
+        # (want_sens), from the rows as stored, instead of a second pass over T; FP16 rows are quantised by the
+        # same pass (row maximum at 2^14, one group after the FP32 values) and their sensitivity is the column
+        # sums of the codes the OSEM projects
         rows = build_compton_rows(None, det, coor, lst0.clone(), 0.0, 0.0, cl.e0, 0.02, tmax, tmin, tsum, dev,
                                   noise=(n1, n2), sigma_model="fixed", fixed_sigma=sigma, store_dtype=store_dtype,
-                                  sync_counts=False, want_sens=fused)
+                                  sync_counts=False, want_sens=True)
         last_rows[0] = rows
         st = ListModeStore.from_rows(rows, name)
-        s = rows.sens if fused else st.backproject_ones()
+        s = rows.sens
         x = torch.ones_like(s)
         st.osem(x, s, args.iters, args.subsets)
         return st, x
 
     results = {}
-    for name, sd in (("bf16", torch.bfloat16), ("fp32", torch.float32), ("fp16", torch.float32)):
+    for name, sd in (("bf16", torch.bfloat16), ("fp32", torch.float32), ("fp16", torch.float16)):
         if name != "bf16" and args.skip_fp32:
             continue
         for _ in range(args.warmup):
@@ -987,7 +986,7 @@ def run_jsccd(args):
             "osem_launches_per_step": launches,
             "osem_bytes_per_launch": subset_bytes,
             "osem_GBs": subset_bytes / (proj_ms * 1e-3) / 1e9,
-            "sensitivity": "formed in the cone pass (want_sens)" if name != "fp16" else "column-sum pass",
+            "sensitivity": "formed in the cone pass (want_sens)",
             "osem_iter_s": 1.0 / osem_s,
             "osem_graph_iter_s": 1.0 / osem_graph_s,
             "osem_graph_equals_eager": graph_equal,
@@ -1068,14 +1067,13 @@ def run_jsccd(args):
         kept_ref, x_ref = fp64_oracle_sample(cl, n1_host.numpy(), n2_host.numpy(), n_s, args.iters, args.subsets,
                                              sigma)
         errs = {}
-        for name, sd in (("bf16", torch.bfloat16), ("fp16", torch.float32), ("fp32", torch.float32)):
-            fused = name != "fp16"
+        for name, sd in (("bf16", torch.bfloat16), ("fp16", torch.float16), ("fp32", torch.float32)):
             rows_s = build_compton_rows(None, det, coor, lst0[:n_s].clone(), 0.0, 0.0, cl.e0, 0.02, tmax, tmin, tsum,
                                         dev, noise=(n1[:n_s], n2[:n_s]), sigma_model="fixed", fixed_sigma=sigma,
-                                        store_dtype=sd, sync_counts=False, want_sens=fused)
+                                        store_dtype=sd, sync_counts=False, want_sens=True)
             kept_ok = bool(np.array_equal(rows_s.kept_event_index().cpu().numpy(), kept_ref))
             st = ListModeStore.from_rows(rows_s, name)
-            s_s = rows_s.sens if fused else st.backproject_ones()
+            s_s = rows_s.sens
             x_s = st.osem(torch.ones_like(s_s), s_s, args.iters, args.subsets)
             e = _rel(x_s.cpu().numpy(), x_ref)
             errs[name] = {"rel_l2": e, "bar": 1e-4 if name == "fp32" else 1e-3,

@@ -106,27 +106,32 @@ int jscc_cone_rows_counted(const void* events, const int32_t* ev_idx, const int3
 
 /* jscc_cone_rows_counted (n_rows_dev may be NULL: all n_rows_max rows) that
  * also returns the list-mode sensitivity of the rows it writes:
- *   sens[p] = sum over kept rows i of out_t[i, p] / S_i
+ *   sens[p] = sum over kept rows i of T_i[p] / S_i
  * with keep_i and S_i as jscc_rows_keep_flags decides them at min_support --
  * torch.sum(t, dim=0) of the normalised, kept T that main_plane.py:348-355
- * scales into Sensi_d (and recon_osem_plane.py:253-256 divides by).  Each row
- * is read back as stored (FP32 or BF16) by the thread that wrote it, during
- * the cone arithmetic of the work-group's next rows, so the sums cost no
- * second pass over T in HBM.  Persistent grid of jscc_cone_sens_grid(n_vox)
- * work-groups taking groups of 8 rows from a ticket counter: the grouping of
- * the FP32 sums, and so the last bits of sens, can differ between runs (rows
- * and statistics do not).  out_dtype FP32 or BF16, first-hit factor
+ * scales into Sensi_d (and recon_osem_plane.py:253-256 divides by) -- taken
+ * from the rows as stored.  Each row is read back by the thread that wrote it,
+ * during the cone arithmetic of the work-group's next rows, so the sums cost no
+ * second pass over T in HBM.  out_dtype FP32 or BF16: rows as jscc_cone_rows
+ * stores them.  out_dtype FP16: the FP32 values wait in a ring in the workspace
+ * and the read-back stores the codes of jscc_quantize_rows (half(t / q),
+ * q = max / 2^14) with row_scale[i] = q / S_i, row_scale * code ~ t / S; a
+ * dropped row's codes are zeros; sens is then the column sums of
+ * row_scale * code, the rows the OSEM projects.  Persistent grid of
+ * jscc_cone_sens_grid(n_vox) work-groups taking groups of 8 rows from a ticket
+ * counter: the grouping of the FP32 sums, and so the last bits of sens, can
+ * differ between runs (rows, codes and statistics do not).  First-hit factor
  * sysmat[arow] or none (no tap tables), no t_c output, n_vox <= 32768.
- * workspace: jscc_cone_sens_workspace_bytes(n_vox) bytes, 16-byte aligned.
- * Statistics equal jscc_cone_rows's up to the order of the per-row sums
- * (another work-group shape); the row values are the same bits. */
+ * workspace: jscc_cone_sens_workspace_bytes(n_vox, out_dtype) bytes, 16-byte
+ * aligned.  Statistics equal jscc_cone_rows's up to the order of the per-row
+ * sums (another work-group shape); the row values are the same bits. */
 int jscc_cone_sens_grid(int64_t n_vox);
-int64_t jscc_cone_sens_workspace_bytes(int64_t n_vox);
+int64_t jscc_cone_sens_workspace_bytes(int64_t n_vox, int out_dtype);
 int jscc_cone_rows_sens(const void* events, const int32_t* ev_idx, const int32_t* n_rows_dev, int64_t n_rows_max,
                         const float* coor_x, const float* coor_y, const float* coor_z, int64_t n_vox,
                         const float* sysmat, int64_t ld_sysmat, int sigma_model, double fixed_sigma, void* out_t,
-                        int out_dtype, int64_t ld_out, double* stats, double min_support, float* sens,
-                        void* workspace, int64_t workspace_bytes, void* stream);
+                        int out_dtype, int64_t ld_out, float* row_scale, double* stats, double min_support,
+                        float* sens, void* workspace, int64_t workspace_bytes, void* stream);
 
 /* keep[i] = S finite and nonzero and S^2/Q >= min_support
  * (process_list_plane_strict.py:241-246, 145-146); inv_sum (optional) = 1/S

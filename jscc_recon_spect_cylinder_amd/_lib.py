@@ -36,9 +36,9 @@ SIGNATURES = {
     "jscc_cone_rows_counted": (c_int, [vp, vp, vp, c_i64, vp, vp, vp, c_i64, vp, c_i64, vp, vp, vp, vp, c_int, c_f64,
                                        vp, c_int, c_i64, vp, c_i64, vp, vp]),
     "jscc_cone_sens_grid": (c_int, [c_i64]),
-    "jscc_cone_sens_workspace_bytes": (c_i64, [c_i64]),
+    "jscc_cone_sens_workspace_bytes": (c_i64, [c_i64, c_int]),
     "jscc_cone_rows_sens": (c_int, [vp, vp, vp, c_i64, vp, vp, vp, c_i64, vp, c_i64, c_int, c_f64, vp, c_int, c_i64,
-                                    vp, c_f64, vp, vp, c_i64, vp]),
+                                    vp, vp, c_f64, vp, vp, c_i64, vp]),
     "jscc_rows_keep_flags": (c_int, [vp, c_i64, c_f64, vp, vp, vp]),
     "jscc_rows_gather_normalize": (c_int, [vp, c_int, c_i64, c_i64, vp, c_i64, vp, c_int, vp, vp, c_i64, c_i64, vp]),
     "jscc_arows_normalize": (c_int, [vp, c_i64, c_i64, vp, c_i64, vp, c_i64, vp]),

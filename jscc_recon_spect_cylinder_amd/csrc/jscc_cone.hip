@@ -476,23 +476,35 @@ __global__ void k_sens_prep(int* ticket, float* zrow, int n) {
   for (int j = i; j < n; j += gridDim.x * blockDim.x) zrow[j] = 0.0f;
 }
 
+// FP16 rows (OUT_DT = JSCC_DT_FP16): the cone values of a group go as FP32 into
+// a ring of two group slots per work-group; the read-back during the next group
+// quantises them like jscc_quantize_rows (code = half(t / q), q = max / 2^14,
+// row_scale = q / S, both known by then), stores the codes and adds
+// row_scale x code -- the column sums of the rows the OSEM will project.  A
+// dropped row is read from the zero row (its codes are stored as zeros; the OSEM
+// never reads it); before the first group the codes go to a sink row.
 template <int SIG, int AF, int OUT_DT, int THREADS>
 __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) k_cone_rows_sens(
     const JsccEvent* __restrict__ ev, const int32_t* __restrict__ ev_idx, int n_grid,
     const int32_t* __restrict__ n_rows_dev, const float* __restrict__ cx, const float* __restrict__ cy,
     const float* __restrict__ cz, int P, const float* __restrict__ A, int64_t lda, ConeConsts cc,
     void* __restrict__ out_t, int64_t ldt, double* __restrict__ stats, double min_support,
-    float* __restrict__ partial, int64_t ldp, int* __restrict__ ticket, const float* __restrict__ zrow) {
+    float* __restrict__ partial, int64_t ldp, int* __restrict__ ticket, const float* __restrict__ zrow,
+    uint16_t* __restrict__ sink, float* __restrict__ ring, float* __restrict__ row_scale) {
   constexpr int E = 4;              // events evaluated together (scalar registers)
   constexpr int H = kSensHalves;    // E-event halves per group: one barrier round per H * E rows
   constexpr int R = H * E;          // rows per group
   constexpr int NW = THREADS / JSCC_WAVE;
-  using Raw = typename std::conditional<OUT_DT == JSCC_DT_F32, float4, uint2>::type;
+  constexpr bool Q16 = OUT_DT == JSCC_DT_FP16;
+  constexpr int TILE_DT = Q16 ? JSCC_DT_F32 : OUT_DT;  // what the tile loop stores
+  using Raw = typename std::conditional<TILE_DT == JSCC_DT_F32, float4, uint2>::type;
   extern __shared__ float4 sacc[];  // [ldp / 4] column sums; quad q is owned by thread q % THREADS
   __shared__ double red[4][NW][R];
   __shared__ double q64[R], wq[NW];
   __shared__ float4 fsh4[H];  // the factors f of the rows of the work-group's last finished group
+  __shared__ float4 qsh4[H];  // FP16 rows: 1 / q of those rows
   float* fsh = reinterpret_cast<float*>(fsh4);
+  float* qsh = reinterpret_cast<float*>(qsh4);
   __shared__ int next_g;
   const int tid = threadIdx.x;
   int n_rows = n_grid;
@@ -509,7 +521,10 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4)
   const int n_quads = (int)(ldp / 4);
   for (int q = tid; q < n_quads; q += THREADS) sacc[q] = make_float4(0.f, 0.f, 0.f, 0.f);
   const int n_groups = (n_rows + R - 1) / R;
-  if (tid < H) fsh4[tid] = make_float4(0.f, 0.f, 0.f, 0.f);  // no previous group: nothing is added
+  if (tid < H) {  // no previous group: nothing is added (or stored, for FP16 rows)
+    fsh4[tid] = make_float4(0.f, 0.f, 0.f, 0.f);
+    qsh4[tid] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
   if (tid == 0) next_g = atomicAdd(ticket, 1);
   __syncthreads();
   int g = next_g;
@@ -519,6 +534,8 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4)
   float pf[R];
 #pragma unroll
   for (int e = 0; e < R; ++e) pf[e] = 0.0f;
+  int slot = 0;  // FP16 rows: the ring slot this group's FP32 values go to
+  float pf_last = 0.0f;  // f of the previous group's last row below the count
 
   while (g < n_groups) {
     const int row0 = g * R;
@@ -529,24 +546,44 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4)
       valid[e] = (row0 + e) < n_rows;
       accS[e] = 0.0f; accQ[e] = 0.0f; accC[e] = 0.0f; accM[e] = 0.0f;
     }
+    float* ring_cur = Q16 ? ring + ((int64_t)blockIdx.x * 2 + slot) * R * ldp : nullptr;
+    const float* ring_prev = Q16 ? ring + ((int64_t)blockIdx.x * 2 + (slot ^ 1)) * R * ldp : nullptr;
 #pragma unroll
     for (int h = 0; h < H; ++h) {
       ConeEv evs[E];
-      int rrow[E];
+      int rrow[E];  // rows the tile loop stores to (ring slots for FP16 rows)
 #pragma unroll
       for (int e = 0; e < E; ++e) {
-        rrow[e] = valid[h * E + e] ? (row0 + h * E + e) : (n_rows - 1);
-        evs[e] = load_event(ev[ev_idx[rrow[e]]], lda);
+        const int r = valid[h * E + e] ? (row0 + h * E + e) : (n_rows - 1);
+        evs[e] = load_event(ev[ev_idx[r]], lda);
+        rrow[e] = Q16 ? r - row0 : r;
       }
-      // rows of the previous group's half h, as stored; a dropped row (f = 0, possibly holding NaN) and every
-      // row before the first group read the zero row instead, so the additions below need no branch
+      // rows of the previous group's half h: as stored (FP32 ring slots for FP16 rows); a dropped row (f = 0,
+      // possibly holding NaN) and every row before the first group read the zero row instead, so the additions
+      // below need no branch
       const Raw* prow[E];
+      uint16_t* dst16[E];  // FP16 rows: where the previous group's codes go
 #pragma unroll
-      for (int e = 0; e < E; ++e)
-        prow[e] = pf[h * E + e] != 0.0f ? reinterpret_cast<const Raw*>(reinterpret_cast<const char*>(out_t) +
-                                                                       (int64_t)(prev_row0 + h * E + e) * ldt *
-                                                                           (int64_t)(sizeof(Raw) / 4))
-                                        : reinterpret_cast<const Raw*>(zrow);
+      for (int e = 0; e < E; ++e) {
+        const int pe = h * E + e;
+        if constexpr (Q16) {
+          // a slot past the count holds nothing: it reads (and stores) the group's last row again, with that
+          // row's f deciding between its values and the zero row (its own f, 0, keeps it out of the sums)
+          const int last = n_rows - 1 - prev_row0;
+          const int le = pe < last ? pe : last;
+          const float pfe = pe < last ? pf[pe] : pf_last;
+          prow[e] = pfe != 0.0f ? reinterpret_cast<const Raw*>(ring_prev + (int64_t)le * ldp)
+                                : reinterpret_cast<const Raw*>(zrow);
+          dst16[e] = prev_row0 < 0 ? sink
+                                   : reinterpret_cast<uint16_t*>(out_t) + (int64_t)(prev_row0 + le) * ldt;
+        } else {
+          prow[e] = pf[pe] != 0.0f ? reinterpret_cast<const Raw*>(reinterpret_cast<const char*>(out_t) +
+                                                                  (int64_t)(prev_row0 + pe) * ldt *
+                                                                      (int64_t)(sizeof(Raw) / 4))
+                                   : reinterpret_cast<const Raw*>(zrow);
+          dst16[e] = nullptr;
+        }
+      }
       for (int q = tid; q * kConeVec < P; q += THREADS) {
         const int base = q * kConeVec;
         const uint32_t coff = (uint32_t)base * 4u;
@@ -562,19 +599,36 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4)
 #pragma unroll
         for (int e = 0; e < E; ++e)
           rb[e] = *reinterpret_cast<const Raw*>(reinterpret_cast<const char*>(prow[e]) + qoff);
-        cone_rows_tile_at<SIG, AF, OUT_DT, false, E>(evs, rrow, base, P, vx4, vy4, vz4, A, nullptr, nullptr,
-                                                     nullptr, nullptr, cc, out_t, ldt, nullptr, 0, accS + h * E,
-                                                     accQ + h * E, accC + h * E, accM + h * E);
+        cone_rows_tile_at<SIG, AF, TILE_DT, false, E>(evs, rrow, base, P, vx4, vy4, vz4, A, nullptr, nullptr,
+                                                      nullptr, nullptr, cc, Q16 ? (void*)ring_cur : out_t,
+                                                      Q16 ? ldp : ldt, nullptr, 0, accS + h * E, accQ + h * E,
+                                                      accC + h * E, accM + h * E);
         float4 a = sacc[q];
         float w[4] = {a.x, a.y, a.z, a.w};
-        // f of the previous group's rows read from LDS (a vector register operand; the scalar registers are
-        // taken by the event parameters and row bases).  fsh holds them until the next group's finalisation.
+        // f (and 1 / q) of the previous group's rows read from LDS (vector register operands; the scalar
+        // registers are taken by the event parameters and row bases).  fsh / qsh hold them until the next
+        // group's finalisation.
         const float4 f4 = fsh4[h];
         const float fv[E] = {f4.x, f4.y, f4.z, f4.w};
+        float iq[E];
+        if constexpr (Q16) {
+          const float4 q4 = qsh4[h];
+          iq[0] = q4.x; iq[1] = q4.y; iq[2] = q4.z; iq[3] = q4.w;
+        }
 #pragma unroll
         for (int e = 0; e < E; ++e) {
           float v[4];
-          if constexpr (OUT_DT == JSCC_DT_F32) {
+          if constexpr (Q16) {
+            // the codes of jscc_quantize_rows: half(t / q), round to nearest even in one step
+            _Float16 c[4];
+            c[0] = (_Float16)(rb[e].x * iq[e]); c[1] = (_Float16)(rb[e].y * iq[e]);
+            c[2] = (_Float16)(rb[e].z * iq[e]); c[3] = (_Float16)(rb[e].w * iq[e]);
+            uint2 pk;
+            pk.x = (uint32_t)__builtin_bit_cast(uint16_t, c[0]) | ((uint32_t)__builtin_bit_cast(uint16_t, c[1]) << 16);
+            pk.y = (uint32_t)__builtin_bit_cast(uint16_t, c[2]) | ((uint32_t)__builtin_bit_cast(uint16_t, c[3]) << 16);
+            *reinterpret_cast<uint2*>(reinterpret_cast<char*>(dst16[e]) + (uint32_t)base * 2u) = pk;
+            v[0] = (float)c[0]; v[1] = (float)c[1]; v[2] = (float)c[2]; v[3] = (float)c[3];
+          } else if constexpr (TILE_DT == JSCC_DT_F32) {
             v[0] = rb[e].x; v[1] = rb[e].y; v[2] = rb[e].z; v[3] = rb[e].w;
           } else {
             v[0] = __uint_as_float(rb[e].x << 16); v[1] = __uint_as_float(rb[e].x & 0xffff0000u);
@@ -629,7 +683,7 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4)
     }
     if (tid < R) {
       const int e = tid;
-      float f = 0.0f;
+      float f = 0.0f, inv_q = 0.0f;
       if (row0 + e < n_rows) {
         double v[4];
 #pragma unroll
@@ -640,36 +694,68 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4)
           v[k] = t;
           stats[(int64_t)(row0 + e) * 4 + k] = t;
         }
-        if (jscc_row_keep(v[0], v[1], v[3], min_support)) f = (float)(1.0 / v[0]);
+        const bool keep = jscc_row_keep(v[0], v[1], v[3], min_support);
+        if constexpr (Q16) {  // the per-row scale of jscc_quantize_rows (row max at 2^14)
+          const double q = v[3] > 0.0 ? v[3] / 16384.0 : 1.0;
+          row_scale[row0 + e] = (float)(q / v[0]);
+          if (keep) {  // a dropped row reads the zero row: 1 / q stays 0 (it may not be finite for such a row)
+            f = (float)(q / v[0]);
+            inv_q = (float)(1.0 / q);
+          }
+        } else {
+          if (keep) f = (float)(1.0 / v[0]);
+        }
       }
       fsh[e] = f;
+      qsh[e] = inv_q;
     }
     __syncthreads();
 #pragma unroll
     for (int e = 0; e < R; ++e) pf[e] = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(fsh[e])));
+    {
+      const int last = n_rows - 1 - row0 < R - 1 ? n_rows - 1 - row0 : R - 1;
+      pf_last = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(fsh[last])));
+    }
     prev_row0 = row0;
+    slot ^= 1;
     g = g_next;
   }
   // the rows of the work-group's last group
   if (prev_row0 >= 0) {
+    const float* ring_prev = Q16 ? ring + ((int64_t)blockIdx.x * 2 + (slot ^ 1)) * R * ldp : nullptr;
 #pragma unroll
     for (int e = 0; e < R; ++e) {
-      if (pf[e] != 0.0f) {
-        const Raw* pr = reinterpret_cast<const Raw*>(reinterpret_cast<const char*>(out_t) +
-                                                     (int64_t)(prev_row0 + e) * ldt * (int64_t)(sizeof(Raw) / 4));
+      if (prev_row0 + e < n_rows && (Q16 || pf[e] != 0.0f)) {
+        const float iq = qsh[e];
+        const Raw* pr = Q16 ? reinterpret_cast<const Raw*>(ring_prev + (int64_t)e * ldp)
+                            : reinterpret_cast<const Raw*>(reinterpret_cast<const char*>(out_t) +
+                                                           (int64_t)(prev_row0 + e) * ldt * (int64_t)(sizeof(Raw) / 4));
         for (int q = tid; q * kConeVec < P; q += THREADS) {
           const Raw rw = pr[q];
           float v[4];
-          if constexpr (OUT_DT == JSCC_DT_F32) {
+          if constexpr (Q16) {
+            _Float16 c[4];
+            c[0] = (_Float16)(rw.x * iq); c[1] = (_Float16)(rw.y * iq);
+            c[2] = (_Float16)(rw.z * iq); c[3] = (_Float16)(rw.w * iq);
+            if (pf[e] == 0.0f) c[0] = c[1] = c[2] = c[3] = (_Float16)0.0f;  // a dropped row: zeros
+            uint2 pk;
+            pk.x = (uint32_t)__builtin_bit_cast(uint16_t, c[0]) | ((uint32_t)__builtin_bit_cast(uint16_t, c[1]) << 16);
+            pk.y = (uint32_t)__builtin_bit_cast(uint16_t, c[2]) | ((uint32_t)__builtin_bit_cast(uint16_t, c[3]) << 16);
+            *reinterpret_cast<uint2*>(reinterpret_cast<uint16_t*>(out_t) + (int64_t)(prev_row0 + e) * ldt +
+                                      (int64_t)q * kConeVec) = pk;
+            v[0] = (float)c[0]; v[1] = (float)c[1]; v[2] = (float)c[2]; v[3] = (float)c[3];
+          } else if constexpr (TILE_DT == JSCC_DT_F32) {
             v[0] = rw.x; v[1] = rw.y; v[2] = rw.z; v[3] = rw.w;
           } else {
             v[0] = __uint_as_float(rw.x << 16); v[1] = __uint_as_float(rw.x & 0xffff0000u);
             v[2] = __uint_as_float(rw.y << 16); v[3] = __uint_as_float(rw.y & 0xffff0000u);
           }
-          float4 a = sacc[q];
-          a.x = fmaf(pf[e], v[0], a.x); a.y = fmaf(pf[e], v[1], a.y);
-          a.z = fmaf(pf[e], v[2], a.z); a.w = fmaf(pf[e], v[3], a.w);
-          sacc[q] = a;
+          if (pf[e] != 0.0f) {
+            float4 a = sacc[q];
+            a.x = fmaf(pf[e], v[0], a.x); a.y = fmaf(pf[e], v[1], a.y);
+            a.z = fmaf(pf[e], v[2], a.z); a.w = fmaf(pf[e], v[3], a.w);
+            sacc[q] = a;
+          }
         }
       }
     }
@@ -715,11 +801,19 @@ int sens_grid(int64_t P) {
 }
 size_t sens_lds(int64_t P) { return (size_t)((P + 3) / 4) * sizeof(float4); }
 
+// workspace: the work-groups' column sums, the ticket counter (256 bytes), the zero row, the sink row, and for
+// FP16 rows the ring of two slots of 8 FP32 rows per work-group
+int64_t sens_workspace(int64_t P, int out_dtype) {
+  const int64_t ldp = (P + 3) / 4 * 4, G = sens_grid(P);
+  return (G + 2) * ldp * (int64_t)sizeof(float) + 256 +
+         (out_dtype == JSCC_DT_FP16 ? G * 2 * (kSensHalves * 4) * ldp * (int64_t)sizeof(float) : 0);
+}
+
 template <int SIG, int AF, int OUT_DT, int THREADS>
 int launch_sens_t(int G, hipStream_t s, const JsccEvent* ev, const int32_t* idx, int n_grid, const int32_t* n_dev,
                   const float* cx, const float* cy, const float* cz, int P, const float* A, int64_t lda,
                   ConeConsts cc, void* out_t, int64_t ldt, double* stats, double min_support, float* partial,
-                  int64_t ldp, int* ticket) {
+                  int64_t ldp, int* ticket, float* row_scale) {
   static bool attr = false;  // more than 64 KB of dynamic LDS needs the attribute (set once)
   if (sens_lds(P) > 65536 && !attr) {
     JSCC_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_cone_rows_sens<SIG, AF, OUT_DT, THREADS>),
@@ -727,10 +821,12 @@ int launch_sens_t(int G, hipStream_t s, const JsccEvent* ev, const int32_t* idx,
     attr = true;
   }
   float* zrow = reinterpret_cast<float*>(reinterpret_cast<char*>(ticket) + 256);
+  uint16_t* sink = reinterpret_cast<uint16_t*>(zrow + ldp);
+  float* ring = zrow + 2 * ldp;
   hipLaunchKernelGGL(k_sens_prep, dim3(16), dim3(256), 0, s, ticket, zrow, (int)ldp);
   hipLaunchKernelGGL((k_cone_rows_sens<SIG, AF, OUT_DT, THREADS>), dim3(G), dim3(THREADS), sens_lds(P), s, ev, idx,
                      n_grid, n_dev, cx, cy, cz, P, A, lda, cc, out_t, ldt, stats, min_support, partial, ldp, ticket,
-                     zrow);
+                     zrow, sink, ring, row_scale);
   return 0;
 }
 
@@ -738,12 +834,12 @@ template <int SIG, int AF, int OUT_DT>
 int launch_sens(int G, hipStream_t s, const JsccEvent* ev, const int32_t* idx, int n_grid, const int32_t* n_dev,
                 const float* cx, const float* cy, const float* cz, int P, const float* A, int64_t lda, ConeConsts cc,
                 void* out_t, int64_t ldt, double* stats, double min_support, float* partial, int64_t ldp,
-                int* ticket) {
+                int* ticket, float* row_scale) {
   if (sens_threads(P) == 512)
     return launch_sens_t<SIG, AF, OUT_DT, 512>(G, s, ev, idx, n_grid, n_dev, cx, cy, cz, P, A, lda, cc, out_t, ldt,
-                                               stats, min_support, partial, ldp, ticket);
+                                               stats, min_support, partial, ldp, ticket, row_scale);
   return launch_sens_t<SIG, AF, OUT_DT, 1024>(G, s, ev, idx, n_grid, n_dev, cx, cy, cz, P, A, lda, cc, out_t, ldt,
-                                              stats, min_support, partial, ldp, ticket);
+                                              stats, min_support, partial, ldp, ticket, row_scale);
 }
 
 }  // namespace
@@ -839,17 +935,16 @@ extern "C" int jscc_cone_rows_counted(const void* events, const int32_t* ev_idx,
 
 extern "C" int jscc_cone_sens_grid(int64_t n_vox) { return sens_threads(n_vox) ? sens_grid(n_vox) : 0; }
 
-extern "C" int64_t jscc_cone_sens_workspace_bytes(int64_t n_vox) {
-  // the work-groups' column sums, the ticket counter (256 bytes), the zero row
-  return ((int64_t)jscc_cone_sens_grid(n_vox) + 1) * ((n_vox + 3) / 4 * 4) * (int64_t)sizeof(float) + 256;
+extern "C" int64_t jscc_cone_sens_workspace_bytes(int64_t n_vox, int out_dtype) {
+  return sens_threads(n_vox) ? sens_workspace(n_vox, out_dtype) : 0;
 }
 
 extern "C" int jscc_cone_rows_sens(const void* events, const int32_t* ev_idx, const int32_t* n_rows_dev,
                                    int64_t n_rows_max, const float* coor_x, const float* coor_y, const float* coor_z,
                                    int64_t n_vox, const float* sysmat, int64_t ld_sysmat, int sigma_model,
-                                   double fixed_sigma, void* out_t, int out_dtype, int64_t ld_out, double* stats,
-                                   double min_support, float* sens, void* workspace, int64_t workspace_bytes,
-                                   void* stream) {
+                                   double fixed_sigma, void* out_t, int out_dtype, int64_t ld_out, float* row_scale,
+                                   double* stats, double min_support, float* sens, void* workspace,
+                                   int64_t workspace_bytes, void* stream) {
   JSCC_REQUIRE(n_rows_max >= 0 && n_rows_max < ((int64_t)1 << 31), "jscc_cone_rows_sens: n_rows out of range");
   JSCC_REQUIRE(events && ev_idx && coor_x && coor_y && coor_z && stats && out_t && sens,
                "jscc_cone_rows_sens: null pointer");
@@ -858,8 +953,9 @@ extern "C" int jscc_cone_rows_sens(const void* events, const int32_t* ev_idx, co
   JSCC_REQUIRE(((uintptr_t)coor_x | (uintptr_t)coor_y | (uintptr_t)coor_z) % 16 == 0,
                "jscc_cone_rows_sens: coordinate arrays must be 16-byte aligned and padded to a multiple of 4");
   JSCC_REQUIRE(sigma_model == SIG_STRICT || sigma_model == SIG_FIXED, "jscc_cone_rows_sens: bad sigma model");
-  JSCC_REQUIRE(out_dtype == JSCC_DT_F32 || out_dtype == JSCC_DT_BF16,
-               "jscc_cone_rows_sens: rows are stored as FP32 or BF16");
+  JSCC_REQUIRE(out_dtype == JSCC_DT_F32 || out_dtype == JSCC_DT_BF16 || out_dtype == JSCC_DT_FP16,
+               "jscc_cone_rows_sens: rows are stored as FP32, BF16 or FP16 (per-row scale)");
+  JSCC_REQUIRE(out_dtype != JSCC_DT_FP16 || row_scale != nullptr, "jscc_cone_rows_sens: FP16 rows need row_scale");
   JSCC_REQUIRE(ld_out >= n_vox && ld_out % 4 == 0 && (uintptr_t)out_t % 16 == 0,
                "jscc_cone_rows_sens: out_t must be 16-byte aligned with ld >= n_vox, a multiple of 4");
   if (sysmat) {
@@ -867,7 +963,7 @@ extern "C" int jscc_cone_rows_sens(const void* events, const int32_t* ev_idx, co
                  "jscc_cone_rows_sens: sysmat rows must be 16-byte aligned with ld a multiple of 4");
   }
   if (sigma_model == SIG_FIXED) JSCC_REQUIRE(fixed_sigma > 0.0, "jscc_cone_rows_sens: fixed sigma must be positive");
-  JSCC_REQUIRE(workspace && workspace_bytes >= jscc_cone_sens_workspace_bytes(n_vox) && (uintptr_t)workspace % 16 == 0,
+  JSCC_REQUIRE(workspace && workspace_bytes >= sens_workspace(n_vox, out_dtype) && (uintptr_t)workspace % 16 == 0,
                "jscc_cone_rows_sens: workspace too small or misaligned");
   hipStream_t s = (hipStream_t)stream;
   const int P = (int)n_vox;
@@ -888,20 +984,17 @@ extern "C" int jscc_cone_rows_sens(const void* events, const int32_t* ev_idx, co
   jscc_prof_begin(JSCC_TAG_CONE, s);
 #define JSCC_SENS(SIG, AF, DT)                                                                                        \
   rc = launch_sens<SIG, AF, DT>(G, s, ev, ev_idx, n, n_rows_dev, coor_x, coor_y, coor_z, P, sysmat, ld_sysmat, cc, \
-                                out_t, ld_out, stats, min_support, partial, ldp, ticket)
+                                out_t, ld_out, stats, min_support, partial, ldp, ticket, row_scale)
+#define JSCC_SENS_DT(SIG, AF)                                   \
+  if (out_dtype == JSCC_DT_F32) JSCC_SENS(SIG, AF, JSCC_DT_F32); \
+  else if (out_dtype == JSCC_DT_BF16) JSCC_SENS(SIG, AF, JSCC_DT_BF16); \
+  else JSCC_SENS(SIG, AF, JSCC_DT_FP16);
   if (sigma_model == SIG_STRICT) {
-    if (sysmat) {
-      if (out_dtype == JSCC_DT_F32) JSCC_SENS(SIG_STRICT, AF_ROW, JSCC_DT_F32); else JSCC_SENS(SIG_STRICT, AF_ROW, JSCC_DT_BF16);
-    } else {
-      if (out_dtype == JSCC_DT_F32) JSCC_SENS(SIG_STRICT, AF_NONE, JSCC_DT_F32); else JSCC_SENS(SIG_STRICT, AF_NONE, JSCC_DT_BF16);
-    }
+    if (sysmat) { JSCC_SENS_DT(SIG_STRICT, AF_ROW) } else { JSCC_SENS_DT(SIG_STRICT, AF_NONE) }
   } else {
-    if (sysmat) {
-      if (out_dtype == JSCC_DT_F32) JSCC_SENS(SIG_FIXED, AF_ROW, JSCC_DT_F32); else JSCC_SENS(SIG_FIXED, AF_ROW, JSCC_DT_BF16);
-    } else {
-      if (out_dtype == JSCC_DT_F32) JSCC_SENS(SIG_FIXED, AF_NONE, JSCC_DT_F32); else JSCC_SENS(SIG_FIXED, AF_NONE, JSCC_DT_BF16);
-    }
+    if (sysmat) { JSCC_SENS_DT(SIG_FIXED, AF_ROW) } else { JSCC_SENS_DT(SIG_FIXED, AF_NONE) }
   }
+#undef JSCC_SENS_DT
 #undef JSCC_SENS
   if (rc != 0) return rc;
   JSCC_CHECK_LAUNCH();

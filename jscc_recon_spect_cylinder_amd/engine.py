@@ -36,6 +36,11 @@ class ListModeStore:
     @classmethod
     def from_rows(cls, rows: ComptonRows, dtype="bf16") -> "ListModeStore":
         code = ops.dtype_code(dtype)
+        if code == ops.DT_FP16 and rows.row_scale is not None:  # FP16 codes written by the cone pass itself
+            if rows.counts_dev is not None:
+                return cls(rows.store, code, rows.n_vox, rows.kept, -1, rows.row_scale, rows.counts_dev[1],
+                           rows.kept.shape[0])
+            return cls(rows.store, code, rows.n_vox, rows.kept, rows.n_kept, rows.row_scale)
         if code in (ops.DT_FP8, ops.DT_FP16):
             rows.resolve_counts()  # quantisation takes the selected rows
             codes, scale = ops.quantize_rows(rows.store, rows.n_vox, rows.n_sel, rows.stats, code)

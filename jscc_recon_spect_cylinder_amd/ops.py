@@ -279,9 +279,11 @@ def cone_rows(events: torch.Tensor, ev_idx: torch.Tensor, n_rows: int, vox: Voxe
 def cone_rows_sens(events: torch.Tensor, ev_idx: torch.Tensor, n_rows: int, vox: VoxelSoA, min_support: float,
                    sysmat=None, sigma_model: int = SIG_STRICT, fixed_sigma: float = 0.0, out_dtype=torch.bfloat16,
                    ld: int | None = None, n_dev: torch.Tensor | None = None):
-    """cone_rows (FP32 or BF16 rows, no t_c, no tap tables) that also returns the list-mode sensitivity: the
-    column sums of the kept rows divided by their sums, formed from each row as stored while it is still in
-    cache (jscc_cone_rows_sens).  Returns (t_store, stats [n,4] f64, ld, sens [P] f32)."""
+    """cone_rows (FP32, BF16 or FP16-with-row-scale rows, no t_c, no tap tables) that also returns the list-mode
+    sensitivity: the column sums of the kept rows divided by their sums, formed from each row as stored while it
+    is still in cache (jscc_cone_rows_sens).  FP16 rows are the codes of quantize_rows(..., "fp16") of the FP32
+    rows (dropped rows: zeros) with their row scales.  Returns (t_store, stats [n,4] f64, ld, sens [P] f32,
+    row_scale [n] f32 or None)."""
     if n_dev is not None and (n_dev.dtype != torch.int32 or n_dev.device != events.device or n_rows < 1
                               or ev_idx.numel() < n_rows):
         raise ValueError("the row count must be a device int32 and the event list hold n_rows entries")
@@ -289,21 +291,22 @@ def cone_rows_sens(events: torch.Tensor, ev_idx: torch.Tensor, n_rows: int, vox:
     P = vox.n
     ld = ld or round_up(P, 64)
     code = dtype_code(out_dtype)
-    if code not in (DT_F32, DT_BF16):
-        raise ValueError("cone rows are stored as FP32 or BF16")
+    if code not in (DT_F32, DT_BF16, DT_FP16):
+        raise ValueError("cone rows with the sensitivity are stored as FP32, BF16 or FP16")
     out_t = torch.empty((max(n_rows, 1), ld), dtype=_CODE_TORCH[code], device=dev)
     stats = torch.empty((max(n_rows, 1), 4), dtype=torch.float64, device=dev)
     sens = torch.empty(P, dtype=torch.float32, device=dev)
+    row_scale = torch.empty(max(n_rows, 1), dtype=torch.float32, device=dev) if code == DT_FP16 else None
     A = aligned_rows(sysmat) if sysmat is not None else None
-    ws_bytes = int(_lib.lib().jscc_cone_sens_workspace_bytes(P))
+    ws_bytes = int(_lib.lib().jscc_cone_sens_workspace_bytes(P, code))
     ws = _workspace(dev, ws_bytes)
     rc = _lib.lib().jscc_cone_rows_sens(events.data_ptr(), ev_idx.data_ptr(), _p(n_dev), n_rows, vox.x.data_ptr(),
                                         vox.y.data_ptr(), vox.z.data_ptr(), P, _p(A), A.stride(0) if A is not None else 0,
                                         int(sigma_model), float(fixed_sigma), out_t.data_ptr(), code, ld,
-                                        stats.data_ptr(), float(min_support), sens.data_ptr(), ws.data_ptr(),
-                                        ws_bytes, stream_of(dev))
+                                        _p(row_scale), stats.data_ptr(), float(min_support), sens.data_ptr(),
+                                        ws.data_ptr(), ws_bytes, stream_of(dev))
     _lib.check(rc, "jscc_cone_rows_sens")
-    return out_t, stats, ld, sens
+    return out_t, stats, ld, sens, row_scale
 
 
 def read_counts_checked(*counts: torch.Tensor) -> list:

@@ -77,6 +77,8 @@ class ComptonRows:
     counts_dev: tuple | None = None
     # want_sens builds: column sums of the kept rows divided by their sums (the list-mode sensitivity), [P] FP32
     sens: torch.Tensor | None = None
+    # FP16 rows (want_sens with store_dtype=torch.float16): per stored row, value / S = row_scale * code
+    row_scale: torch.Tensor | None = None
 
     def resolve_counts(self) -> "ComptonRows":
         """Read deferred counts (one synchronisation, with the index check of the event pass)."""
@@ -100,6 +102,9 @@ class ComptonRows:
 
     def normalized_t(self) -> torch.Tensor:
         self.resolve_counts()
+        if self.row_scale is not None:  # FP16 codes: value / S = row_scale * code
+            k = self.kept[:self.n_kept].long()
+            return self.store[k, :self.n_vox].float() * self.row_scale[k, None]
         return ops.gather_normalize(self.store, self.n_vox, self.kept, self.n_kept, self.stats, 0)
 
     def normalized_tc(self) -> torch.Tensor:
@@ -184,34 +189,38 @@ def build_compton_rows(sysmat, detector, coor, list_origin, delta_r1, delta_r2, 
     sig = ops.SIG_STRICT if sigma_model == "strict" else ops.SIG_FIXED
     if sig == ops.SIG_FIXED and not fixed_sigma:
         raise ValueError("sigma_model='fixed' needs fixed_sigma (rad)")
-    if want_sens and (taps is not None or want_tc or store_dtype not in (torch.float32, torch.bfloat16)):
-        raise ValueError("want_sens needs an FP32 or BF16 store, no tap tables and no t_c rows")
+    if want_sens and (taps is not None or want_tc or store_dtype not in (torch.float32, torch.bfloat16,
+                                                                         torch.float16)):
+        raise ValueError("want_sens needs an FP32, BF16 or FP16 store, no tap tables and no t_c rows")
+    if store_dtype == torch.float16 and not want_sens:
+        raise ValueError("FP16 rows come from the cone pass with the sensitivity (want_sens=True); otherwise build "
+                         "FP32 rows and quantise them (ListModeStore.from_rows(rows, 'fp16'))")
 
     def rows_of(n_rows, n_dev):
         if want_sens:
-            store, stats, ld, sens = ops.cone_rows_sens(events, sel_idx, n_rows, vox, MIN_EVENT_EFFECTIVE_SUPPORT,
-                                                        sysmat=sysmat_dev, sigma_model=sig,
-                                                        fixed_sigma=fixed_sigma or 0.0, out_dtype=store_dtype,
-                                                        n_dev=n_dev)
-            return store, None, stats, ld, sens
+            store, stats, ld, sens, scale = ops.cone_rows_sens(events, sel_idx, n_rows, vox,
+                                                               MIN_EVENT_EFFECTIVE_SUPPORT, sysmat=sysmat_dev,
+                                                               sigma_model=sig, fixed_sigma=fixed_sigma or 0.0,
+                                                               out_dtype=store_dtype, n_dev=n_dev)
+            return store, None, stats, ld, sens, scale
         store, tc_store, stats, ld = ops.cone_rows(events, sel_idx, n_rows, vox, sysmat=sysmat_dev, taps=taps,
                                                    sigma_model=sig, fixed_sigma=fixed_sigma or 0.0,
                                                    out_dtype=store_dtype, want_tc=want_tc, n_dev=n_dev)
-        return store, tc_store, stats, ld, None
+        return store, tc_store, stats, ld, None, None
 
     if not sync_counts and n > 0:
-        store, tc_store, stats, ld, sens = rows_of(n, sel_count)
+        store, tc_store, stats, ld, sens, scale = rows_of(n, sel_count)
         keep, inv_sum = ops.rows_keep(stats, n, MIN_EVENT_EFFECTIVE_SUPPORT, want_inv_sum=True)
         kept, kept_count = ops.compact(keep)
         return ComptonRows(store, tc_store, ld, vox.n, stats, events, sel_idx, -1, kept, -1, inv_sum, dev, vox,
-                           (sel_count, kept_count), sens)
+                           (sel_count, kept_count), sens, scale)
     n_sel = ops.read_count_checked(sel_count)  # one synchronisation for the count and the index check
-    store, tc_store, stats, ld, sens = rows_of(n_sel, None)
+    store, tc_store, stats, ld, sens, scale = rows_of(n_sel, None)
     keep, inv_sum = ops.rows_keep(stats, n_sel, MIN_EVENT_EFFECTIVE_SUPPORT, want_inv_sum=True)
     kept, kept_count = ops.compact(keep)
     n_kept = int(kept_count.item())
     return ComptonRows(store, tc_store, ld, vox.n, stats, events, sel_idx, n_sel, kept, n_kept, inv_sum, dev, vox,
-                       None, sens)
+                       None, sens, scale)
 
 
 def get_compton_backproj_list_single(sysmat, detector, coor, list_origin, delta_r1, delta_r2, e0, ene_resolution,

@@ -122,15 +122,52 @@ def test_sensitivity_of_no_rows_is_zero_and_bad_requests_raise():
                               torch.from_numpy(cl.list_origin.copy()), 0.0, 0.0, cl.e0, 0.02, tmax, tmin, tsum, DEV,
                               noise="none", sigma_model="fixed", fixed_sigma=sig, store_dtype=torch.float32,
                               want_sens=True)
-    _, _, _, sens = ops.cone_rows_sens(rows.events, rows.sel_idx, 0, rows.vox, 50.0, sigma_model=ops.SIG_FIXED,
-                                       fixed_sigma=sig, out_dtype=torch.float32)
+    _, _, _, sens, _ = ops.cone_rows_sens(rows.events, rows.sel_idx, 0, rows.vox, 50.0, sigma_model=ops.SIG_FIXED,
+                                          fixed_sigma=sig, out_dtype=torch.float32)
     assert sens.shape == (256,) and not torch.any(sens != 0)
-    for kw in ({"want_tc": True}, {"store_dtype": None}):
+    for kw in ({"want_tc": True}, {"store_dtype": None}, {"store_dtype": torch.float16, "want_sens": False}):
         with pytest.raises(ValueError):
             build_compton_rows(None, torch.from_numpy(cl.detector), torch.from_numpy(cl.coor),
                                torch.from_numpy(cl.list_origin.copy()), 0.0, 0.0, cl.e0, 0.02, tmax, tmin, tsum, DEV,
-                               noise="none", sigma_model="fixed", fixed_sigma=sig, want_sens=True, **kw)
+                               noise="none", sigma_model="fixed", fixed_sigma=sig, **{"want_sens": True, **kw})
     big = ops.voxel_soa(torch.from_numpy(np.random.default_rng(0).normal(size=(32772, 3)).astype(np.float32) * 40.0),
                         DEV)
     with pytest.raises(_lib.JsccError, match="voxels"):
         ops.cone_rows_sens(rows.events, rows.sel_idx, 4, big, 50.0, sigma_model=ops.SIG_FIXED, fixed_sigma=sig)
+
+
+@pytest.mark.parametrize("rings,angles,sync", [(16, 64, True), (64, 256, False), (100, 256, False)])
+def test_fp16_rows_of_the_cone_pass_equal_quantised_fp32_rows(rings, angles, sync):
+    """FP16 rows written by the cone pass (the FP32 values quantised one group later, when the row maximum and
+    sum are known): codes bit-identical to quantize_rows of the FP32 rows for every kept row, the same row
+    scales, the sensitivity equal to the column-sum pass over the FP16 store, and the OSEM image of the FP16 store
+    equal to the one of the quantised FP32 rows (1e-4 voxels, device-side counts included)."""
+    from jscc_recon_spect_cylinder_amd import ops, synth
+    from jscc_recon_spect_cylinder_amd.engine import ListModeStore
+    from jscc_recon_spect_cylinder_amd.process_list_plane_strict import build_compton_rows
+    cl = synth.compton_list(n_events=1500, n_rings=rings, n_angles=angles, seed=3 * rings + angles)
+    tmax, tmin, tsum = synth.energy_thresholds(cl.e0)
+    sig = float(np.deg2rad(3.0))
+    P = cl.coor.shape[0]
+    built = {}
+    for sd in (torch.float32, torch.float16):
+        built[sd] = build_compton_rows(None, torch.from_numpy(cl.detector), torch.from_numpy(cl.coor),
+                                       torch.from_numpy(cl.list_origin.copy()), 0.0, 0.0, cl.e0, 0.02, tmax, tmin,
+                                       tsum, DEV, noise="none", sigma_model="fixed", fixed_sigma=sig, store_dtype=sd,
+                                       sync_counts=sync, want_sens=True)
+    r32, r16 = built[torch.float32].resolve_counts(), built[torch.float16].resolve_counts()
+    assert r16.store.dtype == torch.float16 and r16.row_scale is not None
+    assert r32.n_sel == r16.n_sel and torch.equal(r32.kept[:r32.n_kept], r16.kept[:r16.n_kept])
+    n = r16.n_sel
+    codes, scale = ops.quantize_rows(r32.store, P, n, r32.stats, "fp16")
+    kept = r16.kept[:r16.n_kept].long()
+    assert torch.equal(r16.store[kept, :P].view(torch.int16), codes[kept, :P].view(torch.int16))
+    # the same scales for every selected row (FP32 statistics of both passes agree to the rounding of their sums)
+    np.testing.assert_allclose(r16.row_scale[:n].cpu().numpy(), scale[:n].cpu().numpy(), rtol=1e-6)
+    st16 = ListModeStore.from_rows(r16, "fp16")
+    s_pass = st16.backproject_ones()
+    assert _rel(r16.sens.cpu().numpy(), s_pass.cpu().numpy()) < 2e-6
+    st_q = ListModeStore.from_rows(r32, "fp16")
+    x16 = st16.osem(torch.ones(P, device=DEV), r16.sens, 3, 4)
+    xq = st_q.osem(torch.ones(P, device=DEV), st_q.backproject_ones(), 3, 4)
+    assert _rel(x16.cpu().numpy(), xq.cpu().numpy()) < 1e-4
