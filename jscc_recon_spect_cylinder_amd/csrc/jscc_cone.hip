@@ -492,10 +492,10 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4)
     float* __restrict__ partial, int64_t ldp, int* __restrict__ ticket, const float* __restrict__ zrow,
     uint16_t* __restrict__ sink, float* __restrict__ ring, float* __restrict__ row_scale) {
   constexpr int E = 4;              // events evaluated together (scalar registers)
-  constexpr int H = kSensHalves;    // E-event halves per group: one barrier round per H * E rows
+  constexpr bool Q16 = OUT_DT == JSCC_DT_FP16;
+  constexpr int H = Q16 ? 1 : kSensHalves;  // E-event halves per group: one barrier round per H * E rows
   constexpr int R = H * E;          // rows per group
   constexpr int NW = THREADS / JSCC_WAVE;
-  constexpr bool Q16 = OUT_DT == JSCC_DT_FP16;
   constexpr int TILE_DT = Q16 ? JSCC_DT_F32 : OUT_DT;  // what the tile loop stores
   using Raw = typename std::conditional<TILE_DT == JSCC_DT_F32, float4, uint2>::type;
   extern __shared__ float4 sacc[];  // [ldp / 4] column sums; quad q is owned by thread q % THREADS
@@ -546,7 +546,7 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4)
       valid[e] = (row0 + e) < n_rows;
       accS[e] = 0.0f; accQ[e] = 0.0f; accC[e] = 0.0f; accM[e] = 0.0f;
     }
-    float* ring_cur = Q16 ? ring + ((int64_t)blockIdx.x * 2 + slot) * R * ldp : nullptr;
+    float* ring_cur = Q16 ? ring + (int64_t)blockIdx.x * R * ldp : nullptr;  // one slot: read back in the group
     const float* ring_prev = Q16 ? ring + ((int64_t)blockIdx.x * 2 + (slot ^ 1)) * R * ldp : nullptr;
 #pragma unroll
     for (int h = 0; h < H; ++h) {
@@ -596,13 +596,16 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4)
         asm volatile("" ::: "memory");
         Raw rb[E];
         const uint32_t qoff = (uint32_t)q * (uint32_t)sizeof(Raw);  // one 32-bit offset from scalar row bases
+        if constexpr (!Q16) {
 #pragma unroll
-        for (int e = 0; e < E; ++e)
-          rb[e] = *reinterpret_cast<const Raw*>(reinterpret_cast<const char*>(prow[e]) + qoff);
+          for (int e = 0; e < E; ++e)
+            rb[e] = *reinterpret_cast<const Raw*>(reinterpret_cast<const char*>(prow[e]) + qoff);
+        }
         cone_rows_tile_at<SIG, AF, TILE_DT, false, E>(evs, rrow, base, P, vx4, vy4, vz4, A, nullptr, nullptr,
                                                       nullptr, nullptr, cc, Q16 ? (void*)ring_cur : out_t,
                                                       Q16 ? ldp : ldt, nullptr, 0, accS + h * E, accQ + h * E,
                                                       accC + h * E, accM + h * E);
+        if constexpr (Q16) continue;  // FP16 rows are read back after the group's statistics (below)
         float4 a = sacc[q];
         float w[4] = {a.x, a.y, a.z, a.w};
         // f (and 1 / q) of the previous group's rows read from LDS (vector register operands; the scalar
@@ -716,12 +719,48 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4)
       const int last = n_rows - 1 - row0 < R - 1 ? n_rows - 1 - row0 : R - 1;
       pf_last = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(fsh[last])));
     }
+    if constexpr (Q16) {
+      // FP16 rows: this group's FP32 values, still in the L2 / Infinity Cache, quantised and summed now (a
+      // short-lived ring keeps the live values within the caches); a row past the count repeats the last row
+      const int last = n_rows - 1 - row0;
+      float iq[R], fr[R];
+      const float* src[R];
+      uint16_t* dst[R];
+#pragma unroll
+      for (int e = 0; e < R; ++e) {
+        const int le = e < last ? e : last;
+        fr[e] = e < last ? pf[e] : pf_last;
+        iq[e] = qsh[le];
+        src[e] = fr[e] != 0.0f ? ring_cur + (int64_t)le * ldp : zrow;
+        dst[e] = reinterpret_cast<uint16_t*>(out_t) + (int64_t)(row0 + le) * ldt;
+        fr[e] = e < last || e == last ? fr[e] : 0.0f;  // rows past the count add nothing
+      }
+      for (int q = tid; q * kConeVec < P; q += THREADS) {
+        float4 rv[R];
+#pragma unroll
+        for (int e = 0; e < R; ++e) rv[e] = reinterpret_cast<const float4*>(src[e])[q];
+        float4 a = sacc[q];
+#pragma unroll
+        for (int e = 0; e < R; ++e) {
+          _Float16 c[4];
+          c[0] = (_Float16)(rv[e].x * iq[e]); c[1] = (_Float16)(rv[e].y * iq[e]);
+          c[2] = (_Float16)(rv[e].z * iq[e]); c[3] = (_Float16)(rv[e].w * iq[e]);
+          uint2 pk;
+          pk.x = (uint32_t)__builtin_bit_cast(uint16_t, c[0]) | ((uint32_t)__builtin_bit_cast(uint16_t, c[1]) << 16);
+          pk.y = (uint32_t)__builtin_bit_cast(uint16_t, c[2]) | ((uint32_t)__builtin_bit_cast(uint16_t, c[3]) << 16);
+          *reinterpret_cast<uint2*>(reinterpret_cast<char*>(dst[e]) + (uint32_t)q * 8u) = pk;
+          a.x = fmaf(fr[e], (float)c[0], a.x); a.y = fmaf(fr[e], (float)c[1], a.y);
+          a.z = fmaf(fr[e], (float)c[2], a.z); a.w = fmaf(fr[e], (float)c[3], a.w);
+        }
+        sacc[q] = a;
+      }
+    }
     prev_row0 = row0;
     slot ^= 1;
     g = g_next;
   }
-  // the rows of the work-group's last group
-  if (prev_row0 >= 0) {
+  // the rows of the work-group's last group (FP16 rows: already done)
+  if (!Q16 && prev_row0 >= 0) {
     const float* ring_prev = Q16 ? ring + ((int64_t)blockIdx.x * 2 + (slot ^ 1)) * R * ldp : nullptr;
 #pragma unroll
     for (int e = 0; e < R; ++e) {
