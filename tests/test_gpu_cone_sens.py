@@ -138,8 +138,8 @@ def test_sensitivity_of_no_rows_is_zero_and_bad_requests_raise():
 
 @pytest.mark.parametrize("rings,angles,sync", [(16, 64, True), (64, 256, False), (100, 256, False)])
 def test_fp16_rows_of_the_cone_pass_equal_quantised_fp32_rows(rings, angles, sync):
-    """FP16 rows written by the cone pass (the FP32 values quantised one group later, when the row maximum and
-    sum are known): codes bit-identical to quantize_rows of the FP32 rows for every kept row, the same row
+    """FP16 rows written by the cone pass (the FP32 values quantised right after their group's statistics, when
+    the row maximum and sum are known): codes bit-identical to quantize_rows of the FP32 rows for every kept row, the same row
     scales, the sensitivity equal to the column-sum pass over the FP16 store, and the OSEM image of the FP16 store
     equal to the one of the quantised FP32 rows (1e-4 voxels, device-side counts included)."""
     from jscc_recon_spect_cylinder_amd import ops, synth
