@@ -905,13 +905,11 @@ def run_jsccd(args):
     last_rows = [None]
 
     def step(store_dtype, name):
-        # FP32 / BF16 rows: the selected and kept counts stay on the device (sync_counts=False), so the step
-        # has no host synchronisation; the FP16 store is quantised from the selected FP32 rows, which reads
-        # the counts
-        # the sensitivity (column sums of the kept, normalised rows) is formed by the cone pass itself
-        # (want_sens), from the rows as stored, instead of a second pass over T; FP16 rows are quantised by the
-        # same pass (row maximum at 2^14, one group after the FP32 values) and their sensitivity is the column
-        # sums of the codes the OSEM projects
+        # the selected and kept counts stay on the device (sync_counts=False), so the step has no host
+        # synchronisation; the sensitivity (column sums of the kept, normalised rows) is formed by the cone pass
+        # itself (want_sens), from the rows as stored, instead of a second pass over T; FP16 rows are quantised
+        # by the same pass (row maximum at 2^14, right after each group's statistics) and their sensitivity is
+        # the column sums of the codes the OSEM projects
         rows = build_compton_rows(None, det, coor, lst0.clone(), 0.0, 0.0, cl.e0, 0.02, tmax, tmin, tsum, dev,
                                   noise=(n1, n2), sigma_model="fixed", fixed_sigma=sigma, store_dtype=store_dtype,
                                   sync_counts=False, want_sens=True)

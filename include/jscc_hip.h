@@ -235,7 +235,12 @@ int jscc_coarse_weight(const float* packed, int64_t ld_packed, const int32_t* ro
  * flags bit 0: y is indexed by the stored row id (random bin subsets given as
  * a row list, recon_osem_local_cntstat.py:11-25, without copying A);
  * bit 1: the block weight is passed through nan_to_num(0,0,0) before out_scale
- * (recon_osem_local_sparse_jsccsd_only.py:13-19). */
+ * (recon_osem_local_sparse_jsccsd_only.py:13-19).
+ * Any n_cols up to jscc_proj_max_cols: rows are read in whole vectors of
+ * jscc_proj_vec_width(dtype) elements, so ld must be a multiple of it that
+ * covers n_cols rounded up to it, and every row must extend that far in
+ * memory; what the row holds past n_cols is never used (the ragged last
+ * vector is cleared past n_cols before the dot products). */
 int jscc_proj_vec_width(int dtype);
 int jscc_proj_max_cols(int dtype);
 /* The largest number of work-groups (and partial rows in the workspace) a

@@ -1262,6 +1262,6 @@ extern "C" int jscc_coarse_weight(const float* packed, int64_t ld_packed, const 
   if (rc != 0) return rc;
   JSCC_CHECK_LAUNCH();
   const JsccEm none = {nullptr, nullptr, 0, 0.0f, 0, 1.0f, nullptr, 0.0f};
-  return jscc_reduce_partials(partial, G, (int)P, inv_perm, inv_perm_stride, out_scale, accumulate, 1, w, w_len, err,
-                              s, none);
+  return jscc_reduce_partials(partial, G, (int)P, P, inv_perm, inv_perm_stride, out_scale, accumulate, 1, w, w_len,
+                              err, s, none);
 }

@@ -200,12 +200,13 @@ struct JsccEm {
 };
 
 // Deterministic reduction of per-work-group partial back-projections [G, P]
-// into w (inverse-rotation gather, scale, accumulate, optional nan_to_num,
-// optional EM update); the buffer must hold jscc_partials_bytes(G, P).
-int jscc_reduce_partials(float* part, int G, int P, const int64_t* inv_perm, int64_t ips, float scale,
+// (row stride ldp: a multiple of 4 covering P) into w (inverse-rotation
+// gather, scale, accumulate, optional nan_to_num, optional EM update); the
+// buffer must hold jscc_partials_bytes(G, ldp).
+int jscc_reduce_partials(float* part, int G, int P, int64_t ldp, const int64_t* inv_perm, int64_t ips, float scale,
                          int accumulate, int sanitize, float* w, int64_t w_len, int32_t* err, hipStream_t s,
                          const JsccEm& em);
-int64_t jscc_partials_bytes(int G, int64_t P);
+int64_t jscc_partials_bytes(int G, int64_t ldp);
 
 // Built-in kernel timer: when enabled, hot kernels are bracketed by HIP event
 // pairs recorded on their launch stream (tag = kernel family).

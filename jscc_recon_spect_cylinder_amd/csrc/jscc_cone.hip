@@ -1270,7 +1270,8 @@ extern "C" int jscc_cone_proj(int mode, const void* events, const int32_t* ev_id
   JSCC_CHECK_LAUNCH();
   if (mode == CP_FWD) return 0;
   const JsccEm none = {nullptr, nullptr, 0, 0.0f, 0, 1.0f, nullptr, 0.0f};
-  return jscc_reduce_partials((float*)workspace, G, (int)n_vox, nullptr, 0, out_scale, accumulate, 0, w, w_len, err, s,
+  return jscc_reduce_partials((float*)workspace, G, (int)n_vox, n_vox, nullptr, 0, out_scale, accumulate, 0, w, w_len,
+                              err, s,
                               none);
 }
 

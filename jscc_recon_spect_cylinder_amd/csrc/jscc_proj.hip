@@ -90,7 +90,8 @@ struct ProjSeg {
 struct ProjArgs {
   const void* M;             // matrix rows
   int64_t ld;                // row stride (elements)
-  int P;                     // columns (multiple of the vector width)
+  int P;                     // columns (any count; rows are read in whole vectors past it)
+  int ldp;                   // row stride of the partial sums (P rounded up to 16)
   const int32_t* rows;       // optional stored-row index list
   int64_t n_rows;            // rows in this call
   int64_t row_base;          // offset into rows / stored row ids
@@ -144,6 +145,21 @@ struct RowBuf {
   typename Vec<DT>::T raw[R][NCH];
 };
 
+// A column count that is not a multiple of the vector width leaves a ragged last
+// vector per row: the thread that holds it keeps only the first nb bytes of it
+// (nb = 16 everywhere else), so the row's padding -- memory a store need not
+// have written -- cannot reach the sums as 0 * inf or NaN.
+template <typename VT>
+__device__ __forceinline__ void clear_tail(VT& v, int nb) {
+  jscc_u4v u = __builtin_bit_cast(jscc_u4v, v);
+#pragma unroll
+  for (int d = 0; d < 4; ++d) {
+    const int k = nb - 4 * d;
+    u[d] &= k >= 4 ? 0xffffffffu : (k <= 0 ? 0u : (1u << (8 * k)) - 1u);
+  }
+  v = __builtin_bit_cast(VT, u);
+}
+
 // Issue the loads of rows [i0, i0 + R) of the staged chunk starting at c0.
 // Buffer loads: the resource's byte range is the row (or empty for slots at or
 // past c1), so out-of-range lanes and dead slots read zero without a branch or
@@ -179,13 +195,29 @@ __device__ __forceinline__ void fetch_rows(const ProjArgs& a, const RowMeta& met
 
 // One block of R rows: dot products with the LDS image, work-group reduction,
 // ratio / coefficient, and accumulation of coef * row into the registers.
-template <int DT, int NCH, int R, int MODE>
+// TAIL (a column count that is not a multiple of the vector width, modes with a dot product): tail_c is the chunk
+// holding the rows' ragged last vector (uniform), tail_nb the bytes of it this thread keeps.  A separate
+// instantiation, so the loop of every other launch carries none of it.
+template <int DT, int NCH, int R, int MODE, bool TAIL>
 __device__ __forceinline__ void process_rows(const ProjArgs& a, const RowMeta& meta, int64_t i0, int64_t c0,
                                              int64_t c1, int tid, RowBuf<DT, NCH, R>& b, const jscc_f4v* xs4,
                                              float (&red)[2][R][kProjWaves], int& buf,
-                                             float (&wr)[NCH][Vec<DT>::N]) {
+                                             float (&wr)[NCH][Vec<DT>::N], int tail_c, int tail_nb) {
   constexpr int VN = Vec<DT>::N;
   float coefq[R];
+  if constexpr (TAIL) {
+    // every chunk is masked (all ones outside the tail chunk): a test of c against tail_c would let the
+    // compiler index the row buffer dynamically, i.e. move it to scratch memory.  The masks are formed here
+    // for each buffer (from an opaque copy of tail_nb), not kept live across the loop.
+    int nb = tail_nb;
+    asm volatile("" : "+v"(nb));
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+      const int nbc = NCH == 1 || c == tail_c ? nb : 16;  // one chunk: it is the tail's
+#pragma unroll
+      for (int q = 0; q < R; ++q) clear_tail(b.raw[q][c], nbc);
+    }
+  }
   if (MODE == MODE_RATIO || MODE == MODE_FWD) {
     float d[R];
 #pragma unroll
@@ -269,7 +301,7 @@ __device__ __forceinline__ void process_rows(const ProjArgs& a, const RowMeta& m
 
 // The pointers are separate __restrict__ parameters (not struct members) so
 // the compiler knows the inputs are not written by this kernel.
-template <int DT, int NCH, int R, int MODE>
+template <int DT, int NCH, int R, int MODE, bool TAIL>
 __global__ void __launch_bounds__(kProjThreads)
     k_proj_fused(const void* __restrict__ M, const int32_t* __restrict__ rows, const float* __restrict__ row_scale,
                  const float* __restrict__ x, const int64_t* __restrict__ x_perm, const float* __restrict__ y,
@@ -286,8 +318,12 @@ __global__ void __launch_bounds__(kProjThreads)
     a.n_rows1 = a.n_rows;
   }
   constexpr int VN = Vec<DT>::N;
+  constexpr int EB = DT == JSCC_DT_F32 ? 4 : (DT == JSCC_DT_FP8 ? 1 : 2);
   const int tid = threadIdx.x;
-  const int nvec = a.P / VN;
+  const int nvec = (a.P + VN - 1) / VN;
+  const int last_cols = a.P - (nvec - 1) * VN;  // columns of the last vector
+  const int tail_c = TAIL ? (nvec - 1) / kProjThreads : -1;
+  const int tail_nb = TAIL && tid == (nvec - 1) % kProjThreads ? last_cols * EB : 16;
 
   // this work-group's contiguous share of the rows
   const int64_t per = a.n_rows / gridDim.x, rem = a.n_rows % gridDim.x;
@@ -359,11 +395,20 @@ __global__ void __launch_bounds__(kProjThreads)
         const int vi = c * kProjThreads + tid;
 #pragma unroll
         for (int k = 0; k < VN; ++k) {
-          const int64_t p = (int64_t)(vi < nvec ? vi : 0) * VN + k;
-          const int64_t v = x_perm[p * a.x_perm_stride];
-          const bool ok = v >= 1 && v <= a.x_len;
-          bad |= (ok || vi >= nvec) ? 0 : 1;
-          src[c][k] = ok ? (int32_t)(v - 1) : 0;
+          if constexpr (TAIL) {  // per column: the last vector is cut by P (src -1: a zero pixel)
+            const int64_t p = (int64_t)vi * VN + k;
+            const bool in = p < a.P;
+            const int64_t v = x_perm[(in ? p : 0) * a.x_perm_stride];
+            const bool ok = v >= 1 && v <= a.x_len;
+            bad |= (ok || !in) ? 0 : 1;
+            src[c][k] = ok && in ? (int32_t)(v - 1) : -1;
+          } else {  // per vector: P is a multiple of VN
+            const int64_t p = (int64_t)(vi < nvec ? vi : 0) * VN + k;
+            const int64_t v = x_perm[p * a.x_perm_stride];
+            const bool ok = v >= 1 && v <= a.x_len;
+            bad |= (ok || vi >= nvec) ? 0 : 1;
+            src[c][k] = ok ? (int32_t)(v - 1) : 0;
+          }
         }
       }
       if (bad) atomicOr(err, 2);
@@ -372,7 +417,7 @@ __global__ void __launch_bounds__(kProjThreads)
         const int vi = c * kProjThreads + tid;
         float xv[VN];
 #pragma unroll
-        for (int k = 0; k < VN; ++k) xv[k] = vi < nvec ? x[src[c][k]] : 0.0f;
+        for (int k = 0; k < VN; ++k) xv[k] = (TAIL ? src[c][k] >= 0 : vi < nvec) ? x[src[c][k]] : 0.0f;
 #pragma unroll
         for (int kk = 0; kk < VN / 4; ++kk) {
           jscc_f4v t;
@@ -381,14 +426,27 @@ __global__ void __launch_bounds__(kProjThreads)
         }
       }
     } else {
-      // x is P contiguous floats, 16-byte aligned
+      // x is P contiguous floats, 16-byte aligned: whole quads below P, zeros past it
       const jscc_f4v* x4 = reinterpret_cast<const jscc_f4v*>(x);
 #pragma unroll
       for (int c = 0; c < NCH; ++c) {
         const int vi = c * kProjThreads + tid;
 #pragma unroll
-        for (int kk = 0; kk < VN / 4; ++kk)
-          xs4[(c * (VN / 4) + kk) * kProjThreads + tid] = vi < nvec ? x4[vi * (VN / 4) + kk] : jscc_f4v(0.0f);
+        for (int kk = 0; kk < VN / 4; ++kk) {
+          const int q = vi * (VN / 4) + kk;
+          xs4[(c * (VN / 4) + kk) * kProjThreads + tid] =
+              (TAIL ? (q + 1) * 4 <= a.P : vi < nvec) ? x4[q] : jscc_f4v(0.0f);
+        }
+      }
+      if (TAIL && (a.P & 3)) {  // the quad that P cuts, read by element in the thread whose slot holds it
+        const int q = a.P / 4, vi = q / (VN / 4);
+        if (tid == vi % kProjThreads) {
+          jscc_f4v t = jscc_f4v(0.0f);
+          t.x = x[4 * q];
+          if (4 * q + 1 < a.P) t.y = x[4 * q + 1];
+          if (4 * q + 2 < a.P) t.z = x[4 * q + 2];
+          xs4[((vi / kProjThreads) * (VN / 4) + q % (VN / 4)) * kProjThreads + tid] = t;
+        }
       }
     }
   }
@@ -404,16 +462,16 @@ __global__ void __launch_bounds__(kProjThreads)
     int64_t i0 = c0;
     for (int64_t st = 0; st + 2 <= n_steps; st += 2) {
       fetch_rows<DT, NCH, R>(a, meta, i0 + R, c0, c1, nvec, tid, bb);
-      process_rows<DT, NCH, R, MODE>(a, meta, i0, c0, c1, tid, ba, xs4, red, buf, wr);
+      process_rows<DT, NCH, R, MODE, TAIL>(a, meta, i0, c0, c1, tid, ba, xs4, red, buf, wr, tail_c, tail_nb);
       fetch_rows<DT, NCH, R>(a, meta, i0 + 2 * R, c0, c1, nvec, tid, ba);
-      process_rows<DT, NCH, R, MODE>(a, meta, i0 + R, c0, c1, tid, bb, xs4, red, buf, wr);
+      process_rows<DT, NCH, R, MODE, TAIL>(a, meta, i0 + R, c0, c1, tid, bb, xs4, red, buf, wr, tail_c, tail_nb);
       i0 += 2 * R;
     }
-    if (n_steps & 1) process_rows<DT, NCH, R, MODE>(a, meta, i0, c0, c1, tid, ba, xs4, red, buf, wr);
+    if (n_steps & 1) process_rows<DT, NCH, R, MODE, TAIL>(a, meta, i0, c0, c1, tid, ba, xs4, red, buf, wr, tail_c, tail_nb);
   }
 
   if (MODE != MODE_FWD) {
-    float* dst = partial + (int64_t)blockIdx.x * a.P;
+    float* dst = partial + (int64_t)blockIdx.x * a.ldp;
 #pragma unroll
     for (int c = 0; c < NCH; ++c) {
       const int vi = c * kProjThreads + tid;
@@ -522,7 +580,7 @@ __global__ void __launch_bounds__(kProjThreads)
   const int G = gridDim.x;
   const int nvec = o.P / VN;
   ProjArgs a;
-  a.M = M; a.ld = o.ld; a.P = o.P; a.rows = rows; a.n_rows = 0; a.row_base = 0; a.row_scale = row_scale;
+  a.M = M; a.ld = o.ld; a.P = o.P; a.ldp = o.P; a.rows = rows; a.n_rows = 0; a.row_base = 0; a.row_scale = row_scale;
   a.x = x; a.x_perm = nullptr; a.x_perm_stride = 0; a.x_len = o.P; a.y = nullptr; a.y_stride = 0; a.coef = nullptr;
   a.eps = o.eps; a.use_eps = o.use_eps; a.fwd_out = nullptr; a.partial = partial; a.err = err; a.y_by_row = 0;
   a.n_rows1 = 0; a.num_scale1 = 1.0f; a.seg2 = ProjSeg{};
@@ -580,12 +638,12 @@ __global__ void __launch_bounds__(kProjThreads)
       int64_t i0 = c0;
       for (int64_t st = 0; st + 2 <= n_steps; st += 2) {
         fetch_rows<DT, NCH, R>(a, meta, i0 + R, c0, c1, nvec, tid, bb);
-        process_rows<DT, NCH, R, MODE_RATIO>(a, meta, i0, c0, c1, tid, ba, xs4, red, buf, wr);
+        process_rows<DT, NCH, R, MODE_RATIO, false>(a, meta, i0, c0, c1, tid, ba, xs4, red, buf, wr, -1, 16);
         fetch_rows<DT, NCH, R>(a, meta, i0 + 2 * R, c0, c1, nvec, tid, ba);
-        process_rows<DT, NCH, R, MODE_RATIO>(a, meta, i0 + R, c0, c1, tid, bb, xs4, red, buf, wr);
+        process_rows<DT, NCH, R, MODE_RATIO, false>(a, meta, i0 + R, c0, c1, tid, bb, xs4, red, buf, wr, -1, 16);
         i0 += 2 * R;
       }
-      if (n_steps & 1) process_rows<DT, NCH, R, MODE_RATIO>(a, meta, i0, c0, c1, tid, ba, xs4, red, buf, wr);
+      if (n_steps & 1) process_rows<DT, NCH, R, MODE_RATIO, false>(a, meta, i0, c0, c1, tid, ba, xs4, red, buf, wr, -1, 16);
     }
     {
       float* dst = partial + (int64_t)g * o.P;
@@ -641,10 +699,11 @@ __global__ void __launch_bounds__(kProjThreads)
   }
 }
 
-// partial [G, P] -> partial2 [ceil(G/16), P]: 16 partial rows per output row,
-// so that enough work-groups stream the partials at full rate
+// partial [G, P] (row stride ldp) -> partial2 [ceil(G/16), P] (same stride): 16
+// partial rows per output row, so that enough work-groups stream the partials at
+// full rate
 constexpr int kRedRows = 16;
-__global__ void __launch_bounds__(256) k_reduce_stage1(const float* __restrict__ part, int G, int P,
+__global__ void __launch_bounds__(256) k_reduce_stage1(const float* __restrict__ part, int G, int P, int64_t ldp,
                                                        float* __restrict__ out) {
   const int col = (blockIdx.x * 256 + threadIdx.x) * 4;
   if (col >= P) return;
@@ -653,14 +712,14 @@ __global__ void __launch_bounds__(256) k_reduce_stage1(const float* __restrict__
   float4 v[kRedRows];
 #pragma unroll
   for (int k = 0; k < kRedRows; ++k)
-    v[k] = g0 + k < g1 ? *reinterpret_cast<const float4*>(part + (int64_t)(g0 + k) * P + col)
+    v[k] = g0 + k < g1 ? *reinterpret_cast<const float4*>(part + (int64_t)(g0 + k) * ldp + col)
                        : make_float4(0.f, 0.f, 0.f, 0.f);
   float4 s = v[0];
 #pragma unroll
   for (int k = 1; k < kRedRows; ++k) {
     s.x += v[k].x; s.y += v[k].y; s.z += v[k].z; s.w += v[k].w;
   }
-  *reinterpret_cast<float4*>(out + (int64_t)blockIdx.y * P + col) = s;
+  *reinterpret_cast<float4*>(out + (int64_t)blockIdx.y * ldp + col) = s;
 }
 
 typedef JsccEm EmEpilogue;
@@ -689,7 +748,7 @@ __device__ __forceinline__ void finish_weight(float sum, int64_t j, float scale,
 
 // w[j] = (accumulate ? w[j] : 0) + scale * sum_g part[g][src(j)],
 // src(j) = inv_perm[j]-1 (the reference's index_select with RotMatInv)
-__global__ void __launch_bounds__(256) k_reduce_stage2(const float* __restrict__ part, int G, int P,
+__global__ void __launch_bounds__(256) k_reduce_stage2(const float* __restrict__ part, int G, int P, int64_t ldp,
                                                        const int64_t* __restrict__ inv_perm, int64_t ips,
                                                        float scale, int accumulate, int sanitize,
                                                        float* __restrict__ w, int64_t w_len,
@@ -705,17 +764,18 @@ __global__ void __launch_bounds__(256) k_reduce_stage2(const float* __restrict__
   }
   float sum = 0.0f;
 #pragma unroll 8
-  for (int g = 0; g < G; ++g) sum += part[(int64_t)g * P + src];
+  for (int g = 0; g < G; ++g) sum += part[(int64_t)g * ldp + src];
   finish_weight(sum, j, scale, accumulate, sanitize, w, em);
 }
 
 // Without a rotation gather the columns are reduced in one pass: a block owns
 // 64 columns; 16 row groups of 16 threads (one float4 column each) sum every
-// 16th partial row, then the groups are combined in LDS in a fixed order.
+// 16th partial row, then the groups are combined in LDS in a fixed order.  The
+// partial rows (stride ldp, a multiple of 4) extend past P to whole quads.
 constexpr int kDirCols = 64;
-__global__ void __launch_bounds__(256) k_reduce_direct(const float* __restrict__ part, int G, int P, float scale,
-                                                       int accumulate, int sanitize, float* __restrict__ w,
-                                                       EmEpilogue em) {
+__global__ void __launch_bounds__(256) k_reduce_direct(const float* __restrict__ part, int G, int P, int64_t ldp,
+                                                       float scale, int accumulate, int sanitize,
+                                                       float* __restrict__ w, EmEpilogue em) {
   constexpr int TX = kDirCols / 4, TY = 256 / TX;
   __shared__ jscc_f4v acc[TY][TX];
   const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
@@ -723,7 +783,7 @@ __global__ void __launch_bounds__(256) k_reduce_direct(const float* __restrict__
   jscc_f4v sum = jscc_f4v(0.0f);
   if (col < P) {
 #pragma unroll 8
-    for (int g = ty; g < G; g += TY) sum += *reinterpret_cast<const jscc_f4v*>(part + (int64_t)g * P + col);
+    for (int g = ty; g < G; g += TY) sum += *reinterpret_cast<const jscc_f4v*>(part + (int64_t)g * ldp + col);
   }
   acc[ty][tx] = sum;
   __syncthreads();
@@ -732,9 +792,9 @@ __global__ void __launch_bounds__(256) k_reduce_direct(const float* __restrict__
 #pragma unroll
     for (int k = 1; k < TY; ++k) t += acc[k][tx];
     finish_weight(t.x, col, scale, accumulate, sanitize, w, em);
-    finish_weight(t.y, col + 1, scale, accumulate, sanitize, w, em);
-    finish_weight(t.z, col + 2, scale, accumulate, sanitize, w, em);
-    finish_weight(t.w, col + 3, scale, accumulate, sanitize, w, em);
+    if (col + 1 < P) finish_weight(t.y, col + 1, scale, accumulate, sanitize, w, em);
+    if (col + 2 < P) finish_weight(t.z, col + 2, scale, accumulate, sanitize, w, em);
+    if (col + 3 < P) finish_weight(t.w, col + 3, scale, accumulate, sanitize, w, em);
   }
 }
 
@@ -787,26 +847,33 @@ __global__ void __launch_bounds__(256) k_sens_gather(const double* __restrict__ 
   out[j] = (float)(acc / (double)R);
 }
 
-template <int DT, int NCH, int R, int MODE>
+template <int DT, int NCH, int R, int MODE, bool TAIL = false>
 int launch_one(const ProjArgs& a, int G, hipStream_t s) {
   // image slots in LDS: NCH chunks x 512 threads x VN floats (only for the modes that read x)
   const size_t lds = (MODE == MODE_RATIO || MODE == MODE_FWD) ? (size_t)NCH * kProjThreads * Vec<DT>::N * 4 : 0;
   static bool attr_set = false;
   if (lds > 65536 && !attr_set) {
-    JSCC_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_proj_fused<DT, NCH, R, MODE>),
+    JSCC_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_proj_fused<DT, NCH, R, MODE, TAIL>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     attr_set = true;
   }
-  hipLaunchKernelGGL((k_proj_fused<DT, NCH, R, MODE>), dim3(G), dim3(kProjThreads), lds, s, a.M, a.rows, a.row_scale,
+  hipLaunchKernelGGL((k_proj_fused<DT, NCH, R, MODE, TAIL>), dim3(G), dim3(kProjThreads), lds, s, a.M, a.rows, a.row_scale,
                      a.x, a.x_perm, a.y, a.coef, a.fwd_out, a.partial, a.err, a);
   return 0;
 }
 
+// Modes with a dot product over a ragged last vector take the TAIL instantiation; the column-sum and
+// coefficient modes only carry the padding into partial columns past P, which the reduction never reads out.
+// (The TAIL instantiations hold at most 4 rows per buffer: the masks of 8 spill registers.)
 template <int DT, int NCH, int R>
 int launch_fused(int mode, const ProjArgs& a, int G, hipStream_t s) {
+  const bool tail = a.P % Vec<DT>::N != 0;
+  constexpr int RT = R > 4 ? 4 : R;
   switch (mode) {
-    case MODE_RATIO: return launch_one<DT, NCH, R, MODE_RATIO>(a, G, s);
-    case MODE_FWD: return launch_one<DT, NCH, R, MODE_FWD>(a, G, s);
+    case MODE_RATIO:
+      return tail ? launch_one<DT, NCH, RT, MODE_RATIO, true>(a, G, s) : launch_one<DT, NCH, R, MODE_RATIO>(a, G, s);
+    case MODE_FWD:
+      return tail ? launch_one<DT, NCH, RT, MODE_FWD, true>(a, G, s) : launch_one<DT, NCH, R, MODE_FWD>(a, G, s);
     case MODE_COEF: return launch_one<DT, NCH, R, MODE_COEF>(a, G, s);
     case MODE_ONES: return launch_one<DT, NCH, R, MODE_ONES>(a, G, s);
     default: jscc_set_error("jscc_proj: bad mode %d", mode); return -2;
@@ -833,7 +900,7 @@ int dispatch_nch(int mode, const ProjArgs& a, int G, hipStream_t s) {
   // chunk counts up to the widest row jscc_proj_max_cols admits (32768 columns: 8 FP32, 4 BF16 / FP16 and
   // 2 FP8 vectors per thread); no wider shape is instantiated
   constexpr int kMaxNch = 32768 / VN / kProjThreads;
-  const int nch = (a.P / VN + kProjThreads - 1) / kProjThreads;
+  const int nch = ((a.P + VN - 1) / VN + kProjThreads - 1) / kProjThreads;
   if (nch <= 1) return launch_fused<DT, 1, rows_per_buf(VN, 1)>(mode, a, G, s);
   if (nch <= 2) return launch_fused<DT, 2, rows_per_buf(VN, 2)>(mode, a, G, s);
   if constexpr (kMaxNch >= 4) {
@@ -932,9 +999,13 @@ static int proj_grid_for(int dtype, int64_t n_rows, hipStream_t s) {
   return proj_grid_rows(n_rows, fp8 ? 4 : 8, fp8, s);
 }
 
+// row stride of a projection launch's partial sums: P rounded up to the widest vector (16 columns), so that a
+// ragged last vector of any storage type writes inside its row
+static int64_t proj_ldp(int64_t P) { return (P + 15) / 16 * 16; }
+
 // workspace bytes a launch of `dtype` over n_rows rows on stream s needs
 static int64_t proj_need(int dtype, int64_t n_rows, int64_t P, hipStream_t s) {
-  return jscc_partials_bytes(proj_grid_for(dtype, n_rows, s), P);
+  return jscc_partials_bytes(proj_grid_for(dtype, n_rows, s), proj_ldp(P));
 }
 
 extern "C" int jscc_proj_grid(int64_t n_rows) {
@@ -942,9 +1013,7 @@ extern "C" int jscc_proj_grid(int64_t n_rows) {
 }
 
 extern "C" int64_t jscc_proj_workspace_bytes(int64_t n_rows, int64_t P) {
-  const int64_t G = jscc_proj_grid(n_rows);
-  const int64_t G2 = (G + kRedRows - 1) / kRedRows;
-  return (G + G2) * P * (int64_t)sizeof(float);
+  return jscc_partials_bytes(jscc_proj_grid(n_rows), proj_ldp(P));
 }
 
 // Fused projection of one block of rows.
@@ -969,9 +1038,9 @@ static int proj_impl(int mode, const void* mat, int dtype, int64_t ld, int64_t n
   JSCC_REQUIRE(err != nullptr, "jscc_proj: null error word");
   const int vw = jscc_proj_vec_width(dtype);
   JSCC_REQUIRE(vw > 0, "jscc_proj: bad dtype %d", dtype);
-  JSCC_REQUIRE(n_cols % vw == 0 && ld % vw == 0 && ld >= n_cols,
-               "jscc_proj: columns (%lld) and row stride (%lld) must be multiples of %d", (long long)n_cols,
-               (long long)ld, vw);
+  JSCC_REQUIRE(ld % vw == 0 && ld >= (n_cols + vw - 1) / vw * vw,
+               "jscc_proj: the row stride (%lld) must be a multiple of %d covering the %lld columns", (long long)ld,
+               vw, (long long)n_cols);
   JSCC_REQUIRE(n_cols <= jscc_proj_max_cols(dtype), "jscc_proj: too many columns (%lld)", (long long)n_cols);
   JSCC_REQUIRE(dtype != JSCC_DT_FP8 || row_scale != nullptr, "jscc_proj: FP8 rows need a per-row scale");
   hipStream_t s = (hipStream_t)stream;
@@ -997,7 +1066,7 @@ static int proj_impl(int mode, const void* mat, int dtype, int64_t ld, int64_t n
                  "jscc_proj: workspace too small");
   }
   ProjArgs a;
-  a.M = mat; a.ld = ld; a.P = (int)n_cols; a.rows = rows; a.n_rows = n_total; a.row_base = row_base;
+  a.M = mat; a.ld = ld; a.P = (int)n_cols; a.ldp = (int)proj_ldp(n_cols); a.rows = rows; a.n_rows = n_total; a.row_base = row_base;
   a.n_rows1 = n_rows; a.num_scale1 = num_scale1; a.seg2 = seg2 ? *seg2 : ProjSeg{};
   a.row_scale = row_scale; a.x = x; a.x_perm = x_perm; a.x_perm_stride = x_perm_stride; a.x_len = x_len;
   a.y = y; a.y_stride = y_stride; a.coef = coef; a.eps = eps; a.use_eps = use_eps; a.fwd_out = fwd_out;
@@ -1015,8 +1084,8 @@ static int proj_impl(int mode, const void* mat, int dtype, int64_t ld, int64_t n
   JSCC_CHECK_LAUNCH();
   if (mode == 1) return 0;
 
-  return jscc_reduce_partials((float*)workspace, G, (int)n_cols, inv_perm, inv_perm_stride, out_scale, accumulate,
-                              (flags & 2) ? 1 : 0, w, w_len, err, s, em);
+  return jscc_reduce_partials((float*)workspace, G, (int)n_cols, proj_ldp(n_cols), inv_perm, inv_perm_stride,
+                              out_scale, accumulate, (flags & 2) ? 1 : 0, w, w_len, err, s, em);
 }
 
 extern "C" int jscc_proj(int mode, const void* mat, int dtype, int64_t ld, int64_t n_cols, const int32_t* rows,
@@ -1194,7 +1263,7 @@ extern "C" int jscc_proj_joint_em(const void* mat, int dtype, int64_t ld, int64_
   JSCC_REQUIRE(mat != nullptr && ((uintptr_t)mat % 16) == 0, "jscc_proj_joint_em: matrix must be 16-byte aligned");
   JSCC_REQUIRE(n_rows2 == 0 || (mat2 != nullptr && ((uintptr_t)mat2 % 16) == 0),
                "jscc_proj_joint_em: second matrix must be 16-byte aligned");
-  JSCC_REQUIRE(n_rows2 == 0 || (ld2 % vw == 0 && ld2 >= n_cols),
+  JSCC_REQUIRE(n_rows2 == 0 || (ld2 % vw == 0 && ld2 >= (n_cols + vw - 1) / vw * vw),
                "jscc_proj_joint_em: second row stride (%lld) must be a multiple of %d covering the columns",
                (long long)ld2, vw);
   JSCC_REQUIRE(n_rows2 == 0 || dtype != JSCC_DT_FP8 || row_scale2 != nullptr,
@@ -1219,37 +1288,37 @@ extern "C" int jscc_proj_joint_em(const void* mat, int dtype, int64_t ld, int64_
                    workspace, workspace_bytes, err, stream, em, n_rows2, n_rows2 > 0 ? &g : nullptr, c1);
 }
 
-// partial [G, P] (followed by room for ceil(G/16) more rows) -> w, deterministic order
-int jscc_reduce_partials(float* part, int G, int P, const int64_t* inv_perm, int64_t ips, float scale,
+// partial [G, P] with row stride ldp (followed by room for ceil(G/16) more rows) -> w, deterministic order
+int jscc_reduce_partials(float* part, int G, int P, int64_t ldp, const int64_t* inv_perm, int64_t ips, float scale,
                          int accumulate, int sanitize, float* w, int64_t w_len, int32_t* err, hipStream_t s,
                          const JsccEm& em) {
-  JSCC_REQUIRE(P % 4 == 0, "jscc_reduce_partials: columns must be a multiple of 4");
+  JSCC_REQUIRE(ldp % 4 == 0 && ldp >= (P + 3) / 4 * 4, "jscc_reduce_partials: partial row stride");
   if (!inv_perm) {
     JSCC_REQUIRE(w_len == P, "jscc_reduce_partials: weight length mismatch");
     hipLaunchKernelGGL(k_reduce_direct, dim3((unsigned)((P + kDirCols - 1) / kDirCols)), dim3(256), 0, s, part, G, P,
-                       scale, accumulate, sanitize, w, em);
+                       ldp, scale, accumulate, sanitize, w, em);
     JSCC_CHECK_LAUNCH();
     return 0;
   }
   int Gfin = G;
   if (G > kRedRows) {
     const int G2 = (G + kRedRows - 1) / kRedRows;
-    float* part2 = part + (int64_t)G * P;
-    dim3 grid1((unsigned)((P / 4 + 255) / 256), (unsigned)G2);
-    hipLaunchKernelGGL(k_reduce_stage1, grid1, dim3(256), 0, s, part, G, P, part2);
+    float* part2 = part + (int64_t)G * ldp;
+    dim3 grid1((unsigned)(((P + 3) / 4 + 255) / 256), (unsigned)G2);
+    hipLaunchKernelGGL(k_reduce_stage1, grid1, dim3(256), 0, s, part, G, P, ldp, part2);
     JSCC_CHECK_LAUNCH();
     part = part2;
     Gfin = G2;
   }
-  hipLaunchKernelGGL(k_reduce_stage2, dim3((unsigned)((w_len + 255) / 256)), dim3(256), 0, s, part, Gfin, P,
+  hipLaunchKernelGGL(k_reduce_stage2, dim3((unsigned)((w_len + 255) / 256)), dim3(256), 0, s, part, Gfin, P, ldp,
                      inv_perm, ips, scale, accumulate, sanitize, w, w_len, err, em);
   JSCC_CHECK_LAUNCH();
   return 0;
 }
 
-int64_t jscc_partials_bytes(int G, int64_t P) {
+int64_t jscc_partials_bytes(int G, int64_t ldp) {
   const int64_t G2 = (G + kRedRows - 1) / kRedRows;
-  return ((int64_t)G + G2) * P * (int64_t)sizeof(float);
+  return ((int64_t)G + G2) * ldp * (int64_t)sizeof(float);
 }
 
 // ------------------------------------------------------------------ FP64 sums

@@ -667,16 +667,23 @@ def _workspace(dev: torch.device, nbytes: int) -> torch.Tensor:
 def proj_ready(mat: torch.Tensor, dtype_c: int) -> torch.Tensor:
     """Matrix rows in the layout the projection kernel streams: unit column stride,
     16-byte aligned rows whose stride is a multiple of the vector width.  The
-    column count itself must be a multiple of the vector width (16 for every
-    polar grid of the reference)."""
+    kernel reads each row in whole vectors, so the rows must also extend (in
+    memory) to the column count rounded up to the vector width; a matrix that
+    does not -- e.g. a contiguous [M, P] with P not a multiple of it -- is copied
+    into zero-padded rows and returned as the [M, P] view of them."""
     vw = _lib.lib().jscc_proj_vec_width(dtype_c)
-    P = mat.shape[1]
-    if P % vw != 0:
-        raise ValueError(f"{P} columns: the projection kernels need a multiple of {vw} pixels")
-    if mat.stride(1) == 1 and mat.stride(0) % vw == 0 and mat.data_ptr() % 16 == 0:
+    M, P = mat.shape
+    Pv = round_up(P, vw)
+    esz = mat.element_size()
+    if (mat.stride(1) == 1 and mat.stride(0) % vw == 0 and mat.data_ptr() % 16 == 0 and
+            (M == 0 or (mat.storage_offset() + (M - 1) * mat.stride(0) + Pv) * esz <= mat.untyped_storage().nbytes())):
         return mat
-    # a fresh allocation is 16-byte aligned; a contiguous but offset view must be copied
-    return mat.clone(memory_format=torch.contiguous_format) if mat.is_contiguous() else mat.contiguous()
+    if P % vw == 0:
+        # a fresh allocation is 16-byte aligned; a contiguous but offset view must be copied
+        return mat.clone(memory_format=torch.contiguous_format) if mat.is_contiguous() else mat.contiguous()
+    padded = torch.zeros((M, Pv), dtype=mat.dtype, device=mat.device)
+    padded[:, :P] = mat
+    return padded[:, :P]
 
 
 def proj(mode: int, mat: torch.Tensor, **kw) -> None:
