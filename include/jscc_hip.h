@@ -384,7 +384,12 @@ int jscc_sensitivity(const float* A, int64_t M, int64_t lda, int64_t P, const in
  * dsig_out (optional, fixed-sigma model): d(T_i x)/d sigma.  Used for lists
  * whose T exceeds HBM, the lazy per-iteration recompute of
  * distributed/python/recon_osem_dist_sparse_jsccsd_only_lazy.py:46-125, and
- * the adjoint of the differentiable cone operator (no reference counterpart). */
+ * the adjoint of the differentiable cone operator (no reference counterpart).
+ * The kernels run on whole voxel quads: P4 = n_vox rounded up to 4.  The
+ * coordinates, x and each sysmat row hold P4 values (16-byte aligned; x past
+ * n_vox finite); a count that is not a multiple of 4 needs a sysmat whose
+ * rows are zero past n_vox (ld_sysmat 0: one row for every event, e.g. ones
+ * then zeros), so the padded voxels carry zero weight; w gets n_vox values. */
 int jscc_cone_proj_grid(int64_t n_rows);
 int64_t jscc_cone_proj_workspace_bytes(int64_t n_rows, int64_t n_vox);
 int jscc_cone_proj(int mode, const void* events, const int32_t* ev_idx, int64_t n_rows, const float* coor_x,
@@ -405,7 +410,10 @@ int jscc_cone_proj(int mode, const void* events, const int32_t* ev_idx, int64_t 
  *           independent reconstructions); w (optional) receives the weights;
  *   mode 1 (column sums): w[b, :] = sum of the normalised rows of b (the
  *           list-mode sensitivity of main_plane.py:348-355).
- * x, sens, w: FP32 [B, ld*] with 16-byte aligned rows.  The launch shape is
+ * x, sens, w: FP32 [B, ld*] with 16-byte aligned rows; x rows hold n_vox
+ * rounded up to 4 finite values, and a count that is not a multiple of 4
+ * needs a sysmat zero past n_vox, as for jscc_cone_proj (updates and sums
+ * cover the first n_vox columns).  The launch shape is
  * what an autotuner varies: groups_per_cu, work-groups per CU (0: 4), and
  * tile, one of jscc_cone_batch_tiles(n_vox) tile shapes (threads per work-
  * group, voxel quads per thread, rows per step; jscc_cone_batch_tile_shape

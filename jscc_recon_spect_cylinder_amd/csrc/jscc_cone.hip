@@ -1215,7 +1215,7 @@ extern "C" int jscc_cone_proj_grid(int64_t n_rows) {
 }
 
 extern "C" int64_t jscc_cone_proj_workspace_bytes(int64_t n_rows, int64_t n_vox) {
-  return jscc_partials_bytes(jscc_cone_proj_grid(n_rows), n_vox);
+  return jscc_partials_bytes(jscc_cone_proj_grid(n_rows), (n_vox + 3) / 4 * 4);
 }
 
 extern "C" int jscc_cone_proj(int mode, const void* events, const int32_t* ev_idx, int64_t n_rows,
@@ -1225,15 +1225,23 @@ extern "C" int jscc_cone_proj(int mode, const void* events, const int32_t* ev_id
                               float* fwd_out, float* dsig_out, float out_scale, int accumulate, float* w,
                               int64_t w_len, void* workspace, int64_t workspace_bytes, int32_t* err, void* stream) {
   JSCC_REQUIRE(mode >= 0 && mode <= 2, "jscc_cone_proj: bad mode");
-  JSCC_REQUIRE(n_rows >= 0 && n_rows < ((int64_t)1 << 31) && n_vox > 0 && n_vox % 4 == 0,
-               "jscc_cone_proj: voxel count must be a positive multiple of 4");
+  // the kernels run on whole voxel quads: P4 = n_vox rounded up to 4 (image, factor rows and partial sums
+  // extend that far); a ragged count needs a factor row that is zero past n_vox, so the padded voxels of the
+  // last quad carry zero weight
+  JSCC_REQUIRE(n_rows >= 0 && n_rows < ((int64_t)1 << 31) && n_vox > 0, "jscc_cone_proj: bad shape");
+  JSCC_REQUIRE(n_vox % 4 == 0 || sysmat != nullptr,
+               "jscc_cone_proj: %lld voxels (not a multiple of 4) need a factor row that is zero past them",
+               (long long)n_vox);
+  const int64_t P4 = (n_vox + 3) / 4 * 4;
   JSCC_REQUIRE(sigma_model == SIG_STRICT || sigma_model == SIG_FIXED, "jscc_cone_proj: bad sigma model");
   JSCC_REQUIRE(dsig_out == nullptr || sigma_model == SIG_FIXED,
                "jscc_cone_proj: d/d sigma exists for the fixed-sigma model only");
   JSCC_REQUIRE(events && ev_idx && coor_x && coor_y && coor_z && x && err, "jscc_cone_proj: null pointer");
   JSCC_REQUIRE(((uintptr_t)coor_x | (uintptr_t)coor_y | (uintptr_t)coor_z | (uintptr_t)x) % 16 == 0,
                "jscc_cone_proj: coordinates and image must be 16-byte aligned");
-  if (sysmat) JSCC_REQUIRE(ld_sysmat % 4 == 0 && (uintptr_t)sysmat % 16 == 0, "jscc_cone_proj: sysmat alignment");
+  if (sysmat)
+    JSCC_REQUIRE(ld_sysmat % 4 == 0 && (ld_sysmat == 0 || ld_sysmat >= P4) && (uintptr_t)sysmat % 16 == 0,
+                 "jscc_cone_proj: sysmat alignment");
   if (mode == CP_FWD) JSCC_REQUIRE(fwd_out != nullptr, "jscc_cone_proj: null forward output");
   if (mode == CP_COEF) JSCC_REQUIRE(coef != nullptr, "jscc_cone_proj: null coefficients");
   if (mode != CP_FWD) JSCC_REQUIRE(w != nullptr && w_len == n_vox, "jscc_cone_proj: weight must have n_vox entries");
@@ -1245,10 +1253,10 @@ extern "C" int jscc_cone_proj(int mode, const void* events, const int32_t* ev_id
   }
   const int G = jscc_cone_proj_grid(n_rows);
   if (mode != CP_FWD)
-    JSCC_REQUIRE(workspace && workspace_bytes >= jscc_partials_bytes(G, n_vox), "jscc_cone_proj: workspace too small");
+    JSCC_REQUIRE(workspace && workspace_bytes >= jscc_partials_bytes(G, P4), "jscc_cone_proj: workspace too small");
   ConeProjArgs a;
   a.ev = (const JsccEvent*)events; a.ev_idx = ev_idx; a.n_rows = n_rows;
-  a.cx = coor_x; a.cy = coor_y; a.cz = coor_z; a.P = (int)n_vox; a.A = sysmat; a.lda = ld_sysmat;
+  a.cx = coor_x; a.cy = coor_y; a.cz = coor_z; a.P = (int)P4; a.A = sysmat; a.lda = ld_sysmat;
   a.cc = make_consts(sigma_model, fixed_sigma);
   a.x = x; a.coef = coef; a.eps = eps; a.use_eps = use_eps; a.normalize = normalize;
   a.fwd_out = fwd_out; a.dsig_out = dsig_out; a.partial = (float*)workspace;
@@ -1270,7 +1278,7 @@ extern "C" int jscc_cone_proj(int mode, const void* events, const int32_t* ev_id
   JSCC_CHECK_LAUNCH();
   if (mode == CP_FWD) return 0;
   const JsccEm none = {nullptr, nullptr, 0, 0.0f, 0, 1.0f, nullptr, 0.0f};
-  return jscc_reduce_partials((float*)workspace, G, (int)n_vox, n_vox, nullptr, 0, out_scale, accumulate, 0, w, w_len,
+  return jscc_reduce_partials((float*)workspace, G, (int)n_vox, P4, nullptr, 0, out_scale, accumulate, 0, w, w_len,
                               err, s,
                               none);
 }
@@ -1500,8 +1508,10 @@ __global__ void __launch_bounds__(BLOCK)
 // Column reduction per reconstruction (blockIdx.y = b), 64 columns per block:
 // 16 groups of 16 threads sum every 16th contributing slot, then combine in
 // LDS in a fixed order.  OSEM: x_b <- em(x_b, w, s_b); SUM: w_out_b <- w.
+// P: the partial row stride (whole quads); only the first n_vox columns are written.
 __global__ void __launch_bounds__(256) k_reduce_batched(const float* __restrict__ part, int G, int64_t n_rows,
-                                                        const int64_t* __restrict__ row_off, int P, int mode,
+                                                        const int64_t* __restrict__ row_off, int P, int n_vox,
+                                                        int mode,
                                                         float* __restrict__ x, int64_t ldx,
                                                         const float* __restrict__ sens, int64_t lds, int em_mode,
                                                         float em_eps, float* __restrict__ w, int64_t ldw) {
@@ -1530,6 +1540,7 @@ __global__ void __launch_bounds__(256) k_reduce_batched(const float* __restrict_
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const int64_t j = col + k;
+      if (j >= n_vox) break;
       if (w) w[(int64_t)b * ldw + j] = tv[k];
       if (mode == BP_OSEM) {
         float* xp = x + (int64_t)b * ldx + j;
@@ -1609,7 +1620,7 @@ extern "C" int jscc_cone_batch_groups(int64_t n_rows, int groups_per_cu) {
 
 extern "C" int64_t jscc_cone_batch_workspace_bytes(int64_t n_rows, int n_recon, int64_t n_vox, int groups_per_cu) {
   const int64_t G = jscc_cone_batch_groups(n_rows, groups_per_cu);
-  return (G + (n_recon > 0 ? n_recon : 1) - 1) * n_vox * (int64_t)sizeof(float);
+  return (G + (n_recon > 0 ? n_recon : 1) - 1) * ((n_vox + 3) / 4 * 4) * (int64_t)sizeof(float);
 }
 
 extern "C" int jscc_cone_proj_batched(int mode, const void* events, const int32_t* ev_idx, int64_t n_rows,
@@ -1621,16 +1632,22 @@ extern "C" int jscc_cone_proj_batched(int mode, const void* events, const int32_
                                       void* workspace, int64_t workspace_bytes, int32_t* err, void* stream) {
   JSCC_REQUIRE(mode == BP_OSEM || mode == BP_SUM, "jscc_cone_proj_batched: bad mode");
   JSCC_REQUIRE(n_recon >= 1 && n_recon <= 65535, "jscc_cone_proj_batched: bad reconstruction count");
-  JSCC_REQUIRE(n_rows >= 0 && n_rows < ((int64_t)1 << 31) && n_vox > 0 && n_vox % 4 == 0,
-               "jscc_cone_proj_batched: voxel count must be a positive multiple of 4");
+  // whole voxel quads, as jscc_cone_proj: a ragged count needs a factor row that is zero past it
+  JSCC_REQUIRE(n_rows >= 0 && n_rows < ((int64_t)1 << 31) && n_vox > 0, "jscc_cone_proj_batched: bad shape");
+  JSCC_REQUIRE(n_vox % 4 == 0 || sysmat != nullptr,
+               "jscc_cone_proj_batched: %lld voxels (not a multiple of 4) need a factor row that is zero past them",
+               (long long)n_vox);
+  const int64_t P4 = (n_vox + 3) / 4 * 4;
   JSCC_REQUIRE(sigma_model == SIG_STRICT || sigma_model == SIG_FIXED, "jscc_cone_proj_batched: bad sigma model");
   if (sigma_model == SIG_FIXED) JSCC_REQUIRE(fixed_sigma > 0.0, "jscc_cone_proj_batched: fixed sigma must be positive");
   JSCC_REQUIRE(events && ev_idx && row_off && coor_x && coor_y && coor_z && err, "jscc_cone_proj_batched: null pointer");
   JSCC_REQUIRE(((uintptr_t)coor_x | (uintptr_t)coor_y | (uintptr_t)coor_z) % 16 == 0,
                "jscc_cone_proj_batched: coordinates must be 16-byte aligned");
-  if (sysmat) JSCC_REQUIRE(ld_sysmat % 4 == 0 && (uintptr_t)sysmat % 16 == 0, "jscc_cone_proj_batched: sysmat alignment");
+  if (sysmat)
+    JSCC_REQUIRE(ld_sysmat % 4 == 0 && (ld_sysmat == 0 || ld_sysmat >= P4) && (uintptr_t)sysmat % 16 == 0,
+                 "jscc_cone_proj_batched: sysmat alignment");
   if (mode == BP_OSEM) {
-    JSCC_REQUIRE(x && sens && ldx >= n_vox && ld_sens >= n_vox && ldx % 4 == 0 && (uintptr_t)x % 16 == 0,
+    JSCC_REQUIRE(x && sens && ldx >= P4 && ld_sens >= n_vox && ldx % 4 == 0 && (uintptr_t)x % 16 == 0,
                  "jscc_cone_proj_batched: images [n_recon, ldx] (16-byte aligned rows) and sensitivities needed");
     JSCC_REQUIRE(em_mode >= 0 && em_mode <= 2, "jscc_cone_proj_batched: bad EM mode");
   } else {
@@ -1645,7 +1662,7 @@ extern "C" int jscc_cone_proj_batched(int mode, const void* events, const int32_
   hipStream_t s = (hipStream_t)stream;
   ConeBatchArgs a;
   a.ev = (const JsccEvent*)events; a.ev_idx = ev_idx; a.n_rows = n_rows; a.row_off = row_off; a.n_recon = n_recon;
-  a.cx = coor_x; a.cy = coor_y; a.cz = coor_z; a.P = (int)n_vox; a.A = sysmat; a.lda = ld_sysmat;
+  a.cx = coor_x; a.cy = coor_y; a.cz = coor_z; a.P = (int)P4; a.A = sysmat; a.lda = ld_sysmat;
   a.cc = make_consts(sigma_model, fixed_sigma);
   a.x = x; a.ldx = ldx; a.eps = eps; a.use_eps = use_eps; a.normalize = normalize; a.partial = (float*)workspace;
   int rc;
@@ -1660,9 +1677,9 @@ extern "C" int jscc_cone_proj_batched(int mode, const void* events, const int32_
   jscc_prof_end(JSCC_TAG_CONE_PROJ, s);
   if (rc != 0) return rc;
   JSCC_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_reduce_batched, dim3((unsigned)((n_vox + 63) / 64), (unsigned)n_recon), dim3(256), 0, s,
-                     (const float*)workspace, G, n_rows, row_off, (int)n_vox, mode, x, ldx, sens, ld_sens, em_mode,
-                     em_eps, w, ldw);
+  hipLaunchKernelGGL(k_reduce_batched, dim3((unsigned)((P4 + 63) / 64), (unsigned)n_recon), dim3(256), 0, s,
+                     (const float*)workspace, G, n_rows, row_off, (int)P4, (int)n_vox, mode, x, ldx, sens, ld_sens,
+                     em_mode, em_eps, w, ldw);
   JSCC_CHECK_LAUNCH();
   return 0;
 }

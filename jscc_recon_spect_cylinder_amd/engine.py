@@ -596,7 +596,8 @@ class BatchedLazyStore:
 
     def osem(self, x: torch.Tensor, s: torch.Tensor, n_iter: int, n_subsets: int, eps=None,
              em_mode: int = ops.EM_PLAIN, graph: bool = False) -> torch.Tensor:
-        """OSEM of all B images x [B, >=P] (updated in place) with sensitivities s [B, >=P].
+        """OSEM of all B images x [B, >= P rounded up to 4] (updated in place; the columns past P are read,
+        never written, and must be finite) with sensitivities s [B, >=P].
         graph=True captures one iteration (2 launches per subset) as a HIP graph and replays it."""
         if graph:
             gi = self.capture(x, s, n_subsets, eps, em_mode)

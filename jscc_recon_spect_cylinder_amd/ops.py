@@ -977,6 +977,23 @@ def sensitivity(A: torch.Tensor, inv: torch.Tensor | None) -> torch.Tensor:
 CP_RATIO, CP_FWD, CP_COEF = 0, 1, 2
 
 
+_ragged_rows: dict = {}
+
+
+def ragged_factor_row(P: int, device) -> torch.Tensor:
+    """[1, P rounded up to 4] FP32: ones on the P voxels, zeros past them.  The no-store projections run on
+    whole voxel quads; for a voxel count that is not a multiple of 4 (and no system-matrix factor, whose rows
+    are zero-padded already) this row, read with stride 0 as every event's first-hit factor, gives the padded
+    voxels of the last quad zero weight."""
+    key = (int(P), str(device))
+    row = _ragged_rows.get(key)
+    if row is None:
+        row = torch.zeros((1, round_up(P, 4)), dtype=torch.float32, device=device)
+        row[0, :P] = 1.0
+        _ragged_rows[key] = row
+    return row
+
+
 def cone_proj(mode: int, events: torch.Tensor, ev_idx: torch.Tensor, n_rows: int, vox: VoxelSoA, x: torch.Tensor,
               *, sysmat=None, sigma_model: int = SIG_FIXED, fixed_sigma: float = 0.0, coef=None, eps=None,
               normalize: bool = True, fwd_out=None, dsig_out=None, out_scale: float = 1.0, accumulate=False,
@@ -984,8 +1001,6 @@ def cone_proj(mode: int, events: torch.Tensor, ev_idx: torch.Tensor, n_rows: int
     """No-store cone projection (rows recomputed in registers), see include/jscc_hip.h."""
     dev = events.device
     P = vox.n
-    if P % 4 != 0:
-        raise ValueError("the recompute kernel needs a multiple of 4 voxels")
     if x.numel() != P or x.dtype != torch.float32 or not x.is_contiguous() or x.device != dev:
         raise ValueError("image must be a contiguous float32 vector with one value per voxel")
     if ev_idx.dtype != torch.int32 or ev_idx.numel() < n_rows or not ev_idx.is_contiguous():
@@ -993,11 +1008,19 @@ def cone_proj(mode: int, events: torch.Tensor, ev_idx: torch.Tensor, n_rows: int
     for t, need in ((coef, n_rows), (fwd_out, n_rows), (dsig_out, n_rows), (w, P)):
         if t is not None and (t.numel() < need or t.dtype != torch.float32 or t.device != dev):
             raise ValueError("cone_proj operand has the wrong size, dtype or device")
-    A = None
+    A, lda = None, 0
     if sysmat is not None:
         A = aligned_rows(sysmat, 4, dev)
         if A.shape[1] != P:
             raise ValueError("sysmat columns do not match the voxels")
+        lda = A.stride(0)
+    elif P % 4:
+        A = ragged_factor_row(P, dev)
+    if P % 4 or x.data_ptr() % 16:
+        # the kernels read the image in whole 16-byte quads: a zero-padded copy
+        xq = torch.zeros(round_up(P, 4), dtype=torch.float32, device=dev)
+        xq[:P] = x
+        x = xq
     ws = None
     nbytes = 0
     if mode != CP_FWD:
@@ -1006,7 +1029,7 @@ def cone_proj(mode: int, events: torch.Tensor, ev_idx: torch.Tensor, n_rows: int
         nbytes = ws.numel()
     rc = _lib.lib().jscc_cone_proj(
         mode, events.data_ptr(), ev_idx.data_ptr(), n_rows, vox.x.data_ptr(), vox.y.data_ptr(), vox.z.data_ptr(),
-        P, _p(A), A.stride(0) if A is not None else 0, int(sigma_model), float(fixed_sigma), x.data_ptr(), _p(coef),
+        P, _p(A), lda, int(sigma_model), float(fixed_sigma), x.data_ptr(), _p(coef),
         float(eps) if eps is not None else 0.0, 1 if eps is not None else 0, 1 if normalize else 0, _p(fwd_out),
         _p(dsig_out), float(out_scale), 1 if accumulate else 0, _p(w), w.numel() if w is not None else 0,
         _p(ws), nbytes, err_word(dev).data_ptr(), stream_of(dev))
@@ -1035,16 +1058,22 @@ def cone_proj_batched(mode: int, events: torch.Tensor, ev_idx: torch.Tensor, row
         if t is not None and (t.ndim != 2 or t.shape[0] != B or t.shape[1] < P or t.dtype != torch.float32
                               or t.stride(1) != 1 or t.device != dev):
             raise ValueError("images / sensitivities / weights must be float32 [B, >= P] with unit column stride")
-    A = None
+    if x is not None and (x.shape[1] < round_up(P, 4) or x.stride(0) % 4 != 0):
+        raise ValueError("images must be [B, >= P rounded up to 4] with a row stride that is a multiple of 4 "
+                         "(the kernels read whole voxel quads; values past P must be finite)")
+    A, lda = None, 0
     if sysmat is not None:
         A = aligned_rows(sysmat, 4, dev)
+        lda = A.stride(0)
+    elif P % 4:
+        A = ragged_factor_row(P, dev)
     nbytes = _lib.lib().jscc_cone_batch_workspace_bytes(max(n_rows, 1), B, P, int(groups_per_cu))
     ws = workspace if workspace is not None else _workspace(dev, nbytes)
     if ws.numel() < nbytes:
         raise ValueError("workspace too small")
     rc = _lib.lib().jscc_cone_proj_batched(
         mode, events.data_ptr(), ev_idx.data_ptr(), n_rows, row_off.data_ptr(), B, vox.x.data_ptr(),
-        vox.y.data_ptr(), vox.z.data_ptr(), P, _p(A), A.stride(0) if A is not None else 0, int(sigma_model),
+        vox.y.data_ptr(), vox.z.data_ptr(), P, _p(A), lda, int(sigma_model),
         float(fixed_sigma), _p(x), x.stride(0) if x is not None else 0, float(eps) if eps is not None else 0.0,
         1 if eps is not None else 0, 1 if normalize else 0, _p(sens), sens.stride(0) if sens is not None else 0,
         int(em_mode), float(em_eps), _p(w), w.stride(0) if w is not None else 0, int(groups_per_cu), int(tile),

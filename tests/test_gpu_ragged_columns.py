@@ -246,3 +246,63 @@ def test_joint_osem_on_ragged_columns(dtype):
         xr = osem.osem_joint_mode([decA[lo:hi] for lo, hi in ab], [y[lo:hi, None].astype(np.float64) for lo, hi in ab],
                                   [[[decT[lo:hi]]] for lo, hi in tb], ident, ident, xr, s, 0.5)
     assert rel(x.cpu().numpy(), xr) < 1e-4
+
+
+# ------------------------------------------------------------------ no-store projections (T recomputed)
+def _ragged_rows(n, seed, store, sysmat=None, model="fixed"):
+    from jscc_recon_spect_cylinder_amd import synth
+    from jscc_recon_spect_cylinder_amd.process_list_plane_strict import build_compton_rows
+    cl = _ragged_list(n=n, seed=seed)
+    tmax, tmin, tsum = synth.energy_thresholds(cl.e0)
+    return build_compton_rows(sysmat, torch.from_numpy(cl.detector), torch.from_numpy(cl.coor),
+                              torch.from_numpy(cl.list_origin.copy()), 0.0, 0.0, cl.e0, 0.05, tmax, tmin, tsum, DEV,
+                              noise="none", sigma_model=model, fixed_sigma=SIG, store_dtype=store)
+
+
+@pytest.mark.parametrize("model", ["fixed", "strict"])
+def test_lazy_store_on_a_ragged_grid_equals_stored_rows(model):
+    """The no-store projection on 1001 voxels (whole quads: the padded voxels weighted zero by a factor row --
+    the ragged factor row without a system matrix, the zero-padded system-matrix rows with one) against the
+    stored FP32 rows: forward, column sums and 2 x 4 OSEM."""
+    from jscc_recon_spect_cylinder_amd import ops
+    from jscc_recon_spect_cylinder_amd.engine import LazyListStore, ListModeStore
+    print(f"ragged lazy store {model}", flush=True)
+    rng = np.random.default_rng(3)
+    A = torch.from_numpy((rng.random((800, 1001)) + 0.05).astype(np.float32)) if model == "strict" else None
+    rows = _ragged_rows(400, 21, torch.float32, sysmat=A, model=model)  # 800 crystals: one A row each
+    P = rows.n_vox
+    assert P % 4 == 1
+    st = ListModeStore.from_rows(rows, "fp32")
+    lz = LazyListStore.from_rows(rows, sysmat=A.to(DEV) if A is not None else None,
+                                 sigma_model=ops.SIG_FIXED if model == "fixed" else ops.SIG_STRICT, fixed_sigma=SIG)
+    x = torch.from_numpy((rng.random(P) + 0.5).astype(np.float32)).to(DEV)
+    f_lz, _ = lz.forward(x)
+    assert rel(f_lz.cpu().numpy(), st.forward(x).cpu().numpy()) < 1e-5
+    s_st = st.backproject_ones()
+    s_lz = lz.backproject_ones(x)
+    assert s_lz.shape == (P,)
+    assert rel(s_lz.cpu().numpy(), s_st.cpu().numpy()) < 1e-5
+    x1 = st.osem(torch.ones_like(s_st), s_st, 2, 4)
+    x2 = lz.osem(torch.ones_like(s_st), s_st, 2, 4)
+    assert rel(x2.cpu().numpy(), x1.cpu().numpy()) < 1e-5
+
+
+def test_batched_store_on_a_ragged_grid_matches_single_lists():
+    """Three lists of uneven length on 1001 voxels in one launch per subset step (images [B, 1004]) against
+    each list's stored-row OSEM."""
+    from jscc_recon_spect_cylinder_amd import ops
+    from jscc_recon_spect_cylinder_amd.engine import BatchedLazyStore, ListModeStore
+    print("ragged batched store", flush=True)
+    built = [_ragged_rows(n, 50 + b, torch.float32) for b, n in enumerate((500, 1400, 80))]
+    bs = BatchedLazyStore.from_rows(built, sigma_model=ops.SIG_FIXED, fixed_sigma=SIG)
+    P = built[0].n_vox
+    s = bs.sensitivity()
+    x = torch.ones((len(built), ops.round_up(P, 4)), device=DEV)
+    bs.osem(x, s, 3, 4)
+    for b, r in enumerate(built):
+        st = ListModeStore.from_rows(r, "fp32")
+        sb = st.backproject_ones()
+        assert rel(s[b, :P].cpu().numpy(), sb.cpu().numpy()) < 1e-5
+        xb = st.osem(torch.ones(P, device=DEV), sb, 3, 4)
+        assert rel(x[b, :P].cpu().numpy(), xb.cpu().numpy()) < 1e-5
+        assert torch.all(x[b, P:] == 1.0)  # the padding is never updated
