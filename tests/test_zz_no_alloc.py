@@ -29,11 +29,17 @@ def _matrix(dtype, M, P, rng):
         codes, scale = ops.quantize_rows(base, P, M, None, dtype)
         return codes, scale
     scale = torch.from_numpy((rng.random(M) + 0.5).astype(np.float32)).to(DEV)
-    return (base.to(torch.bfloat16) if dtype == "bf16" else base), scale
+    m = base.to(torch.bfloat16) if dtype == "bf16" else base
+    if P % 64:  # rows padded to whole vectors (their padding poisoned: it never reaches the sums)
+        padded = torch.full((M, ops.round_up(P, 64)), float("nan"), dtype=m.dtype, device=DEV)
+        padded[:, :P] = m
+        m = padded
+    return m, scale
 
 
 @pytest.mark.parametrize("dtype,P,M", [("fp32", 16384, 3000), ("bf16", 16384, 3000), ("fp16", 25600, 3000),
-                                       ("fp8", 4096, 3000), ("bf16", 32768, 600)])
+                                       ("fp8", 4096, 3000), ("bf16", 32768, 600), ("fp32", 1001, 1500),
+                                       ("bf16", 9001, 1500)])
 def test_noalloc_subset_step_equals_workspace_form(dtype, P, M):
     from jscc_recon_spect_cylinder_amd import _lib, ops
     lib = _lib.lib()
