@@ -171,3 +171,37 @@ def test_fp16_rows_of_the_cone_pass_equal_quantised_fp32_rows(rings, angles, syn
     x16 = st16.osem(torch.ones(P, device=DEV), r16.sens, 3, 4)
     xq = st_q.osem(torch.ones(P, device=DEV), st_q.backproject_ones(), 3, 4)
     assert _rel(x16.cpu().numpy(), xq.cpu().numpy()) < 1e-4
+
+
+def test_fp16_strict_rows_with_first_hit_factor_equal_quantised_fp32_rows():
+    """The strict sigma model with the first-hit factor on the reference grid layout (the reference-written
+    golden list, 1280 voxels): FP16 rows written by the cone pass have the codes and scales of quantize_rows on
+    the FP32 rows, the sensitivity of the column-sum pass over them, and column sums within FP16 rounding
+    (2^-11 per element) of the reference-written normalised T."""
+    from jscc_recon_spect_cylinder_amd import ops
+    from jscc_recon_spect_cylinder_amd.engine import ListModeStore
+    from jscc_recon_spect_cylinder_amd.process_list_plane_strict import build_compton_rows
+    g = np.load(STRICT)
+    det = np.load(FIX)["detector"]
+    coor = g["coor"]
+    A = golden_sysmat(det, coor)
+    dr1, dr2, e0, res, tmax, tmin, tsum = g["params"]
+    built = {}
+    for sd in (torch.float32, torch.float16):
+        built[sd] = build_compton_rows(torch.from_numpy(A), torch.from_numpy(det), torch.from_numpy(coor),
+                                       torch.from_numpy(g["list_in"].copy()), dr1, dr2, e0, res, tmax, tmin, tsum,
+                                       DEV, noise=(torch.from_numpy(g["noise1"]).to(DEV),
+                                                   torch.from_numpy(g["noise2"]).to(DEV)),
+                                       store_dtype=sd, want_sens=True)
+    r32, r16 = built[torch.float32].resolve_counts(), built[torch.float16].resolve_counts()
+    P = coor.shape[0]
+    assert r32.n_kept == r16.n_kept == g["t"].shape[0]
+    assert torch.equal(r32.kept[:r32.n_kept], r16.kept[:r16.n_kept])
+    n = r16.n_sel
+    codes, scale = ops.quantize_rows(r32.store, P, n, r32.stats, "fp16")
+    kept = r16.kept[:r16.n_kept].long()
+    assert torch.equal(r16.store[kept, :P].view(torch.int16), codes[kept, :P].view(torch.int16))
+    np.testing.assert_allclose(r16.row_scale[:n].cpu().numpy(), scale[:n].cpu().numpy(), rtol=1e-6)
+    s_pass = ListModeStore.from_rows(r16, "fp16").backproject_ones().cpu().numpy()
+    assert _rel(r16.sens.cpu().numpy(), s_pass) < 2e-6
+    assert _rel(r16.sens.cpu().numpy(), g["t"].astype(np.float64).sum(0)) < 5e-4
