@@ -205,3 +205,37 @@ def test_fp16_strict_rows_with_first_hit_factor_equal_quantised_fp32_rows():
     s_pass = ListModeStore.from_rows(r16, "fp16").backproject_ones().cpu().numpy()
     assert _rel(r16.sens.cpu().numpy(), s_pass) < 2e-6
     assert _rel(r16.sens.cpu().numpy(), g["t"].astype(np.float64).sum(0)) < 5e-4
+
+
+@pytest.mark.parametrize("store", [torch.float32, torch.bfloat16, torch.float16])
+def test_strict_first_hit_pass_at_the_reference_grid_size(store):
+    """The strict sigma model with a first-hit factor at 25600 voxels (the size of the reference's
+    511keV_RotateNum20 grid; 1024-thread work-groups): FP32 / BF16 rows and statistics as the plain pass,
+    the sensitivity as its column sums; FP16 codes and scales as quantize_rows of the FP32 rows."""
+    from jscc_recon_spect_cylinder_amd import ops, synth
+    from jscc_recon_spect_cylinder_amd.engine import ListModeStore
+    from jscc_recon_spect_cylinder_amd.process_list_plane_strict import build_compton_rows
+    print(f"strict first-hit 25600 {store}", flush=True)
+    rng = np.random.default_rng(100)
+    P = 100 * 256
+    A = torch.from_numpy((rng.random((800, P)) + 0.05).astype(np.float32))  # 2 crystals per event
+    cl = synth.compton_list(n_events=400, n_rings=100, n_angles=256, seed=140)
+    tmax, tmin, tsum = synth.energy_thresholds(cl.e0)
+
+    def build(sd, want):
+        return build_compton_rows(A, torch.from_numpy(cl.detector), torch.from_numpy(cl.coor),
+                                  torch.from_numpy(cl.list_origin.copy()), 0.0, 0.0, cl.e0, 0.05, tmax, tmin, tsum,
+                                  DEV, noise="none", sigma_model="strict", store_dtype=sd, want_sens=want)
+    if store != torch.float16:
+        got = _check_against_plain_pass(build(store, False), build(store, True), P)
+        assert np.all(np.isfinite(got))
+        return
+    r32, r16 = build(torch.float32, True).resolve_counts(), build(torch.float16, True).resolve_counts()
+    assert r16.n_kept > 20 and torch.equal(r32.kept[:r32.n_kept], r16.kept[:r16.n_kept])
+    n = r16.n_sel
+    codes, scale = ops.quantize_rows(r32.store, P, n, r32.stats, "fp16")
+    kept = r16.kept[:r16.n_kept].long()
+    assert torch.equal(r16.store[kept, :P].view(torch.int16), codes[kept, :P].view(torch.int16))
+    np.testing.assert_allclose(r16.row_scale[:n].cpu().numpy(), scale[:n].cpu().numpy(), rtol=1e-6)
+    s_pass = ListModeStore.from_rows(r16, "fp16").backproject_ones().cpu().numpy()
+    assert _rel(r16.sens.cpu().numpy(), s_pass) < 2e-6
