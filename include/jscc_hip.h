@@ -111,16 +111,17 @@ int jscc_cone_rows_counted(const void* events, const int32_t* ev_idx, const int3
  * torch.sum(t, dim=0) of the normalised, kept T that main_plane.py:348-355
  * scales into Sensi_d (and recon_osem_plane.py:253-256 divides by) -- taken
  * from the rows as stored.  Each row is read back by the thread that wrote it,
- * during the cone arithmetic of the work-group's next rows, so the sums cost no
- * second pass over T in HBM.  out_dtype FP32 or BF16: rows as jscc_cone_rows
- * stores them.  out_dtype FP16: the FP32 values wait in a ring in the workspace
- * and the read-back stores the codes of jscc_quantize_rows (half(t / q),
- * q = max / 2^14) with row_scale[i] = q / S_i, row_scale * code ~ t / S; a
- * dropped row's codes are zeros; sens is then the column sums of
- * row_scale * code, the rows the OSEM projects.  Persistent grid of
- * jscc_cone_sens_grid(n_vox) work-groups taking groups of 8 rows from a ticket
- * counter: the grouping of the FP32 sums, and so the last bits of sens, can
- * differ between runs (rows, codes and statistics do not).  First-hit factor
+ * so the sums cost no second pass over T in HBM.  out_dtype FP32 or BF16: rows
+ * as jscc_cone_rows stores them, read back during the cone arithmetic of the
+ * work-group's next rows.  out_dtype FP16: the FP32 values of a group wait in
+ * a slot of the workspace and, right after the group's statistics, are read
+ * back as the codes of jscc_quantize_rows (half(t / q), q = max / 2^14) with
+ * row_scale[i] = q / S_i, row_scale * code ~ t / S; a dropped row's codes are
+ * zeros; sens is then the column sums of row_scale * code, the rows the OSEM
+ * projects.  Persistent grid of jscc_cone_sens_grid(n_vox) work-groups taking
+ * groups of rows (8; 4 for FP16) from a ticket counter: the grouping of the
+ * FP32 sums, and so the last bits of sens, can differ between runs (rows,
+ * codes and statistics do not).  First-hit factor
  * sysmat[arow] or none (no tap tables), no t_c output, n_vox <= 32768.
  * workspace: jscc_cone_sens_workspace_bytes(n_vox, out_dtype) bytes, 16-byte
  * aligned.  Statistics equal jscc_cone_rows's up to the order of the per-row
