@@ -492,7 +492,7 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4)
     const float* __restrict__ cz, int P, const float* __restrict__ A, int64_t lda, ConeConsts cc,
     void* __restrict__ out_t, int64_t ldt, double* __restrict__ stats, double min_support,
     float* __restrict__ partial, int64_t ldp, int* __restrict__ ticket, const float* __restrict__ zrow,
-    float* __restrict__ ring, float* __restrict__ row_scale) {
+    float* __restrict__ ring, float* __restrict__ row_scale) {  // ring: the FP16 rows' group slots
   constexpr int E = 4;              // events evaluated together (scalar registers)
   constexpr bool Q16 = OUT_DT == JSCC_DT_FP16;
   constexpr int H = Q16 ? 1 : kSensHalves;  // E-event halves per group: one barrier round per H * E rows
@@ -551,7 +551,7 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4)
 #pragma unroll
     for (int h = 0; h < H; ++h) {
       ConeEv evs[E];
-      int rrow[E];  // rows the tile loop stores to (ring slots for FP16 rows)
+      int rrow[E];  // rows the tile loop stores to (rows of the group's slot for FP16 rows)
 #pragma unroll
       for (int e = 0; e < E; ++e) {
         const int r = valid[h * E + e] ? (row0 + h * E + e) : (n_rows - 1);
