@@ -724,8 +724,12 @@ __global__ void __launch_bounds__(256) k_reduce_stage1(const float* __restrict__
 
 typedef JsccEm EmEpilogue;
 
+// The weight combination and accumulation are rounded operation by operation, as the reference's separate
+// tensor operations are ((2 - alpha) * weight_compton + alpha * weight_single, weight += ...): no contraction
+// into fused multiply-adds, which the compiler could otherwise place differently in each kernel.
 __device__ __forceinline__ void finish_weight(float sum, int64_t j, float scale, int accumulate, int sanitize,
                                               float* __restrict__ w, const EmEpilogue& em) {
+#pragma clang fp contract(off)
   if (sanitize && !(sum == sum && fabsf(sum) != __builtin_inff())) sum = 0.0f;  // nan_to_num(0, 0, 0)
   const float add = scale == 1.0f ? sum : scale * sum;
   const float wj = accumulate ? w[j] + add : add;
@@ -739,6 +743,31 @@ __device__ __forceinline__ void finish_weight(float sum, int64_t j, float scale,
       wc = em.c1 * wc;
     }
     em.x[j] = jscc_em_value(em.x[j], wc, em.s[j], em.mode, em.eps);
+    w[j] = em.zero_w ? 0.0f : wj;
+    if (em.w2 && em.zero_w) em.w2[j] = 0.0f;
+  } else {
+    w[j] = wj;
+  }
+}
+
+// finish_weight with the operands (w, x, s, w2 of column j) already loaded (the k_reduce_direct form: they
+// are loaded before the partial sums, so that their round trip overlaps the reduction's)
+__device__ __forceinline__ void finish_weight_pre(float sum, int64_t j, float scale, int accumulate, int sanitize,
+                                                  float* __restrict__ w, const EmEpilogue& em, float wv, float xv,
+                                                  float sv, float w2v) {
+#pragma clang fp contract(off)
+  if (sanitize && !(sum == sum && fabsf(sum) != __builtin_inff())) sum = 0.0f;  // nan_to_num(0, 0, 0)
+  const float add = scale == 1.0f ? sum : scale * sum;
+  const float wj = accumulate ? wv + add : add;
+  if (em.x) {
+    float wc = wj;
+    if (em.w2) {
+      wc = em.c1 * wc;
+      wc = wc + em.c2 * w2v;
+    } else if (em.c1 != 1.0f) {
+      wc = em.c1 * wc;
+    }
+    em.x[j] = jscc_em_value(xv, wc, sv, em.mode, em.eps);
     w[j] = em.zero_w ? 0.0f : wj;
     if (em.w2 && em.zero_w) em.w2[j] = 0.0f;
   } else {
@@ -780,6 +809,22 @@ __global__ void __launch_bounds__(256) k_reduce_direct(const float* __restrict__
   __shared__ jscc_f4v acc[TY][TX];
   const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
   const int col = blockIdx.x * kDirCols + tx * 4;
+  // the operands of the update, loaded before the partial sums so that their round trip overlaps them
+  float pw[4] = {0.f, 0.f, 0.f, 0.f}, px[4] = {0.f, 0.f, 0.f, 0.f}, ps[4] = {1.f, 1.f, 1.f, 1.f};
+  float pw2[4] = {0.f, 0.f, 0.f, 0.f};
+  if (ty == 0 && col < P) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (col + k < P) {
+        if (accumulate) pw[k] = w[col + k];
+        if (em.x) {
+          px[k] = em.x[col + k];
+          ps[k] = em.s[col + k];
+          if (em.w2) pw2[k] = em.w2[col + k];
+        }
+      }
+    }
+  }
   jscc_f4v sum = jscc_f4v(0.0f);
   if (col < P) {
 #pragma unroll 8
@@ -791,10 +836,10 @@ __global__ void __launch_bounds__(256) k_reduce_direct(const float* __restrict__
     jscc_f4v t = acc[0][tx];
 #pragma unroll
     for (int k = 1; k < TY; ++k) t += acc[k][tx];
-    finish_weight(t.x, col, scale, accumulate, sanitize, w, em);
-    if (col + 1 < P) finish_weight(t.y, col + 1, scale, accumulate, sanitize, w, em);
-    if (col + 2 < P) finish_weight(t.z, col + 2, scale, accumulate, sanitize, w, em);
-    if (col + 3 < P) finish_weight(t.w, col + 3, scale, accumulate, sanitize, w, em);
+    finish_weight_pre(t.x, col, scale, accumulate, sanitize, w, em, pw[0], px[0], ps[0], pw2[0]);
+    if (col + 1 < P) finish_weight_pre(t.y, col + 1, scale, accumulate, sanitize, w, em, pw[1], px[1], ps[1], pw2[1]);
+    if (col + 2 < P) finish_weight_pre(t.z, col + 2, scale, accumulate, sanitize, w, em, pw[2], px[2], ps[2], pw2[2]);
+    if (col + 3 < P) finish_weight_pre(t.w, col + 3, scale, accumulate, sanitize, w, em, pw[3], px[3], ps[3], pw2[3]);
   }
 }
 
@@ -802,6 +847,7 @@ __global__ void __launch_bounds__(256) k_em_update(float* __restrict__ x, const 
                                                    const float* __restrict__ w2, float c2,
                                                    const float* __restrict__ s, int64_t n, int mode, float eps,
                                                    float* __restrict__ zero_after) {
+#pragma clang fp contract(off)
   const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (j >= n) return;
   // w = c1*w1 (+ c2*w2 when w2 is given), evaluated like the reference's
